@@ -1,0 +1,202 @@
+/*
+ * mcfd.h — C-ABI of the MI355X (gfx950) residual hot path.
+ *
+ * Plain C types only: device pointers are `double*` / `int32_t*`, streams are
+ * `hipStream_t` passed as `void*`. Every call returns an int status (0 = ok,
+ * negative = error; `mcfd_last_error` gives the message). Calls are
+ * stream-ordered and never synchronise the host, except `mcfd_status` and
+ * `mcfd_sync`.
+ *
+ * Ownership: the caller owns every state buffer (U, R, dt). A mesh handle owns
+ * the device copy of the geometry and its scratch buffers; it is used by one
+ * stream at a time.
+ *
+ * The Python seams these entry points replace are plain module functions in
+ * the reference's `src/meshcfd/core/solver.py`; a ctypes binding
+ * (meshcfd_amd/native.py) calls them from the solver driver.
+ */
+#ifndef MCFD_H
+#define MCFD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCFD_ABI_VERSION 1
+
+/* Flux-face kinds (face_kind). */
+enum {
+  MCFD_FACE_INTERIOR = 0,
+  MCFD_FACE_PERIODIC = 1,
+  MCFD_FACE_WALL = 2,          /* Wall and Symmetry: mirrored velocity */
+  MCFD_FACE_VELOCITY_INLET = 3,
+  MCFD_FACE_PRESSURE_OUTLET = 4,
+  MCFD_FACE_FAR_FIELD = 5,
+  MCFD_FACE_TRANSMISSIVE = 6
+};
+
+/* Accumulation modes (low 2 bits of acc_item; face index in the high bits). */
+enum {
+  MCFD_ACC_OWNER = 0,      /* interior face, owner side:  (-L/A) * F          */
+  MCFD_ACC_NEIGHBOUR = 1,  /* interior face, neighbour:   ( L/A) * F          */
+  MCFD_ACC_BOUNDARY = 2,   /* boundary / periodic a side: -(F*L)/A            */
+  MCFD_ACC_PERIODIC_B = 3  /* periodic b side:             (F*L)/A            */
+};
+
+/* Error codes reported by mcfd_status (bit i of the mask = code i). */
+enum {
+  MCFD_ERR_STATE = 1,          /* "Euler state has non-finite or non-positive density or pressure." */
+  MCFD_ERR_HLLC = 2,           /* "HLLC flux produced a non-finite value." */
+  MCFD_ERR_ROE_SOUND = 3,      /* "Roe average has a non-positive sound speed." */
+  MCFD_ERR_ROE = 4,            /* "Roe flux produced a non-finite value." */
+  MCFD_ERR_FAR_SOUND = 5,      /* "Far-field characteristics produced invalid sound speed." */
+  MCFD_ERR_FAR_STATE = 6,      /* "Far-field characteristics produced an invalid state." */
+  MCFD_ERR_RESIDUAL = 7,       /* "Spatial residual contains non-finite values." */
+  MCFD_ERR_LOCAL_DT = 8,       /* "Local stable time steps must be positive and finite." */
+  MCFD_ERR_GLOBAL_DT = 9       /* "The stable time step is non-positive or non-finite." */
+};
+
+/*
+ * Geometry in device layout (host pointers; copied by mcfd_mesh_create).
+ * All per-cell lists are CSR (ptr has n_cells+1 entries) and hold their
+ * entries in the reference's summation order.
+ */
+typedef struct McfdMeshDesc {
+  int32_t n_cells;
+  int32_t n_faces;          /* flux faces: interior, periodic pairs, boundary */
+  int32_t n_grad;           /* LSQ directed edges */
+  int32_t n_lim;            /* limiter entries */
+  int32_t n_acc;            /* accumulation items */
+  int32_t n_dts;            /* spectral-radius items */
+  int32_t n_bc;             /* boundary records */
+  int32_t reserved;
+  const double* area;       /* [n_cells] */
+  const int32_t* grad_ptr;  /* [n_cells+1] */
+  const int32_t* grad_nb;   /* [n_grad] neighbour cell */
+  const double* grad_cx;    /* [n_grad] LSQ coefficient, x */
+  const double* grad_cy;    /* [n_grad] LSQ coefficient, y */
+  const int32_t* lim_ptr;   /* [n_cells+1] */
+  const double* lim_dx;     /* [n_lim] face midpoint - centroid */
+  const double* lim_dy;
+  const int32_t* face_left;   /* [n_faces] owner (periodic: owner of face a) */
+  const int32_t* face_right;  /* [n_faces] neighbour (periodic: owner of b), -1 on boundary */
+  const int32_t* face_kind;   /* [n_faces] MCFD_FACE_* */
+  const int32_t* face_bc;     /* [n_faces] boundary record or -1 */
+  const double* face_nx;      /* [n_faces] unit normal (periodic: of face a) */
+  const double* face_ny;
+  const double* face_len;     /* [n_faces] length */
+  const double* face_lx;      /* [n_faces] left offset (face midpoint - left centroid) */
+  const double* face_ly;
+  const double* face_rx;      /* [n_faces] right offset */
+  const double* face_ry;
+  const int32_t* acc_ptr;     /* [n_cells+1] */
+  const int32_t* acc_item;    /* [n_acc] face*4 + MCFD_ACC_* */
+  const int32_t* dts_ptr;     /* [n_cells+1] */
+  const double* dts_nx;       /* [n_dts] face normal */
+  const double* dts_ny;
+  const double* dts_len;      /* [n_dts] face length */
+  const int32_t* dts_bc;      /* [n_dts] boundary record or -1 */
+  const int32_t* bc_kind;     /* [n_bc] MCFD_FACE_* */
+  const double* bc_state;     /* [n_bc*4] exterior primitive; outlet uses [3] = p */
+  const double* bc_wall;      /* [n_bc*2] wall velocity */
+} McfdMeshDesc;
+
+typedef struct McfdSettings {
+  int32_t flux;            /* 0 Rusanov, 1 HLLC, 2 Roe */
+  int32_t reconstruction;  /* 0 first order, 1 MUSCL linear */
+  int32_t limiter;         /* 0 Barth-Jespersen, 1 Minmod, 2 Van Leer, 3 Venkatakrishnan */
+  int32_t reserved;
+  double gamma;
+  double cfl;
+  double acceleration_x;
+  double acceleration_y;
+} McfdSettings;
+
+/* Stage update ops (McfdStage.op). h = dt (scalar dt[0], or dt[i] if dt_per_cell). */
+enum {
+  MCFD_OP_EULER = 0,   /* out = U + (c0*h)*R                                   */
+  MCFD_OP_SSP = 1,     /* out = c0*U0 + c1*(U + h*R)                           */
+  MCFD_OP_RK4 = 2,     /* out = c0*U0 + c1*U + (c2*h)*R                        */
+  MCFD_OP_RK4_FINAL = 3, /* out = c0*U0 + c1*U1 + (c2*h)*R1 + c3*U + (c4*h)*R  */
+  MCFD_OP_RESIDUAL = 4 /* R only (written to r_out)                            */
+};
+
+typedef struct McfdStage {
+  int32_t op;
+  int32_t dt_per_cell;
+  const double* u;       /* state the residual is evaluated at, (C,4) */
+  const double* u0;      /* (C,4) or NULL */
+  const double* u1;      /* (C,4) or NULL (RK4 final: stage three) */
+  const double* r1;      /* (C,4) or NULL (RK4 final: residual at stage three) */
+  const double* dt;      /* device: [1] or [C] */
+  double* out;           /* (C,4) updated state, may be NULL for MCFD_OP_RESIDUAL */
+  double* r_out;         /* (C,4) residual copy, or NULL */
+  double c[5];
+} McfdStage;
+
+typedef struct McfdMesh McfdMesh;
+
+/* Library / ABI version. */
+int mcfd_version(void);
+
+/* Message of the last failing call on this thread. */
+int mcfd_last_error(char* buffer, size_t size);
+
+/* Upload a mesh (replaces `_build_geometry`'s output `_Geometry`, solver.py:122-176). */
+int mcfd_mesh_create(const McfdMeshDesc* desc, McfdMesh** mesh);
+int mcfd_mesh_destroy(McfdMesh* mesh);
+
+/* R = residual(U) (replaces `_residual`, solver.py:2437-2718). */
+int mcfd_residual(McfdMesh* mesh, const McfdSettings* settings, const double* u, double* r,
+                  void* stream);
+
+/* Per-cell stable steps and their minimum (replaces `_stable_time_steps` /
+ * `_stable_time_step`, solver.py:4117-4343). dt_cell [C], dt_min [1], device. */
+int mcfd_time_steps(McfdMesh* mesh, const McfdSettings* settings, const double* u,
+                    double* dt_cell, double* dt_min, void* stream);
+
+/* clock = [time, dt]: dt = min(dt_source[0], end_time - time) (dt_source NULL:
+ * fixed_dt), checked positive/finite, then time += dt (solver.py:452-465, 812). */
+int mcfd_advance_clock(McfdMesh* mesh, const double* dt_source, double fixed_dt,
+                       double end_time, double* clock, void* stream);
+
+/* One residual evaluation fused with an explicit stage update (the inlined
+ * stages of `solve_case`, solver.py:532-756, and `_explicit_rk4_step`,
+ * solver.py:1128-1182). */
+int mcfd_stage(McfdMesh* mesh, const McfdSettings* settings, const McfdStage* stage,
+               void* stream);
+
+/* Per-component sum of R^2 over cells into sums[4] (device), deterministic. */
+int mcfd_sum_squares(McfdMesh* mesh, const double* r, double* sums, void* stream);
+
+/* Synchronise `stream`, read and clear the device status: mask of MCFD_ERR_*
+ * bits and the lowest failing cell/face index. Returns 0. */
+int mcfd_status(McfdMesh* mesh, void* stream, int32_t* mask, int32_t* index);
+
+/* Kernel kinds timed by mcfd_timing. */
+enum {
+  MCFD_KERNEL_PRIMITIVE = 0,
+  MCFD_KERNEL_GRADIENT = 1,
+  MCFD_KERNEL_FLUX = 2,
+  MCFD_KERNEL_UPDATE = 3,
+  MCFD_KERNEL_TIME_STEP = 4,
+  MCFD_KERNEL_REDUCE = 5,
+  MCFD_KERNEL_STAGE = 6,
+  MCFD_KERNEL_KINDS = 8
+};
+
+/* Per-launch HIP-event timing on the launch stream, for benchmarking.
+ * enable = 1: start recording (clears totals); enable = 0: stop.
+ * On return (after synchronising the recorded events) total_ms[k] and
+ * counts[k], k < MCFD_KERNEL_KINDS, hold the summed durations and launch
+ * counts per kernel kind since recording started. Either array may be NULL. */
+int mcfd_timing(McfdMesh* mesh, int32_t enable, double* total_ms, int32_t* counts);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MCFD_H */

@@ -1,0 +1,227 @@
+"""Device-resident residual / time-step / stage driver over the C-ABI.
+
+`DeviceMesh` uploads a case's geometry once (`mcfd_mesh_create`) and runs the
+reference solver's per-step sequence (`solve_case`, `solver.py:437-812`) on
+device buffers:
+
+    time step:  mcfd_time_steps -> (global) mcfd_advance_clock
+    stages:     mcfd_stage(op)  -- residual + stage update fused per stage
+
+PyTorch (ROCm) is used only to own device memory and to name the stream; the
+arithmetic is in libmcfdhip.so.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import native
+from .geometry import SolverError, build_geometry
+from .layout import build_layout
+from .model import FLUX_CODES, LIMITER_CODES, RECONSTRUCTION_CODES, Case, NumericalSettings
+
+RK4_COEFFS = (
+    (0.391752226571890,),
+    (0.444370493651235, 0.555629506348765, 0.368410593050371),
+    (0.620101851488403, 0.379898148511597, 0.251891774271694),
+    (0.178079954393132, 0.821920045606868, 0.544974750228521),
+    (0.517231671970585, 0.096059710526147, 0.063692468666290, 0.386708617503269, 0.226007483236906),
+)
+
+
+def native_settings(s: NumericalSettings) -> native.McfdSettings:
+    if s.reconstruction not in RECONSTRUCTION_CODES:
+        raise SolverError(f"Unsupported reconstruction '{s.reconstruction}'.")
+    if s.reconstruction == "MUSCL parabolic":
+        raise SolverError("MUSCL parabolic reconstruction is not implemented by the GPU residual.")
+    if s.limiter not in LIMITER_CODES:
+        raise SolverError(f"Unsupported limiter '{s.limiter}'.")
+    if s.flux not in FLUX_CODES:
+        raise SolverError(f"Unsupported convective flux '{s.flux}'.")
+    if s.viscosity_model != "Inviscid":
+        raise SolverError("Viscous residuals are not implemented by the GPU residual.")
+    return native.McfdSettings(
+        flux=FLUX_CODES[s.flux],
+        reconstruction=RECONSTRUCTION_CODES[s.reconstruction],
+        limiter=LIMITER_CODES[s.limiter],
+        reserved=0,
+        gamma=s.gamma,
+        cfl=s.cfl,
+        acceleration_x=s.acceleration_x,
+        acceleration_y=s.acceleration_y,
+    )
+
+
+def _ptr(t: torch.Tensor | None):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+class DeviceMesh:
+    """One mesh uploaded to one GPU, plus the step buffers of its case."""
+
+    def __init__(self, case: Case, device: str | torch.device = "cuda:0", geometry=None, layout=None):
+        if not torch.cuda.is_available():
+            raise native.NativeError("No ROCm device is visible: the GPU residual cannot run.")
+        self.lib = native.load()
+        self.case = case
+        self.settings = case.numerics
+        self.device = torch.device(device)
+        self.geometry = geometry if geometry is not None else build_geometry(case)
+        self.layout = layout if layout is not None else build_layout(self.geometry)
+        self.n = self.layout.n_cells
+        with torch.cuda.device(self.device):
+            desc, keep = native.mesh_desc(self.layout)
+            handle = C.c_void_p()
+            native.check(self.lib.mcfd_mesh_create(C.byref(desc), C.byref(handle)))
+            del keep
+        self.handle = handle
+        self.native_settings = native_settings(self.settings)
+        opts = dict(dtype=torch.float64, device=self.device)
+        n = self.n
+        self.u = torch.empty((n, 4), **opts)
+        self.s1 = torch.empty((n, 4), **opts)
+        self.s2 = torch.empty((n, 4), **opts)
+        self.s3 = torch.empty((n, 4), **opts)
+        self.r3 = torch.empty((n, 4), **opts)
+        self.r0 = torch.empty((n, 4), **opts)
+        self.dt_cell = torch.empty((n,), **opts)
+        self.dt_min = torch.empty((1,), **opts)
+        self.clock = torch.zeros((2,), **opts)
+        self.sums = torch.zeros((4,), **opts)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.mcfd_mesh_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- plumbing -----------------------------------------------------------
+    @property
+    def stream(self):
+        return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def check_status(self, context: str = "") -> None:
+        mask = C.c_int32()
+        index = C.c_int32()
+        native.check(self.lib.mcfd_status(self.handle, self.stream, C.byref(mask), C.byref(index)))
+        if mask.value:
+            code = min(k for k in native.ERROR_MESSAGES if mask.value & (1 << k))
+            raise SolverError(f"{context}{native.ERROR_MESSAGES[code]}")
+
+    def set_state(self, conservative) -> None:
+        self.u.copy_(torch.as_tensor(np.ascontiguousarray(conservative, dtype=np.float64)))
+
+    def state(self) -> np.ndarray:
+        return self.u.cpu().numpy()
+
+    def _stage(self, op, u, out=None, u0=None, u1=None, r1=None, dt=None, per_cell=False, coeffs=(), r_out=None):
+        st = native.McfdStage()
+        st.op = op
+        st.dt_per_cell = 1 if per_cell else 0
+        st.u = _ptr(u)
+        st.u0 = _ptr(u0)
+        st.u1 = _ptr(u1)
+        st.r1 = _ptr(r1)
+        st.dt = _ptr(dt)
+        st.out = _ptr(out)
+        st.r_out = _ptr(r_out)
+        for k, v in enumerate(coeffs):
+            st.c[k] = v
+        native.check(self.lib.mcfd_stage(self.handle, C.byref(self.native_settings), C.byref(st), self.stream))
+
+    # -- seams (same meaning as the reference's module functions) ------------
+    def residual(self, u: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """`_residual` (`solver.py:2437-2718`) on a device state (C,4)."""
+        out = torch.empty_like(u) if out is None else out
+        self._stage(native.OP_RESIDUAL, u, r_out=out)
+        return out
+
+    def time_steps(self, u: torch.Tensor) -> torch.Tensor:
+        """`_stable_time_steps` (`solver.py:4132-4343`): per-cell steps into
+        ``dt_cell`` and their minimum into ``dt_min``."""
+        native.check(
+            self.lib.mcfd_time_steps(
+                self.handle, C.byref(self.native_settings), _ptr(u), _ptr(self.dt_cell), _ptr(self.dt_min), self.stream
+            )
+        )
+        return self.dt_cell
+
+    def sum_squares(self, r: torch.Tensor) -> torch.Tensor:
+        native.check(self.lib.mcfd_sum_squares(self.handle, _ptr(r), _ptr(self.sums), self.stream))
+        return self.sums
+
+    # -- one explicit step --------------------------------------------------
+    def step(self, time: float | None = None, residual_out: torch.Tensor | None = None) -> None:
+        """Advance ``self.u`` by one explicit step on the stream; no host sync.
+
+        Global modes keep [time, dt] in ``self.clock`` on the device. When
+        ``time`` is given the clock's time is set to it first.
+        """
+        s = self.settings
+        mode = s.time_step_mode
+        local = mode == "Local (steady)"
+        if time is not None:
+            self.clock[0] = time
+        if local:
+            self.time_steps(self.u)
+            dt, per_cell = self.dt_cell, True
+        elif mode == "Global fixed":
+            native.check(self.lib.mcfd_advance_clock(self.handle, None, s.fixed_time_step, s.end_time, _ptr(self.clock), self.stream))
+            dt, per_cell = self.clock[1:], False
+        else:
+            self.time_steps(self.u)
+            native.check(
+                self.lib.mcfd_advance_clock(self.handle, _ptr(self.dt_min), 0.0, s.end_time, _ptr(self.clock), self.stream)
+            )
+            dt, per_cell = self.clock[1:], False
+        ti = s.time_integration
+        u = self.u
+        if ti == "Explicit Euler":
+            self._stage(native.OP_EULER, u, out=self.s1, dt=dt, per_cell=per_cell, coeffs=(1.0,), r_out=residual_out)
+            self.u, self.s1 = self.s1, u
+        elif ti == "Explicit RK2":
+            self._stage(native.OP_EULER, u, out=self.s1, dt=dt, per_cell=per_cell, coeffs=(1.0,), r_out=residual_out)
+            self._stage(native.OP_SSP, self.s1, out=self.s2, u0=u, dt=dt, per_cell=per_cell, coeffs=(0.5, 0.5))
+            self.u, self.s2 = self.s2, u
+        elif ti == "Explicit RK3":
+            self._stage(native.OP_EULER, u, out=self.s1, dt=dt, per_cell=per_cell, coeffs=(1.0,), r_out=residual_out)
+            self._stage(native.OP_SSP, self.s1, out=self.s2, u0=u, dt=dt, per_cell=per_cell, coeffs=(0.75, 0.25))
+            self._stage(native.OP_SSP, self.s2, out=self.s1, u0=u, dt=dt, per_cell=per_cell, coeffs=(1.0 / 3.0, 2.0 / 3.0))
+            self.u, self.s1 = self.s1, u
+        elif ti == "Explicit RK4":
+            c = RK4_COEFFS
+            if residual_out is not None:
+                self.residual(u, out=residual_out)
+            self._stage(native.OP_EULER, u, out=self.s1, dt=dt, per_cell=per_cell, coeffs=c[0])
+            self._stage(native.OP_RK4, self.s1, out=self.s2, u0=u, dt=dt, per_cell=per_cell, coeffs=c[1])
+            self._stage(native.OP_RK4, self.s2, out=self.s3, u0=u, dt=dt, per_cell=per_cell, coeffs=c[2], r_out=None)
+            # stage four needs R(stage three) kept for the final combination
+            self._stage(native.OP_RK4, self.s3, out=self.s1, u0=u, dt=dt, per_cell=per_cell, coeffs=c[3], r_out=self.r3)
+            # final: a*S2 + b*S3 + (c*dt)*R3 + d*S4 + (e*dt)*R(S4)
+            self._stage(native.OP_RK4_FINAL, self.s1, out=u, u0=self.s2, u1=self.s3, r1=self.r3, dt=dt, per_cell=per_cell, coeffs=c[4])
+        else:
+            raise SolverError(f"Unsupported time integration '{ti}' for the explicit GPU path.")
+
+    def run(self, steps: int, time: float = 0.0, check_every: int = 0) -> float:
+        """Advance ``steps`` steps; returns the final time (one host sync)."""
+        self.clock[0] = time
+        for k in range(steps):
+            self.step()
+            if check_every and (k + 1) % check_every == 0:
+                self.check_status(f"Solver failed at step {k + 1}: ")
+        self.check_status("Solver failed: ")
+        return float(self.clock[0].item())
+
+    def timing(self, enable: bool):
+        total = (C.c_double * 8)()
+        counts = (C.c_int32 * 8)()
+        native.check(self.lib.mcfd_timing(self.handle, 1 if enable else 0, total, counts))
+        return {native.KERNEL_NAMES[k]: (total[k], counts[k]) for k in range(8) if counts[k]}

@@ -1,0 +1,322 @@
+"""Seeded mesh generators for the workloads this package is measured on.
+
+Numbering follows the reference's structured-mesh builders
+(`benchmarks/common.py:53-113`, `tests/test_solver.py:57-131`): cells are
+visited in row-major order, each cell's edges in its CCW vertex order, and the
+first cell to meet an edge owns it; the face keeps that cell's CCW node order.
+Boundary patches are numbered bottom 0, right 1, top 2, left 3.
+
+All generators are vectorised (NumPy) so that 1M-4M cell meshes build in
+seconds; the reference builds its meshes with Python loops.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .model import (
+    BoundaryCondition,
+    Case,
+    FiniteVolumeMesh,
+    MeshPatch,
+    NumericalSettings,
+    PeriodicBoundaryPair,
+)
+
+
+def _faces_from_cells(cells: np.ndarray, sizes: np.ndarray):
+    """Derive faces, owners and neighbours from CCW cell vertex lists.
+
+    Edges are enumerated cell by cell, edge by edge; the first occurrence of an
+    undirected edge creates the face (so face ids grow in first-visit order).
+    """
+    cell_count, width = cells.shape
+    slots = np.arange(width)
+    valid = slots[None, :] < sizes[:, None]
+    nxt = np.where(slots[None, :] + 1 < sizes[:, None], slots[None, :] + 1, 0)
+    start = cells[valid]
+    end = np.take_along_axis(cells, nxt, axis=1)[valid]
+    edge_cell = np.broadcast_to(np.arange(cell_count)[:, None], cells.shape)[valid]
+    lo = np.minimum(start, end)
+    hi = np.maximum(start, end)
+    node_count = int(cells.max()) + 1
+    key = lo.astype(np.int64) * node_count + hi
+    order = np.argsort(key, kind="stable")
+    sorted_key = key[order]
+    first = np.ones(len(order), dtype=bool)
+    first[1:] = sorted_key[1:] != sorted_key[:-1]
+    group = np.cumsum(first) - 1  # group id per sorted edge
+    first_pos = order[first]  # edge index of first occurrence, per group
+    counts = np.diff(np.append(np.flatnonzero(first), len(order)))
+    if np.any(counts > 2):
+        raise ValueError("An edge is shared by more than two cells.")
+    # Faces numbered by first visit.
+    face_rank = np.empty(len(first_pos), dtype=np.int64)
+    face_rank[np.argsort(first_pos, kind="stable")] = np.arange(len(first_pos))
+    second = np.full(len(first_pos), -1, dtype=np.int64)
+    has_second = counts == 2
+    second_edge = order[np.flatnonzero(first)[has_second] + 1]
+    second[has_second] = edge_cell[second_edge]
+    face_count = len(first_pos)
+    faces = np.empty((face_count, 2), dtype=np.int64)
+    owners = np.empty(face_count, dtype=np.int64)
+    neighbours = np.empty(face_count, dtype=np.int64)
+    faces[face_rank, 0] = start[first_pos]
+    faces[face_rank, 1] = end[first_pos]
+    owners[face_rank] = edge_cell[first_pos]
+    neighbours[face_rank] = second
+    del group
+    return faces, owners, neighbours
+
+
+def _grid_mesh(
+    nx: int,
+    ny: int,
+    x0: float,
+    x1: float,
+    y0: float,
+    y1: float,
+    jitter: float,
+    rng: np.random.Generator | None,
+    split_fraction: float = 0.0,
+):
+    hx = (x1 - x0) / nx
+    hy = (y1 - y0) / ny
+    i = np.arange(nx + 1)
+    j = np.arange(ny + 1)
+    xs = x0 + hx * i
+    ys = y0 + hy * j
+    xs[-1] = x1
+    ys[-1] = y1
+    px = np.broadcast_to(xs[None, :], (ny + 1, nx + 1)).copy()
+    py = np.broadcast_to(ys[:, None], (ny + 1, nx + 1)).copy()
+    if jitter > 0.0 and nx > 1 and ny > 1:
+        offsets = rng.uniform(-jitter, jitter, size=(ny - 1, nx - 1, 2))
+        px[1:-1, 1:-1] += offsets[..., 0] * hx
+        py[1:-1, 1:-1] += offsets[..., 1] * hy
+    points = np.column_stack((px.ravel(), py.ravel()))
+
+    jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    bl = (jj * (nx + 1) + ii).ravel()
+    br = bl + 1
+    tr = br + nx + 1
+    tl = bl + nx + 1
+    quad_count = nx * ny
+    if split_fraction > 0.0:
+        split_count = int(round(split_fraction * quad_count))
+        chosen = rng.choice(quad_count, size=split_count, replace=False)
+        diagonal = rng.integers(0, 2, size=quad_count)
+        split = np.zeros(quad_count, dtype=bool)
+        split[chosen] = True
+    else:
+        split = np.zeros(quad_count, dtype=bool)
+        diagonal = np.zeros(quad_count, dtype=np.int64)
+    per_quad = np.where(split, 2, 1)
+    first_cell = np.concatenate(([0], np.cumsum(per_quad)[:-1]))
+    cell_count = int(per_quad.sum())
+    cells = np.full((cell_count, 4), -1, dtype=np.int64)
+    sizes = np.empty(cell_count, dtype=np.int64)
+    q = ~split
+    cells[first_cell[q]] = np.column_stack((bl[q], br[q], tr[q], tl[q]))
+    sizes[first_cell[q]] = 4
+    d0 = split & (diagonal == 0)  # diagonal bl-tr
+    c = first_cell[d0]
+    cells[c, :3] = np.column_stack((bl[d0], br[d0], tr[d0]))
+    cells[c + 1, :3] = np.column_stack((bl[d0], tr[d0], tl[d0]))
+    d1 = split & (diagonal == 1)  # diagonal br-tl
+    c = first_cell[d1]
+    cells[c, :3] = np.column_stack((bl[d1], br[d1], tl[d1]))
+    cells[c + 1, :3] = np.column_stack((br[d1], tr[d1], tl[d1]))
+    sizes[first_cell[split]] = 3
+    sizes[first_cell[split] + 1] = 3
+
+    faces, owners, neighbours = _faces_from_cells(cells, sizes)
+    node_i = np.tile(i, ny + 1)
+    node_j = np.repeat(j, nx + 1)
+    a, b = faces[:, 0], faces[:, 1]
+    boundary = neighbours < 0
+    patch_ids = np.full(len(faces), -1, dtype=np.int64)
+    patch_ids[boundary & (node_j[a] == 0) & (node_j[b] == 0)] = 0
+    patch_ids[boundary & (node_i[a] == nx) & (node_i[b] == nx)] = 1
+    patch_ids[boundary & (node_j[a] == ny) & (node_j[b] == ny)] = 2
+    patch_ids[boundary & (node_i[a] == 0) & (node_i[b] == 0)] = 3
+    if np.any(patch_ids[boundary] < 0):
+        raise ValueError("Boundary face not on the rectangle's sides.")
+    patches = tuple(
+        MeshPatch(pid, name, np.flatnonzero(patch_ids == pid))
+        for pid, name in enumerate(("bottom", "right", "top", "left"))
+    )
+    return FiniteVolumeMesh(
+        points=points,
+        cells=cells,
+        cell_sizes=sizes,
+        faces=faces,
+        face_owners=owners,
+        face_neighbours=neighbours,
+        face_patch_ids=patch_ids,
+        patches=patches,
+    )
+
+
+def rectangle_mesh(
+    nx: int,
+    ny: int,
+    bounds: tuple[float, float, float, float] = (0.0, 1.0, 0.0, 1.0),
+    jitter: float = 0.0,
+    seed: int | None = None,
+    split_fraction: float = 0.0,
+) -> FiniteVolumeMesh:
+    """Quad grid on a rectangle, interior nodes jittered by U(-jitter*h, jitter*h).
+
+    With ``split_fraction`` > 0 that fraction of quads (sampled without
+    replacement) is split into two CCW triangles along a random diagonal; the
+    two triangles take consecutive cell ids in the quad's place. The jitter is
+    drawn before the split choice from the same generator.
+    """
+    rng = np.random.default_rng(seed) if (jitter > 0.0 or split_fraction > 0.0) else None
+    x0, x1, y0, y1 = bounds
+    return _grid_mesh(nx, ny, x0, x1, y0, y1, jitter, rng, split_fraction)
+
+
+def cell_centroids(mesh: FiniteVolumeMesh) -> np.ndarray:
+    """Shoelace centroids in the reference's summation order
+    (`initial_conditions.py:56-75`, identical to `solver.py:1655-1672`)."""
+    pts = mesh.points
+    cells, sizes = mesh.cells, mesh.cell_sizes
+    twice = np.zeros(len(cells))
+    xs = np.zeros(len(cells))
+    ys = np.zeros(len(cells))
+    for k in range(cells.shape[1]):
+        active = sizes > k
+        nk = np.where(k + 1 < sizes, k + 1, 0)
+        a = cells[:, k]
+        b = cells[np.arange(len(cells)), nk]
+        x0 = pts[np.where(active, a, 0), 0]
+        y0 = pts[np.where(active, a, 0), 1]
+        x1 = pts[np.where(active, b, 0), 0]
+        y1 = pts[np.where(active, b, 0), 1]
+        cross = x0 * y1 - x1 * y0
+        twice = np.where(active, twice + cross, twice)
+        xs = np.where(active, xs + (x0 + x1) * cross, xs)
+        ys = np.where(active, ys + (y0 + y1) * cross, ys)
+    return np.column_stack((xs / (3.0 * twice), ys / (3.0 * twice)))
+
+
+# ---------------------------------------------------------------------------
+# Initial states
+# ---------------------------------------------------------------------------
+
+
+def isentropic_vortex(
+    centers: np.ndarray,
+    strength: float = 5.0,
+    gamma: float = 1.4,
+    free_stream=(1.0, 1.0, 1.0, 1.0),
+    center=(0.0, 0.0),
+) -> np.ndarray:
+    """Shu's isentropic vortex sampled at cell centroids -> (C,4) primitive.
+
+    Perturbations: du = -b/(2pi) e^((1-r^2)/2) y, dv = b/(2pi) e^((1-r^2)/2) x,
+    dT = -(gamma-1) b^2/(8 gamma pi^2) e^(1-r^2), with T = p/rho and the free
+    stream rho = p = 1 so that rho = T^(1/(gamma-1)), p = rho^gamma.
+    """
+    rho_inf, u_inf, v_inf, p_inf = free_stream
+    x = centers[:, 0] - center[0]
+    y = centers[:, 1] - center[1]
+    r2 = x * x + y * y
+    bump = np.exp(0.5 * (1.0 - r2))
+    du = -strength / (2.0 * math.pi) * bump * y
+    dv = strength / (2.0 * math.pi) * bump * x
+    t_inf = p_inf / rho_inf
+    dt = -(gamma - 1.0) * strength**2 / (8.0 * gamma * math.pi**2) * bump * bump
+    temperature = t_inf + dt
+    entropy = p_inf / rho_inf**gamma
+    rho = (temperature / (entropy)) ** (1.0 / (gamma - 1.0))
+    p = rho * temperature
+    return np.column_stack((rho, u_inf + du, v_inf + dv, p))
+
+
+FOUR_QUADRANT_STATES = (
+    (1.5, 0.0, 0.0, 1.5),  # x > xc, y > yc
+    (0.5323, 1.206, 0.0, 0.3),  # x < xc, y > yc
+    (0.138, 1.206, 1.206, 0.029),  # x < xc, y < yc
+    (0.5323, 0.0, 1.206, 0.3),  # x > xc, y < yc
+)
+
+
+def four_quadrant(centers: np.ndarray, split=(0.5, 0.5)) -> np.ndarray:
+    x = centers[:, 0]
+    y = centers[:, 1]
+    right = x > split[0]
+    top = y > split[1]
+    quadrant = np.where(top, np.where(right, 0, 1), np.where(right, 3, 2))
+    return np.asarray(FOUR_QUADRANT_STATES, dtype=float)[quadrant]
+
+
+# ---------------------------------------------------------------------------
+# Workload cases
+# ---------------------------------------------------------------------------
+
+
+def periodic_case(mesh: FiniteVolumeMesh, numerics: NumericalSettings) -> Case:
+    """All four sides periodic, paired right<->left (x) and bottom<->top (y),
+    as the reference's `make_case` does (`benchmarks/common.py:116-131`)."""
+    x = mesh.points[:, 0]
+    y = mesh.points[:, 1]
+    bcs = [
+        BoundaryCondition(p.id, p.name, kind="Periodic") for p in mesh.patches
+    ]
+    pairs = [
+        PeriodicBoundaryPair(1, 3, "x", float(x.max() - x.min())),
+        PeriodicBoundaryPair(0, 2, "y", float(y.max() - y.min())),
+    ]
+    return Case(mesh=mesh, boundary_conditions=bcs, periodic_pairs=pairs, numerics=numerics)
+
+
+def vortex_workload(n: int = 64, steps: int = 100):
+    """Isentropic vortex (strength 5) on the periodic square [-5,5]^2, n x n
+    quads, interior nodes jittered by U(-0.2h, 0.2h) from seed 0; Rusanov,
+    first order, explicit Euler, CFL 0.5."""
+    mesh = rectangle_mesh(n, n, (-5.0, 5.0, -5.0, 5.0), jitter=0.2, seed=0)
+    numerics = NumericalSettings(
+        reconstruction="First order",
+        flux="Rusanov",
+        time_integration="Explicit Euler",
+        time_step_mode="Global adaptive",
+        cfl=0.5,
+        max_iterations=steps,
+        end_time=1.0e30,
+        residual_tolerance=1.0e-300,
+        convergence_history_interval=max(steps + 1, 10),
+    )
+    case = periodic_case(mesh, numerics)
+    primitive = isentropic_vortex(cell_centroids(mesh))
+    return case, primitive
+
+
+def riemann_workload(n: int = 1000, steps: int = 200, split_fraction: float = 0.3):
+    """Four-quadrant Riemann problem on [0,1]^2: n x n quads, interior jitter
+    U(-0.15h, 0.15h) and 30% of quads split into triangles, both from seed 1;
+    HLLC + MUSCL linear (Barth-Jespersen), SSP-RK3, CFL 0.8, transmissive
+    boundaries; quadrant split point (0.5, 0.5)."""
+    mesh = rectangle_mesh(
+        n, n, (0.0, 1.0, 0.0, 1.0), jitter=0.15, seed=1, split_fraction=split_fraction
+    )
+    numerics = NumericalSettings(
+        reconstruction="MUSCL linear",
+        limiter="Barth-Jespersen",
+        flux="HLLC",
+        time_integration="Explicit RK3",
+        time_step_mode="Global adaptive",
+        cfl=0.8,
+        max_iterations=steps,
+        end_time=1.0e30,
+        residual_tolerance=1.0e-300,
+        convergence_history_interval=max(steps + 1, 10),
+    )
+    bcs = [BoundaryCondition(p.id, p.name, kind="Transmissive") for p in mesh.patches]
+    case = Case(mesh=mesh, boundary_conditions=bcs, numerics=numerics)
+    primitive = four_quadrant(cell_centroids(mesh))
+    return case, primitive

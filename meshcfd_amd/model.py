@@ -1,0 +1,150 @@
+"""Case, mesh and numerical-settings types of the residual hot path.
+
+The field names and string values follow the reference data model
+(`src/meshcfd/core/model.py:9-34` enums, `:60-74` FiniteVolumeMesh,
+`:77-93` BoundaryCondition/PeriodicBoundaryPair, `:95-129` NumericalSettings),
+so a caller of the reference solver passes the same settings here.
+
+Meshes are held as NumPy arrays instead of nested tuples: cells are a padded
+``(C, max_vertices)`` int array plus a per-cell vertex count, which lets the
+geometry builder run vectorised at millions of cells.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+BOUNDARY_TYPES = (
+    "Wall",
+    "Velocity Inlet",
+    "Pressure Inlet",
+    "Pressure Outlet",
+    "Mass Flow Inlet",
+    "Mass Flow Outlet",
+    "Symmetry",
+    "Far Field",
+    "Periodic",
+    # No counterpart in the reference: exterior state = interior reconstructed
+    # state, so the face flux is the numerical flux F(W_L, W_L).
+    "Transmissive",
+)
+RECONSTRUCTION_SCHEMES = ("First order", "MUSCL linear", "MUSCL parabolic")
+LIMITER_SCHEMES = ("Barth-Jespersen", "Venkatakrishnan", "Minmod", "Van Leer")
+FLUX_SCHEMES = ("HLLC", "Roe", "Rusanov")
+TIME_INTEGRATION_SCHEMES = (
+    "Explicit Euler",
+    "Explicit RK2",
+    "Explicit RK3",
+    "Explicit RK4",
+    "Implicit Euler",
+)
+TIME_STEP_MODES = ("Global adaptive", "Global fixed", "Local (steady)")
+
+# Integer codes shared with the C-ABI (include/mcfd.h).
+LIMITER_CODES = {"Barth-Jespersen": 0, "Minmod": 1, "Van Leer": 2, "Venkatakrishnan": 3}
+FLUX_CODES = {"Rusanov": 0, "HLLC": 1, "Roe": 2}
+RECONSTRUCTION_CODES = {"First order": 0, "MUSCL linear": 1, "MUSCL parabolic": 2}
+
+
+@dataclass
+class MeshPatch:
+    id: int
+    name: str
+    face_ids: np.ndarray
+
+
+@dataclass
+class FiniteVolumeMesh:
+    """Polygon mesh (reference `model.py:60-74`) with array storage.
+
+    ``cells[i, :cell_sizes[i]]`` lists the CCW vertex ids of cell ``i``.
+    ``faces[f] = (n0, n1)`` in the owner's CCW order; ``face_neighbours[f]``
+    is ``-1`` on a boundary face.
+    """
+
+    points: np.ndarray  # (N, 2) float64
+    cells: np.ndarray  # (C, V) int64, padded with -1
+    cell_sizes: np.ndarray  # (C,) int64
+    faces: np.ndarray  # (F, 2) int64
+    face_owners: np.ndarray  # (F,) int64
+    face_neighbours: np.ndarray  # (F,) int64
+    face_patch_ids: np.ndarray  # (F,) int64, -1 on interior faces
+    patches: tuple[MeshPatch, ...]
+
+    @property
+    def cell_count(self) -> int:
+        return int(self.cells.shape[0])
+
+    @property
+    def face_count(self) -> int:
+        return int(self.faces.shape[0])
+
+
+@dataclass
+class BoundaryCondition:
+    patch_id: int
+    patch_name: str
+    kind: str = "Wall"
+    values: dict = field(default_factory=dict)
+    thermal_condition: str = "Temperature"
+    mechanical_condition: str = "No-slip"
+
+
+@dataclass(frozen=True)
+class PeriodicBoundaryPair:
+    patch_a_id: int
+    patch_b_id: int
+    axis: str
+    pitch: float
+
+
+@dataclass
+class NumericalSettings:
+    """Reference `model.py:95-129`, same defaults."""
+
+    reconstruction: str = "MUSCL linear"
+    limiter: str = "Barth-Jespersen"
+    flux: str = "HLLC"
+    time_integration: str = "Explicit RK2"
+    time_step_mode: str = "Global adaptive"
+    cfl: float = 0.5
+    fixed_time_step: float = 0.001
+    gmres_tolerance: float = 1e-6
+    gmres_max_iterations: int = 300
+    gmres_krylov_dimension: int = 30
+    implicit_preconditioner: str = "LU-SGS"
+    max_iterations: int = 1000
+    end_time: float = 1.0
+    residual_tolerance: float = 1e-6
+    convergence_history_interval: int = 10
+    gamma: float = 1.4
+    gas_constant: float = 287.05
+    viscosity_model: str = "Inviscid"
+    dynamic_viscosity: float = 1.8e-5
+    sutherland_reference_viscosity: float = 1.716e-5
+    sutherland_reference_temperature: float = 273.15
+    sutherland_temperature: float = 110.4
+    prandtl_number: float = 0.72
+    acceleration_x: float = 0.0
+    acceleration_y: float = 0.0
+
+
+@dataclass
+class Case:
+    """A mesh with its boundary conditions, periodic pairs and numerics."""
+
+    mesh: FiniteVolumeMesh
+    boundary_conditions: list[BoundaryCondition]
+    periodic_pairs: list[PeriodicBoundaryPair] = field(default_factory=list)
+    numerics: NumericalSettings = field(default_factory=NumericalSettings)
+    name: str = "case"
+
+
+@dataclass(frozen=True)
+class InitialState:
+    density: float = 1.225
+    velocity_x: float = 0.0
+    velocity_y: float = 0.0
+    pressure: float = 101325.0
