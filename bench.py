@@ -1,0 +1,256 @@
+"""Benchmark of the residual hot path on one MI355X.
+
+Workload: four-quadrant Riemann problem on [0,1]^2, 1000x1000 quads with
+0.15h jitter and 30% of quads split into triangles (seed 1): 1,300,000 cells,
+HLLC + limited MUSCL-linear (Barth-Jespersen), SSP-RK3, CFL 0.8, FP64,
+transmissive boundaries. One "step" = one SSP-RK3 step (CFL time step + three
+residual/stage evaluations) over the whole mesh.
+
+Prints one JSON line. ``value`` = cells * steps / seconds / 1e6
+(Mcell-updates/s) with the state resident in HBM; ``roofline`` reports the
+dominant kernel against the 8 TB/s HBM peak from its algorithmic bytes; the
+``stage`` figures use B_alg per cell-update over the whole step;
+``cpu_baseline`` times the CPU oracle on one host core on a bounded sample.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(layout, reconstruction: str, integrator: str):
+    """B_alg per cell-update and per-kernel compulsory bytes per launch.
+
+    Per stage: 32 (U in) + 32 (U out) + 32*[stage reads U0] + g_cell +
+    g_face*F/C with MUSCL g_cell = 24 (1/A + centroid), g_face = 40
+    (owner+neighbour int32, n*L 2xfp64, face midpoint 2xfp64); first order
+    g_cell = 8, g_face = 24.
+    """
+    c = layout.n_cells
+    f = layout.n_faces
+    muscl = reconstruction != "First order"
+    g_cell, g_face = (24.0, 40.0) if muscl else (8.0, 24.0)
+    base = 64.0 + g_cell + g_face * f / c
+    stages = {"Explicit Euler": [0], "Explicit RK2": [0, 1], "Explicit RK3": [0, 1, 1]}[integrator]
+    b_alg = sum(base + 32.0 * u0 for u0 in stages)
+    n_grad = len(layout.grad_nb)
+    n_lim = len(layout.lim_dx)
+    n_acc = len(layout.acc_item)
+    n_dts = len(layout.dts_nx)
+    kernels = {
+        # U in (32), W out (32)
+        "primitive": 64.0 * c,
+        # W (32) + ptrs (8) + per LSQ entry nb+cx+cy (20) + per limiter entry dx,dy (16) + G,PHI out (96)
+        "gradient": (32 + 8 + 96) * c + 20.0 * n_grad + 16.0 * n_lim,
+        # per face: ids, kind, bc (16) + n, L (24) + offsets (32) + flux out (32); W,G,PHI once per cell (128)
+        "flux": 104.0 * f + (128.0 if muscl else 32.0) * c,
+        # per cell: ptr (4) + area (8) + U, U0, out (96); per item 4; per face flux + length once (40)
+        "update": (4 + 8 + 96) * c + 4.0 * n_acc + 40.0 * f,
+        # per cell: W (32) + ptr (4) + area (8) + dt out (8); per entry n, L, bc (28)
+        "time_step": (32 + 4 + 8 + 8) * c + 28.0 * n_dts,
+    }
+    return b_alg, kernels
+
+
+def cpu_baseline(case, geometry, primitive, steps: int):
+    """Time the CPU oracle (NumPy restatement of the reference) on one core."""
+    from oracle.euler import primitive_to_conservative
+    from oracle.solver import Oracle
+
+    old = os.sched_getaffinity(0)
+    core = min(old)
+    os.sched_setaffinity(0, {core})
+    try:
+        oracle = Oracle(geometry, case.numerics)
+        u = primitive_to_conservative(primitive, case.numerics.gamma)
+        oracle.residual(u)  # warm-up residual (benchmarks/run_benchmarks.py:266-267)
+        t0 = time.perf_counter()
+        t = 0.0
+        for _ in range(steps):
+            u, dt, _, _ = oracle.step(u, t)
+            t += dt
+        elapsed = time.perf_counter() - t0
+    finally:
+        os.sched_setaffinity(0, old)
+    cells = geometry.cell_count
+    return {
+        "value": cells * steps / elapsed / 1e6,
+        "unit": "Mcell-updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{steps} SSP-RK3 step(s) of the full {cells}-cell mesh, NumPy/SciPy oracle, pinned to 1 core",
+        "seconds": elapsed,
+        "host_cores": os.cpu_count(),
+        "cpu_model": platform.processor() or platform.machine(),
+    }
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=1000, help="quads per side")
+    ap.add_argument("--cpu-steps", type=int, default=1, help="SSP-RK3 steps timed on the CPU oracle (0 = skip)")
+    ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument(
+        "--copy-probe",
+        type=int,
+        default=0,
+        metavar="BYTES",
+        help="also run the plain copy kernel k_copy over BYTES (HBM counter calibration)",
+    )
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(
+            json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is design-only", "n_gpus": args.gpus}),
+            flush=True,
+        )
+        return 2
+
+    import torch
+
+    from meshcfd_amd import meshgen
+    from meshcfd_amd.device import DeviceMesh
+    from meshcfd_amd.geometry import build_geometry
+    from meshcfd_amd.layout import build_layout
+
+    if args.copy_probe:
+        import ctypes
+
+        from meshcfd_amd import native
+
+        lib = native.load()
+        count = args.copy_probe // 8
+        src = torch.ones(count, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        for _ in range(3):
+            native.check(lib.mcfd_copy(ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()), count, s))
+        torch.cuda.synchronize()
+        del src, dst
+
+    t0 = time.perf_counter()
+    case, primitive = meshgen.riemann_workload(args.n, args.steps)
+    geometry = build_geometry(case)
+    layout = build_layout(geometry)
+    setup_s = time.perf_counter() - t0
+    print(f"[bench] setup {setup_s:.1f}s: {layout.n_cells} cells, {layout.n_faces} flux faces", file=sys.stderr, flush=True)
+
+    dm = DeviceMesh(case, "cuda:0", geometry=geometry, layout=layout)
+    gamma = case.numerics.gamma
+    rho, u, v, p = primitive.T
+    cons = np.column_stack((rho, rho * u, rho * v, p / (gamma - 1.0) + 0.5 * rho * (u**2 + v**2)))
+    dm.set_state(cons)
+    dm.clock.zero_()
+    for _ in range(args.warmup):
+        dm.step()
+    dm.check_status("warm-up: ")
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for _ in range(args.steps):
+        dm.step()
+    stop.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop)
+    dm.check_status("timed run: ")
+    cells = layout.n_cells
+    value = cells * args.steps / (ms / 1e3) / 1e6
+    print(f"[bench] {args.steps} steps in {ms:.2f} ms", file=sys.stderr, flush=True)
+
+    b_alg, kernel_bytes = algorithmic_bytes(layout, case.numerics.reconstruction, case.numerics.time_integration)
+    stage_gbs = b_alg * cells * args.steps / (ms / 1e3) / 1e9
+
+    roofline = None
+    kernel_times = {}
+    if not args.no_kernel_timing:
+        dm.timing(True)
+        ksteps = max(1, min(args.steps, 50))
+        for _ in range(ksteps):
+            dm.step()
+        torch.cuda.synchronize()
+        kernel_times = dm.timing(False)
+        dominant = max((k for k in kernel_times if k in kernel_bytes), key=lambda k: kernel_times[k][0])
+        total, count = kernel_times[dominant]
+        avg_ms = total / count
+        achieved = kernel_bytes[dominant] / (avg_ms / 1e3) / 1e9
+        traffic = None
+        tfile = ROOT / "profiles" / "traffic.json"
+        if tfile.exists():
+            try:
+                tdata = json.loads(tfile.read_text())
+                traffic = tdata.get("kernels", {}).get(dominant)
+            except Exception:
+                traffic = None
+        roofline = {
+            "bound": "hbm",
+            "kernel": f"k_{dominant}",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "alg_bytes_per_launch": kernel_bytes[dominant],
+            "avg_launch_ms": round(avg_ms, 5),
+        }
+
+    cpu = None
+    if args.cpu_steps > 0:
+        print("[bench] CPU baseline ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(case, geometry, primitive, args.cpu_steps)
+
+    line = {
+        "metric": "Mcell-updates/s (HLLC+MUSCL RK3, FP64)",
+        "value": round(value, 2),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded mesh and initial state)",
+        "config": {
+            "workload": f"four-quadrant Riemann, {args.n}x{args.n} quads + 30% triangle split, {cells} cells, "
+            "HLLC + MUSCL-linear (Barth-Jespersen), SSP-RK3, CFL 0.8, transmissive BCs",
+            "cells": cells,
+            "flux_faces": layout.n_faces,
+            "parallelism": "single GPU",
+        },
+        "roofline": roofline,
+        "stage": {
+            "b_alg_bytes_per_cell_update": round(b_alg, 2),
+            "achieved_GBs": round(stage_gbs, 1),
+            "frac": round(stage_gbs / HBM_PEAK_GBS, 4),
+        },
+        "kernel_ms": {k: round(t / c, 5) for k, (t, c) in kernel_times.items()},
+        "cpu_baseline": cpu,
+        "setup_s": round(setup_s, 2),
+    }
+    print(json.dumps(line), flush=True)
+    dm.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -172,6 +172,10 @@ int mcfd_stage(McfdMesh* mesh, const McfdSettings* settings, const McfdStage* st
 /* Per-component sum of R^2 over cells into sums[4] (device), deterministic. */
 int mcfd_sum_squares(McfdMesh* mesh, const double* r, double* sums, void* stream);
 
+/* dst[0:count] = src[0:count] (device, count even): a plain streaming copy,
+ * the reference kernel for HBM byte counters (FETCH_SIZE / WRITE_SIZE). */
+int mcfd_copy(const double* src, double* dst, size_t count, void* stream);
+
 /* Synchronise `stream`, read and clear the device status: mask of MCFD_ERR_*
  * bits and the lowest failing cell/face index. Returns 0. */
 int mcfd_status(McfdMesh* mesh, void* stream, int32_t* mask, int32_t* index);

@@ -624,6 +624,14 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x < 4) out[threadIdx.x] = red[threadIdx.x][0];
 }
 
+// Plain streaming copy of n doubles: the reference point for HBM counters.
+__global__ void __launch_bounds__(kBlock)
+    k_copy(long n2, const double2* __restrict__ src, double2* __restrict__ dst) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long stride = (long)gridDim.x * blockDim.x;
+  for (; i < n2; i += stride) dst[i] = src[i];
+}
+
 __global__ void k_status_init(int* status) {
   if (threadIdx.x == 0) {
     status[0] = 0;
@@ -920,6 +928,17 @@ int mcfd_sum_squares(McfdMesh* m, const double* r, double* sums, void* stream) {
   int nb = blocks(m->n_cells);
   hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(kBlock), 0, s, m->n_cells, r, m->partial);
   hipLaunchKernelGGL(k_sumsq_final, dim3(1), dim3(256), 0, s, nb, m->partial, sums);
+  HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int mcfd_copy(const double* src, double* dst, size_t count, void* stream) {
+  if (!src || !dst || (count & 1)) return fail("mcfd_copy: NULL pointer or odd count");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  long n2 = (long)(count / 2);
+  int grid = 256 * 8 * 4;
+  hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, s, n2,
+                     reinterpret_cast<const double2*>(src), reinterpret_cast<double2*>(dst));
   HIP_CHECK(hipGetLastError());
   return 0;
 }
