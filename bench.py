@@ -52,6 +52,8 @@ def algorithmic_bytes(layout, reconstruction: str, integrator: str):
     n_acc = len(layout.acc_item)
     n_dts = len(layout.dts_nx)
     kernels = {
+        # fused stage kernel: B_alg of one stage (averaged over the step's stages)
+        "stage": b_alg * c / len(stages),
         # U in (32), W out (32)
         "primitive": 64.0 * c,
         # W (32) + ptrs (8) + per LSQ entry nb+cx+cy (20) + per limiter entry dx,dy (16) + G,PHI out (96)
@@ -107,6 +109,8 @@ def main(argv=None) -> int:
     ap.add_argument("--n", type=int, default=1000, help="quads per side")
     ap.add_argument("--cpu-steps", type=int, default=1, help="SSP-RK3 steps timed on the CPU oracle (0 = skip)")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--unfused", action="store_true", help="four-kernel path instead of the fused stage kernel")
+    ap.add_argument("--block-cells", type=int, default=128, help="cells per workgroup of the fused stage kernel")
     ap.add_argument(
         "--copy-probe",
         type=int,
@@ -151,7 +155,9 @@ def main(argv=None) -> int:
     setup_s = time.perf_counter() - t0
     print(f"[bench] setup {setup_s:.1f}s: {layout.n_cells} cells, {layout.n_faces} flux faces", file=sys.stderr, flush=True)
 
-    dm = DeviceMesh(case, "cuda:0", geometry=geometry, layout=layout)
+    dm = DeviceMesh(
+        case, "cuda:0", geometry=geometry, layout=layout, fused=not args.unfused, block_cells=args.block_cells
+    )
     gamma = case.numerics.gamma
     rho, u, v, p = primitive.T
     cons = np.column_stack((rho, rho * u, rho * v, p / (gamma - 1.0) + 0.5 * rho * (u**2 + v**2)))
@@ -236,6 +242,7 @@ def main(argv=None) -> int:
             "cells": cells,
             "flux_faces": layout.n_faces,
             "parallelism": "single GPU",
+            "path": "four-kernel" if args.unfused else f"fused stage kernel, {args.block_cells}-cell blocks",
         },
         "roofline": roofline,
         "stage": {

@@ -104,6 +104,32 @@ typedef struct McfdMeshDesc {
   const double* bc_wall;      /* [n_bc*2] wall velocity */
 } McfdMeshDesc;
 
+/*
+ * Cell blocks for the fused stage kernel (host pointers; copied). Block b owns
+ * cells [b*block_cells, min((b+1)*block_cells, n_cells)); its local cells are
+ * own cells, then ring-1 and ring-2 halo cells. info[b] = {cells_start, n_own,
+ * n_ring1, n_ring2, faces_start, n_faces, gnb_start, gptr_start}.
+ */
+typedef struct McfdBlockDesc {
+  int32_t n_blocks;
+  int32_t block_cells;
+  int32_t max_local;        /* max local cells of a block */
+  int32_t max_grad;         /* max own + ring-1 cells of a block */
+  int32_t max_faces;        /* max faces of a block */
+  int32_t n_local;          /* length of cells */
+  int32_t n_gnb_ptr;        /* length of gnb_ptr */
+  int32_t n_gnb;            /* length of gnb */
+  int32_t n_block_faces;    /* length of faces (face_lr has twice as many) */
+  int32_t reserved;
+  const int32_t* info;      /* [n_blocks*8] */
+  const int32_t* cells;     /* local -> global cell */
+  const int32_t* gnb_ptr;   /* per own/ring-1 local cell: first LSQ entry in gnb (block relative) */
+  const uint16_t* gnb;      /* LSQ neighbours as local cell ids, reference entry order */
+  const int32_t* faces;     /* local face -> flux face */
+  const uint16_t* face_lr;  /* [n_block_faces*2] local left/right cell (0xFFFF: none) */
+  const uint16_t* acc_local;/* [n_acc] accumulation items as local face slot*4 + mode */
+} McfdBlockDesc;
+
 typedef struct McfdSettings {
   int32_t flux;            /* 0 Rusanov, 1 HLLC, 2 Roe */
   int32_t reconstruction;  /* 0 first order, 1 MUSCL linear */
@@ -137,6 +163,53 @@ typedef struct McfdStage {
   double c[5];
 } McfdStage;
 
+/*
+ * Implicit-Euler preconditioner structure (host pointers; device cell ids).
+ * Records are face pairs (interior faces, periodic pairs) and boundary faces
+ * (every face with neighbour < 0, periodic ones included, as the reference
+ * counts them again in the diagonal). Per cell: diagonal items in the
+ * reference's add.at order; off-diagonal items sorted by the natural
+ * (mesh-order) index of their column, lower columns first, off_split[i] the
+ * first upper item. Item = record*2 + side (0 owner side, 1 neighbour side).
+ */
+typedef struct McfdImplicitDesc {
+  int32_t n_records;
+  int32_t reserved;
+  const int32_t* rec_o;
+  const int32_t* rec_n;        /* -1 for boundary records */
+  const double* rec_nox;       /* normal paired with the owner-side state */
+  const double* rec_noy;
+  const double* rec_nnx;       /* normal paired with the neighbour-side state */
+  const double* rec_nny;
+  const double* rec_len;
+  const int32_t* diag_ptr;     /* [n_cells+1] */
+  const int32_t* diag_item;
+  const int32_t* off_ptr;      /* [n_cells+1] */
+  const int32_t* off_split;    /* [n_cells] */
+  const int32_t* off_item;
+  const int32_t* off_nb;       /* device cell of the column */
+  const int32_t* natural_order;/* [n_cells] device cell at each natural (mesh) position */
+} McfdImplicitDesc;
+
+/* Matrix-free implicit operator and GMRES arguments (device pointers). */
+typedef struct McfdGmres {
+  const double* state;    /* (C,4) Newton iterate s */
+  const double* rate;     /* (C,4) first-order residual at s */
+  const double* rhs;      /* (C,4) right-hand side b */
+  double* x;              /* (C,4) solution */
+  const double* dt;       /* [1] or [C] */
+  int32_t dt_per_cell;
+  int32_t krylov;         /* restart length m (<= 31) */
+  int32_t max_iterations;
+  int32_t iterations;     /* out: inner iterations used */
+  int32_t info;           /* out: 0 converged, 1 not converged */
+  int32_t reserved;
+  double sigma[4];        /* variable scales */
+  double state_norm;      /* ||state / sigma||_2 */
+  double tolerance;       /* relative tolerance rtol */
+  double residual;        /* out: final true residual ||b - A x|| */
+} McfdGmres;
+
 typedef struct McfdMesh McfdMesh;
 
 /* Library / ABI version. */
@@ -148,6 +221,11 @@ int mcfd_last_error(char* buffer, size_t size);
 /* Upload a mesh (replaces `_build_geometry`'s output `_Geometry`, solver.py:122-176). */
 int mcfd_mesh_create(const McfdMeshDesc* desc, McfdMesh** mesh);
 int mcfd_mesh_destroy(McfdMesh* mesh);
+
+/* Upload cell blocks and switch mcfd_stage / mcfd_residual to the fused
+ * one-kernel-per-stage path; NULL switches back to the four-kernel path.
+ * No counterpart in the reference (its residual is whole-array NumPy passes). */
+int mcfd_mesh_set_blocks(McfdMesh* mesh, const McfdBlockDesc* blocks);
 
 /* R = residual(U) (replaces `_residual`, solver.py:2437-2718). */
 int mcfd_residual(McfdMesh* mesh, const McfdSettings* settings, const double* u, double* r,
@@ -168,6 +246,41 @@ int mcfd_advance_clock(McfdMesh* mesh, const double* dt_source, double fixed_dt,
  * solver.py:1128-1182). */
 int mcfd_stage(McfdMesh* mesh, const McfdSettings* settings, const McfdStage* stage,
                void* stream);
+
+/* Upload the LU-SGS structure (builds the level schedules); NULL frees it. */
+int mcfd_mesh_set_implicit(McfdMesh* mesh, const McfdImplicitDesc* desc);
+
+/* Assemble S from the first-order spectral radii at state u with steps dt
+ * (replaces `_implicit_preconditioner`'s assembly, solver.py:1185-1301). */
+int mcfd_lusgs_assemble(McfdMesh* mesh, const McfdSettings* settings, const double* u,
+                        const double* dt, int32_t dt_per_cell, void* stream);
+
+/* out = U^-1 (D (L^-1 v)) (replaces `apply_lu_sgs`, solver.py:1315-1319). */
+int mcfd_lusgs_apply(McfdMesh* mesh, const double* v, double* out, void* stream);
+
+/* out = v - dt * J v / sigma, one-sided difference of the first-order residual
+ * (replaces `_matrix_free_implicit_operator`'s matvec, solver.py:1543-1563). */
+int mcfd_jacobian_apply(McfdMesh* mesh, const McfdSettings* jacobian_settings, McfdGmres* op,
+                        const double* v, double* out, void* stream);
+
+/* Restarted left-preconditioned GMRES on the matrix-free operator with the
+ * assembled LU-SGS (replaces `_solve_gmres` + scipy gmres, solver.py:1489-1529).
+ * Synchronises the stream once per inner iteration. */
+int mcfd_gmres(McfdMesh* mesh, const McfdSettings* jacobian_settings, McfdGmres* args,
+               void* stream);
+
+/* scaled = (u - u0 - dt*r)/sigma; norms[0] = max|scaled|, norms[1] =
+ * max|(u - u0)/sigma| (solver.py:1415-1424). scaled may be NULL. */
+int mcfd_newton_equation(McfdMesh* mesh, const double* u, const double* u0, const double* r,
+                         const double* dt, int32_t dt_per_cell, const double* sigma_host,
+                         double* scaled, double* norms, void* stream);
+
+/* out = u + (damping*corr)*sigma (solver.py:1465-1468). */
+int mcfd_newton_candidate(McfdMesh* mesh, const double* u, const double* corr, double damping,
+                          const double* sigma_host, double* out, void* stream);
+
+/* out[0] = ||a||_2 over the (C,4) array (device), deterministic. */
+int mcfd_norm2(McfdMesh* mesh, const double* a, double* out, void* stream);
 
 /* Per-component sum of R^2 over cells into sums[4] (device), deterministic. */
 int mcfd_sum_squares(McfdMesh* mesh, const double* r, double* sums, void* stream);

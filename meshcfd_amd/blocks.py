@@ -1,0 +1,180 @@
+"""Cell blocks with two-ring halos for the fused stage kernel.
+
+The device mesh is cut into blocks of ``B`` consecutive cells in its
+space-filling-curve order. One workgroup owns one block and evaluates a whole
+stage for it out of LDS:
+
+    local cells = [own cells (B)] + [ring 1] + [ring 2]
+
+* ring 1: cells outside the block that share a face (or a periodic pair) with
+  an own cell — their gradients and limiters are needed for the own cells'
+  face reconstructions;
+* ring 2: cells outside block + ring 1 that neighbour a ring-1 cell — only
+  their primitive states are needed (ring-1 gradients and min/max).
+
+Per block the layout lists the local cells' global ids, every LSQ entry of an
+own or ring-1 cell as a local neighbour id (in the reference's entry order),
+the flux faces touching an own cell (interior faces between two blocks are
+evaluated by both, with identical inputs, so both get the same bits), and the
+accumulation items re-encoded as local face slots.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from .layout import DeviceLayout
+
+INFO_FIELDS = 8  # cells_start, n_own, n_ring1, n_ring2, faces_start, n_faces, gnb_start, gptr_start
+
+
+@dataclass
+class BlockLayout:
+    block_cells: int
+    n_blocks: int
+    info: np.ndarray  # (n_blocks, 8) int32
+    cells: np.ndarray  # int32 global ids of local cells, per block
+    gnb_ptr: np.ndarray  # int32, per local cell of own+ring1: start into gnb (relative to block gnb_start)
+    gnb: np.ndarray  # uint16 local neighbour ids of the LSQ entries
+    faces: np.ndarray  # int32 global flux-face ids, per block
+    face_lr: np.ndarray  # uint16 (n, 2) local left / right ids (0xFFFF: boundary)
+    acc_local: np.ndarray  # uint16, per global acc item: local face slot * 4 + mode
+    max_local: int
+    max_faces: int
+    max_grad_cells: int
+
+    def nbytes(self) -> int:
+        return int(sum(v.nbytes for v in vars(self).values() if isinstance(v, np.ndarray)))
+
+
+def _lookup(keys_sorted: np.ndarray, values: np.ndarray, query: np.ndarray) -> np.ndarray:
+    pos = np.searchsorted(keys_sorted, query)
+    if np.any(pos >= len(keys_sorted)) or np.any(keys_sorted[np.minimum(pos, len(keys_sorted) - 1)] != query):
+        raise ValueError("block halo lookup failed")
+    return values[pos]
+
+
+def build_blocks(L: DeviceLayout, block_cells: int = 128) -> BlockLayout:
+    n = L.n_cells
+    B = block_cells
+    nb = (n + B - 1) // B
+    blk = np.arange(n, dtype=np.int64) // B
+    gptr = L.grad_ptr.astype(np.int64)
+    deg = np.diff(gptr)
+    src = np.repeat(np.arange(n, dtype=np.int64), deg)
+    nbr = L.grad_nb.astype(np.int64)
+
+    # ring 1: (block, cell) pairs for neighbours outside the block
+    cross = blk[src] != blk[nbr]
+    r1 = np.unique(blk[src[cross]] * n + nbr[cross])
+    r1_b, r1_c = r1 // n, r1 % n
+    # ring 2: neighbours of ring-1 cells outside the block and outside ring 1
+    cnt = deg[r1_c]
+    exp_b = np.repeat(r1_b, cnt)
+    starts = gptr[r1_c]
+    offs = np.arange(cnt.sum()) - np.repeat(np.cumsum(cnt) - cnt, cnt)
+    exp_c = nbr[np.repeat(starts, cnt) + offs]
+    cand = exp_b * n + exp_c
+    keep = (blk[exp_c] != exp_b) & ~np.isin(cand, r1)
+    r2 = np.unique(cand[keep])
+    r2_b, r2_c = r2 // n, r2 % n
+
+    n_own = np.bincount(blk, minlength=nb)
+    n_r1 = np.bincount(r1_b, minlength=nb)
+    n_r2 = np.bincount(r2_b, minlength=nb)
+    n_loc = n_own + n_r1 + n_r2
+    cstart = np.concatenate(([0], np.cumsum(n_loc)[:-1]))
+    # local ids: own cells 0..n_own-1, ring 1 next, ring 2 last
+    own_local = np.arange(n) - blk * B
+    r1_rank = np.arange(len(r1)) - np.concatenate(([0], np.cumsum(n_r1)[:-1]))[r1_b]
+    r2_rank = np.arange(len(r2)) - np.concatenate(([0], np.cumsum(n_r2)[:-1]))[r2_b]
+    r1_local = n_own[r1_b] + r1_rank
+    r2_local = n_own[r2_b] + n_r1[r2_b] + r2_rank
+    cells = np.empty(int(n_loc.sum()), dtype=np.int64)
+    cells[cstart[blk] + own_local] = np.arange(n)
+    cells[cstart[r1_b] + r1_local] = r1_c
+    cells[cstart[r2_b] + r2_local] = r2_c
+    # (block, global cell) -> local id lookup
+    keys = np.concatenate((blk * n + np.arange(n), r1, r2))
+    vals = np.concatenate((own_local, r1_local, r2_local))
+    o = np.argsort(keys)
+    keys, vals = keys[o], vals[o]
+
+    # LSQ entries of own + ring-1 cells as local neighbour ids
+    gc_b = np.concatenate((blk, r1_b))  # block of each gradient cell
+    gc_c = np.concatenate((np.arange(n), r1_c))
+    gc_local = np.concatenate((own_local, r1_local))
+    o = np.lexsort((gc_local, gc_b))
+    gc_b, gc_c, gc_local = gc_b[o], gc_c[o], gc_local[o]
+    gcnt = deg[gc_c]
+    g_b = np.repeat(gc_b, gcnt)
+    g_off = np.arange(gcnt.sum()) - np.repeat(np.cumsum(gcnt) - gcnt, gcnt)
+    g_nb = nbr[np.repeat(gptr[gc_c], gcnt) + g_off]
+    gnb = _lookup(keys, vals, g_b * n + g_nb)
+    if gnb.max(initial=0) >= 0xFFFF:
+        raise ValueError("block too large for 16-bit local ids")
+    n_gcells = n_own + n_r1
+    gstart_cells = np.concatenate(([0], np.cumsum(n_gcells)[:-1]))
+    ecount = np.bincount(g_b, weights=None, minlength=nb) if len(g_b) else np.zeros(nb, np.int64)
+    gnb_start = np.concatenate(([0], np.cumsum(ecount)[:-1])).astype(np.int64)
+    cell_first = np.concatenate(([0], np.cumsum(gcnt)[:-1]))
+    gnb_ptr = cell_first - gnb_start[gc_b]  # relative to the block's gnb_start
+
+    # flux faces per block: the left cell's block, plus the right cell's block if different
+    fl = L.face_left.astype(np.int64)
+    fr = L.face_right.astype(np.int64)
+    nf = len(fl)
+    fb_l = blk[fl]
+    has_r = fr >= 0
+    fb_r = np.where(has_r, blk[np.maximum(fr, 0)], -1)
+    second = has_r & (fb_r != fb_l)
+    pf_b = np.concatenate((fb_l, fb_r[second]))
+    pf_k = np.concatenate((np.arange(nf), np.flatnonzero(second)))
+    o = np.lexsort((pf_k, pf_b))
+    pf_b, pf_k = pf_b[o], pf_k[o]
+    nfb = np.bincount(pf_b, minlength=nb)
+    fstart = np.concatenate(([0], np.cumsum(nfb)[:-1]))
+    slot = np.arange(len(pf_k)) - fstart[pf_b]
+    lr = np.empty((len(pf_k), 2), dtype=np.int64)
+    lr[:, 0] = _lookup(keys, vals, pf_b * n + fl[pf_k])
+    rr = fr[pf_k]
+    lr[:, 1] = 0xFFFF
+    hr = rr >= 0
+    lr[hr, 1] = _lookup(keys, vals, pf_b[hr] * n + rr[hr])
+    # accumulation items -> local slots of the owning cell's block
+    fkeys = pf_b * nf + pf_k
+    fo = np.argsort(fkeys)
+    fkeys_s, slots_s = fkeys[fo], slot[fo]
+    aptr = L.acc_ptr.astype(np.int64)
+    acell = np.repeat(np.arange(n), np.diff(aptr))
+    item = L.acc_item.astype(np.int64)
+    aslot = _lookup(fkeys_s, slots_s, blk[acell] * nf + (item >> 2))
+    if aslot.max(initial=0) >= (1 << 14):
+        raise ValueError("too many faces per block for 14-bit slots")
+    acc_local = (aslot << 2) | (item & 3)
+
+    info = np.zeros((nb, INFO_FIELDS), dtype=np.int64)
+    info[:, 0] = cstart
+    info[:, 1] = n_own
+    info[:, 2] = n_r1
+    info[:, 3] = n_r2
+    info[:, 4] = fstart
+    info[:, 5] = nfb
+    info[:, 6] = gnb_start
+    info[:, 7] = gstart_cells
+    return BlockLayout(
+        block_cells=B,
+        n_blocks=nb,
+        info=info.astype(np.int32),
+        cells=cells.astype(np.int32),
+        gnb_ptr=gnb_ptr.astype(np.int32),
+        gnb=gnb.astype(np.uint16),
+        faces=pf_k.astype(np.int32),
+        face_lr=lr.astype(np.uint16),
+        acc_local=acc_local.astype(np.uint16),
+        max_local=int(n_loc.max()),
+        max_faces=int(nfb.max()),
+        max_grad_cells=int(n_gcells.max()),
+    )

@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "../../include/mcfd.h"
@@ -69,6 +70,19 @@ __device__ __forceinline__ void report(int* status, int code, int index) {
 }
 
 __device__ __forceinline__ bool finite(double x) { return isfinite(x); }
+
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
+// XCD b % 8). Remap so that each XCD walks one contiguous range of blocks:
+// with cells in space-filling-curve order, an XCD's L2 then holds one compact
+// region of the mesh and the neighbour gathers hit in it. Bijective for any
+// grid size.
+__device__ __forceinline__ int xcd_block() {
+  int nb = gridDim.x;
+  int b = blockIdx.x;
+  int q = nb >> 3, r = nb & 7;
+  int x = b & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
 
 // numpy.minimum / maximum for non-NaN operands.
 __device__ __forceinline__ double vmin(double a, double b) { return (a <= b) ? a : b; }
@@ -248,7 +262,7 @@ __device__ P4 far_field(const P4& in, const P4& out, double nx, double ny, doubl
 __global__ void __launch_bounds__(kBlock)
     k_primitive(int n, const double* __restrict__ U, double* __restrict__ W, double g,
                 int* __restrict__ status) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   P4 q = load4(U, i);
   double rho = q.r;
@@ -267,7 +281,7 @@ __global__ void __launch_bounds__(kBlock)
                const double* __restrict__ gcy, const int* __restrict__ lptr,
                const double* __restrict__ ldx, const double* __restrict__ ldy, int code,
                double* __restrict__ G, double* __restrict__ PHI) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   P4 w = load4(W, i);
   double wv[4] = {w.r, w.u, w.v, w.p};
@@ -366,7 +380,7 @@ __global__ void __launch_bounds__(kBlock)
     k_flux(int nf, FaceArrays fa, const double* __restrict__ W, const double* __restrict__ G,
            const double* __restrict__ PHI, int scheme, int muscl, double g,
            double* __restrict__ FLUX, int* __restrict__ status) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int k = xcd_block() * blockDim.x + threadIdx.x;
   if (k >= nf) return;
   int kind = fa.kind[k];
   double nx = fa.nx[k], ny = fa.ny[k];
@@ -432,7 +446,7 @@ __global__ void __launch_bounds__(kBlock)
              const double* __restrict__ FLUX, const double* __restrict__ flen,
              const double* __restrict__ area, const double* __restrict__ W, double ax, double ay,
              StageArgs s, int* __restrict__ status) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double A = area[i];
   double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
@@ -511,6 +525,593 @@ __global__ void __launch_bounds__(kBlock)
   store4(s.out, i, o0, o1, o2, o3);
 }
 
+
+// ---------------------------------------------------------------------------
+// Fused stage: one workgroup evaluates a whole residual + stage update for a
+// block of consecutive cells out of LDS.
+//   P0  primitive states of own + ring-1 + ring-2 cells  -> LDS
+//   P1  LSQ gradient + limiter of own + ring-1 cells     -> LDS
+//   P2  flux of every face touching an own cell          -> LDS
+//   P3  per own cell: gather in reference order, stage update -> HBM
+// Arithmetic is the same as the unfused kernels (same helpers, same order).
+// ---------------------------------------------------------------------------
+
+struct BlockArrays {
+  const int* info;          // [nb][8]
+  const int* cells;         // local -> global cell
+  const int* gnb_ptr;       // per gradient cell, start in gnb (block relative)
+  const unsigned short* gnb;
+  const int* faces;         // local face -> global flux face
+  const unsigned short* face_lr;
+  const unsigned short* acc_local;
+};
+
+struct GeomArrays {
+  const double* area;
+  const int* grad_ptr;
+  const double* grad_cx;
+  const double* grad_cy;
+  const int* lim_ptr;
+  const double* lim_dx;
+  const double* lim_dy;
+  const int* acc_ptr;
+  const double* face_len;
+};
+
+__device__ __forceinline__ P4 lds4(const double* s, int j) {
+  return P4{s[4 * j], s[4 * j + 1], s[4 * j + 2], s[4 * j + 3]};
+}
+
+__device__ __forceinline__ P4 rec_lds(const double* sW, const double* sG, const double* sP, int c,
+                                      double dx, double dy, bool muscl) {
+  P4 w = lds4(sW, c);
+  if (!muscl) return w;
+  const double* g = sG + 8 * c;
+  const double* f = sP + 4 * c;
+  P4 o;
+  o.r = w.r + f[0] * (g[0] * dx + g[1] * dy);
+  o.u = w.u + f[1] * (g[2] * dx + g[3] * dy);
+  o.v = w.v + f[2] * (g[4] * dx + g[5] * dy);
+  o.p = w.p + f[3] * (g[6] * dx + g[7] * dy);
+  return o;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_stage_fused(int block_cells, int max_local, int max_grad, int max_faces, BlockArrays ba,
+                  GeomArrays ga, FaceArrays fa, int scheme, int muscl_i, int limiter, double g,
+                  double ax, double ay, StageArgs s, int* __restrict__ status) {
+  extern __shared__ double smem[];
+  const bool muscl = muscl_i != 0;
+  const int b = xcd_block();
+  const int* inf = ba.info + 8 * b;
+  const int cs = inf[0], n_own = inf[1], n_r1 = inf[2], n_r2 = inf[3];
+  const int fs = inf[4], n_faces = inf[5], gs = inf[6], gps = inf[7];
+  const int first = b * block_cells;
+  const int n_loc = n_own + n_r1 + n_r2;
+  const int n_grad = n_own + n_r1;
+  double* sW = smem;
+  double* sG = sW + 4 * max_local;
+  double* sP = sG + (muscl ? 8 * max_grad : 0);
+  double* sF = sP + (muscl ? 4 * max_grad : 0);
+  double* sL = sF + 4 * max_faces;
+  double* sU = sL + max_faces;
+  const double* __restrict__ U = s.u;
+
+  // P0: primitive states (euler.py:20-40)
+  for (int j = threadIdx.x; j < n_loc; j += blockDim.x) {
+    int c = ba.cells[cs + j];
+    P4 q = load4(U, c);
+    double rho = q.r;
+    double u = q.u / rho;
+    double v = q.v / rho;
+    double p = (g - 1.0) * (q.p - 0.5 * (q.u * q.u + q.v * q.v) / rho);
+    if (j < n_own) {
+      if (!finite(rho) || !finite(u) || !finite(v) || !finite(p) || !(rho > 0.0) || !(p > 0.0))
+        report(status, MCFD_ERR_STATE, c);
+      sU[4 * j] = q.r;
+      sU[4 * j + 1] = q.u;
+      sU[4 * j + 2] = q.v;
+      sU[4 * j + 3] = q.p;
+    }
+    sW[4 * j] = rho;
+    sW[4 * j + 1] = u;
+    sW[4 * j + 2] = v;
+    sW[4 * j + 3] = p;
+  }
+  __syncthreads();
+
+  // P1: LSQ gradient (_kernels.py:252-262) + limiter (_kernels.py:9-67)
+  if (muscl) {
+    for (int j = threadIdx.x; j < n_grad; j += blockDim.x) {
+      int c = ba.cells[cs + j];
+      double wv[4] = {sW[4 * j], sW[4 * j + 1], sW[4 * j + 2], sW[4 * j + 3]};
+      double gx[4] = {0.0, 0.0, 0.0, 0.0};
+      double gy[4] = {0.0, 0.0, 0.0, 0.0};
+      double mn[4] = {wv[0], wv[1], wv[2], wv[3]};
+      double mx[4] = {wv[0], wv[1], wv[2], wv[3]};
+      int e0 = ga.grad_ptr[c], e1 = ga.grad_ptr[c + 1];
+      const unsigned short* nbl = ba.gnb + gs + ba.gnb_ptr[gps + j] - e0;
+      for (int e = e0; e < e1; ++e) {
+        int o = nbl[e];
+        double ov[4] = {sW[4 * o], sW[4 * o + 1], sW[4 * o + 2], sW[4 * o + 3]};
+        double cx = ga.grad_cx[e], cy = ga.grad_cy[e];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          double d = ov[k] - wv[k];
+          gx[k] = gx[k] + cx * d;
+          gy[k] = gy[k] + cy * d;
+          if (ov[k] < mn[k]) mn[k] = ov[k];
+          if (ov[k] > mx[k]) mx[k] = ov[k];
+        }
+      }
+      double phi[4] = {1.0, 1.0, 1.0, 1.0};
+      int l0 = ga.lim_ptr[c], l1 = ga.lim_ptr[c + 1];
+      for (int e = l0; e < l1; ++e) {
+        double dx = ga.lim_dx[e], dy = ga.lim_dy[e];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          double inc = gx[k] * dx + gy[k] * dy;
+          double tol = 1e-14 * vmax(fabs(wv[k]), 1.0);
+          double ratio = 1.0;
+          if (inc > tol)
+            ratio = (mx[k] - wv[k]) / inc;
+          else if (inc < -tol)
+            ratio = (mn[k] - wv[k]) / inc;
+          ratio = (0.0 > ratio) ? 0.0 : ratio;
+          double f;
+          if (limiter <= 1) {
+            f = (ratio < 1.0) ? ratio : 1.0;
+          } else if (limiter == 2) {
+            if (ratio > 0.0) {
+              double t = 2.0 * ratio / (1.0 + ratio);
+              f = (t < 1.0) ? t : 1.0;
+            } else {
+              f = 0.0;
+            }
+          } else {
+            const double eps = 1e-12;
+            f = (ratio * ratio + 2.0 * ratio + eps) / (ratio * ratio + ratio + 2.0 + eps);
+            f = (f < 1.0) ? f : 1.0;
+          }
+          if (f < phi[k]) phi[k] = f;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        sG[8 * j + 2 * k] = gx[k];
+        sG[8 * j + 2 * k + 1] = gy[k];
+        sP[4 * j + k] = phi[k];
+      }
+    }
+    __syncthreads();
+  }
+
+  // P2: face fluxes
+  for (int f = threadIdx.x; f < n_faces; f += blockDim.x) {
+    int k = ba.faces[fs + f];
+    int l = ba.face_lr[2 * (fs + f)];
+    int r = ba.face_lr[2 * (fs + f) + 1];
+    int kind = fa.kind[k];
+    double nx = fa.nx[k], ny = fa.ny[k];
+    P4 L = rec_lds(sW, sG, sP, l, fa.lx[k], fa.ly[k], muscl);
+    P4 R;
+    int err = 0;
+    if (kind <= MCFD_FACE_PERIODIC) {
+      R = rec_lds(sW, sG, sP, r, fa.rx[k], fa.ry[k], muscl);
+    } else {
+      int bi = fa.bc[k];
+      const double* st = fa.bc_state + 4 * (long)bi;
+      if (kind == MCFD_FACE_WALL) {
+        double rx = L.u - fa.bc_wall[2 * (long)bi];
+        double ry = L.v - fa.bc_wall[2 * (long)bi + 1];
+        double un = rx * nx + ry * ny;
+        R = L;
+        R.u = L.u - 2.0 * un * nx;
+        R.v = L.v - 2.0 * un * ny;
+      } else if (kind == MCFD_FACE_VELOCITY_INLET) {
+        R = P4{st[0], st[1], st[2], st[3]};
+      } else if (kind == MCFD_FACE_PRESSURE_OUTLET) {
+        R = L;
+        R.p = st[3];
+      } else if (kind == MCFD_FACE_FAR_FIELD) {
+        R = far_field(L, P4{st[0], st[1], st[2], st[3]}, nx, ny, g, &err);
+      } else {
+        R = L;
+      }
+    }
+    F4 fl;
+    if (scheme == 1) {
+      fl = hllc(L, R, nx, ny, g, kind == MCFD_FACE_INTERIOR);
+      if (kind != MCFD_FACE_INTERIOR &&
+          !(finite(fl.a) && finite(fl.b) && finite(fl.c) && finite(fl.d)))
+        err = MCFD_ERR_HLLC;
+    } else if (scheme == 2) {
+      fl = roe(L, R, nx, ny, g, &err);
+      if (!err && !(finite(fl.a) && finite(fl.b) && finite(fl.c) && finite(fl.d)))
+        err = MCFD_ERR_ROE;
+    } else {
+      fl = rusanov(L, R, nx, ny, g);
+    }
+    if (err) report(status, err, k);
+    sF[4 * f] = fl.a;
+    sF[4 * f + 1] = fl.b;
+    sF[4 * f + 2] = fl.c;
+    sF[4 * f + 3] = fl.d;
+    sL[f] = ga.face_len[k];
+  }
+  __syncthreads();
+
+  // P3: gather (solver.py:2501-2704) + body force + stage update
+  for (int j = threadIdx.x; j < n_own; j += blockDim.x) {
+    int i = first + j;
+    double A = ga.area[i];
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+    int e0 = ga.acc_ptr[i], e1 = ga.acc_ptr[i + 1];
+    for (int e = e0; e < e1; ++e) {
+      int item = ba.acc_local[e];
+      int slot = item >> 2;
+      int mode = item & 3;
+      double L = sL[slot];
+      P4 F = lds4(sF, slot);
+      if (mode == MCFD_ACC_OWNER) {
+        double c = -L / A;
+        r0 = r0 + c * F.r;
+        r1 = r1 + c * F.u;
+        r2 = r2 + c * F.v;
+        r3 = r3 + c * F.p;
+      } else if (mode == MCFD_ACC_NEIGHBOUR) {
+        double c = L / A;
+        r0 = r0 + c * F.r;
+        r1 = r1 + c * F.u;
+        r2 = r2 + c * F.v;
+        r3 = r3 + c * F.p;
+      } else if (mode == MCFD_ACC_BOUNDARY) {
+        r0 = r0 + (-(F.r * L)) / A;
+        r1 = r1 + (-(F.u * L)) / A;
+        r2 = r2 + (-(F.v * L)) / A;
+        r3 = r3 + (-(F.p * L)) / A;
+      } else {
+        r0 = r0 + (F.r * L) / A;
+        r1 = r1 + (F.u * L) / A;
+        r2 = r2 + (F.v * L) / A;
+        r3 = r3 + (F.p * L) / A;
+      }
+    }
+    if (ax != 0.0 || ay != 0.0) {
+      P4 w = lds4(sW, j);
+      r1 = r1 + w.r * ax;
+      r2 = r2 + w.r * ay;
+      r3 = r3 + w.r * (ax * w.u + ay * w.v);
+    }
+    if (!(finite(r0) && finite(r1) && finite(r2) && finite(r3)))
+      report(status, MCFD_ERR_RESIDUAL, i);
+    if (s.r_out) store4(s.r_out, i, r0, r1, r2, r3);
+    if (s.op == MCFD_OP_RESIDUAL) continue;
+    double h = s.dt_per_cell ? s.dt[i] : s.dt[0];
+    P4 u = lds4(sU, j);
+    double o0, o1, o2, o3;
+    if (s.op == MCFD_OP_EULER) {
+      double k = s.c0 * h;
+      o0 = u.r + k * r0;
+      o1 = u.u + k * r1;
+      o2 = u.v + k * r2;
+      o3 = u.p + k * r3;
+    } else if (s.op == MCFD_OP_SSP) {
+      P4 z = load4(s.u0, i);
+      o0 = s.c0 * z.r + s.c1 * (u.r + h * r0);
+      o1 = s.c0 * z.u + s.c1 * (u.u + h * r1);
+      o2 = s.c0 * z.v + s.c1 * (u.v + h * r2);
+      o3 = s.c0 * z.p + s.c1 * (u.p + h * r3);
+    } else if (s.op == MCFD_OP_RK4) {
+      P4 z = load4(s.u0, i);
+      double k = s.c2 * h;
+      o0 = s.c0 * z.r + s.c1 * u.r + k * r0;
+      o1 = s.c0 * z.u + s.c1 * u.u + k * r1;
+      o2 = s.c0 * z.v + s.c1 * u.v + k * r2;
+      o3 = s.c0 * z.p + s.c1 * u.p + k * r3;
+    } else {
+      P4 z = load4(s.u0, i);
+      P4 y = load4(s.u1, i);
+      P4 q = load4(s.r1, i);
+      double k2 = s.c2 * h, k4 = s.c4 * h;
+      o0 = s.c0 * z.r + s.c1 * y.r + k2 * q.r + s.c3 * u.r + k4 * r0;
+      o1 = s.c0 * z.u + s.c1 * y.u + k2 * q.u + s.c3 * u.u + k4 * r1;
+      o2 = s.c0 * z.v + s.c1 * y.v + k2 * q.v + s.c3 * u.v + k4 * r2;
+      o3 = s.c0 * z.p + s.c1 * y.p + k2 * q.p + s.c3 * u.p + k4 * r3;
+    }
+    store4(s.out, i, o0, o1, o2, o3);
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// Implicit Euler: LU-SGS preconditioner (solver.py:1185-1323), matrix-free
+// Jacobian (solver.py:1532-1567) and BLAS-1 helpers for GMRES.
+// ---------------------------------------------------------------------------
+
+struct LusgsArrays {
+  int n_rec;
+  const int* rec_o;      // device cell on the owner side
+  const int* rec_n;      // device cell on the neighbour side, -1 for boundary records
+  const double* nox;     // normal used with the owner-side state
+  const double* noy;
+  const double* nnx;     // normal used with the neighbour-side state
+  const double* nny;
+  const double* len;
+  const int* diag_ptr;   // per cell: diagonal items in reference order
+  const int* diag_item;  // record*2 + side
+  const int* off_ptr;    // per cell: off-diagonal items, lower (natural order) first
+  const int* off_split;  // per cell: first upper item
+  const int* off_item;   // record*2 + side
+  const int* off_nb;     // device cell of the off-diagonal column
+  int n_levels_f, n_levels_b;
+  const int* lvl_ptr_f;
+  const int* lvl_cells_f;
+  const int* lvl_ptr_b;
+  const int* lvl_cells_b;
+  double* coef;          // per record
+  double* diag;          // per cell
+  double* off_val;       // per off item
+};
+
+// Per record: 0.5*max(|u_o.n_o|+c_o, |u_n.n_n|+c_n)*L for face pairs, the
+// owner's |u.n|+c for boundary records (solver.py:1214-1299).
+__global__ void __launch_bounds__(kBlock)
+    k_lusgs_coef(LusgsArrays la, const double* __restrict__ W, double g) {
+  int r = xcd_block() * blockDim.x + threadIdx.x;
+  if (r >= la.n_rec) return;
+  P4 wo = load4(W, la.rec_o[r]);
+  double ro = fabs(wo.u * la.nox[r] + wo.v * la.noy[r]) + sound(wo, g);
+  int n = la.rec_n[r];
+  if (n < 0) {
+    la.coef[r] = ro;
+    return;
+  }
+  P4 wn = load4(W, n);
+  double rn = fabs(wn.u * la.nnx[r] + wn.v * la.nny[r]) + sound(wn, g);
+  la.coef[r] = 0.5 * vmax(ro, rn) * la.len[r];
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_lusgs_assemble(int n, LusgsArrays la, const double* __restrict__ area,
+                     const double* __restrict__ dt, int dt_per_cell) {
+  int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double h = dt_per_cell ? dt[i] : dt[0];
+  double A = area[i];
+  double d = 1.0;
+  for (int e = la.diag_ptr[i]; e < la.diag_ptr[i + 1]; ++e) {
+    int r = la.diag_item[e] >> 1;
+    if (la.rec_n[r] < 0)
+      d = d + h * la.coef[r] * la.len[r] / A;
+    else
+      d = d + h * la.coef[r] / A;
+  }
+  la.diag[i] = d;
+  for (int e = la.off_ptr[i]; e < la.off_ptr[i + 1]; ++e) {
+    int r = la.off_item[e] >> 1;
+    la.off_val[e] = -(h * la.coef[r]) / A;
+  }
+}
+
+// Level-scheduled triangular sweeps in one workgroup: every cell of a level
+// depends only on cells of earlier levels, so the sweep equals the
+// sequential natural-order substitution. Forward: y = L^-1 v. Backward (in
+// place): x = U^-1 (max(diag, 1e-14) * y).
+__global__ void __launch_bounds__(1024)
+    k_lusgs_forward(LusgsArrays la, const double* __restrict__ v, double* __restrict__ y) {
+  for (int lev = 0; lev < la.n_levels_f; ++lev) {
+    int a = la.lvl_ptr_f[lev], b = la.lvl_ptr_f[lev + 1];
+    for (int idx = a + threadIdx.x; idx < b; idx += blockDim.x) {
+      int i = la.lvl_cells_f[idx];
+      P4 s = load4(v, i);
+      for (int e = la.off_ptr[i]; e < la.off_split[i]; ++e) {
+        double c = la.off_val[e];
+        P4 o = load4(y, la.off_nb[e]);
+        s.r = s.r - c * o.r;
+        s.u = s.u - c * o.u;
+        s.v = s.v - c * o.v;
+        s.p = s.p - c * o.p;
+      }
+      double d = la.diag[i];
+      store4(y, i, s.r / d, s.u / d, s.v / d, s.p / d);
+    }
+    __threadfence();
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_lusgs_backward(LusgsArrays la, double* __restrict__ x) {
+  for (int lev = 0; lev < la.n_levels_b; ++lev) {
+    int a = la.lvl_ptr_b[lev], b = la.lvl_ptr_b[lev + 1];
+    for (int idx = a + threadIdx.x; idx < b; idx += blockDim.x) {
+      int i = la.lvl_cells_b[idx];
+      double d = la.diag[i];
+      double dm = vmax(d, 1e-14);
+      P4 y = load4(x, i);
+      P4 s{dm * y.r, dm * y.u, dm * y.v, dm * y.p};
+      for (int e = la.off_split[i]; e < la.off_ptr[i + 1]; ++e) {
+        double c = la.off_val[e];
+        P4 o = load4(x, la.off_nb[e]);
+        s.r = s.r - c * o.r;
+        s.u = s.u - c * o.u;
+        s.v = s.v - c * o.v;
+        s.p = s.p - c * o.p;
+      }
+      store4(x, i, s.r / d, s.u / d, s.v / d, s.p / d);
+    }
+    __threadfence();
+    __syncthreads();
+  }
+}
+
+// Deterministic dot product: fixed-shape block partials, then one block.
+__global__ void __launch_bounds__(kBlock)
+    k_dot_partial(long n, const double* __restrict__ a, const double* __restrict__ b,
+                  double* __restrict__ partial) {
+  __shared__ double red[kBlock];
+  double acc = 0.0;
+  long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    acc += a[i] * b[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_dot_final(int nb, const double* __restrict__ partial, double* __restrict__ out, int sqrt_out) {
+  __shared__ double red[kBlock];
+  double acc = 0.0;
+  for (int j = threadIdx.x; j < nb; j += blockDim.x) acc += partial[j];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sqrt_out ? sqrt(red[0]) : red[0];
+}
+
+// y -= (*coef) * x  (MGS projection with a device-resident coefficient)
+__global__ void __launch_bounds__(kBlock)
+    k_axpy_dev(long n, const double* __restrict__ x, double* __restrict__ y,
+               const double* __restrict__ coef) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = y[i] - coef[0] * x[i];
+}
+
+// y = a*x + b*y (host scalars)
+__global__ void __launch_bounds__(kBlock)
+    k_axpby(long n, double a, const double* __restrict__ x, double b, double* __restrict__ y) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a * x[i] + b * y[i];
+}
+
+// out = x * s
+__global__ void __launch_bounds__(kBlock)
+    k_scale(long n, const double* __restrict__ x, double s, double* __restrict__ out) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = x[i] * s;
+}
+
+// x += sum_k y_k V_k, k = 0..m-1 in order (y on the host, passed by value)
+struct Coeffs32 {
+  double y[32];
+};
+__global__ void __launch_bounds__(kBlock)
+    k_combine(long n, int m, const double* __restrict__ V, Coeffs32 c, double* __restrict__ x) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int k = 0; k < m; ++k) acc = acc + c.y[k] * V[(long)k * n + i];
+  x[i] = x[i] + acc;
+}
+
+// perturbed = s + (eps*sigma_k) * d   (solver.py:1553-1556)
+__global__ void __launch_bounds__(kBlock)
+    k_perturb(int n, const double* __restrict__ s, const double* __restrict__ d, double eps,
+              double s0, double s1, double s2, double s3, double* __restrict__ out) {
+  int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  P4 a = load4(s, i), b = load4(d, i);
+  store4(out, i, a.r + eps * s0 * b.r, a.u + eps * s1 * b.u, a.v + eps * s2 * b.v,
+         a.p + eps * s3 * b.p);
+}
+
+// out = d - (h * ((rp - r) / eps)) / sigma   (solver.py:1557-1563)
+__global__ void __launch_bounds__(kBlock)
+    k_jv_finish(int n, const double* __restrict__ d, const double* __restrict__ rp,
+                const double* __restrict__ r, double eps, const double* __restrict__ dt,
+                int dt_per_cell, double s0, double s1, double s2, double s3,
+                double* __restrict__ out) {
+  int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double h = dt_per_cell ? dt[i] : dt[0];
+  P4 a = load4(d, i), p = load4(rp, i), q = load4(r, i);
+  store4(out, i, a.r - h * ((p.r - q.r) / eps) / s0, a.u - h * ((p.u - q.u) / eps) / s1,
+         a.v - h * ((p.v - q.v) / eps) / s2, a.p - h * ((p.p - q.p) / eps) / s3);
+}
+
+// Newton equation (solver.py:1415-1424): scaled = (u - u0 - h*R)/sigma and the
+// max-norms of scaled and of (u - u0)/sigma, per block then final.
+__global__ void __launch_bounds__(kBlock)
+    k_newton_equation(int n, const double* __restrict__ u, const double* __restrict__ u0,
+                      const double* __restrict__ R, const double* __restrict__ dt, int dt_per_cell,
+                      double s0, double s1, double s2, double s3, double* __restrict__ scaled,
+                      double* __restrict__ partial) {
+  __shared__ double r1[kBlock];
+  __shared__ double r2[kBlock];
+  int i = xcd_block() * blockDim.x + threadIdx.x;
+  double m1 = 0.0, m2 = 0.0;
+  if (i < n) {
+    double h = dt_per_cell ? dt[i] : dt[0];
+    P4 a = load4(u, i), b = load4(u0, i), r = load4(R, i);
+    double e0 = (a.r - b.r - h * r.r) / s0, e1 = (a.u - b.u - h * r.u) / s1;
+    double e2 = (a.v - b.v - h * r.v) / s2, e3 = (a.p - b.p - h * r.p) / s3;
+    if (scaled) store4(scaled, i, e0, e1, e2, e3);
+    m1 = fmax(fmax(fabs(e0), fabs(e1)), fmax(fabs(e2), fabs(e3)));
+    m2 = fmax(fmax(fabs((a.r - b.r) / s0), fabs((a.u - b.u) / s1)),
+              fmax(fabs((a.v - b.v) / s2), fabs((a.p - b.p) / s3)));
+    if (!finite(e0) || !finite(e1) || !finite(e2) || !finite(e3)) m1 = INFINITY;
+  }
+  r1[threadIdx.x] = m1;
+  r2[threadIdx.x] = m2;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      r1[threadIdx.x] = fmax(r1[threadIdx.x], r1[threadIdx.x + s]);
+      r2[threadIdx.x] = fmax(r2[threadIdx.x], r2[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[2 * xcd_block()] = r1[0];
+    partial[2 * xcd_block() + 1] = r2[0];
+  }
+}
+
+__global__ void __launch_bounds__(1024)
+    k_max2_final(int nb, const double* __restrict__ partial, double* __restrict__ out) {
+  __shared__ double r1[1024];
+  __shared__ double r2[1024];
+  double m1 = 0.0, m2 = 0.0;
+  for (int j = threadIdx.x; j < nb; j += blockDim.x) {
+    m1 = fmax(m1, partial[2 * j]);
+    m2 = fmax(m2, partial[2 * j + 1]);
+  }
+  r1[threadIdx.x] = m1;
+  r2[threadIdx.x] = m2;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      r1[threadIdx.x] = fmax(r1[threadIdx.x], r1[threadIdx.x + s]);
+      r2[threadIdx.x] = fmax(r2[threadIdx.x], r2[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = r1[0];
+    out[1] = r2[0];
+  }
+}
+
+// candidate = state + (damping*corr)*sigma   (solver.py:1465-1468)
+__global__ void __launch_bounds__(kBlock)
+    k_candidate(int n, const double* __restrict__ u, const double* __restrict__ corr,
+                double damping, double s0, double s1, double s2, double s3,
+                double* __restrict__ out) {
+  int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  P4 a = load4(u, i), c = load4(corr, i);
+  store4(out, i, a.r + damping * c.r * s0, a.u + damping * c.u * s1, a.v + damping * c.v * s2,
+         a.p + damping * c.p * s3);
+}
+
 // Spectral-radius sum and local step (solver.py:4132-4343, inviscid part).
 __global__ void __launch_bounds__(kBlock)
     k_time_steps(int n, const double* __restrict__ W, const int* __restrict__ dptr,
@@ -521,7 +1122,7 @@ __global__ void __launch_bounds__(kBlock)
                  double* __restrict__ dt_cell, double* __restrict__ partial,
                  int* __restrict__ status) {
   __shared__ double red[kBlock];
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_block() * blockDim.x + threadIdx.x;
   double dt = INFINITY;
   if (i < n) {
     P4 w = load4(W, i);
@@ -559,7 +1160,7 @@ __global__ void __launch_bounds__(kBlock)
     if (threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) partial[xcd_block()] = red[0];
 }
 
 __global__ void __launch_bounds__(1024)
@@ -593,7 +1194,7 @@ __global__ void k_clock(const double* __restrict__ src, double fixed_dt, double 
 __global__ void __launch_bounds__(kBlock)
     k_sumsq(int n, const double* __restrict__ R, double* __restrict__ partial) {
   __shared__ double red[4][kBlock];
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_block() * blockDim.x + threadIdx.x;
   P4 r = (i < n) ? load4(R, i) : P4{0.0, 0.0, 0.0, 0.0};
   red[0][threadIdx.x] = r.r * r.r;
   red[1][threadIdx.x] = r.u * r.u;
@@ -605,7 +1206,7 @@ __global__ void __launch_bounds__(kBlock)
       for (int k = 0; k < 4; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x < 4) partial[4 * (long)blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < 4) partial[4 * (long)xcd_block() + threadIdx.x] = red[threadIdx.x][0];
 }
 
 __global__ void __launch_bounds__(256)
@@ -674,6 +1275,24 @@ struct McfdMesh {
   double* FLUX = nullptr;
   double* partial = nullptr;
   int* status = nullptr;
+  // fused-stage blocks
+  bool fused = false;
+  int nb = 0, block_cells = 0, max_local = 0, max_grad = 0, max_faces = 0;
+  BlockArrays blocks{};
+  std::vector<void*> block_allocations;
+  // implicit
+  bool implicit = false;
+  LusgsArrays lusgs{};
+  std::vector<void*> implicit_allocations;
+  int gmres_m = 0;
+  double* gm_V = nullptr;       // (m+1) * 4C Krylov basis
+  double* gm_w = nullptr;       // 4C
+  double* gm_r = nullptr;       // 4C
+  double* gm_av = nullptr;      // 4C
+  double* gm_pert = nullptr;    // 4C
+  double* gm_rp = nullptr;      // 4C
+  double* gm_scalars = nullptr; // 64
+  double* gm_partial = nullptr; // dot partials
   // timing
   bool timing = false;
   std::vector<hipEvent_t> event_pool;
@@ -706,7 +1325,53 @@ int scratch(McfdMesh* m, T** dst, size_t count) {
   return 0;
 }
 
+void free_blocks(McfdMesh* m) {
+  for (void* p : m->block_allocations) (void)hipFree(p);
+  m->block_allocations.clear();
+  m->fused = false;
+  m->blocks = BlockArrays{};
+}
+
+template <typename T>
+int upload_block(McfdMesh* m, T** dst, const T* src, size_t count) {
+  void* p = nullptr;
+  HIP_CHECK(hipMalloc(&p, sizeof(T) * (count ? count : 1)));
+  m->block_allocations.push_back(p);
+  if (count) {
+    if (!src) return fail("mcfd_mesh_set_blocks: missing array");
+    HIP_CHECK(hipMemcpy(p, src, sizeof(T) * count, hipMemcpyHostToDevice));
+  }
+  *dst = static_cast<T*>(p);
+  return 0;
+}
+
+size_t fused_lds_bytes(const McfdMesh* m, bool muscl) {
+  size_t d = 4 * (size_t)m->max_local + (muscl ? 12 * (size_t)m->max_grad : 0) +
+             5 * (size_t)m->max_faces + 4 * (size_t)m->block_cells;
+  return d * sizeof(double);
+}
+
+void free_implicit(McfdMesh* m) {
+  for (void* p : m->implicit_allocations) (void)hipFree(p);
+  m->implicit_allocations.clear();
+  m->implicit = false;
+  m->lusgs = LusgsArrays{};
+  m->gmres_m = 0;
+}
+
+template <typename T>
+int upload_implicit(McfdMesh* m, T** dst, const T* src, size_t count) {
+  void* p = nullptr;
+  HIP_CHECK(hipMalloc(&p, sizeof(T) * (count ? count : 1)));
+  m->implicit_allocations.push_back(p);
+  if (count && src) HIP_CHECK(hipMemcpy(p, src, sizeof(T) * count, hipMemcpyHostToDevice));
+  *dst = static_cast<T*>(p);
+  return 0;
+}
+
 void free_mesh(McfdMesh* m) {
+  free_implicit(m);
+  free_blocks(m);
   for (void* p : m->allocations) (void)hipFree(p);
   for (hipEvent_t e : m->event_pool) (void)hipEventDestroy(e);
   delete m;
@@ -763,6 +1428,239 @@ int residual_front(McfdMesh* m, const McfdSettings* st, const double* u, hipStre
                        m->faces, m->W, m->G, m->PHI, st->flux, muscl ? 1 : 0, st->gamma,
                        m->FLUX, m->status);
   }
+  HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+
+namespace {
+
+constexpr int kDotBlocks = 1024;
+
+int dot_dev(McfdMesh* m, long n, const double* a, const double* b, double* out, bool sqrt_out,
+            hipStream_t s) {
+  hipLaunchKernelGGL(k_dot_partial, dim3(kDotBlocks), dim3(kBlock), 0, s, n, a, b, m->gm_partial);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kBlock), 0, s, kDotBlocks, m->gm_partial, out,
+                     sqrt_out ? 1 : 0);
+  HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int read_scalars(double* host, const double* dev, int count, hipStream_t s) {
+  HIP_CHECK(hipMemcpyAsync(host, dev, sizeof(double) * count, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  return 0;
+}
+
+int lblocks(long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+int lusgs_apply(McfdMesh* m, const double* v, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_lusgs_forward, dim3(1), dim3(1024), 0, s, m->lusgs, v, out);
+  hipLaunchKernelGGL(k_lusgs_backward, dim3(1), dim3(1024), 0, s, m->lusgs, out);
+  HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int ensure_gmres(McfdMesh* m, int krylov) {
+  if (krylov <= m->gmres_m) return 0;
+  if (krylov > 31) return fail("mcfd_gmres: Krylov dimension above 31 is not supported");
+  long N = 4L * m->n_cells;
+  double* bufs[7] = {nullptr};
+  size_t sizes[7] = {(size_t)(krylov + 1) * N, (size_t)N, (size_t)N, (size_t)N, (size_t)N, (size_t)N, 64};
+  for (int k = 0; k < 7; ++k) {
+    if (upload_implicit(m, &bufs[k], (const double*)nullptr, sizes[k])) return -1;
+  }
+  if (upload_implicit(m, &m->gm_partial, (const double*)nullptr, (size_t)kDotBlocks)) return -1;
+  m->gm_V = bufs[0];
+  m->gm_w = bufs[1];
+  m->gm_r = bufs[2];
+  m->gm_av = bufs[3];
+  m->gm_pert = bufs[4];
+  m->gm_rp = bufs[5];
+  m->gm_scalars = bufs[6];
+  m->gmres_m = krylov;
+  return 0;
+}
+
+struct JacobianOp {
+  McfdMesh* m;
+  const McfdSettings* st;
+  const double* state;
+  const double* rate;
+  const double* dt;
+  int per_cell;
+  double sigma[4];
+  double base;  // ||state / sigma||
+};
+
+// out = v - dt * J v / sigma with J v = (R1(s + eps*sigma*v) - R1(s)) / eps
+int jacobian_apply(JacobianOp& op, const double* v, double* out, hipStream_t s) {
+  McfdMesh* m = op.m;
+  long N = 4L * m->n_cells;
+  double dn = 0.0;
+  if (dot_dev(m, N, v, v, m->gm_scalars + 40, true, s)) return -1;
+  if (read_scalars(&dn, m->gm_scalars + 40, 1, s)) return -1;
+  if (dn == 0.0) {
+    HIP_CHECK(hipMemcpyAsync(out, v, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+    return 0;
+  }
+  const double fd = 1.4901161193847656e-08;  // sqrt(2^-52)
+  double eps = fd * (op.base > 1.0 ? op.base : 1.0) / dn;
+  int nb = blocks(m->n_cells);
+  hipLaunchKernelGGL(k_perturb, dim3(nb), dim3(kBlock), 0, s, m->n_cells, op.state, v, eps,
+                     op.sigma[0], op.sigma[1], op.sigma[2], op.sigma[3], m->gm_pert);
+  if (mcfd_residual(m, op.st, m->gm_pert, m->gm_rp, s)) return -1;
+  hipLaunchKernelGGL(k_jv_finish, dim3(nb), dim3(kBlock), 0, s, m->n_cells, v, m->gm_rp, op.rate,
+                     eps, op.dt, op.per_cell, op.sigma[0], op.sigma[1], op.sigma[2], op.sigma[3],
+                     out);
+  HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+// LAPACK dlartg (3.10+): [c s; -s c] [f; g] = [r; 0]
+void lartg(double f, double g, double& c, double& sn, double& r) {
+  if (g == 0.0) {
+    c = 1.0;
+    sn = 0.0;
+    r = f;
+  } else if (f == 0.0) {
+    c = 0.0;
+    sn = g > 0 ? 1.0 : -1.0;
+    r = fabs(g);
+  } else {
+    double d = sqrt(f * f + g * g);
+    c = fabs(f) / d;
+    r = copysign(d, f);
+    sn = g / r;
+  }
+}
+
+// One call of restarted, left-preconditioned GMRES with maxiter = 1 (the
+// algorithm of scipy.sparse.linalg.gmres, SciPy 1.15): MGS + Givens, inner
+// stop on the preconditioned residual, success on the true residual.
+int gmres_cycle(JacobianOp& op, const double* b, double* x, double rtol, int restart,
+                bool x_nonzero, int& inner, int& info, double& rnorm, hipStream_t s) {
+  McfdMesh* m = op.m;
+  long N = 4L * m->n_cells;
+  int nbl = lblocks(N);
+  double* V = m->gm_V;
+  double* sc = m->gm_scalars;
+  double host[40];
+  inner = 0;
+  if (dot_dev(m, N, b, b, sc + 0, true, s)) return -1;
+  double bnrm2 = 0.0;
+  if (read_scalars(&bnrm2, sc, 1, s)) return -1;
+  if (bnrm2 == 0.0) {
+    HIP_CHECK(hipMemcpyAsync(x, b, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+    info = 0;
+    rnorm = 0.0;
+    return 0;
+  }
+  double atol = rtol * bnrm2;
+  // ptol = ||M b|| * min(1, atol / bnrm2)
+  if (lusgs_apply(m, b, m->gm_w, s)) return -1;
+  if (dot_dev(m, N, m->gm_w, m->gm_w, sc + 1, true, s)) return -1;
+  double mb = 0.0;
+  if (read_scalars(&mb, sc + 1, 1, s)) return -1;
+  double ratio = atol / bnrm2;
+  double ptol = mb * (ratio < 1.0 ? ratio : 1.0);
+  const double eps = 2.220446049250313e-16;
+  // r = b - A x (or b when x == 0)
+  if (x_nonzero) {
+    if (jacobian_apply(op, x, m->gm_av, s)) return -1;
+    HIP_CHECK(hipMemcpyAsync(m->gm_r, b, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_axpby, dim3(nbl), dim3(kBlock), 0, s, N, -1.0, m->gm_av, 1.0, m->gm_r);
+    if (dot_dev(m, N, m->gm_r, m->gm_r, sc + 2, true, s)) return -1;
+    double r0 = 0.0;
+    if (read_scalars(&r0, sc + 2, 1, s)) return -1;
+    if (r0 < atol) {
+      info = 0;
+      rnorm = r0;
+      return 0;
+    }
+  } else {
+    HIP_CHECK(hipMemcpyAsync(m->gm_r, b, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+  }
+  // v0 = M r / ||M r||
+  if (lusgs_apply(m, m->gm_r, V, s)) return -1;
+  if (dot_dev(m, N, V, V, sc + 3, true, s)) return -1;
+  double t0 = 0.0;
+  if (read_scalars(&t0, sc + 3, 1, s)) return -1;
+  hipLaunchKernelGGL(k_scale, dim3(nbl), dim3(kBlock), 0, s, N, V, 1.0 / t0, V);
+  std::vector<double> h((size_t)restart * (restart + 1), 0.0);  // h[col*(restart+1) + row]
+  std::vector<double> S(restart + 1, 0.0), gc(restart, 0.0), gs(restart, 0.0);
+  S[0] = t0;
+  bool breakdown = false;
+  double presid = 0.0;
+  int col = 0;
+  for (col = 0; col < restart; ++col) {
+    double* vc = V + (long)col * N;
+    double* vn = V + (long)(col + 1) * N;
+    if (jacobian_apply(op, vc, m->gm_av, s)) return -1;
+    if (lusgs_apply(m, m->gm_av, m->gm_w, s)) return -1;
+    // h0 = ||w||, then modified Gram-Schmidt with device coefficients
+    if (dot_dev(m, N, m->gm_w, m->gm_w, sc + 4, true, s)) return -1;
+    for (int k = 0; k <= col; ++k) {
+      if (dot_dev(m, N, V + (long)k * N, m->gm_w, sc + 5 + k, false, s)) return -1;
+      hipLaunchKernelGGL(k_axpy_dev, dim3(nbl), dim3(kBlock), 0, s, N, V + (long)k * N, m->gm_w,
+                         sc + 5 + k);
+    }
+    if (dot_dev(m, N, m->gm_w, m->gm_w, sc + 5 + col + 1, true, s)) return -1;
+    if (read_scalars(host, sc + 4, col + 3, s)) return -1;
+    double h0 = host[0];
+    double* hc = h.data() + (size_t)col * (restart + 1);
+    for (int k = 0; k <= col; ++k) hc[k] = host[1 + k];
+    double h1 = host[col + 2];
+    hc[col + 1] = h1;
+    if (h1 <= eps * h0) {
+      hc[col + 1] = 0.0;
+      breakdown = true;
+      HIP_CHECK(hipMemcpyAsync(vn, m->gm_w, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+    } else {
+      hipLaunchKernelGGL(k_scale, dim3(nbl), dim3(kBlock), 0, s, N, m->gm_w, 1.0 / h1, vn);
+    }
+    for (int k = 0; k < col; ++k) {
+      double c = gc[k], sn = gs[k];
+      double n0 = hc[k], n1 = hc[k + 1];
+      hc[k] = c * n0 + sn * n1;
+      hc[k + 1] = -sn * n0 + c * n1;
+    }
+    double c, sn, mag;
+    lartg(hc[col], hc[col + 1], c, sn, mag);
+    gc[col] = c;
+    gs[col] = sn;
+    hc[col] = mag;
+    hc[col + 1] = 0.0;
+    double tmp = -sn * S[col];
+    S[col] = c * S[col];
+    S[col + 1] = tmp;
+    presid = fabs(tmp);
+    inner += 1;
+    if (presid <= ptol || breakdown) break;
+  }
+  if (col == restart) col = restart - 1;
+  // solve the upper-triangular system
+  if (h[(size_t)col * (restart + 1) + col] == 0.0) S[col] = 0.0;
+  Coeffs32 y{};
+  for (int k = 0; k <= col; ++k) y.y[k] = S[k];
+  for (int k = col; k > 0; --k) {
+    if (y.y[k] != 0.0) {
+      y.y[k] /= h[(size_t)k * (restart + 1) + k];
+      double t = y.y[k];
+      for (int i = 0; i < k; ++i) y.y[i] -= t * h[(size_t)k * (restart + 1) + i];
+    }
+  }
+  if (y.y[0] != 0.0) y.y[0] /= h[0];
+  hipLaunchKernelGGL(k_combine, dim3(nbl), dim3(kBlock), 0, s, N, col + 1, V, y, x);
+  // true residual
+  if (jacobian_apply(op, x, m->gm_av, s)) return -1;
+  HIP_CHECK(hipMemcpyAsync(m->gm_r, b, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(k_axpby, dim3(nbl), dim3(kBlock), 0, s, N, -1.0, m->gm_av, 1.0, m->gm_r);
+  if (dot_dev(m, N, m->gm_r, m->gm_r, sc + 2, true, s)) return -1;
+  if (read_scalars(&rnorm, sc + 2, 1, s)) return -1;
+  info = (rnorm <= atol) ? 0 : 1;
   HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -844,6 +1742,45 @@ int mcfd_mesh_create(const McfdMeshDesc* d, McfdMesh** out) {
   return 0;
 }
 
+int mcfd_mesh_set_blocks(McfdMesh* m, const McfdBlockDesc* d) {
+  if (!m) return fail("mcfd_mesh_set_blocks: NULL mesh");
+  (void)hipDeviceSynchronize();
+  free_blocks(m);
+  if (!d) return 0;
+  if (d->n_blocks <= 0 || d->block_cells <= 0 ||
+      (long)d->n_blocks * d->block_cells < m->n_cells ||
+      (long)(d->n_blocks - 1) * d->block_cells >= m->n_cells)
+    return fail("mcfd_mesh_set_blocks: block count does not cover the mesh");
+  m->nb = d->n_blocks;
+  m->block_cells = d->block_cells;
+  m->max_local = d->max_local;
+  m->max_grad = d->max_grad;
+  m->max_faces = d->max_faces;
+  if (fused_lds_bytes(m, true) > 64 * 1024) {
+    free_blocks(m);
+    return fail("mcfd_mesh_set_blocks: block needs more than 64 KiB of LDS; use smaller blocks");
+  }
+  int rc = 0;
+  BlockArrays& b = m->blocks;
+  rc |= upload_block(m, const_cast<int**>(&b.info), d->info, 8 * (size_t)d->n_blocks);
+  rc |= upload_block(m, const_cast<int**>(&b.cells), d->cells, (size_t)d->n_local);
+  rc |= upload_block(m, const_cast<int**>(&b.gnb_ptr), d->gnb_ptr, (size_t)d->n_gnb_ptr);
+  rc |= upload_block(m, const_cast<unsigned short**>(&b.gnb), d->gnb, (size_t)d->n_gnb);
+  rc |= upload_block(m, const_cast<int**>(&b.faces), d->faces, (size_t)d->n_block_faces);
+  rc |= upload_block(m, const_cast<unsigned short**>(&b.face_lr), d->face_lr,
+                     2 * (size_t)d->n_block_faces);
+  rc |= upload_block(m, const_cast<unsigned short**>(&b.acc_local), d->acc_local, (size_t)m->n_acc);
+  if (rc) {
+    std::string msg = g_last_error;
+    free_blocks(m);
+    return fail(msg);
+  }
+  HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_stage_fused),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
+  m->fused = true;
+  return 0;
+}
+
 int mcfd_mesh_destroy(McfdMesh* m) {
   if (!m) return 0;
   (void)hipDeviceSynchronize();
@@ -873,9 +1810,22 @@ int mcfd_stage(McfdMesh* m, const McfdSettings* st, const McfdStage* sg, void* s
     return fail("mcfd_stage: u1 and r1 are required for the final SSPRK(5,4) combination");
   if (sg->op == MCFD_OP_RESIDUAL && !sg->r_out) return fail("mcfd_residual: output is NULL");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (residual_front(m, st, sg->u, s)) return -1;
   StageArgs a{sg->op, sg->dt_per_cell, sg->u, sg->u0, sg->u1, sg->r1, sg->dt, sg->out,
               sg->r_out, sg->c[0], sg->c[1], sg->c[2], sg->c[3], sg->c[4]};
+  if (m->fused) {
+    bool muscl = st->reconstruction == 1 && m->n_grad > 0;
+    size_t lds = fused_lds_bytes(m, muscl);
+    GeomArrays ga{m->area,    m->grad_ptr, m->grad_cx, m->grad_cy,  m->lim_ptr,
+                  m->lim_dx,  m->lim_dy,   m->acc_ptr, m->face_len};
+    TimedLaunch t(m, s, MCFD_KERNEL_STAGE);
+    hipLaunchKernelGGL(k_stage_fused, dim3(m->nb), dim3(kBlock), lds, s, m->block_cells,
+                       m->max_local, m->max_grad, m->max_faces, m->blocks, ga, m->faces,
+                       st->flux, muscl ? 1 : 0, st->limiter, st->gamma, st->acceleration_x,
+                       st->acceleration_y, a, m->status);
+    HIP_CHECK(hipGetLastError());
+    return 0;
+  }
+  if (residual_front(m, st, sg->u, s)) return -1;
   {
     TimedLaunch t(m, s, MCFD_KERNEL_UPDATE);
     hipLaunchKernelGGL(k_update, dim3(blocks(m->n_cells)), dim3(kBlock), 0, s, m->n_cells,
@@ -941,6 +1891,180 @@ int mcfd_copy(const double* src, double* dst, size_t count, void* stream) {
                      reinterpret_cast<const double2*>(src), reinterpret_cast<double2*>(dst));
   HIP_CHECK(hipGetLastError());
   return 0;
+}
+
+
+int mcfd_mesh_set_implicit(McfdMesh* m, const McfdImplicitDesc* d) {
+  if (!m) return fail("mcfd_mesh_set_implicit: NULL mesh");
+  (void)hipDeviceSynchronize();
+  free_implicit(m);
+  if (!d) return 0;
+  int n = m->n_cells;
+  // Level schedules of the natural-order triangular solves: a cell's level
+  // is one more than the largest level among the cells it depends on.
+  std::vector<int> order(d->natural_order, d->natural_order + n);  // position -> device cell
+  std::vector<int> lev_f(n, 0), lev_b(n, 0);
+  int nlf = 0, nlb = 0;
+  for (int k = 0; k < n; ++k) {
+    int i = order[k];
+    int l = 0;
+    for (int e = d->off_ptr[i]; e < d->off_split[i]; ++e) l = std::max(l, lev_f[d->off_nb[e]] + 1);
+    lev_f[i] = l;
+    nlf = std::max(nlf, l + 1);
+  }
+  for (int k = n - 1; k >= 0; --k) {
+    int i = order[k];
+    int l = 0;
+    for (int e = d->off_split[i]; e < d->off_ptr[i + 1]; ++e)
+      l = std::max(l, lev_b[d->off_nb[e]] + 1);
+    lev_b[i] = l;
+    nlb = std::max(nlb, l + 1);
+  }
+  auto bucket = [n](const std::vector<int>& lev, int nl, std::vector<int>& ptr,
+                    std::vector<int>& cells) {
+    ptr.assign(nl + 1, 0);
+    for (int i = 0; i < n; ++i) ptr[lev[i] + 1]++;
+    for (int l = 0; l < nl; ++l) ptr[l + 1] += ptr[l];
+    cells.assign(n, 0);
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (int i = 0; i < n; ++i) cells[fill[lev[i]]++] = i;
+  };
+  std::vector<int> pf, cf, pb, cb;
+  bucket(lev_f, nlf, pf, cf);
+  bucket(lev_b, nlb, pb, cb);
+  LusgsArrays& la = m->lusgs;
+  la.n_rec = d->n_records;
+  la.n_levels_f = nlf;
+  la.n_levels_b = nlb;
+  int rc = 0;
+  size_t R = d->n_records, NO = (size_t)d->off_ptr[n], ND = (size_t)d->diag_ptr[n];
+  rc |= upload_implicit(m, const_cast<int**>(&la.rec_o), d->rec_o, R);
+  rc |= upload_implicit(m, const_cast<int**>(&la.rec_n), d->rec_n, R);
+  rc |= upload_implicit(m, const_cast<double**>(&la.nox), d->rec_nox, R);
+  rc |= upload_implicit(m, const_cast<double**>(&la.noy), d->rec_noy, R);
+  rc |= upload_implicit(m, const_cast<double**>(&la.nnx), d->rec_nnx, R);
+  rc |= upload_implicit(m, const_cast<double**>(&la.nny), d->rec_nny, R);
+  rc |= upload_implicit(m, const_cast<double**>(&la.len), d->rec_len, R);
+  rc |= upload_implicit(m, const_cast<int**>(&la.diag_ptr), d->diag_ptr, (size_t)n + 1);
+  rc |= upload_implicit(m, const_cast<int**>(&la.diag_item), d->diag_item, ND);
+  rc |= upload_implicit(m, const_cast<int**>(&la.off_ptr), d->off_ptr, (size_t)n + 1);
+  rc |= upload_implicit(m, const_cast<int**>(&la.off_split), d->off_split, (size_t)n);
+  rc |= upload_implicit(m, const_cast<int**>(&la.off_item), d->off_item, NO);
+  rc |= upload_implicit(m, const_cast<int**>(&la.off_nb), d->off_nb, NO);
+  rc |= upload_implicit(m, const_cast<int**>(&la.lvl_ptr_f), pf.data(), pf.size());
+  rc |= upload_implicit(m, const_cast<int**>(&la.lvl_cells_f), cf.data(), cf.size());
+  rc |= upload_implicit(m, const_cast<int**>(&la.lvl_ptr_b), pb.data(), pb.size());
+  rc |= upload_implicit(m, const_cast<int**>(&la.lvl_cells_b), cb.data(), cb.size());
+  rc |= upload_implicit(m, &la.coef, (const double*)nullptr, R);
+  rc |= upload_implicit(m, &la.diag, (const double*)nullptr, (size_t)n);
+  rc |= upload_implicit(m, &la.off_val, (const double*)nullptr, NO);
+  if (rc) {
+    std::string msg = g_last_error;
+    free_implicit(m);
+    return fail(msg);
+  }
+  m->implicit = true;
+  return 0;
+}
+
+int mcfd_lusgs_assemble(McfdMesh* m, const McfdSettings* st, const double* u, const double* dt,
+                        int32_t dt_per_cell, void* stream) {
+  if (!m || !u || !dt) return fail("mcfd_lusgs_assemble: NULL argument");
+  if (!m->implicit) return fail("mcfd_lusgs_assemble: call mcfd_mesh_set_implicit first");
+  if (check_settings(st)) return -1;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int n = m->n_cells;
+  hipLaunchKernelGGL(k_primitive, dim3(blocks(n)), dim3(kBlock), 0, s, n, u, m->W, st->gamma,
+                     m->status);
+  hipLaunchKernelGGL(k_lusgs_coef, dim3(blocks(m->lusgs.n_rec)), dim3(kBlock), 0, s, m->lusgs,
+                     m->W, st->gamma);
+  hipLaunchKernelGGL(k_lusgs_assemble, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->lusgs, m->area,
+                     dt, dt_per_cell);
+  HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int mcfd_lusgs_apply(McfdMesh* m, const double* v, double* out, void* stream) {
+  if (!m || !v || !out) return fail("mcfd_lusgs_apply: NULL argument");
+  if (!m->implicit) return fail("mcfd_lusgs_apply: call mcfd_mesh_set_implicit first");
+  return lusgs_apply(m, v, out, static_cast<hipStream_t>(stream));
+}
+
+int mcfd_jacobian_apply(McfdMesh* m, const McfdSettings* jst, McfdGmres* g, const double* v,
+                        double* out, void* stream) {
+  if (!m || !g || !v || !out) return fail("mcfd_jacobian_apply: NULL argument");
+  if (check_settings(jst)) return -1;
+  if (ensure_gmres(m, 1)) return -1;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  JacobianOp op{m, jst, g->state, g->rate, g->dt, g->dt_per_cell,
+                {g->sigma[0], g->sigma[1], g->sigma[2], g->sigma[3]}, g->state_norm};
+  return jacobian_apply(op, v, out, s);
+}
+
+int mcfd_gmres(McfdMesh* m, const McfdSettings* jst, McfdGmres* g, void* stream) {
+  if (!m || !g || !g->state || !g->rate || !g->rhs || !g->x || !g->dt)
+    return fail("mcfd_gmres: NULL argument");
+  if (!m->implicit) return fail("mcfd_gmres: call mcfd_mesh_set_implicit first");
+  if (check_settings(jst)) return -1;
+  if (g->krylov <= 0 || g->max_iterations <= 0) return fail("mcfd_gmres: bad iteration limits");
+  if (ensure_gmres(m, g->krylov)) return -1;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  long N = 4L * m->n_cells;
+  HIP_CHECK(hipMemsetAsync(g->x, 0, sizeof(double) * N, s));
+  JacobianOp op{m, jst, g->state, g->rate, g->dt, g->dt_per_cell,
+                {g->sigma[0], g->sigma[1], g->sigma[2], g->sigma[3]}, g->state_norm};
+  int used = 0;
+  bool nonzero = false;
+  g->info = 1;
+  g->iterations = 0;
+  while (used < g->max_iterations) {
+    int restart = std::min(g->krylov, g->max_iterations - used);
+    int inner = 0, info = 1;
+    double rnorm = 0.0;
+    if (gmres_cycle(op, g->rhs, g->x, g->tolerance, restart, nonzero, inner, info, rnorm, s))
+      return -1;
+    nonzero = true;
+    used += inner;
+    g->iterations = used;
+    g->residual = rnorm;
+    if (info == 0) {
+      g->info = 0;
+      return 0;
+    }
+    if (inner == 0) break;
+  }
+  g->info = 1;
+  return 0;
+}
+
+int mcfd_newton_equation(McfdMesh* m, const double* u, const double* u0, const double* r,
+                         const double* dt, int32_t dt_per_cell, const double* sigma,
+                         double* scaled, double* norms, void* stream) {
+  if (!m || !u || !u0 || !r || !dt || !sigma || !norms)
+    return fail("mcfd_newton_equation: NULL argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int nb = blocks(m->n_cells);
+  hipLaunchKernelGGL(k_newton_equation, dim3(nb), dim3(kBlock), 0, s, m->n_cells, u, u0, r, dt,
+                     dt_per_cell, sigma[0], sigma[1], sigma[2], sigma[3], scaled, m->partial);
+  hipLaunchKernelGGL(k_max2_final, dim3(1), dim3(1024), 0, s, nb, m->partial, norms);
+  HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int mcfd_newton_candidate(McfdMesh* m, const double* u, const double* corr, double damping,
+                          const double* sigma, double* out, void* stream) {
+  if (!m || !u || !corr || !sigma || !out) return fail("mcfd_newton_candidate: NULL argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_candidate, dim3(blocks(m->n_cells)), dim3(kBlock), 0, s, m->n_cells, u,
+                     corr, damping, sigma[0], sigma[1], sigma[2], sigma[3], out);
+  HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int mcfd_norm2(McfdMesh* m, const double* a, double* out, void* stream) {
+  if (!m || !a || !out) return fail("mcfd_norm2: NULL argument");
+  if (ensure_gmres(m, 1)) return -1;
+  return dot_dev(m, 4L * m->n_cells, a, a, out, true, static_cast<hipStream_t>(stream));
 }
 
 int mcfd_status(McfdMesh* m, void* stream, int32_t* mask, int32_t* index) {

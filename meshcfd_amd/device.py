@@ -62,7 +62,15 @@ def _ptr(t: torch.Tensor | None):
 class DeviceMesh:
     """One mesh uploaded to one GPU, plus the step buffers of its case."""
 
-    def __init__(self, case: Case, device: str | torch.device = "cuda:0", geometry=None, layout=None):
+    def __init__(
+        self,
+        case: Case,
+        device: str | torch.device = "cuda:0",
+        geometry=None,
+        layout=None,
+        fused: bool = True,
+        block_cells: int = 128,
+    ):
         if not torch.cuda.is_available():
             raise native.NativeError("No ROCm device is visible: the GPU residual cannot run.")
         self.lib = native.load()
@@ -78,6 +86,9 @@ class DeviceMesh:
             native.check(self.lib.mcfd_mesh_create(C.byref(desc), C.byref(handle)))
             del keep
         self.handle = handle
+        self.blocks = None
+        if fused:
+            self.set_blocks(block_cells)
         self.native_settings = native_settings(self.settings)
         opts = dict(dtype=torch.float64, device=self.device)
         n = self.n
@@ -91,6 +102,20 @@ class DeviceMesh:
         self.dt_min = torch.empty((1,), **opts)
         self.clock = torch.zeros((2,), **opts)
         self.sums = torch.zeros((4,), **opts)
+
+    def set_blocks(self, block_cells: int | None) -> None:
+        """Switch to the fused one-kernel-per-stage path with blocks of
+        ``block_cells`` cells (None: back to the four-kernel path)."""
+        from .blocks import build_blocks
+
+        if block_cells is None:
+            native.check(self.lib.mcfd_mesh_set_blocks(self.handle, None))
+            self.blocks = None
+            return
+        self.blocks = build_blocks(self.layout, block_cells)
+        desc, keep = native.block_desc(self.blocks)
+        native.check(self.lib.mcfd_mesh_set_blocks(self.handle, C.byref(desc)))
+        del keep
 
     def close(self):
         if getattr(self, "handle", None):
@@ -116,11 +141,23 @@ class DeviceMesh:
             code = min(k for k in native.ERROR_MESSAGES if mask.value & (1 << k))
             raise SolverError(f"{context}{native.ERROR_MESSAGES[code]}")
 
+    # Device cells are stored in the layout's (Hilbert) order; these convert
+    # between mesh order (what callers see) and device order.
+    def to_device_order(self, array) -> torch.Tensor:
+        a = np.asarray(array, dtype=np.float64)[self.layout.cell_order]
+        return torch.as_tensor(np.ascontiguousarray(a), device=self.device)
+
+    def to_mesh_order(self, tensor: torch.Tensor) -> np.ndarray:
+        a = tensor.detach().cpu().numpy()
+        out = np.empty_like(a)
+        out[self.layout.cell_order] = a
+        return out
+
     def set_state(self, conservative) -> None:
-        self.u.copy_(torch.as_tensor(np.ascontiguousarray(conservative, dtype=np.float64)))
+        self.u.copy_(self.to_device_order(conservative))
 
     def state(self) -> np.ndarray:
-        return self.u.cpu().numpy()
+        return self.to_mesh_order(self.u)
 
     def _stage(self, op, u, out=None, u0=None, u1=None, r1=None, dt=None, per_cell=False, coeffs=(), r_out=None):
         st = native.McfdStage()
@@ -143,6 +180,38 @@ class DeviceMesh:
         out = torch.empty_like(u) if out is None else out
         self._stage(native.OP_RESIDUAL, u, r_out=out)
         return out
+
+    def _stage_first_order_residual(self, u, out, settings):
+        native.check(self.lib.mcfd_residual(self.handle, C.byref(settings), _ptr(u), _ptr(out), self.stream))
+        return out
+
+    def step_implicit(self, time: float) -> float:
+        """One implicit-Euler step of `solve_case` (`solver.py:447-513`);
+        returns the new time (host-synchronising: Newton needs the norms)."""
+        from .implicit import ImplicitStepper
+
+        if getattr(self, "_implicit", None) is None:
+            self._implicit = ImplicitStepper(self)
+        s = self.settings
+        mode = s.time_step_mode
+        if mode == "Local (steady)":
+            self.time_steps(self.u)
+            self.check_status()
+            dt, per_cell = self.dt_cell.clone(), True
+        elif mode == "Global fixed":
+            dt = torch.full((1,), min(s.fixed_time_step, s.end_time - time), dtype=torch.float64, device=self.device)
+            per_cell = False
+        else:
+            self.time_steps(self.u)
+            self.check_status()
+            value = min(float(self.dt_min.item()), s.end_time - time)
+            dt = torch.full((1,), value, dtype=torch.float64, device=self.device)
+            per_cell = False
+        if not bool(torch.all(torch.isfinite(dt))) or bool(torch.any(dt <= 0.0)):
+            raise SolverError("The stable time step is non-positive or non-finite.")
+        state, accepted = self._implicit.step(self.u, dt, per_cell, allow_reduction=(mode != "Global fixed"))
+        self.u.copy_(state)
+        return time + float(accepted.mean().item())
 
     def time_steps(self, u: torch.Tensor) -> torch.Tensor:
         """`_stable_time_steps` (`solver.py:4132-4343`): per-cell steps into

@@ -28,9 +28,40 @@ FACE_PRESSURE_OUTLET, FACE_FAR_FIELD, FACE_TRANSMISSIVE = 4, 5, 6
 ACC_OWNER, ACC_NEIGHBOUR, ACC_BOUNDARY, ACC_PERIODIC_B = 0, 1, 2, 3
 
 
+def hilbert_order(points: np.ndarray, bits: int = 16) -> np.ndarray:
+    """Permutation sorting points along a Hilbert curve (2^bits grid).
+
+    Cells that are close in this order are close in space, so a contiguous
+    block of cells (one workgroup) touches a compact neighbourhood and its
+    neighbour gathers hit the same L2 lines.
+    """
+    lo = points.min(axis=0)
+    span = np.maximum(points.max(axis=0) - lo, 1e-300)
+    side = (1 << bits) - 1
+    q = np.clip(((points - lo) / span.max() * side).astype(np.int64), 0, side)
+    x, y = q[:, 0].copy(), q[:, 1].copy()
+    d = np.zeros(len(points), dtype=np.int64)
+    s = 1 << (bits - 1)
+    while s > 0:
+        rx = ((x & s) > 0).astype(np.int64)
+        ry = ((y & s) > 0).astype(np.int64)
+        d += s * s * ((3 * rx) ^ ry)
+        # rotate the quadrant
+        flip = ry == 0
+        swap_x = flip & (rx == 1)
+        x = np.where(swap_x, side - x, x)
+        y = np.where(swap_x, side - y, y)
+        x2 = np.where(flip, y, x)
+        y2 = np.where(flip, x, y)
+        x, y = x2, y2
+        s >>= 1
+    return np.argsort(d, kind="stable")
+
+
 @dataclass
 class DeviceLayout:
     n_cells: int
+    cell_order: np.ndarray  # device cell j holds mesh cell cell_order[j]
     area: np.ndarray
     grad_ptr: np.ndarray
     grad_nb: np.ndarray
@@ -78,10 +109,17 @@ def _csr(keys: np.ndarray, n: int):
     return ptr.astype(np.int32), order
 
 
-def build_layout(g: Geometry) -> DeviceLayout:
+def build_layout(g: Geometry, reorder: bool = True) -> DeviceLayout:
+    """Device layout; with ``reorder`` cells are renumbered along a Hilbert
+    curve and flux faces sorted by their (renumbered) left cell. Per-cell
+    lists keep the reference summation order, which depends on original face
+    ids only, so reordering changes no result bit."""
     n = g.cell_count
     if n >= 2**31 - 1:
         raise ValueError("Mesh too large for 32-bit cell indices.")
+    order_cells = hilbert_order(g.centers) if reorder else np.arange(n)
+    new_id = np.empty(n, dtype=np.int64)
+    new_id[order_cells] = np.arange(n)
     c = g.centers
     fc = g.face_centers
 
@@ -140,56 +178,74 @@ def build_layout(g: Geometry) -> DeviceLayout:
     bc_wall = np.concatenate([gr[8] for gr in groups[2:]]) if n_bc else np.zeros((0, 2))
     bc_kind = kinds[bstart:].copy()
 
+    # --- renumbering: faces sorted by their renumbered left cell ----------------
+    fperm = np.argsort(new_id[left], kind="stable")  # new face -> old face
+    fnew = np.empty(nf, dtype=np.int64)
+    fnew[fperm] = np.arange(nf)
+
     # --- accumulation items ------------------------------------------------
     k_int = np.arange(n_int, dtype=np.int64)
-    # interior: row order ascending face index over owner and neighbour entries
+    # interior: row order ascending (original) face index over owner and neighbour entries
     int_cells = np.concatenate((io, ine))
-    int_items = np.concatenate((k_int * 4 + ACC_OWNER, k_int * 4 + ACC_NEIGHBOUR))
-    int_face = np.concatenate((k_int, k_int))
-    o = np.lexsort((int_face, int_cells))
-    int_cells, int_items = int_cells[o], int_items[o]
+    int_k = np.concatenate((k_int, k_int))
+    int_mode = np.concatenate((np.full(n_int, ACC_OWNER), np.full(n_int, ACC_NEIGHBOUR)))
+    o = np.lexsort((int_k, int_cells))
+    int_cells, int_k, int_mode = int_cells[o], int_k[o], int_mode[o]
     k_per = n_int + np.arange(n_per, dtype=np.int64)
     k_bnd = bstart + np.arange(n_bc, dtype=np.int64)
     b_owners = left[bstart:].astype(np.int64)
     seq_cells = np.concatenate((oa, ob, b_owners))
-    seq_items = np.concatenate((k_per * 4 + ACC_BOUNDARY, k_per * 4 + ACC_PERIODIC_B, k_bnd * 4 + ACC_BOUNDARY))
+    seq_k = np.concatenate((k_per, k_per, k_bnd))
+    seq_mode = np.concatenate(
+        (np.full(n_per, ACC_BOUNDARY), np.full(n_per, ACC_PERIODIC_B), np.full(n_bc, ACC_BOUNDARY))
+    )
     # interior entries (already cell-grouped, ascending face) precede the sequence entries
     all_cells = np.concatenate((int_cells, seq_cells))
-    all_items = np.concatenate((int_items, seq_items))
-    acc_ptr, order = _csr(all_cells, n)
+    all_items = fnew[np.concatenate((int_k, seq_k))] * 4 + np.concatenate((int_mode, seq_mode))
+    acc_ptr, order = _csr(new_id[all_cells], n)
     acc_item = all_items[order]
     if acc_item.size and acc_item.max() >= 2**31:
         raise ValueError("Too many faces for 32-bit accumulation items.")
 
     # --- LSQ and limiter entries -------------------------------------------
-    grad_ptr, order = _csr(g.gradient_sources, n)
-    grad_nb = g.gradient_neighbours[order].astype(np.int32)
+    grad_ptr, order = _csr(new_id[g.gradient_sources], n)
+    grad_nb = new_id[g.gradient_neighbours[order]].astype(np.int32)
     grad_cx = g.gradient_coefficients_x[order]
     grad_cy = g.gradient_coefficients_y[order]
-    lim_ptr, order = _csr(g.limiter_cells, n)
+    lim_ptr, order = _csr(new_id[g.limiter_cells], n)
     lim_dx = g.limiter_dx[order]
     lim_dy = g.limiter_dy[order]
 
     # --- spectral-radius entries -------------------------------------------
     F = len(g.owners)
     mesh_bc = np.full(F, -1, np.int64)
-    # boundary records for faces whose exterior state enters the time step
-    off = bstart - bstart  # records are numbered from the first boundary flux face
+    # boundary records (numbered from the first boundary flux face) of faces
+    # whose exterior state enters the time step
     rec = 0
     for kind, faces in zip(
         (FACE_WALL, FACE_VELOCITY_INLET, FACE_PRESSURE_OUTLET, FACE_FAR_FIELD, FACE_TRANSMISSIVE), bc_faces
     ):
         if kind in (FACE_VELOCITY_INLET, FACE_PRESSURE_OUTLET, FACE_FAR_FIELD):
-            mesh_bc[faces] = off + rec + np.arange(len(faces))
+            mesh_bc[faces] = rec + np.arange(len(faces))
         rec += len(faces)
     d_cells = np.concatenate((g.owners, ine))
     d_faces = np.concatenate((np.arange(F), inter))
     d_bc = np.concatenate((mesh_bc, np.full(len(inter), -1, np.int64)))
-    dts_ptr, order = _csr(d_cells, n)
+    dts_ptr, order = _csr(new_id[d_cells], n)
     d_faces = d_faces[order]
+    right_new = np.where(right >= 0, new_id[np.maximum(right, 0)], -1)
+    left = new_id[left][fperm].astype(np.int32)
+    right = right_new[fperm].astype(np.int32)
+    kinds = kinds[fperm]
+    face_bc = face_bc[fperm]
+    normals = normals[fperm]
+    lengths = lengths[fperm]
+    loff = loff[fperm]
+    roff = roff[fperm]
     return DeviceLayout(
         n_cells=n,
-        area=np.ascontiguousarray(g.areas),
+        cell_order=order_cells,
+        area=np.ascontiguousarray(g.areas[order_cells]),
         grad_ptr=grad_ptr,
         grad_nb=grad_nb,
         grad_cx=np.ascontiguousarray(grad_cx),

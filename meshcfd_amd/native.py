@@ -75,6 +75,61 @@ class McfdMeshDesc(C.Structure):
     ]
 
 
+_hp = C.POINTER(C.c_uint16)
+
+
+class McfdBlockDesc(C.Structure):
+    _fields_ = [
+        ("n_blocks", C.c_int32),
+        ("block_cells", C.c_int32),
+        ("max_local", C.c_int32),
+        ("max_grad", C.c_int32),
+        ("max_faces", C.c_int32),
+        ("n_local", C.c_int32),
+        ("n_gnb_ptr", C.c_int32),
+        ("n_gnb", C.c_int32),
+        ("n_block_faces", C.c_int32),
+        ("reserved", C.c_int32),
+        ("info", _ip),
+        ("cells", _ip),
+        ("gnb_ptr", _ip),
+        ("gnb", _hp),
+        ("faces", _ip),
+        ("face_lr", _hp),
+        ("acc_local", _hp),
+    ]
+
+
+def block_desc(bl) -> tuple[McfdBlockDesc, list]:
+    keep = []
+
+    def arr(a, dtype, ctype):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        keep.append(a)
+        return as_ptr(a, ctype)
+
+    desc = McfdBlockDesc(
+        n_blocks=bl.n_blocks,
+        block_cells=bl.block_cells,
+        max_local=bl.max_local,
+        max_grad=bl.max_grad_cells,
+        max_faces=bl.max_faces,
+        n_local=len(bl.cells),
+        n_gnb_ptr=len(bl.gnb_ptr),
+        n_gnb=len(bl.gnb),
+        n_block_faces=len(bl.faces),
+        reserved=0,
+        info=arr(bl.info.reshape(-1), np.int32, C.c_int32),
+        cells=arr(bl.cells, np.int32, C.c_int32),
+        gnb_ptr=arr(bl.gnb_ptr, np.int32, C.c_int32),
+        gnb=arr(bl.gnb, np.uint16, C.c_uint16),
+        faces=arr(bl.faces, np.int32, C.c_int32),
+        face_lr=arr(bl.face_lr.reshape(-1), np.uint16, C.c_uint16),
+        acc_local=arr(bl.acc_local, np.uint16, C.c_uint16),
+    )
+    return desc, keep
+
+
 class McfdSettings(C.Structure):
     _fields_ = [
         ("flux", C.c_int32),
@@ -108,6 +163,7 @@ EXPORTS = {
     "mcfd_last_error": (C.c_int, [C.c_char_p, C.c_size_t]),
     "mcfd_mesh_create": (C.c_int, [C.POINTER(McfdMeshDesc), C.POINTER(C.c_void_p)]),
     "mcfd_mesh_destroy": (C.c_int, [C.c_void_p]),
+    "mcfd_mesh_set_blocks": (C.c_int, [C.c_void_p, C.c_void_p]),
     "mcfd_residual": (C.c_int, [C.c_void_p, C.POINTER(McfdSettings), C.c_void_p, C.c_void_p, C.c_void_p]),
     "mcfd_time_steps": (
         C.c_int,

@@ -267,6 +267,31 @@ class Oracle:
             + scale(d4, 0.226007483236906)
         )
 
+    def solve(self, primitive0, time=0.0):
+        """The explicit part of `solve_case`'s loop (`solver.py:437-999`):
+        end-time test with the 1e-13 relative slack, local-mode steady test on
+        the RMS of an extra residual; returns (conservative, time, steps,
+        completed)."""
+        s = self.s
+        u = primitive_to_conservative(primitive0, s.gamma)
+        steps = 0
+        completed = False
+        local = s.time_step_mode == "Local (steady)"
+        for step in range(1, s.max_iterations + 1):
+            u, dt, _, _ = self.step(u, time)
+            time += dt
+            steps = step
+            if local:
+                r = self.residual(u)
+                if float(np.sqrt(np.mean(r**2))) <= s.residual_tolerance:
+                    completed = True
+                    break
+            elif time >= s.end_time * (1.0 - 1e-13):
+                time = s.end_time
+                completed = True
+                break
+        return u, time, steps, completed
+
     def run(self, primitive0, steps, time=0.0):
         """Advance ``steps`` explicit steps from a primitive state."""
         u = primitive_to_conservative(primitive0, self.s.gamma)

@@ -25,12 +25,17 @@ def rel_err(gpu, ref):
     return np.max(np.abs(gpu - ref), axis=0) / scale
 
 
-def _device(case, cuda_device):
+@pytest.fixture(params=["fused", "unfused"])
+def path(request):
+    return request.param
+
+
+def _device(case, cuda_device, path="fused", block_cells=64):
     from meshcfd_amd.device import DeviceMesh
 
     print(f"case: {case.name} cells={case.mesh.cell_count} flux={case.numerics.flux} "
-          f"rec={case.numerics.reconstruction} ti={case.numerics.time_integration}", flush=True)
-    return DeviceMesh(case, cuda_device)
+          f"rec={case.numerics.reconstruction} ti={case.numerics.time_integration} path={path}", flush=True)
+    return DeviceMesh(case, cuda_device, fused=(path == "fused"), block_cells=block_cells)
 
 
 def _perturbed_primitive(case, seed=3):
@@ -72,20 +77,18 @@ def _case(kind, flux, reconstruction, limiter="Barth-Jespersen", n=12, split=0.3
 @pytest.mark.parametrize("flux", ["Rusanov", "HLLC", "Roe"])
 @pytest.mark.parametrize("reconstruction", ["First order", "MUSCL linear"])
 @pytest.mark.parametrize("kind", ["periodic", "mixed", "farfield", "transmissive"])
-def test_residual_matches_oracle(cuda_device, flux, reconstruction, kind):
-    import torch
+def test_residual_matches_oracle(cuda_device, path, flux, reconstruction, kind):
     from oracle.euler import primitive_to_conservative
     from oracle.solver import Oracle
 
     case = _case(kind, flux, reconstruction)
-    dm = _device(case, cuda_device)
+    dm = _device(case, cuda_device, path)
     prim = _perturbed_primitive(case)
     u = primitive_to_conservative(prim, 1.4)
     ref = Oracle(dm.geometry, case.numerics).residual(u)
-    ut = torch.as_tensor(u, device=cuda_device)
-    r = dm.residual(ut)
+    r = dm.residual(dm.to_device_order(u))
     dm.check_status()
-    gpu = r.cpu().numpy()
+    gpu = dm.to_mesh_order(r)
     err = rel_err(gpu, ref)
     print("residual rel err", err)
     assert np.all(err <= 1e-13), err
@@ -95,23 +98,21 @@ def test_residual_matches_oracle(cuda_device, flux, reconstruction, kind):
 
 
 @pytest.mark.parametrize("limiter", ["Barth-Jespersen", "Minmod", "Van Leer", "Venkatakrishnan"])
-def test_limiters_match_oracle(cuda_device, limiter):
-    import torch
+def test_limiters_match_oracle(cuda_device, path, limiter):
     from oracle.euler import primitive_to_conservative
     from oracle.solver import Oracle
 
     case = _case("periodic", "HLLC", "MUSCL linear", limiter=limiter)
-    dm = _device(case, cuda_device)
+    dm = _device(case, cuda_device, path)
     u = primitive_to_conservative(_perturbed_primitive(case, seed=9), 1.4)
     ref = Oracle(dm.geometry, case.numerics).residual(u)
-    gpu = dm.residual(torch.as_tensor(u, device=cuda_device)).cpu().numpy()
+    gpu = dm.to_mesh_order(dm.residual(dm.to_device_order(u)))
     assert np.array_equal(gpu, ref), rel_err(gpu, ref)
     dm.close()
 
 
 @pytest.mark.parametrize("kind", ["periodic", "mixed", "farfield"])
 def test_time_steps_match_oracle(cuda_device, kind):
-    import torch
     from oracle.euler import conservative_to_primitive, primitive_to_conservative
     from oracle.solver import Oracle
 
@@ -119,9 +120,9 @@ def test_time_steps_match_oracle(cuda_device, kind):
     dm = _device(case, cuda_device)
     u = primitive_to_conservative(_perturbed_primitive(case), 1.4)
     ref = Oracle(dm.geometry, case.numerics).stable_time_steps(conservative_to_primitive(u, 1.4))
-    dm.time_steps(torch.as_tensor(u, device=cuda_device))
+    dm.time_steps(dm.to_device_order(u))
     dm.check_status()
-    gpu = dm.dt_cell.cpu().numpy()
+    gpu = dm.to_mesh_order(dm.dt_cell)
     assert np.max(np.abs(gpu - ref) / ref) <= 1e-14
     if kind != "farfield":
         assert np.array_equal(gpu, ref)
@@ -129,14 +130,14 @@ def test_time_steps_match_oracle(cuda_device, kind):
     dm.close()
 
 
-def test_vortex_100_steps_matches_oracle(cuda_device):
+def test_vortex_100_steps_matches_oracle(cuda_device, path):
     """Isentropic vortex, 64x64 jittered periodic quads, Rusanov, first order,
     explicit Euler, CFL 0.5, 100 steps."""
     from oracle.euler import primitive_to_conservative
     from oracle.solver import Oracle
 
     case, prim = meshgen.vortex_workload(64, 100)
-    dm = _device(case, cuda_device)
+    dm = _device(case, cuda_device, path)
     dm.set_state(primitive_to_conservative(prim, 1.4))
     t_gpu = dm.run(100)
     gpu = dm.state()
@@ -151,7 +152,7 @@ def test_vortex_100_steps_matches_oracle(cuda_device):
 
 @pytest.mark.parametrize("integrator", ["Explicit Euler", "Explicit RK2", "Explicit RK3", "Explicit RK4"])
 @pytest.mark.parametrize("mode", ["Global adaptive", "Local (steady)", "Global fixed"])
-def test_riemann_steps_match_oracle(cuda_device, integrator, mode):
+def test_riemann_steps_match_oracle(cuda_device, path, integrator, mode):
     """Four-quadrant Riemann problem, HLLC + MUSCL linear, 20 steps."""
     from oracle.euler import primitive_to_conservative
     from oracle.solver import Oracle
@@ -161,7 +162,7 @@ def test_riemann_steps_match_oracle(cuda_device, integrator, mode):
     case.numerics.time_step_mode = mode
     case.numerics.fixed_time_step = 2e-4
     case.numerics.end_time = 1e30 if mode != "Global fixed" else 0.00395
-    dm = _device(case, cuda_device)
+    dm = _device(case, cuda_device, path)
     dm.set_state(primitive_to_conservative(prim, 1.4))
     steps = 20
     t_gpu = dm.run(steps)
@@ -174,15 +175,32 @@ def test_riemann_steps_match_oracle(cuda_device, integrator, mode):
     dm.close()
 
 
-def test_reruns_are_bit_identical(cuda_device):
+def test_reruns_are_bit_identical(cuda_device, path):
     from oracle.euler import primitive_to_conservative
 
     case, prim = meshgen.riemann_workload(64, 10)
     out = []
     for _ in range(2):
-        dm = _device(case, cuda_device)
+        dm = _device(case, cuda_device, path)
         dm.set_state(primitive_to_conservative(prim, 1.4))
         dm.run(10)
         out.append(dm.state().tobytes())
         dm.close()
     assert out[0] == out[1]
+
+
+@pytest.mark.parametrize("block_cells", [32, 96, 128, 192])
+def test_fused_block_sizes_match_unfused(cuda_device, block_cells):
+    """Every block size gives the same bits as the four-kernel path."""
+    from oracle.euler import primitive_to_conservative
+
+    case, prim = meshgen.riemann_workload(48, 5)
+    u0 = primitive_to_conservative(prim, 1.4)
+    out = []
+    for path, b in (("unfused", 64), ("fused", block_cells)):
+        dm = _device(case, cuda_device, path, block_cells=b)
+        dm.set_state(u0)
+        dm.run(5)
+        out.append(dm.state())
+        dm.close()
+    assert np.array_equal(out[0], out[1])
