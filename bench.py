@@ -69,9 +69,17 @@ def algorithmic_bytes(layout, reconstruction: str, integrator: str):
 
 
 def cpu_baseline(case, geometry, primitive, steps: int):
-    """Time the CPU oracle (NumPy restatement of the reference) on one core."""
+    """Time the reference-equivalent CPU path on one core: the NumPy/SciPy
+    restatement of the solver with single-threaded C stand-ins for the
+    reference's numba kernels (oracle/mcfd_oracle.c), as the reference runs
+    them compiled."""
+    from oracle import compiled
     from oracle.euler import primitive_to_conservative
     from oracle.solver import Oracle
+
+    use_c = compiled.available()
+    if use_c:
+        compiled.enable()
 
     old = os.sched_getaffinity(0)
     core = min(old)
@@ -88,13 +96,17 @@ def cpu_baseline(case, geometry, primitive, steps: int):
         elapsed = time.perf_counter() - t0
     finally:
         os.sched_setaffinity(0, old)
+        if use_c:
+            compiled.disable()
     cells = geometry.cell_count
     return {
         "value": cells * steps / elapsed / 1e6,
         "unit": "Mcell-updates/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{steps} SSP-RK3 step(s) of the full {cells}-cell mesh, NumPy/SciPy oracle, pinned to 1 core",
+        "sample": f"{steps} SSP-RK3 step(s) of the full {cells}-cell mesh after one warm-up residual; "
+        + ("NumPy/SciPy oracle with C stand-ins for the numba kernels" if use_c else "NumPy/SciPy oracle")
+        + ", pinned to 1 core",
         "seconds": elapsed,
         "host_cores": os.cpu_count(),
         "cpu_model": platform.processor() or platform.machine(),
@@ -107,7 +119,7 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=1000, help="quads per side")
-    ap.add_argument("--cpu-steps", type=int, default=1, help="SSP-RK3 steps timed on the CPU oracle (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=5, help="SSP-RK3 steps timed on the CPU oracle (0 = skip)")
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--unfused", action="store_true", help="four-kernel path instead of the fused stage kernel")
     ap.add_argument("--block-cells", type=int, default=128, help="cells per workgroup of the fused stage kernel")
