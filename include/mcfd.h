@@ -282,6 +282,26 @@ int mcfd_newton_candidate(McfdMesh* mesh, const double* u, const double* corr, d
 /* out[0] = ||a||_2 over the (C,4) array (device), deterministic. */
 int mcfd_norm2(McfdMesh* mesh, const double* a, double* out, void* stream);
 
+/* Upload the reverse-entry map of the LSQ lists (grad_rev[e] = entry of the
+ * neighbour that points back) and allocate the adjoint scratch; NULL frees. */
+int mcfd_mesh_set_adjoint(McfdMesh* mesh, const int32_t* grad_rev);
+
+/* ubar = lambda^T dR/dU at state u: the hand-written reverse of `_residual`
+ * (solver.py:2437-2718) through fluxes, ghost states, reconstruction,
+ * limiter, LSQ gradient and primitive conversion. No counterpart in the
+ * reference (which has no adjoint). */
+int mcfd_residual_vjp(McfdMesh* mesh, const McfdSettings* settings, const double* u,
+                      const double* lambda, double* ubar, void* stream);
+
+/* Batched cases (cells of case c in [case_ptr[c], case_ptr[c+1]), device):
+ * per-cell stable steps, then per case dt = min(case minimum, end_time -
+ * time_c), time_c += dt, and dt written to every cell of the case so the
+ * stage kernels run with per-cell steps. clock = [time, dt] per case. No
+ * counterpart in the reference (one case per run). */
+int mcfd_batch_time_steps(McfdMesh* mesh, const McfdSettings* settings, const double* u,
+                          int32_t n_cases, const int32_t* case_ptr, double end_time,
+                          double* clock, double* dt_cell, void* stream);
+
 /* Per-component sum of R^2 over cells into sums[4] (device), deterministic. */
 int mcfd_sum_squares(McfdMesh* mesh, const double* r, double* sums, void* stream);
 

@@ -1112,6 +1112,591 @@ __global__ void __launch_bounds__(kBlock)
          a.p + damping * c.p * s3);
 }
 
+
+// ---------------------------------------------------------------------------
+// Adjoint residual (vector-Jacobian product).
+//
+// Face fluxes are differentiated in forward mode with 8-direction dual
+// numbers (4 left + 4 right primitive components); every comparison is made
+// on the value part, which is computed with the same arithmetic as the
+// forward kernels, so the derivative is that of the branch the forward pass
+// took. The reverse sweep through reconstruction, limiter, LSQ gradient and
+// primitive conversion is written out by hand as per-cell gathers (no
+// atomics: deterministic).
+// ---------------------------------------------------------------------------
+
+struct D8 {
+  double v;
+  double d[8];
+};
+
+__device__ __forceinline__ D8 dconst(double v) {
+  D8 r;
+  r.v = v;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r.d[k] = 0.0;
+  return r;
+}
+__device__ __forceinline__ D8 operator+(const D8& a, const D8& b) {
+  D8 r;
+  r.v = a.v + b.v;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r.d[k] = a.d[k] + b.d[k];
+  return r;
+}
+__device__ __forceinline__ D8 operator-(const D8& a, const D8& b) {
+  D8 r;
+  r.v = a.v - b.v;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r.d[k] = a.d[k] - b.d[k];
+  return r;
+}
+__device__ __forceinline__ D8 operator-(const D8& a) {
+  D8 r;
+  r.v = -a.v;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r.d[k] = -a.d[k];
+  return r;
+}
+__device__ __forceinline__ D8 operator*(const D8& a, const D8& b) {
+  D8 r;
+  r.v = a.v * b.v;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+  return r;
+}
+__device__ __forceinline__ D8 operator/(const D8& a, const D8& b) {
+  D8 r;
+  r.v = a.v / b.v;
+  double inv = 1.0 / b.v;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) * inv;
+  return r;
+}
+__device__ __forceinline__ D8 operator+(const D8& a, double b) { D8 r = a; r.v = a.v + b; return r; }
+__device__ __forceinline__ D8 operator+(double a, const D8& b) { D8 r = b; r.v = a + b.v; return r; }
+__device__ __forceinline__ D8 operator-(const D8& a, double b) { D8 r = a; r.v = a.v - b; return r; }
+__device__ __forceinline__ D8 operator-(double a, const D8& b) {
+  D8 r;
+  r.v = a - b.v;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r.d[k] = -b.d[k];
+  return r;
+}
+__device__ __forceinline__ D8 operator*(double a, const D8& b) {
+  D8 r;
+  r.v = a * b.v;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r.d[k] = a * b.d[k];
+  return r;
+}
+__device__ __forceinline__ D8 operator*(const D8& a, double b) {
+  D8 r;
+  r.v = a.v * b;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r.d[k] = a.d[k] * b;
+  return r;
+}
+__device__ __forceinline__ D8 operator/(const D8& a, double b) {
+  D8 r;
+  r.v = a.v / b;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r.d[k] = a.d[k] / b;
+  return r;
+}
+__device__ __forceinline__ D8 operator/(double a, const D8& b) { return dconst(a) / b; }
+__device__ __forceinline__ D8 dsqrt(const D8& a) {
+  D8 r;
+  r.v = sqrt(a.v);
+  double h = 0.5 / r.v;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r.d[k] = a.d[k] * h;
+  return r;
+}
+__device__ __forceinline__ D8 dfabs(const D8& a) { return (a.v < 0.0) ? -a : a; }
+__device__ __forceinline__ D8 dpow(const D8& a, double e) {
+  D8 r;
+  r.v = pow(a.v, e);
+  double dv = e * pow(a.v, e - 1.0);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r.d[k] = a.d[k] * dv;
+  return r;
+}
+__device__ __forceinline__ D8 dmin(const D8& a, const D8& b) { return (a.v <= b.v) ? a : b; }
+__device__ __forceinline__ D8 dmax(const D8& a, const D8& b) { return (a.v >= b.v) ? a : b; }
+
+struct P4D {
+  D8 r, u, v, p;
+};
+struct F4D {
+  D8 a, b, c, d;
+};
+
+struct SideD {
+  D8 rho, u, v, p, un, E;
+  D8 U[4];
+  D8 F[4];
+};
+
+__device__ __forceinline__ void side_state_d(const P4D& w, double nx, double ny, double g, SideD& s) {
+  s.rho = w.r;
+  s.u = w.u;
+  s.v = w.v;
+  s.p = w.p;
+  s.un = w.u * nx + w.v * ny;
+  s.E = w.p / (g - 1.0) + 0.5 * w.r * (w.u * w.u + w.v * w.v);
+  s.U[0] = w.r;
+  s.U[1] = w.r * w.u;
+  s.U[2] = w.r * w.v;
+  s.U[3] = s.E;
+  s.F[0] = w.r * s.un;
+  s.F[1] = w.r * w.u * s.un + w.p * nx;
+  s.F[2] = w.r * w.v * s.un + w.p * ny;
+  s.F[3] = (s.E + w.p) * s.un;
+}
+
+__device__ __forceinline__ D8 sound_d(const P4D& w, double g) { return dsqrt(g * w.p / w.r); }
+
+__device__ F4D rusanov_d(const P4D& L, const P4D& R, double nx, double ny, double g) {
+  SideD a, b;
+  side_state_d(L, nx, ny, g, a);
+  side_state_d(R, nx, ny, g, b);
+  D8 s = dmax(dfabs(a.un) + sound_d(L, g), dfabs(b.un) + sound_d(R, g));
+  F4D f;
+  f.a = 0.5 * (a.F[0] + b.F[0] - s * (b.U[0] - a.U[0]));
+  f.b = 0.5 * (a.F[1] + b.F[1] - s * (b.U[1] - a.U[1]));
+  f.c = 0.5 * (a.F[2] + b.F[2] - s * (b.U[2] - a.U[2]));
+  f.d = 0.5 * (a.F[3] + b.F[3] - s * (b.U[3] - a.U[3]));
+  return f;
+}
+
+__device__ F4D hllc_star_d(const SideD& s, const D8& wave, const D8& star, double nx, double ny) {
+  D8 factor = s.rho * (wave - s.un) / (wave - star);
+  D8 vx = s.u + (star - s.un) * nx;
+  D8 vy = s.v + (star - s.un) * ny;
+  D8 energy = factor * (s.E / s.rho + (star - s.un) * (star + s.p / (s.rho * (wave - s.un))));
+  F4D f;
+  f.a = s.F[0] + wave * (factor - s.U[0]);
+  f.b = s.F[1] + wave * (factor * vx - s.U[1]);
+  f.c = s.F[2] + wave * (factor * vy - s.U[2]);
+  f.d = s.F[3] + wave * (energy - s.U[3]);
+  return f;
+}
+
+__device__ F4D hllc_d(const P4D& L, const P4D& R, double nx, double ny, double g, bool fused_order) {
+  SideD a, b;
+  side_state_d(L, nx, ny, g, a);
+  side_state_d(R, nx, ny, g, b);
+  D8 cl = sound_d(L, g), cr = sound_d(R, g);
+  D8 sl = dmin(a.un - cl, b.un - cr);
+  D8 sr = dmax(a.un + cl, b.un + cr);
+  D8 den = a.rho * (sl - a.un) - b.rho * (sr - b.un);
+  if (!(fabs(den.v) > 1e-14)) den = dconst(copysign(1e-14, den.v));
+  D8 star = (b.p - a.p + a.rho * a.un * (sl - a.un) - b.rho * b.un * (sr - b.un)) / den;
+  if (sl.v >= 0.0) return F4D{a.F[0], a.F[1], a.F[2], a.F[3]};
+  if (fused_order) {
+    if (sr.v <= 0.0) return F4D{b.F[0], b.F[1], b.F[2], b.F[3]};
+    if (star.v >= 0.0) return hllc_star_d(a, sl, star, nx, ny);
+    return hllc_star_d(b, sr, star, nx, ny);
+  }
+  if (star.v >= 0.0) return hllc_star_d(a, sl, star, nx, ny);
+  if (sr.v > 0.0) return hllc_star_d(b, sr, star, nx, ny);
+  return F4D{b.F[0], b.F[1], b.F[2], b.F[3]};
+}
+
+__device__ F4D roe_d(const P4D& L, const P4D& R, double nx, double ny, double g) {
+  SideD a, b;
+  side_state_d(L, nx, ny, g, a);
+  side_state_d(R, nx, ny, g, b);
+  D8 sl = dsqrt(L.r), sr = dsqrt(R.r);
+  D8 den = sl + sr;
+  D8 u = (sl * L.u + sr * R.u) / den;
+  D8 v = (sl * L.v + sr * R.v) / den;
+  D8 h = (sl * (L.p / L.r + 0.5 * (L.u * L.u + L.v * L.v)) + sr * (R.p / R.r + 0.5 * (R.u * R.u + R.v * R.v))) / den;
+  D8 un = u * nx + v * ny;
+  D8 ut = u * -ny + v * nx;
+  D8 a2 = (g - 1.0) * (h - 0.5 * (u * u + v * v));
+  D8 c = dsqrt(a2);
+  D8 rho_avg = sl * sr;
+  D8 utl = L.u * -ny + L.v * nx;
+  D8 utr = R.u * -ny + R.v * nx;
+  D8 drho = R.r - L.r, dun = b.un - a.un, dut = utr - utl, dp = R.p - L.p;
+  D8 am = (dp - rho_avg * c * dun) / (2.0 * a2);
+  D8 ap = (dp + rho_avg * c * dun) / (2.0 * a2);
+  D8 ac = drho - dp / a2;
+  D8 as = rho_avg * dut;
+  D8 fix = 0.1 * c;
+  D8 l0 = dfabs(un - c), l1 = dfabs(un), l3 = dfabs(un + c);
+  if (l0.v < fix.v) l0 = 0.5 * (l0 * l0 / fix + fix);
+  if (l1.v < fix.v) l1 = 0.5 * (l1 * l1 / fix + fix);
+  if (l3.v < fix.v) l3 = 0.5 * (l3 * l3 / fix + fix);
+  D8 w0 = l0 * am, w1 = l1 * ac, w2 = l1 * as, w3 = l3 * ap;
+  D8 d0 = w0 + w1 + w3;
+  D8 d1 = (un - c) * w0 + un * w1 + (un + c) * w3;
+  D8 d2 = ut * w0 + ut * w1 + w2 + ut * w3;
+  D8 d3 = (h - un * c) * w0 + 0.5 * (un * un + ut * ut) * w1 + ut * w2 + (h + un * c) * w3;
+  F4D f;
+  f.a = 0.5 * (a.F[0] + b.F[0] - d0);
+  f.b = 0.5 * (a.F[1] + b.F[1] - (d1 * nx - d2 * ny));
+  f.c = 0.5 * (a.F[2] + b.F[2] - (d1 * ny + d2 * nx));
+  f.d = 0.5 * (a.F[3] + b.F[3] - d3);
+  return f;
+}
+
+__device__ P4D far_field_d(const P4D& in, const P4& out, double nx, double ny, double g) {
+  D8 un_i = in.u * nx + in.v * ny;
+  double un_f = out.u * nx + out.v * ny;
+  D8 c_i = sound_d(in, g);
+  double c_f = sound(out, g);
+  if (un_i.v <= -c_i.v) return P4D{dconst(out.r), dconst(out.u), dconst(out.v), dconst(out.p)};
+  if (un_i.v >= c_i.v) return in;
+  D8 r_out = un_i + 2.0 * c_i / (g - 1.0);
+  double r_in = un_f - 2.0 * c_f / (g - 1.0);
+  D8 vb = 0.5 * (r_out + r_in);
+  D8 cb = 0.25 * (g - 1.0) * (r_out - r_in);
+  bool inflow = vb.v < 0.0;
+  P4D src = inflow ? P4D{dconst(out.r), dconst(out.u), dconst(out.v), dconst(out.p)} : in;
+  D8 entropy = src.p / dpow(src.r, g);
+  D8 rho = dpow(cb * cb / (g * entropy), 1.0 / (g - 1.0));
+  D8 p = entropy * dpow(rho, g);
+  double tx = -ny, ty = nx;
+  D8 vt = src.u * tx + src.v * ty;
+  return P4D{rho, vb * nx + vt * tx, vb * ny + vt * ty, p};
+}
+
+__device__ __forceinline__ P4D seed(const P4& w, int base) {
+  P4D r{dconst(w.r), dconst(w.u), dconst(w.v), dconst(w.p)};
+  r.r.d[base] = 1.0;
+  r.u.d[base + 1] = 1.0;
+  r.v.d[base + 2] = 1.0;
+  r.p.d[base + 3] = 1.0;
+  return r;
+}
+
+// Face adjoint: Fbar = sum of the face's residual coefficients times lambda,
+// then (dF/dW_L)^T Fbar and (dF/dW_R)^T Fbar -> FADJ[k][0..7].
+__global__ void __launch_bounds__(kBlock)
+    k_flux_adjoint(int nf, FaceArrays fa, const double* __restrict__ W,
+                   const double* __restrict__ G, const double* __restrict__ PHI,
+                   const double* __restrict__ flen, const double* __restrict__ area,
+                   const double* __restrict__ lam, int scheme, int muscl, double g,
+                   double* __restrict__ FADJ) {
+  int k = xcd_block() * blockDim.x + threadIdx.x;
+  if (k >= nf) return;
+  int kind = fa.kind[k];
+  double nx = fa.nx[k], ny = fa.ny[k];
+  int l = fa.left[k];
+  int r = fa.right[k];
+  double L = flen[k];
+  // dR/dF: owner/left side -L/A (interior: -L/A * F; boundary: -(F*L)/A has the same derivative)
+  P4 ll = load4(lam, l);
+  double cl = -L / area[l];
+  double fb[4] = {cl * ll.r, cl * ll.u, cl * ll.v, cl * ll.p};
+  if (kind <= MCFD_FACE_PERIODIC) {
+    P4 lr = load4(lam, r);
+    double cr = L / area[r];
+    fb[0] += cr * lr.r;
+    fb[1] += cr * lr.u;
+    fb[2] += cr * lr.v;
+    fb[3] += cr * lr.p;
+  }
+  P4 wl = reconstruct(W, G, PHI, l, fa.lx[k], fa.ly[k], muscl != 0);
+  P4D DL = seed(wl, 0);
+  P4D DR;
+  if (kind <= MCFD_FACE_PERIODIC) {
+    P4 wr = reconstruct(W, G, PHI, r, fa.rx[k], fa.ry[k], muscl != 0);
+    DR = seed(wr, 4);
+  } else {
+    int b = fa.bc[k];
+    const double* st = fa.bc_state + 4 * (long)b;
+    if (kind == MCFD_FACE_WALL) {
+      D8 rx = DL.u - fa.bc_wall[2 * (long)b];
+      D8 ry = DL.v - fa.bc_wall[2 * (long)b + 1];
+      D8 un = rx * nx + ry * ny;
+      DR = DL;
+      DR.u = DL.u - 2.0 * un * nx;
+      DR.v = DL.v - 2.0 * un * ny;
+    } else if (kind == MCFD_FACE_VELOCITY_INLET) {
+      DR = P4D{dconst(st[0]), dconst(st[1]), dconst(st[2]), dconst(st[3])};
+    } else if (kind == MCFD_FACE_PRESSURE_OUTLET) {
+      DR = DL;
+      DR.p = dconst(st[3]);
+    } else if (kind == MCFD_FACE_FAR_FIELD) {
+      DR = far_field_d(DL, P4{st[0], st[1], st[2], st[3]}, nx, ny, g);
+    } else {
+      DR = DL;
+    }
+  }
+  F4D f;
+  if (scheme == 1)
+    f = hllc_d(DL, DR, nx, ny, g, kind == MCFD_FACE_INTERIOR);
+  else if (scheme == 2)
+    f = roe_d(DL, DR, nx, ny, g);
+  else
+    f = rusanov_d(DL, DR, nx, ny, g);
+  double out[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+    out[j] = fb[0] * f.a.d[j] + fb[1] * f.b.d[j] + fb[2] * f.c.d[j] + fb[3] * f.d.d[j];
+  double2* q = reinterpret_cast<double2*>(FADJ + 8 * (long)k);
+  q[0] = make_double2(out[0], out[1]);
+  q[1] = make_double2(out[2], out[3]);
+  q[2] = make_double2(out[4], out[5]);
+  q[3] = make_double2(out[6], out[7]);
+}
+
+// Cell pass 1: gather face adjoints (reconstruction backward), limiter
+// backward. Writes per cell: Wbar (local part), gbar, and for each variable
+// the extremum adjoints with the LSQ entry (block-global index) that holds
+// the extremum (-1: the cell itself).
+struct AdjArrays {
+  const int* acc_ptr;
+  const int* acc_item;
+  const double* flx;   // face offsets, for the side of each item
+  const double* fly;
+  const double* frx;
+  const double* fry;
+  const int* grad_ptr;
+  const int* grad_nb;
+  const double* grad_cx;
+  const double* grad_cy;
+  const int* grad_rev;  // reverse entry of each LSQ entry
+  const int* lim_ptr;
+  const double* lim_dx;
+  const double* lim_dy;
+  double* wbar;   // (C,4)
+  double* gbar;   // (C,8)
+  double* xbar;   // (C,8): max-adjoint (4), min-adjoint (4)
+  int* xarg;      // (C,8): entry of max (4), of min (4), -1 = own cell
+};
+
+__device__ __forceinline__ double limiter_value(double ratio, int code, double& dfdr) {
+  double f;
+  dfdr = 0.0;
+  if (code <= 1) {
+    if (ratio < 1.0) {
+      f = ratio;
+      dfdr = 1.0;
+    } else {
+      f = 1.0;
+    }
+  } else if (code == 2) {
+    if (ratio > 0.0) {
+      double t = 2.0 * ratio / (1.0 + ratio);
+      if (t < 1.0) {
+        f = t;
+        dfdr = 2.0 / ((1.0 + ratio) * (1.0 + ratio));
+      } else {
+        f = 1.0;
+      }
+    } else {
+      f = 0.0;
+    }
+  } else {
+    const double eps = 1e-12;
+    double num = ratio * ratio + 2.0 * ratio + eps;
+    double den = ratio * ratio + ratio + 2.0 + eps;
+    f = num / den;
+    if (f < 1.0) {
+      dfdr = ((2.0 * ratio + 2.0) * den - num * (2.0 * ratio + 1.0)) / (den * den);
+    } else {
+      f = 1.0;
+    }
+  }
+  return f;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_cell_adjoint1(int n, const double* __restrict__ W, const double* __restrict__ G,
+                    const double* __restrict__ PHI, const double* __restrict__ FADJ, AdjArrays aa,
+                    int muscl, int code) {
+  int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double wb[4] = {0.0, 0.0, 0.0, 0.0};
+  double pb[4] = {0.0, 0.0, 0.0, 0.0};
+  double gb[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  P4 w = load4(W, i);
+  double wv[4] = {w.r, w.u, w.v, w.p};
+  double gv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double pv[4] = {1.0, 1.0, 1.0, 1.0};
+  if (muscl) {
+    for (int k = 0; k < 8; ++k) gv[k] = G[8 * (long)i + k];
+    for (int k = 0; k < 4; ++k) pv[k] = PHI[4 * (long)i + k];
+  }
+  for (int e = aa.acc_ptr[i]; e < aa.acc_ptr[i + 1]; ++e) {
+    int item = aa.acc_item[e];
+    int f = item >> 2;
+    int mode = item & 3;
+    bool right = (mode == MCFD_ACC_NEIGHBOUR) || (mode == MCFD_ACC_PERIODIC_B);
+    const double* q = FADJ + 8 * (long)f + (right ? 4 : 0);
+    double dx = right ? aa.frx[f] : aa.flx[f];
+    double dy = right ? aa.fry[f] : aa.fly[f];
+    for (int k = 0; k < 4; ++k) {
+      double a = q[k];
+      wb[k] += a;
+      if (muscl) {
+        double inc = gv[2 * k] * dx + gv[2 * k + 1] * dy;
+        pb[k] += a * inc;
+        gb[2 * k] += a * pv[k] * dx;
+        gb[2 * k + 1] += a * pv[k] * dy;
+      }
+    }
+  }
+  double xb[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int xa[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+  if (muscl) {
+    // extrema and their entries (first strict extremum in entry order)
+    double mn[4] = {wv[0], wv[1], wv[2], wv[3]};
+    double mx[4] = {wv[0], wv[1], wv[2], wv[3]};
+    for (int e = aa.grad_ptr[i]; e < aa.grad_ptr[i + 1]; ++e) {
+      P4 o = load4(W, aa.grad_nb[e]);
+      double ov[4] = {o.r, o.u, o.v, o.p};
+      for (int k = 0; k < 4; ++k) {
+        if (ov[k] < mn[k]) { mn[k] = ov[k]; xa[4 + k] = e; }
+        if (ov[k] > mx[k]) { mx[k] = ov[k]; xa[k] = e; }
+      }
+    }
+    // limiter backward: phi = min over entries (first strict minimum below 1)
+    for (int k = 0; k < 4; ++k) {
+      if (pb[k] == 0.0) continue;
+      double best = 1.0;
+      int best_e = -1;
+      double best_d = 0.0, best_ratio = 0.0, best_inc = 0.0, best_dx = 0.0, best_dy = 0.0;
+      int best_side = 0;
+      for (int e = aa.lim_ptr[i]; e < aa.lim_ptr[i + 1]; ++e) {
+        double dx = aa.lim_dx[e], dy = aa.lim_dy[e];
+        double inc = gv[2 * k] * dx + gv[2 * k + 1] * dy;
+        double tol = 1e-14 * vmax(fabs(wv[k]), 1.0);
+        double ratio = 1.0;
+        int side = 0;
+        if (inc > tol) {
+          ratio = (mx[k] - wv[k]) / inc;
+          side = 1;
+        } else if (inc < -tol) {
+          ratio = (mn[k] - wv[k]) / inc;
+          side = -1;
+        }
+        bool clamped = 0.0 > ratio;
+        ratio = clamped ? 0.0 : ratio;
+        double dfdr;
+        double f = limiter_value(ratio, code, dfdr);
+        if (f < best) {
+          best = f;
+          best_e = e;
+          best_d = (side == 0 || clamped) ? 0.0 : dfdr;
+          best_ratio = ratio;
+          best_inc = inc;
+          best_dx = dx;
+          best_dy = dy;
+          best_side = side;
+        }
+      }
+      if (best_e < 0 || best_d == 0.0) continue;
+      double rb = pb[k] * best_d;           // d phi / d ratio
+      double ext_b = rb / best_inc;         // d ratio / d extremum
+      wb[k] -= ext_b;                       // d ratio / d W_i
+      double incb = -rb * best_ratio / best_inc;
+      gb[2 * k] += incb * best_dx;
+      gb[2 * k + 1] += incb * best_dy;
+      if (best_side > 0) xb[k] += ext_b; else xb[4 + k] += ext_b;
+    }
+    // extrema owned by the cell itself flow back to W_i
+    for (int k = 0; k < 4; ++k) {
+      if (xa[k] < 0) wb[k] += xb[k];
+      if (xa[4 + k] < 0) wb[k] += xb[4 + k];
+    }
+  }
+  store4(aa.wbar, i, wb[0], wb[1], wb[2], wb[3]);
+  double* gq = aa.gbar + 8 * (long)i;
+  double* xq = aa.xbar + 8 * (long)i;
+  int* aq = aa.xarg + 8 * (long)i;
+  for (int k = 0; k < 8; ++k) {
+    gq[k] = gb[k];
+    xq[k] = xb[k];
+    aq[k] = xa[k];
+  }
+}
+
+// Cell pass 2: LSQ-gradient backward (own term and the neighbours' terms via
+// the reverse entries), extremum adjoints from neighbours, body force, and
+// the primitive -> conservative chain rule. Writes Ubar.
+__global__ void __launch_bounds__(kBlock)
+    k_cell_adjoint2(int n, const double* __restrict__ U, const double* __restrict__ W,
+                    const double* __restrict__ lam, AdjArrays aa, int muscl, double g, double ax,
+                    double ay, double* __restrict__ ubar) {
+  int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  P4 wb4 = load4(aa.wbar, i);
+  double wb[4] = {wb4.r, wb4.u, wb4.v, wb4.p};
+  if (muscl) {
+    const double* gi = aa.gbar + 8 * (long)i;
+    for (int e = aa.grad_ptr[i]; e < aa.grad_ptr[i + 1]; ++e) {
+      double cx = aa.grad_cx[e], cy = aa.grad_cy[e];
+      int j = aa.grad_nb[e];
+      int r = aa.grad_rev[e];  // entry j -> i
+      const double* gj = aa.gbar + 8 * (long)j;
+      double cxr = aa.grad_cx[r], cyr = aa.grad_cy[r];
+      const double* xj = aa.xbar + 8 * (long)j;
+      const int* aj = aa.xarg + 8 * (long)j;
+      for (int k = 0; k < 4; ++k) {
+        wb[k] -= cx * gi[2 * k] + cy * gi[2 * k + 1];
+        wb[k] += cxr * gj[2 * k] + cyr * gj[2 * k + 1];
+        if (aj[k] == r) wb[k] += xj[k];
+        if (aj[4 + k] == r) wb[k] += xj[4 + k];
+      }
+    }
+  }
+  P4 w = load4(W, i);
+  if (ax != 0.0 || ay != 0.0) {
+    P4 l = load4(lam, i);
+    wb[0] += l.u * ax + l.v * ay + l.p * (ax * w.u + ay * w.v);
+    wb[1] += l.p * w.r * ax;
+    wb[2] += l.p * w.r * ay;
+  }
+  // W = prim(U): rho = U0, u = U1/U0, v = U2/U0, p = (g-1)(U3 - 0.5(U1^2+U2^2)/U0)
+  double rho = w.r, u = w.u, v = w.v;
+  double u0 = wb[0] - wb[1] * u / rho - wb[2] * v / rho + wb[3] * (g - 1.0) * 0.5 * (u * u + v * v);
+  double u1 = wb[1] / rho - wb[3] * (g - 1.0) * u;
+  double u2 = wb[2] / rho - wb[3] * (g - 1.0) * v;
+  double u3 = wb[3] * (g - 1.0);
+  store4(ubar, i, u0, u1, u2, u3);
+  (void)U;
+}
+
+
+// Batched cases: per-case CFL step = min over the case's cell range, clipped
+// to end_time - time_case, then broadcast to the case's cells. One block per
+// case; clock = [time, dt] per case.
+__global__ void __launch_bounds__(1024)
+    k_batch_dt(int n_cases, const int* __restrict__ case_ptr, double end_time,
+               double* __restrict__ clock, double* __restrict__ dt_cell, int* __restrict__ status) {
+  __shared__ double red[1024];
+  __shared__ double shared_dt;
+  int c = blockIdx.x;
+  if (c >= n_cases) return;
+  int a = case_ptr[c], b = case_ptr[c + 1];
+  double m = INFINITY;
+  for (int i = a + threadIdx.x; i < b; i += blockDim.x) m = fmin(m, dt_cell[i]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double t = clock[2 * c];
+    double rest = end_time - t;
+    double dt = (rest < red[0]) ? rest : red[0];
+    if (!finite(dt) || !(dt > 0.0)) report(status, MCFD_ERR_GLOBAL_DT, c);
+    clock[2 * c + 1] = dt;
+    clock[2 * c] = t + dt;
+    shared_dt = dt;
+  }
+  __syncthreads();
+  double dt = shared_dt;
+  for (int i = a + threadIdx.x; i < b; i += blockDim.x) dt_cell[i] = dt;
+}
+
 // Spectral-radius sum and local step (solver.py:4132-4343, inviscid part).
 __global__ void __launch_bounds__(kBlock)
     k_time_steps(int n, const double* __restrict__ W, const int* __restrict__ dptr,
@@ -1293,6 +1878,12 @@ struct McfdMesh {
   double* gm_rp = nullptr;      // 4C
   double* gm_scalars = nullptr; // 64
   double* gm_partial = nullptr; // dot partials
+  // adjoint
+  bool adjoint = false;
+  int* grad_rev = nullptr;
+  double* FADJ = nullptr;
+  AdjArrays adj{};
+  std::vector<void*> adjoint_allocations;
   // timing
   bool timing = false;
   std::vector<hipEvent_t> event_pool;
@@ -1369,7 +1960,24 @@ int upload_implicit(McfdMesh* m, T** dst, const T* src, size_t count) {
   return 0;
 }
 
+void free_adjoint(McfdMesh* m) {
+  for (void* p : m->adjoint_allocations) (void)hipFree(p);
+  m->adjoint_allocations.clear();
+  m->adjoint = false;
+}
+
+template <typename T>
+int alloc_adjoint(McfdMesh* m, T** dst, const T* src, size_t count) {
+  void* p = nullptr;
+  HIP_CHECK(hipMalloc(&p, sizeof(T) * (count ? count : 1)));
+  m->adjoint_allocations.push_back(p);
+  if (count && src) HIP_CHECK(hipMemcpy(p, src, sizeof(T) * count, hipMemcpyHostToDevice));
+  *dst = static_cast<T*>(p);
+  return 0;
+}
+
 void free_mesh(McfdMesh* m) {
+  free_adjoint(m);
   free_implicit(m);
   free_blocks(m);
   for (void* p : m->allocations) (void)hipFree(p);
@@ -2065,6 +2673,88 @@ int mcfd_norm2(McfdMesh* m, const double* a, double* out, void* stream) {
   if (!m || !a || !out) return fail("mcfd_norm2: NULL argument");
   if (ensure_gmres(m, 1)) return -1;
   return dot_dev(m, 4L * m->n_cells, a, a, out, true, static_cast<hipStream_t>(stream));
+}
+
+
+int mcfd_mesh_set_adjoint(McfdMesh* m, const int32_t* grad_rev) {
+  if (!m) return fail("mcfd_mesh_set_adjoint: NULL mesh");
+  (void)hipDeviceSynchronize();
+  free_adjoint(m);
+  if (!grad_rev) return 0;
+  size_t C = m->n_cells;
+  int rc = 0;
+  AdjArrays& a = m->adj;
+  rc |= alloc_adjoint(m, &m->grad_rev, grad_rev, (size_t)m->n_grad);
+  rc |= alloc_adjoint(m, &m->FADJ, (const double*)nullptr, 8 * (size_t)m->n_faces);
+  rc |= alloc_adjoint(m, &a.wbar, (const double*)nullptr, 4 * C);
+  rc |= alloc_adjoint(m, &a.gbar, (const double*)nullptr, 8 * C);
+  rc |= alloc_adjoint(m, &a.xbar, (const double*)nullptr, 8 * C);
+  rc |= alloc_adjoint(m, &a.xarg, (const int*)nullptr, 8 * C);
+  if (rc) {
+    std::string msg = g_last_error;
+    free_adjoint(m);
+    return fail(msg);
+  }
+  a.acc_ptr = m->acc_ptr;
+  a.acc_item = m->acc_item;
+  a.flx = m->faces.lx;
+  a.fly = m->faces.ly;
+  a.frx = m->faces.rx;
+  a.fry = m->faces.ry;
+  a.grad_ptr = m->grad_ptr;
+  a.grad_nb = m->grad_nb;
+  a.grad_cx = m->grad_cx;
+  a.grad_cy = m->grad_cy;
+  a.grad_rev = m->grad_rev;
+  a.lim_ptr = m->lim_ptr;
+  a.lim_dx = m->lim_dx;
+  a.lim_dy = m->lim_dy;
+  m->adjoint = true;
+  return 0;
+}
+
+int mcfd_residual_vjp(McfdMesh* m, const McfdSettings* st, const double* u, const double* lam,
+                      double* ubar, void* stream) {
+  if (!m || !u || !lam || !ubar) return fail("mcfd_residual_vjp: NULL argument");
+  if (!m->adjoint) return fail("mcfd_residual_vjp: call mcfd_mesh_set_adjoint first");
+  if (check_settings(st)) return -1;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int n = m->n_cells;
+  bool muscl = st->reconstruction == 1 && m->n_grad > 0;
+  hipLaunchKernelGGL(k_primitive, dim3(blocks(n)), dim3(kBlock), 0, s, n, u, m->W, st->gamma,
+                     m->status);
+  if (muscl)
+    hipLaunchKernelGGL(k_gradient, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->grad_ptr,
+                       m->grad_nb, m->grad_cx, m->grad_cy, m->lim_ptr, m->lim_dx, m->lim_dy,
+                       st->limiter, m->G, m->PHI);
+  if (m->n_faces)
+    hipLaunchKernelGGL(k_flux_adjoint, dim3(blocks(m->n_faces)), dim3(kBlock), 0, s, m->n_faces,
+                       m->faces, m->W, m->G, m->PHI, m->face_len, m->area, lam, st->flux,
+                       muscl ? 1 : 0, st->gamma, m->FADJ);
+  hipLaunchKernelGGL(k_cell_adjoint1, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->G, m->PHI,
+                     m->FADJ, m->adj, muscl ? 1 : 0, st->limiter);
+  hipLaunchKernelGGL(k_cell_adjoint2, dim3(blocks(n)), dim3(kBlock), 0, s, n, u, m->W, lam,
+                     m->adj, muscl ? 1 : 0, st->gamma, st->acceleration_x, st->acceleration_y,
+                     ubar);
+  HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+
+int mcfd_batch_time_steps(McfdMesh* m, const McfdSettings* st, const double* u, int32_t n_cases,
+                          const int32_t* case_ptr, double end_time, double* clock, double* dt_cell,
+                          void* stream) {
+  if (!m || !u || !case_ptr || !clock || !dt_cell || n_cases <= 0)
+    return fail("mcfd_batch_time_steps: NULL argument or no cases");
+  if (mcfd_time_steps(m, st, u, dt_cell, nullptr, stream)) return -1;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  {
+    TimedLaunch t(m, s, MCFD_KERNEL_REDUCE);
+    hipLaunchKernelGGL(k_batch_dt, dim3(n_cases), dim3(1024), 0, s, n_cases, case_ptr, end_time,
+                       clock, dt_cell, m->status);
+  }
+  HIP_CHECK(hipGetLastError());
+  return 0;
 }
 
 int mcfd_status(McfdMesh* m, void* stream, int32_t* mask, int32_t* index) {

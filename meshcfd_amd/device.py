@@ -102,6 +102,54 @@ class DeviceMesh:
         self.dt_min = torch.empty((1,), **opts)
         self.clock = torch.zeros((2,), **opts)
         self.sums = torch.zeros((4,), **opts)
+        self.case_ptr = None
+        self.n_cases = 0
+        self.case_clock = None
+        self.graph = None
+
+    @classmethod
+    def from_batch(cls, batch, device="cuda:0", fused: bool = True, block_cells: int = 128):
+        """A device mesh for a `batch.Batch`: all instances advance together,
+        each with its own CFL step and clock."""
+        from .batch import batch_case
+
+        dm = cls(batch_case(batch), device, geometry=batch.geometry, layout=batch.layout, fused=fused,
+                 block_cells=block_cells)
+        dm.n_cases = batch.n_cases
+        dm.case_ptr = torch.as_tensor(batch.case_ptr.astype(np.int32), device=dm.device)
+        dm.case_clock = torch.zeros((2 * batch.n_cases,), dtype=torch.float64, device=dm.device)
+        return dm
+
+    def capture(self, steps: int = 1):
+        """Capture `steps` explicit steps in a HIP graph (replayed by
+        `replay()`). Requires a pointer-stable step (RK2/RK3/RK4) and no
+        per-launch timing."""
+        if self.settings.time_integration == "Explicit Euler" and steps % 2:
+            raise ValueError("Euler steps swap buffers: capture an even number of steps")
+        self.timing(False)
+        saved = [t.clone() for t in (self.u, self.clock)] + ([self.case_clock.clone()] if self.case_clock is not None else [])
+        stream = torch.cuda.Stream(self.device)
+        stream.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(stream):
+            for _ in range(2):  # warm up kernels and allocator outside the graph
+                self.step()
+        torch.cuda.current_stream(self.device).wait_stream(stream)
+        torch.cuda.synchronize(self.device)
+        self.u.copy_(saved[0])
+        self.clock.copy_(saved[1])
+        if self.case_clock is not None:
+            self.case_clock.copy_(saved[2])
+        torch.cuda.synchronize(self.device)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=stream):
+            for _ in range(steps):
+                self.step()
+        self.graph = graph
+        self.graph_steps = steps
+        return graph
+
+    def replay(self):
+        self.graph.replay()
 
     def set_blocks(self, block_cells: int | None) -> None:
         """Switch to the fused one-kernel-per-stage path with blocks of
@@ -239,7 +287,15 @@ class DeviceMesh:
         local = mode == "Local (steady)"
         if time is not None:
             self.clock[0] = time
-        if local:
+        if self.case_ptr is not None:
+            native.check(
+                self.lib.mcfd_batch_time_steps(
+                    self.handle, C.byref(self.native_settings), _ptr(self.u), self.n_cases, _ptr(self.case_ptr),
+                    s.end_time, _ptr(self.case_clock), _ptr(self.dt_cell), self.stream,
+                )
+            )
+            dt, per_cell = self.dt_cell, True
+        elif local:
             self.time_steps(self.u)
             dt, per_cell = self.dt_cell, True
         elif mode == "Global fixed":
@@ -257,14 +313,13 @@ class DeviceMesh:
             self._stage(native.OP_EULER, u, out=self.s1, dt=dt, per_cell=per_cell, coeffs=(1.0,), r_out=residual_out)
             self.u, self.s1 = self.s1, u
         elif ti == "Explicit RK2":
+            # the last stage reads U^n only at its own cell, so it may write U^n in place
             self._stage(native.OP_EULER, u, out=self.s1, dt=dt, per_cell=per_cell, coeffs=(1.0,), r_out=residual_out)
-            self._stage(native.OP_SSP, self.s1, out=self.s2, u0=u, dt=dt, per_cell=per_cell, coeffs=(0.5, 0.5))
-            self.u, self.s2 = self.s2, u
+            self._stage(native.OP_SSP, self.s1, out=u, u0=u, dt=dt, per_cell=per_cell, coeffs=(0.5, 0.5))
         elif ti == "Explicit RK3":
             self._stage(native.OP_EULER, u, out=self.s1, dt=dt, per_cell=per_cell, coeffs=(1.0,), r_out=residual_out)
             self._stage(native.OP_SSP, self.s1, out=self.s2, u0=u, dt=dt, per_cell=per_cell, coeffs=(0.75, 0.25))
-            self._stage(native.OP_SSP, self.s2, out=self.s1, u0=u, dt=dt, per_cell=per_cell, coeffs=(1.0 / 3.0, 2.0 / 3.0))
-            self.u, self.s1 = self.s1, u
+            self._stage(native.OP_SSP, self.s2, out=u, u0=u, dt=dt, per_cell=per_cell, coeffs=(1.0 / 3.0, 2.0 / 3.0))
         elif ti == "Explicit RK4":
             c = RK4_COEFFS
             if residual_out is not None:

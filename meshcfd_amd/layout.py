@@ -67,6 +67,7 @@ class DeviceLayout:
     grad_nb: np.ndarray
     grad_cx: np.ndarray
     grad_cy: np.ndarray
+    grad_rev: np.ndarray  # entry of the neighbour's list that points back
     lim_ptr: np.ndarray
     lim_dx: np.ndarray
     lim_dy: np.ndarray
@@ -109,7 +110,7 @@ def _csr(keys: np.ndarray, n: int):
     return ptr.astype(np.int32), order
 
 
-def build_layout(g: Geometry, reorder: bool = True) -> DeviceLayout:
+def build_layout(g: Geometry, reorder: bool = True, order: np.ndarray | None = None) -> DeviceLayout:
     """Device layout; with ``reorder`` cells are renumbered along a Hilbert
     curve and flux faces sorted by their (renumbered) left cell. Per-cell
     lists keep the reference summation order, which depends on original face
@@ -117,7 +118,10 @@ def build_layout(g: Geometry, reorder: bool = True) -> DeviceLayout:
     n = g.cell_count
     if n >= 2**31 - 1:
         raise ValueError("Mesh too large for 32-bit cell indices.")
-    order_cells = hilbert_order(g.centers) if reorder else np.arange(n)
+    if order is not None:
+        order_cells = np.asarray(order, dtype=np.int64)
+    else:
+        order_cells = hilbert_order(g.centers) if reorder else np.arange(n)
     new_id = np.empty(n, dtype=np.int64)
     new_id[order_cells] = np.arange(n)
     c = g.centers
@@ -210,6 +214,10 @@ def build_layout(g: Geometry, reorder: bool = True) -> DeviceLayout:
     # --- LSQ and limiter entries -------------------------------------------
     grad_ptr, order = _csr(new_id[g.gradient_sources], n)
     grad_nb = new_id[g.gradient_neighbours[order]].astype(np.int32)
+    # LSQ entries come in (a -> b, b -> a) pairs: 2k and 2k+1 in the reference list
+    pos = np.empty(len(order), dtype=np.int64)
+    pos[order] = np.arange(len(order))
+    grad_rev = pos[order ^ 1].astype(np.int32)
     grad_cx = g.gradient_coefficients_x[order]
     grad_cy = g.gradient_coefficients_y[order]
     lim_ptr, order = _csr(new_id[g.limiter_cells], n)
@@ -250,6 +258,7 @@ def build_layout(g: Geometry, reorder: bool = True) -> DeviceLayout:
         grad_nb=grad_nb,
         grad_cx=np.ascontiguousarray(grad_cx),
         grad_cy=np.ascontiguousarray(grad_cy),
+        grad_rev=grad_rev,
         lim_ptr=lim_ptr,
         lim_dx=np.ascontiguousarray(lim_dx),
         lim_dy=np.ascontiguousarray(lim_dy),

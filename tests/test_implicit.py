@@ -194,5 +194,7 @@ def test_gpu_implicit_steps_match_oracle(cuda_device, fused, mode):
     print("implicit rel err", err, "time", t, tr)
     assert np.all(err <= 1e-7)
     assert math.isclose(t, tr, rel_tol=1e-12)
-    np.testing.assert_allclose((gpu * g.areas[:, None]).sum(0), (u0 * g.areas[:, None]).sum(0), rtol=1e-10, atol=1e-10)
+    if mode != "Local (steady)":  # per-cell steps are not conservative
+        np.testing.assert_allclose((gpu * g.areas[:, None]).sum(0), (u0 * g.areas[:, None]).sum(0),
+                                   rtol=1e-10, atol=1e-10)
     dm.close()
