@@ -161,6 +161,11 @@ typedef struct McfdStage {
   double* out;           /* (C,4) updated state, may be NULL for MCFD_OP_RESIDUAL */
   double* r_out;         /* (C,4) residual copy, or NULL */
   double c[5];
+  /* Fused path only: also compute the next step's local stable steps from
+   * the updated state into dt_next [C], and their minimum into dt_min[0]
+   * (device; must hold +inf or a larger value beforehand). Or NULL. */
+  double* dt_next;
+  double* dt_min;
 } McfdStage;
 
 /*
@@ -237,9 +242,11 @@ int mcfd_time_steps(McfdMesh* mesh, const McfdSettings* settings, const double* 
                     double* dt_cell, double* dt_min, void* stream);
 
 /* clock = [time, dt]: dt = min(dt_source[0], end_time - time) (dt_source NULL:
- * fixed_dt), checked positive/finite, then time += dt (solver.py:452-465, 812). */
+ * fixed_dt), checked positive/finite, then time += dt (solver.py:452-465, 812).
+ * reset != 0 sets dt_source[0] = +inf after reading it (for dt_min of the
+ * next fused stage). */
 int mcfd_advance_clock(McfdMesh* mesh, const double* dt_source, double fixed_dt,
-                       double end_time, double* clock, void* stream);
+                       double end_time, double* clock, int32_t reset, void* stream);
 
 /* One residual evaluation fused with an explicit stage update (the inlined
  * stages of `solve_case`, solver.py:532-756, and `_explicit_rk4_step`,

@@ -437,7 +437,57 @@ struct StageArgs {
   double* out;
   double* r_out;
   double c0, c1, c2, c3, c4;
+  double* dt_next;                 // per-cell stable step of the updated state, or NULL
+  unsigned long long* dt_min_bits; // running minimum of dt_next (positive doubles order as integers)
+  double cfl;
 };
+
+// Arrays of the spectral-radius sum (solver.py:4132-4343, inviscid part).
+struct DtArrays {
+  const int* ptr;
+  const double* nx;
+  const double* ny;
+  const double* len;
+  const int* bc;
+  const int* bc_kind;
+  const double* bc_state;
+  const double* area;
+};
+
+// Local stable step of one cell from its primitive state: sum over its
+// faces of (|u.n| + c) L with the exterior state's radius at inlet, outlet and
+// far-field faces, then cfl * A / sum. Shared by the time-step kernel and the
+// fused final stage so both give the same bits.
+__device__ double cell_time_step(int i, const P4& w, const DtArrays& d, double g, double cfl,
+                                 int* status) {
+  double c = sound(w, g);
+  double sum = 0.0;
+  int e0 = d.ptr[i], e1 = d.ptr[i + 1];
+  for (int e = e0; e < e1; ++e) {
+    double nx = d.nx[e], ny = d.ny[e];
+    double rad = fabs(w.u * nx + w.v * ny) + c;
+    int b = d.bc[e];
+    if (b >= 0) {
+      int kind = d.bc_kind[b];
+      const double* st = d.bc_state + 4 * (long)b;
+      P4 x = w;
+      int err = 0;
+      if (kind == MCFD_FACE_VELOCITY_INLET)
+        x = P4{st[0], st[1], st[2], st[3]};
+      else if (kind == MCFD_FACE_PRESSURE_OUTLET)
+        x.p = st[3];
+      else if (kind == MCFD_FACE_FAR_FIELD)
+        x = far_field(w, P4{st[0], st[1], st[2], st[3]}, nx, ny, g, &err);
+      if (err) report(status, err, i);
+      double xr = fabs(x.u * nx + x.v * ny) + sound(x, g);
+      rad = vmax(rad, xr);
+    }
+    sum = sum + rad * d.len[e];
+  }
+  double dt = cfl * d.area[i] / sum;
+  if (!finite(dt) || !(dt > 0.0)) report(status, MCFD_ERR_LOCAL_DT, i);
+  return dt;
+}
 
 // Face -> cell gather in the reference order (solver.py:2501-2704), body force
 // (solver.py:2709-2715), finiteness check (:2716-2717), then the stage update.
@@ -578,8 +628,8 @@ __device__ __forceinline__ P4 rec_lds(const double* sW, const double* sG, const 
 
 __global__ void __launch_bounds__(kBlock)
     k_stage_fused(int block_cells, int max_local, int max_grad, int max_faces, BlockArrays ba,
-                  GeomArrays ga, FaceArrays fa, int scheme, int muscl_i, int limiter, double g,
-                  double ax, double ay, StageArgs s, int* __restrict__ status) {
+                  GeomArrays ga, FaceArrays fa, DtArrays da, int scheme, int muscl_i, int limiter,
+                  double g, double ax, double ay, StageArgs s, int* __restrict__ status) {
   extern __shared__ double smem[];
   const bool muscl = muscl_i != 0;
   const int b = xcd_block();
@@ -742,6 +792,7 @@ __global__ void __launch_bounds__(kBlock)
   __syncthreads();
 
   // P3: gather (solver.py:2501-2704) + body force + stage update
+  double dt_local = INFINITY;
   for (int j = threadIdx.x; j < n_own; j += blockDim.x) {
     int i = first + j;
     double A = ga.area[i];
@@ -820,6 +871,30 @@ __global__ void __launch_bounds__(kBlock)
       o3 = s.c0 * z.p + s.c1 * y.p + k2 * q.p + s.c3 * u.p + k4 * r3;
     }
     store4(s.out, i, o0, o1, o2, o3);
+    if (s.dt_next) {
+      // next step's stable step from the updated state (euler.py:20-40 + solver.py:4132-4343)
+      double rho = o0;
+      double uu = o1 / rho;
+      double vv = o2 / rho;
+      double p = (g - 1.0) * (o3 - 0.5 * (o1 * o1 + o2 * o2) / rho);
+      if (!finite(rho) || !finite(uu) || !finite(vv) || !finite(p) || !(rho > 0.0) || !(p > 0.0))
+        report(status, MCFD_ERR_STATE, i);
+      double dt = cell_time_step(i, P4{rho, uu, vv, p}, da, g, s.cfl, status);
+      s.dt_next[i] = dt;
+      dt_local = fmin(dt_local, dt);
+    }
+  }
+  if (s.dt_next) {
+    __syncthreads();  // every thread is done with the flux slots: reuse them
+    double* red = sF;
+    red[threadIdx.x] = dt_local;
+    __syncthreads();
+    for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+      if (threadIdx.x < k) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + k]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0 && red[0] > 0.0 && red[0] < INFINITY)
+      atomicMin(s.dt_min_bits, (unsigned long long)__double_as_longlong(red[0]));
   }
 }
 
@@ -1699,44 +1774,14 @@ __global__ void __launch_bounds__(1024)
 
 // Spectral-radius sum and local step (solver.py:4132-4343, inviscid part).
 __global__ void __launch_bounds__(kBlock)
-    k_time_steps(int n, const double* __restrict__ W, const int* __restrict__ dptr,
-                 const double* __restrict__ dnx, const double* __restrict__ dny,
-                 const double* __restrict__ dlen, const int* __restrict__ dbc,
-                 const int* __restrict__ bc_kind, const double* __restrict__ bc_state,
-                 const double* __restrict__ area, double g, double cfl,
+    k_time_steps(int n, const double* __restrict__ W, DtArrays d, double g, double cfl,
                  double* __restrict__ dt_cell, double* __restrict__ partial,
                  int* __restrict__ status) {
   __shared__ double red[kBlock];
   int i = xcd_block() * blockDim.x + threadIdx.x;
   double dt = INFINITY;
   if (i < n) {
-    P4 w = load4(W, i);
-    double c = sound(w, g);
-    double sum = 0.0;
-    int e0 = dptr[i], e1 = dptr[i + 1];
-    for (int e = e0; e < e1; ++e) {
-      double nx = dnx[e], ny = dny[e];
-      double rad = fabs(w.u * nx + w.v * ny) + c;
-      int b = dbc[e];
-      if (b >= 0) {
-        int kind = bc_kind[b];
-        const double* st = bc_state + 4 * (long)b;
-        P4 x = w;
-        int err = 0;
-        if (kind == MCFD_FACE_VELOCITY_INLET)
-          x = P4{st[0], st[1], st[2], st[3]};
-        else if (kind == MCFD_FACE_PRESSURE_OUTLET)
-          x.p = st[3];
-        else if (kind == MCFD_FACE_FAR_FIELD)
-          x = far_field(w, P4{st[0], st[1], st[2], st[3]}, nx, ny, g, &err);
-        if (err) report(status, err, i);
-        double xr = fabs(x.u * nx + x.v * ny) + sound(x, g);
-        rad = vmax(rad, xr);
-      }
-      sum = sum + rad * dlen[e];
-    }
-    dt = cfl * area[i] / sum;
-    if (!finite(dt) || !(dt > 0.0)) report(status, MCFD_ERR_LOCAL_DT, i);
+    dt = cell_time_step(i, load4(W, i), d, g, cfl, status);
     dt_cell[i] = dt;
   }
   red[threadIdx.x] = dt;
@@ -1762,11 +1807,12 @@ __global__ void __launch_bounds__(1024)
   if (threadIdx.x == 0) out[0] = red[0];
 }
 
-__global__ void k_clock(const double* __restrict__ src, double fixed_dt, double end_time,
-                        double* __restrict__ clock, int* __restrict__ status) {
+__global__ void k_clock(double* __restrict__ src, double fixed_dt, double end_time,
+                        double* __restrict__ clock, int reset, int* __restrict__ status) {
   if (threadIdx.x != 0) return;
   double t = clock[0];
   double v = src ? src[0] : fixed_dt;
+  if (src && reset) src[0] = INFINITY;
   double rest = end_time - t;
   double dt = (rest < v) ? rest : v;
   if (!finite(dt) || !(dt > 0.0)) report(status, MCFD_ERR_GLOBAL_DT, 0);
@@ -1853,6 +1899,7 @@ struct McfdMesh {
   double* dts_len = nullptr;
   int* dts_bc = nullptr;
   int* bc_kind = nullptr;
+  DtArrays dts{};
   // scratch
   double* W = nullptr;
   double* G = nullptr;
@@ -2329,6 +2376,8 @@ int mcfd_mesh_create(const McfdMeshDesc* d, McfdMesh** out) {
   rc |= upload(m, &m->bc_kind, d->bc_kind, d->n_bc);
   rc |= upload(m, const_cast<double**>(&fa.bc_state), d->bc_state, 4 * (size_t)d->n_bc);
   rc |= upload(m, const_cast<double**>(&fa.bc_wall), d->bc_wall, 2 * (size_t)d->n_bc);
+  m->dts = DtArrays{m->dts_ptr, m->dts_nx, m->dts_ny, m->dts_len,
+                    m->dts_bc,  m->bc_kind, fa.bc_state, m->area};
   rc |= scratch(m, &m->W, 4 * C);
   rc |= scratch(m, &m->G, 8 * C);
   rc |= scratch(m, &m->PHI, 4 * C);
@@ -2418,8 +2467,12 @@ int mcfd_stage(McfdMesh* m, const McfdSettings* st, const McfdStage* sg, void* s
     return fail("mcfd_stage: u1 and r1 are required for the final SSPRK(5,4) combination");
   if (sg->op == MCFD_OP_RESIDUAL && !sg->r_out) return fail("mcfd_residual: output is NULL");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  StageArgs a{sg->op, sg->dt_per_cell, sg->u, sg->u0, sg->u1, sg->r1, sg->dt, sg->out,
-              sg->r_out, sg->c[0], sg->c[1], sg->c[2], sg->c[3], sg->c[4]};
+  StageArgs a{sg->op,   sg->dt_per_cell, sg->u,    sg->u0,   sg->u1,   sg->r1,
+              sg->dt,   sg->out,         sg->r_out, sg->c[0], sg->c[1], sg->c[2],
+              sg->c[3], sg->c[4],        sg->dt_next,
+              reinterpret_cast<unsigned long long*>(sg->dt_min), st->cfl};
+  if (sg->dt_next && (!m->fused || !sg->dt_min || sg->op == MCFD_OP_RESIDUAL))
+    return fail("mcfd_stage: dt_next needs the fused path, dt_min and a state update");
   if (m->fused) {
     bool muscl = st->reconstruction == 1 && m->n_grad > 0;
     size_t lds = fused_lds_bytes(m, muscl);
@@ -2427,7 +2480,7 @@ int mcfd_stage(McfdMesh* m, const McfdSettings* st, const McfdStage* sg, void* s
                   m->lim_dx,  m->lim_dy,   m->acc_ptr, m->face_len};
     TimedLaunch t(m, s, MCFD_KERNEL_STAGE);
     hipLaunchKernelGGL(k_stage_fused, dim3(m->nb), dim3(kBlock), lds, s, m->block_cells,
-                       m->max_local, m->max_grad, m->max_faces, m->blocks, ga, m->faces,
+                       m->max_local, m->max_grad, m->max_faces, m->blocks, ga, m->faces, m->dts,
                        st->flux, muscl ? 1 : 0, st->limiter, st->gamma, st->acceleration_x,
                        st->acceleration_y, a, m->status);
     HIP_CHECK(hipGetLastError());
@@ -2457,10 +2510,8 @@ int mcfd_time_steps(McfdMesh* m, const McfdSettings* st, const double* u, double
   }
   {
     TimedLaunch t(m, s, MCFD_KERNEL_TIME_STEP);
-    hipLaunchKernelGGL(k_time_steps, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->dts_ptr,
-                       m->dts_nx, m->dts_ny, m->dts_len, m->dts_bc, m->bc_kind,
-                       m->faces.bc_state, m->area, st->gamma, st->cfl, dt_cell, m->partial,
-                       m->status);
+    hipLaunchKernelGGL(k_time_steps, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->dts,
+                       st->gamma, st->cfl, dt_cell, m->partial, m->status);
   }
   if (dt_min) {
     TimedLaunch t(m, s, MCFD_KERNEL_REDUCE);
@@ -2471,11 +2522,11 @@ int mcfd_time_steps(McfdMesh* m, const McfdSettings* st, const double* u, double
 }
 
 int mcfd_advance_clock(McfdMesh* m, const double* dt_source, double fixed_dt, double end_time,
-                       double* clock, void* stream) {
+                       double* clock, int32_t reset, void* stream) {
   if (!m || !clock) return fail("mcfd_advance_clock: NULL argument");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, s, dt_source, fixed_dt, end_time, clock,
-                     m->status);
+  hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, s, const_cast<double*>(dt_source), fixed_dt,
+                     end_time, clock, reset, m->status);
   HIP_CHECK(hipGetLastError());
   return 0;
 }

@@ -103,6 +103,7 @@ class DeviceMesh:
         self.clock = torch.zeros((2,), **opts)
         self.sums = torch.zeros((4,), **opts)
         self.case_ptr = None
+        self.dt_fresh = False
         self.n_cases = 0
         self.case_clock = None
         self.graph = None
@@ -137,6 +138,7 @@ class DeviceMesh:
         torch.cuda.synchronize(self.device)
         self.u.copy_(saved[0])
         self.clock.copy_(saved[1])
+        self.dt_fresh = False
         if self.case_clock is not None:
             self.case_clock.copy_(saved[2])
         torch.cuda.synchronize(self.device)
@@ -203,11 +205,13 @@ class DeviceMesh:
 
     def set_state(self, conservative) -> None:
         self.u.copy_(self.to_device_order(conservative))
+        self.dt_fresh = False
 
     def state(self) -> np.ndarray:
         return self.to_mesh_order(self.u)
 
-    def _stage(self, op, u, out=None, u0=None, u1=None, r1=None, dt=None, per_cell=False, coeffs=(), r_out=None):
+    def _stage(self, op, u, out=None, u0=None, u1=None, r1=None, dt=None, per_cell=False, coeffs=(), r_out=None,
+               dt_next=None, dt_min=None):
         st = native.McfdStage()
         st.op = op
         st.dt_per_cell = 1 if per_cell else 0
@@ -218,6 +222,8 @@ class DeviceMesh:
         st.dt = _ptr(dt)
         st.out = _ptr(out)
         st.r_out = _ptr(r_out)
+        st.dt_next = _ptr(dt_next)
+        st.dt_min = _ptr(dt_min)
         for k, v in enumerate(coeffs):
             st.c[k] = v
         native.check(self.lib.mcfd_stage(self.handle, C.byref(self.native_settings), C.byref(st), self.stream))
@@ -296,30 +302,41 @@ class DeviceMesh:
             )
             dt, per_cell = self.dt_cell, True
         elif local:
-            self.time_steps(self.u)
+            if not self.dt_fresh:
+                self.time_steps(self.u)
             dt, per_cell = self.dt_cell, True
         elif mode == "Global fixed":
-            native.check(self.lib.mcfd_advance_clock(self.handle, None, s.fixed_time_step, s.end_time, _ptr(self.clock), self.stream))
+            native.check(
+                self.lib.mcfd_advance_clock(self.handle, None, s.fixed_time_step, s.end_time, _ptr(self.clock), 0,
+                                            self.stream)
+            )
             dt, per_cell = self.clock[1:], False
         else:
-            self.time_steps(self.u)
+            if not self.dt_fresh:
+                self.time_steps(self.u)
             native.check(
-                self.lib.mcfd_advance_clock(self.handle, _ptr(self.dt_min), 0.0, s.end_time, _ptr(self.clock), self.stream)
+                self.lib.mcfd_advance_clock(self.handle, _ptr(self.dt_min), 0.0, s.end_time, _ptr(self.clock), 1,
+                                            self.stream)
             )
             dt, per_cell = self.clock[1:], False
         ti = s.time_integration
         u = self.u
+        # the fused final stage also computes the next step's CFL steps
+        fuse_dt = self.blocks is not None and self.case_ptr is None and mode != "Global fixed"
+        nxt = dict(dt_next=self.dt_cell, dt_min=self.dt_min) if fuse_dt else {}
         if ti == "Explicit Euler":
-            self._stage(native.OP_EULER, u, out=self.s1, dt=dt, per_cell=per_cell, coeffs=(1.0,), r_out=residual_out)
+            self._stage(native.OP_EULER, u, out=self.s1, dt=dt, per_cell=per_cell, coeffs=(1.0,), r_out=residual_out,
+                        **nxt)
             self.u, self.s1 = self.s1, u
         elif ti == "Explicit RK2":
             # the last stage reads U^n only at its own cell, so it may write U^n in place
             self._stage(native.OP_EULER, u, out=self.s1, dt=dt, per_cell=per_cell, coeffs=(1.0,), r_out=residual_out)
-            self._stage(native.OP_SSP, self.s1, out=u, u0=u, dt=dt, per_cell=per_cell, coeffs=(0.5, 0.5))
+            self._stage(native.OP_SSP, self.s1, out=u, u0=u, dt=dt, per_cell=per_cell, coeffs=(0.5, 0.5), **nxt)
         elif ti == "Explicit RK3":
             self._stage(native.OP_EULER, u, out=self.s1, dt=dt, per_cell=per_cell, coeffs=(1.0,), r_out=residual_out)
             self._stage(native.OP_SSP, self.s1, out=self.s2, u0=u, dt=dt, per_cell=per_cell, coeffs=(0.75, 0.25))
-            self._stage(native.OP_SSP, self.s2, out=u, u0=u, dt=dt, per_cell=per_cell, coeffs=(1.0 / 3.0, 2.0 / 3.0))
+            self._stage(native.OP_SSP, self.s2, out=u, u0=u, dt=dt, per_cell=per_cell, coeffs=(1.0 / 3.0, 2.0 / 3.0),
+                        **nxt)
         elif ti == "Explicit RK4":
             c = RK4_COEFFS
             if residual_out is not None:
@@ -330,13 +347,16 @@ class DeviceMesh:
             # stage four needs R(stage three) kept for the final combination
             self._stage(native.OP_RK4, self.s3, out=self.s1, u0=u, dt=dt, per_cell=per_cell, coeffs=c[3], r_out=self.r3)
             # final: a*S2 + b*S3 + (c*dt)*R3 + d*S4 + (e*dt)*R(S4)
-            self._stage(native.OP_RK4_FINAL, self.s1, out=u, u0=self.s2, u1=self.s3, r1=self.r3, dt=dt, per_cell=per_cell, coeffs=c[4])
+            self._stage(native.OP_RK4_FINAL, self.s1, out=u, u0=self.s2, u1=self.s3, r1=self.r3, dt=dt, per_cell=per_cell,
+                        coeffs=c[4], **nxt)
         else:
             raise SolverError(f"Unsupported time integration '{ti}' for the explicit GPU path.")
+        self.dt_fresh = fuse_dt
 
     def run(self, steps: int, time: float = 0.0, check_every: int = 0) -> float:
         """Advance ``steps`` steps; returns the final time (one host sync)."""
         self.clock[0] = time
+        self.dt_fresh = False
         for k in range(steps):
             self.step()
             if check_every and (k + 1) % check_every == 0:

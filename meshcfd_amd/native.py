@@ -155,6 +155,8 @@ class McfdStage(C.Structure):
         ("out", C.c_void_p),
         ("r_out", C.c_void_p),
         ("c", C.c_double * 5),
+        ("dt_next", C.c_void_p),
+        ("dt_min", C.c_void_p),
     ]
 
 
@@ -171,7 +173,7 @@ EXPORTS = {
     ),
     "mcfd_advance_clock": (
         C.c_int,
-        [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p],
+        [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_int32, C.c_void_p],
     ),
     "mcfd_stage": (C.c_int, [C.c_void_p, C.POINTER(McfdSettings), C.POINTER(McfdStage), C.c_void_p]),
     "mcfd_sum_squares": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -57,6 +57,14 @@ def counters(directory: str, counter: str) -> dict[str, float]:
     return {k: sum(v) / len(v) for k, v in acc.items()}
 
 
+def all_counters(directory: str) -> dict[str, dict[str, float]]:
+    """Mean value per dispatch of every collected counter, by kernel."""
+    acc = defaultdict(lambda: defaultdict(list))
+    for row in _rows(directory, "counter_collection.csv"):
+        acc[_short(row.get("Kernel_Name", ""))][row.get("Counter_Name")].append(float(row["Counter_Value"]))
+    return {k: {c: sum(v) / len(v) for c, v in d.items()} for k, d in acc.items()}
+
+
 def traffic(fetch_dir: str, write_dir: str, copy_bytes: int) -> dict:
     fetch = counters(fetch_dir, "FETCH_SIZE")
     write = counters(write_dir, "WRITE_SIZE")
@@ -84,6 +92,9 @@ def main(argv=None) -> int:
     sub = ap.add_subparsers(dest="cmd", required=True)
     a = sub.add_parser("stats")
     a.add_argument("dir")
+    c = sub.add_parser("counters")
+    c.add_argument("dir")
+    c.add_argument("--kernel", default=None)
     b = sub.add_parser("traffic")
     b.add_argument("fetch_dir")
     b.add_argument("write_dir")
@@ -95,6 +106,12 @@ def main(argv=None) -> int:
         print(f"{'kernel':<22}{'calls':>8}{'total_ms':>12}{'avg_ms':>12}{'%':>8}")
         for r in rows:
             print(f"{r['kernel']:<22}{r['calls']:>8}{r['total_ms']:>12.3f}{r['avg_ms']:>12.5f}{r['percent']:>8.2f}")
+        return 0
+    if args.cmd == "counters":
+        for k, d in sorted(all_counters(args.dir).items()):
+            if args.kernel and args.kernel not in k:
+                continue
+            print(k, " ".join(f"{c}={v:.4g}" for c, v in sorted(d.items())))
         return 0
     t = traffic(args.fetch_dir, args.write_dir, args.copy_bytes)
     text = json.dumps(t, indent=1)

@@ -49,7 +49,7 @@ def test_ctypes_structs_match_header_layout():
     # 8 int32 counts + 29 pointers
     assert C.sizeof(native.McfdMeshDesc) == 8 * 4 + 29 * 8
     assert C.sizeof(native.McfdSettings) == 4 * 4 + 4 * 8
-    assert C.sizeof(native.McfdStage) == 8 + 7 * 8 + 5 * 8
+    assert C.sizeof(native.McfdStage) == 8 + 7 * 8 + 5 * 8 + 2 * 8
 
 
 @pytest.mark.parametrize("reorder", [False, True])
