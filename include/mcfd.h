@@ -102,6 +102,8 @@ typedef struct McfdMeshDesc {
   const int32_t* bc_kind;     /* [n_bc] MCFD_FACE_* */
   const double* bc_state;     /* [n_bc*4] exterior primitive; outlet uses [3] = p */
   const double* bc_wall;      /* [n_bc*2] wall velocity */
+  const double* acc_coef;     /* [n_acc] -L/A (owner) or L/A (neighbour) of interior
+                                 items, the reference's CSR data values; 0 otherwise */
 } McfdMeshDesc;
 
 /*

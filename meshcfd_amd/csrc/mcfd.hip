@@ -274,6 +274,52 @@ __global__ void __launch_bounds__(kBlock)
   store4(W, i, rho, u, v, p);
 }
 
+
+// One limiter entry (_kernels.py:36-65): phi_k = min(phi_k, f(r)). For
+// Barth-Jespersen / Minmod (f = min(r, 1)) an entry whose ratio provably is
+// >= phi_k cannot lower phi_k and its division is skipped:
+// |num| > fl(phi*|inc|)*(1+1e-14) implies fl(num/inc) > phi. The phi values
+// are identical to evaluating every entry.
+__device__ __forceinline__ void limiter_entry(double inc, double wk, double mxk, double mnk, int code,
+                                              double& phik) {
+  double tol = 1e-14 * vmax(fabs(wk), 1.0);
+  double num;
+  if (inc > tol) {
+    num = mxk - wk;
+  } else if (inc < -tol) {
+    num = mnk - wk;
+  } else {
+    if (code <= 2) return;  // ratio = 1: f = 1 for these limiters
+    const double eps = 1e-12;
+    double f = (1.0 * 1.0 + 2.0 * 1.0 + eps) / (1.0 * 1.0 + 1.0 + 2.0 + eps);
+    f = (f < 1.0) ? f : 1.0;
+    if (f < phik) phik = f;
+    return;
+  }
+  if (code <= 1) {
+    bool same = (num > 0.0 && inc > 0.0) || (num < 0.0 && inc < 0.0);
+    if (same && fabs(num) > phik * fabs(inc) * (1.0 + 1e-14)) return;
+  }
+  double ratio = num / inc;
+  ratio = (0.0 > ratio) ? 0.0 : ratio;
+  double f;
+  if (code <= 1) {
+    f = (ratio < 1.0) ? ratio : 1.0;
+  } else if (code == 2) {
+    if (ratio > 0.0) {
+      double t = 2.0 * ratio / (1.0 + ratio);
+      f = (t < 1.0) ? t : 1.0;
+    } else {
+      f = 0.0;
+    }
+  } else {
+    const double eps = 1e-12;
+    f = (ratio * ratio + 2.0 * ratio + eps) / (ratio * ratio + ratio + 2.0 + eps);
+    f = (f < 1.0) ? f : 1.0;
+  }
+  if (f < phik) phik = f;
+}
+
 // LSQ gradient (_kernels.py:252-262) + limiter factor (_kernels.py:9-67).
 __global__ void __launch_bounds__(kBlock)
     k_gradient(int n, const double* __restrict__ W, const int* __restrict__ gptr,
@@ -309,30 +355,7 @@ __global__ void __launch_bounds__(kBlock)
     double dx = ldx[e], dy = ldy[e];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      double inc = gx[k] * dx + gy[k] * dy;
-      double tol = 1e-14 * vmax(fabs(wv[k]), 1.0);
-      double ratio = 1.0;
-      if (inc > tol)
-        ratio = (mx[k] - wv[k]) / inc;
-      else if (inc < -tol)
-        ratio = (mn[k] - wv[k]) / inc;
-      ratio = (0.0 > ratio) ? 0.0 : ratio;
-      double f;
-      if (code <= 1) {
-        f = (ratio < 1.0) ? ratio : 1.0;
-      } else if (code == 2) {
-        if (ratio > 0.0) {
-          double t = 2.0 * ratio / (1.0 + ratio);
-          f = (t < 1.0) ? t : 1.0;
-        } else {
-          f = 0.0;
-        }
-      } else {
-        const double eps = 1e-12;
-        f = (ratio * ratio + 2.0 * ratio + eps) / (ratio * ratio + ratio + 2.0 + eps);
-        f = (f < 1.0) ? f : 1.0;
-      }
-      if (f < phi[k]) phi[k] = f;
+      limiter_entry(gx[k] * dx + gy[k] * dy, wv[k], mx[k], mn[k], code, phi[k]);
     }
   }
   double2* gq = reinterpret_cast<double2*>(G + 8 * (long)i);
@@ -493,7 +516,8 @@ __device__ double cell_time_step(int i, const P4& w, const DtArrays& d, double g
 // (solver.py:2709-2715), finiteness check (:2716-2717), then the stage update.
 __global__ void __launch_bounds__(kBlock)
     k_update(int n, const int* __restrict__ aptr, const int* __restrict__ aitem,
-             const double* __restrict__ FLUX, const double* __restrict__ flen,
+             const double* __restrict__ acoef, const double* __restrict__ FLUX,
+             const double* __restrict__ flen,
              const double* __restrict__ area, const double* __restrict__ W, double ax, double ay,
              StageArgs s, int* __restrict__ status) {
   int i = xcd_block() * blockDim.x + threadIdx.x;
@@ -505,21 +529,17 @@ __global__ void __launch_bounds__(kBlock)
     int item = aitem[e];
     int f = item >> 2;
     int mode = item & 3;
-    double L = flen[f];
     P4 F = load4(FLUX, f);
-    if (mode == MCFD_ACC_OWNER) {
-      double c = -L / A;
+    if (mode <= MCFD_ACC_NEIGHBOUR) {
+      double c = acoef[e];  // (-L)/A or L/A, precomputed like the reference's CSR data
       r0 = r0 + c * F.r;
       r1 = r1 + c * F.u;
       r2 = r2 + c * F.v;
       r3 = r3 + c * F.p;
-    } else if (mode == MCFD_ACC_NEIGHBOUR) {
-      double c = L / A;
-      r0 = r0 + c * F.r;
-      r1 = r1 + c * F.u;
-      r2 = r2 + c * F.v;
-      r3 = r3 + c * F.p;
-    } else if (mode == MCFD_ACC_BOUNDARY) {
+      continue;
+    }
+    double L = flen[f];
+    if (mode == MCFD_ACC_BOUNDARY) {
       r0 = r0 + (-(F.r * L)) / A;
       r1 = r1 + (-(F.u * L)) / A;
       r2 = r2 + (-(F.v * L)) / A;
@@ -606,6 +626,7 @@ struct GeomArrays {
   const double* lim_dy;
   const int* acc_ptr;
   const double* face_len;
+  const double* acc_coef;
 };
 
 __device__ __forceinline__ P4 lds4(const double* s, int j) {
@@ -643,8 +664,9 @@ __global__ void __launch_bounds__(kBlock)
   double* sG = sW + 4 * max_local;
   double* sP = sG + (muscl ? 8 * max_grad : 0);
   double* sF = sP + (muscl ? 4 * max_grad : 0);
-  double* sL = sF + 4 * max_faces;
-  double* sU = sL + max_faces;
+  // the flux slots hold at least kBlock doubles: the final stage reuses them
+  // for the block reduction of the next time step
+  double* sL = sF + (4 * max_faces > kBlock ? 4 * max_faces : kBlock);
   const double* __restrict__ U = s.u;
 
   // P0: primitive states (euler.py:20-40)
@@ -658,10 +680,6 @@ __global__ void __launch_bounds__(kBlock)
     if (j < n_own) {
       if (!finite(rho) || !finite(u) || !finite(v) || !finite(p) || !(rho > 0.0) || !(p > 0.0))
         report(status, MCFD_ERR_STATE, c);
-      sU[4 * j] = q.r;
-      sU[4 * j + 1] = q.u;
-      sU[4 * j + 2] = q.v;
-      sU[4 * j + 3] = q.p;
     }
     sW[4 * j] = rho;
     sW[4 * j + 1] = u;
@@ -680,50 +698,55 @@ __global__ void __launch_bounds__(kBlock)
       double mn[4] = {wv[0], wv[1], wv[2], wv[3]};
       double mx[4] = {wv[0], wv[1], wv[2], wv[3]};
       int e0 = ga.grad_ptr[c], e1 = ga.grad_ptr[c + 1];
+      int l0 = ga.lim_ptr[c], l1 = ga.lim_ptr[c + 1];
       const unsigned short* nbl = ba.gnb + gs + ba.gnb_ptr[gps + j] - e0;
-      for (int e = e0; e < e1; ++e) {
-        int o = nbl[e];
-        double ov[4] = {sW[4 * o], sW[4 * o + 1], sW[4 * o + 2], sW[4 * o + 3]};
-        double cx = ga.grad_cx[e], cy = ga.grad_cy[e];
+      // entries in batches of 4: the batch's loads are issued together, the
+      // sums still run in entry order
+      for (int base = e0; base < e1; base += 4) {
+        int ob[4];
+        double cxb[4], cyb[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          double d = ov[k] - wv[k];
-          gx[k] = gx[k] + cx * d;
-          gy[k] = gy[k] + cy * d;
-          if (ov[k] < mn[k]) mn[k] = ov[k];
-          if (ov[k] > mx[k]) mx[k] = ov[k];
+        for (int q = 0; q < 4; ++q) {
+          int e = base + q;
+          bool ok = e < e1;
+          ob[q] = ok ? (int)nbl[e] : 0;
+          cxb[q] = ok ? ga.grad_cx[e] : 0.0;
+          cyb[q] = ok ? ga.grad_cy[e] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (base + q >= e1) break;
+          int o = ob[q];
+          double ov[4] = {sW[4 * o], sW[4 * o + 1], sW[4 * o + 2], sW[4 * o + 3]};
+          double cx = cxb[q], cy = cyb[q];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            double d = ov[k] - wv[k];
+            gx[k] = gx[k] + cx * d;
+            gy[k] = gy[k] + cy * d;
+            if (ov[k] < mn[k]) mn[k] = ov[k];
+            if (ov[k] > mx[k]) mx[k] = ov[k];
+          }
         }
       }
       double phi[4] = {1.0, 1.0, 1.0, 1.0};
-      int l0 = ga.lim_ptr[c], l1 = ga.lim_ptr[c + 1];
-      for (int e = l0; e < l1; ++e) {
-        double dx = ga.lim_dx[e], dy = ga.lim_dy[e];
+      for (int base = l0; base < l1; base += 4) {
+        double dxb[4], dyb[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          int e = base + q;
+          bool ok = e < l1;
+          dxb[q] = ok ? ga.lim_dx[e] : 0.0;
+          dyb[q] = ok ? ga.lim_dy[e] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+        if (base + q >= l1) break;
+        double dx = dxb[q], dy = dyb[q];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          double inc = gx[k] * dx + gy[k] * dy;
-          double tol = 1e-14 * vmax(fabs(wv[k]), 1.0);
-          double ratio = 1.0;
-          if (inc > tol)
-            ratio = (mx[k] - wv[k]) / inc;
-          else if (inc < -tol)
-            ratio = (mn[k] - wv[k]) / inc;
-          ratio = (0.0 > ratio) ? 0.0 : ratio;
-          double f;
-          if (limiter <= 1) {
-            f = (ratio < 1.0) ? ratio : 1.0;
-          } else if (limiter == 2) {
-            if (ratio > 0.0) {
-              double t = 2.0 * ratio / (1.0 + ratio);
-              f = (t < 1.0) ? t : 1.0;
-            } else {
-              f = 0.0;
-            }
-          } else {
-            const double eps = 1e-12;
-            f = (ratio * ratio + 2.0 * ratio + eps) / (ratio * ratio + ratio + 2.0 + eps);
-            f = (f < 1.0) ? f : 1.0;
-          }
-          if (f < phi[k]) phi[k] = f;
+          limiter_entry(gx[k] * dx + gy[k] * dy, wv[k], mx[k], mn[k], limiter, phi[k]);
+        }
         }
       }
 #pragma unroll
@@ -802,21 +825,17 @@ __global__ void __launch_bounds__(kBlock)
       int item = ba.acc_local[e];
       int slot = item >> 2;
       int mode = item & 3;
-      double L = sL[slot];
       P4 F = lds4(sF, slot);
-      if (mode == MCFD_ACC_OWNER) {
-        double c = -L / A;
+      if (mode <= MCFD_ACC_NEIGHBOUR) {
+        double c = ga.acc_coef[e];
         r0 = r0 + c * F.r;
         r1 = r1 + c * F.u;
         r2 = r2 + c * F.v;
         r3 = r3 + c * F.p;
-      } else if (mode == MCFD_ACC_NEIGHBOUR) {
-        double c = L / A;
-        r0 = r0 + c * F.r;
-        r1 = r1 + c * F.u;
-        r2 = r2 + c * F.v;
-        r3 = r3 + c * F.p;
-      } else if (mode == MCFD_ACC_BOUNDARY) {
+        continue;
+      }
+      double L = sL[slot];
+      if (mode == MCFD_ACC_BOUNDARY) {
         r0 = r0 + (-(F.r * L)) / A;
         r1 = r1 + (-(F.u * L)) / A;
         r2 = r2 + (-(F.v * L)) / A;
@@ -839,7 +858,7 @@ __global__ void __launch_bounds__(kBlock)
     if (s.r_out) store4(s.r_out, i, r0, r1, r2, r3);
     if (s.op == MCFD_OP_RESIDUAL) continue;
     double h = s.dt_per_cell ? s.dt[i] : s.dt[0];
-    P4 u = lds4(sU, j);
+    P4 u = load4(U, i);
     double o0, o1, o2, o3;
     if (s.op == MCFD_OP_EULER) {
       double k = s.c0 * h;
@@ -1893,6 +1912,7 @@ struct McfdMesh {
   double* face_len = nullptr;
   int* acc_ptr = nullptr;
   int* acc_item = nullptr;
+  double* acc_coef = nullptr;
   int* dts_ptr = nullptr;
   double* dts_nx = nullptr;
   double* dts_ny = nullptr;
@@ -1984,8 +2004,9 @@ int upload_block(McfdMesh* m, T** dst, const T* src, size_t count) {
 }
 
 size_t fused_lds_bytes(const McfdMesh* m, bool muscl) {
-  size_t d = 4 * (size_t)m->max_local + (muscl ? 12 * (size_t)m->max_grad : 0) +
-             5 * (size_t)m->max_faces + 4 * (size_t)m->block_cells;
+  size_t flux_slots = 4 * (size_t)m->max_faces > (size_t)kBlock ? 4 * (size_t)m->max_faces : kBlock;
+  size_t d = 4 * (size_t)m->max_local + (muscl ? 12 * (size_t)m->max_grad : 0) + flux_slots +
+             (size_t)m->max_faces;
   return d * sizeof(double);
 }
 
@@ -2368,6 +2389,7 @@ int mcfd_mesh_create(const McfdMeshDesc* d, McfdMesh** out) {
   rc |= upload(m, const_cast<double**>(&fa.ry), d->face_ry, F);
   rc |= upload(m, &m->acc_ptr, d->acc_ptr, C + 1);
   rc |= upload(m, &m->acc_item, d->acc_item, d->n_acc);
+  rc |= upload(m, &m->acc_coef, d->acc_coef, d->n_acc);
   rc |= upload(m, &m->dts_ptr, d->dts_ptr, C + 1);
   rc |= upload(m, &m->dts_nx, d->dts_nx, d->n_dts);
   rc |= upload(m, &m->dts_ny, d->dts_ny, d->n_dts);
@@ -2476,8 +2498,8 @@ int mcfd_stage(McfdMesh* m, const McfdSettings* st, const McfdStage* sg, void* s
   if (m->fused) {
     bool muscl = st->reconstruction == 1 && m->n_grad > 0;
     size_t lds = fused_lds_bytes(m, muscl);
-    GeomArrays ga{m->area,    m->grad_ptr, m->grad_cx, m->grad_cy,  m->lim_ptr,
-                  m->lim_dx,  m->lim_dy,   m->acc_ptr, m->face_len};
+    GeomArrays ga{m->area,   m->grad_ptr, m->grad_cx,  m->grad_cy, m->lim_ptr,
+                  m->lim_dx, m->lim_dy,   m->acc_ptr,  m->face_len, m->acc_coef};
     TimedLaunch t(m, s, MCFD_KERNEL_STAGE);
     hipLaunchKernelGGL(k_stage_fused, dim3(m->nb), dim3(kBlock), lds, s, m->block_cells,
                        m->max_local, m->max_grad, m->max_faces, m->blocks, ga, m->faces, m->dts,
@@ -2490,7 +2512,7 @@ int mcfd_stage(McfdMesh* m, const McfdSettings* st, const McfdStage* sg, void* s
   {
     TimedLaunch t(m, s, MCFD_KERNEL_UPDATE);
     hipLaunchKernelGGL(k_update, dim3(blocks(m->n_cells)), dim3(kBlock), 0, s, m->n_cells,
-                       m->acc_ptr, m->acc_item, m->FLUX, m->face_len, m->area, m->W,
+                       m->acc_ptr, m->acc_item, m->acc_coef, m->FLUX, m->face_len, m->area, m->W,
                        st->acceleration_x, st->acceleration_y, a, m->status);
   }
   HIP_CHECK(hipGetLastError());

@@ -70,6 +70,7 @@ class DeviceMesh:
         layout=None,
         fused: bool = True,
         block_cells: int = 128,
+        fuse_time_step: bool = False,
     ):
         if not torch.cuda.is_available():
             raise native.NativeError("No ROCm device is visible: the GPU residual cannot run.")
@@ -86,6 +87,7 @@ class DeviceMesh:
             native.check(self.lib.mcfd_mesh_create(C.byref(desc), C.byref(handle)))
             del keep
         self.handle = handle
+        self.fuse_time_step = fuse_time_step
         self.blocks = None
         if fused:
             self.set_blocks(block_cells)
@@ -322,7 +324,9 @@ class DeviceMesh:
         ti = s.time_integration
         u = self.u
         # the fused final stage also computes the next step's CFL steps
-        fuse_dt = self.blocks is not None and self.case_ptr is None and mode != "Global fixed"
+        fuse_dt = (
+            self.fuse_time_step and self.blocks is not None and self.case_ptr is None and mode != "Global fixed"
+        )
         nxt = dict(dt_next=self.dt_cell, dt_min=self.dt_min) if fuse_dt else {}
         if ti == "Explicit Euler":
             self._stage(native.OP_EULER, u, out=self.s1, dt=dt, per_cell=per_cell, coeffs=(1.0,), r_out=residual_out,

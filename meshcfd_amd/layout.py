@@ -84,6 +84,7 @@ class DeviceLayout:
     face_ry: np.ndarray
     acc_ptr: np.ndarray
     acc_item: np.ndarray
+    acc_coef: np.ndarray  # -L/A_owner, L/A_neighbour (solver.py:2157-2174); 0 on boundary items
     dts_ptr: np.ndarray
     dts_nx: np.ndarray
     dts_ny: np.ndarray
@@ -208,6 +209,11 @@ def build_layout(g: Geometry, reorder: bool = True, order: np.ndarray | None = N
     all_items = fnew[np.concatenate((int_k, seq_k))] * 4 + np.concatenate((int_mode, seq_mode))
     acc_ptr, order = _csr(new_id[all_cells], n)
     acc_item = all_items[order]
+    # CSR data of the divergence operator: (-L)/A for the owner, L/A for the neighbour
+    lint = g.face_lengths[inter]
+    int_coef = np.where(int_mode == ACC_OWNER, -lint[int_k] / g.areas[int_cells], lint[int_k] / g.areas[int_cells])
+    all_coef = np.concatenate((int_coef, np.zeros(len(seq_cells))))
+    acc_coef = all_coef[order]
     if acc_item.size and acc_item.max() >= 2**31:
         raise ValueError("Too many faces for 32-bit accumulation items.")
 
@@ -275,6 +281,7 @@ def build_layout(g: Geometry, reorder: bool = True, order: np.ndarray | None = N
         face_ry=np.ascontiguousarray(roff[:, 1]),
         acc_ptr=acc_ptr,
         acc_item=acc_item.astype(np.int32),
+        acc_coef=np.ascontiguousarray(acc_coef),
         dts_ptr=dts_ptr,
         dts_nx=np.ascontiguousarray(g.face_normals[d_faces, 0]),
         dts_ny=np.ascontiguousarray(g.face_normals[d_faces, 1]),

@@ -72,6 +72,7 @@ class McfdMeshDesc(C.Structure):
         ("bc_kind", _ip),
         ("bc_state", _dp),
         ("bc_wall", _dp),
+        ("acc_coef", _dp),
     ]
 
 
@@ -281,5 +282,6 @@ def mesh_desc(layout) -> tuple[McfdMeshDesc, list]:
         bc_kind=i(L.bc_kind),
         bc_state=d(L.bc_state),
         bc_wall=d(L.bc_wall),
+        acc_coef=d(L.acc_coef),
     )
     return desc, keep

@@ -46,8 +46,8 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
 def test_ctypes_structs_match_header_layout():
     import ctypes as C
 
-    # 8 int32 counts + 29 pointers
-    assert C.sizeof(native.McfdMeshDesc) == 8 * 4 + 29 * 8
+    # 8 int32 counts + 30 pointers
+    assert C.sizeof(native.McfdMeshDesc) == 8 * 4 + 30 * 8
     assert C.sizeof(native.McfdSettings) == 4 * 4 + 4 * 8
     assert C.sizeof(native.McfdStage) == 8 + 7 * 8 + 5 * 8 + 2 * 8
 
