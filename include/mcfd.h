@@ -134,7 +134,7 @@ typedef struct McfdBlockDesc {
 
 typedef struct McfdSettings {
   int32_t flux;            /* 0 Rusanov, 1 HLLC, 2 Roe */
-  int32_t reconstruction;  /* 0 first order, 1 MUSCL linear */
+  int32_t reconstruction;  /* 0 first order, 1 MUSCL linear, 2 MUSCL parabolic */
   int32_t limiter;         /* 0 Barth-Jespersen, 1 Minmod, 2 Van Leer, 3 Venkatakrishnan */
   int32_t reserved;
   double gamma;
@@ -290,6 +290,15 @@ int mcfd_newton_candidate(McfdMesh* mesh, const double* u, const double* corr, d
 
 /* out[0] = ||a||_2 over the (C,4) array (device), deterministic. */
 int mcfd_norm2(McfdMesh* mesh, const double* a, double* out, void* stream);
+
+/* MUSCL parabolic data (host pointers, device cell ids): per-cell CSR of the
+ * two-ring stencil (ptr [C+1], nb [n_entries]), per entry the 5 fit
+ * coefficients P_e with (grad, curvature) = sum_e P_e (W_e - W_i), and the
+ * per-cell full-rank flag [C]. Replaces `_parabolic_stencil` and the per-call
+ * normal-equation solve of `_parabolic_reconstruction` (solver.py:2241-2372).
+ * NULL ptr frees. */
+int mcfd_mesh_set_parabolic(McfdMesh* mesh, int32_t n_entries, const int32_t* ptr,
+                            const int32_t* nb, const double* coef, const int32_t* full);
 
 /* Upload the reverse-entry map of the LSQ lists (grad_rev[e] = entry of the
  * neighbour that points back) and allocate the adjoint scratch; NULL frees. */

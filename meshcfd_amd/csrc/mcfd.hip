@@ -366,6 +366,83 @@ __global__ void __launch_bounds__(kBlock)
   store4(PHI, i, phi[0], phi[1], phi[2], phi[3]);
 }
 
+
+// MUSCL parabolic (solver.py:2322-2415): linear LSQ gradient and limiter as
+// k_gradient; for full-rank cells the quadratic fit replaces them:
+// gradient/curvature = sum_e P_e (W_e - W_i) with P_e precomputed per stencil
+// entry, min/max over the two-ring stencil, quadratic face increments.
+__global__ void __launch_bounds__(kBlock)
+    k_gradient_parabolic(int n, const double* __restrict__ W, const int* __restrict__ gptr,
+                         const int* __restrict__ gnb, const double* __restrict__ gcx,
+                         const double* __restrict__ gcy, const int* __restrict__ lptr,
+                         const double* __restrict__ ldx, const double* __restrict__ ldy, int code,
+                         const int* __restrict__ pptr, const int* __restrict__ pnb,
+                         const double* __restrict__ pcoef, const int* __restrict__ pfull,
+                         double* __restrict__ G, double* __restrict__ K, double* __restrict__ PHI) {
+  int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  P4 w = load4(W, i);
+  double wv[4] = {w.r, w.u, w.v, w.p};
+  double gx[4] = {0.0, 0.0, 0.0, 0.0};
+  double gy[4] = {0.0, 0.0, 0.0, 0.0};
+  double kk[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // [k][xx, xy, yy]
+  double mn[4] = {wv[0], wv[1], wv[2], wv[3]};
+  double mx[4] = {wv[0], wv[1], wv[2], wv[3]};
+  double phi[4] = {1.0, 1.0, 1.0, 1.0};
+  if (pfull[i]) {
+    double c[5][4] = {{0}};
+    for (int e = pptr[i]; e < pptr[i + 1]; ++e) {
+      P4 o = load4(W, pnb[e]);
+      double ov[4] = {o.r, o.u, o.v, o.p};
+      const double* pe = pcoef + 5 * (long)e;
+      for (int k = 0; k < 4; ++k) {
+        double d = ov[k] - wv[k];
+        for (int a = 0; a < 5; ++a) c[a][k] = c[a][k] + pe[a] * d;
+        if (ov[k] < mn[k]) mn[k] = ov[k];
+        if (ov[k] > mx[k]) mx[k] = ov[k];
+      }
+    }
+    for (int k = 0; k < 4; ++k) {
+      gx[k] = c[0][k];
+      gy[k] = c[1][k];
+      kk[3 * k] = c[2][k];
+      kk[3 * k + 1] = c[3][k];
+      kk[3 * k + 2] = c[4][k];
+    }
+    for (int e = lptr[i]; e < lptr[i + 1]; ++e) {
+      double dx = ldx[e], dy = ldy[e];
+      for (int k = 0; k < 4; ++k) {
+        double inc = gx[k] * dx + gy[k] * dy + 0.5 * kk[3 * k] * (dx * dx) +
+                     kk[3 * k + 1] * (dx * dy) + 0.5 * kk[3 * k + 2] * (dy * dy);
+        limiter_entry(inc, wv[k], mx[k], mn[k], code, phi[k]);
+      }
+    }
+  } else {
+    for (int e = gptr[i]; e < gptr[i + 1]; ++e) {
+      P4 o = load4(W, gnb[e]);
+      double ov[4] = {o.r, o.u, o.v, o.p};
+      double cx = gcx[e], cy = gcy[e];
+      for (int k = 0; k < 4; ++k) {
+        double d = ov[k] - wv[k];
+        gx[k] = gx[k] + cx * d;
+        gy[k] = gy[k] + cy * d;
+        if (ov[k] < mn[k]) mn[k] = ov[k];
+        if (ov[k] > mx[k]) mx[k] = ov[k];
+      }
+    }
+    for (int e = lptr[i]; e < lptr[i + 1]; ++e) {
+      double dx = ldx[e], dy = ldy[e];
+      for (int k = 0; k < 4; ++k)
+        limiter_entry(gx[k] * dx + gy[k] * dy, wv[k], mx[k], mn[k], code, phi[k]);
+    }
+  }
+  double2* gq = reinterpret_cast<double2*>(G + 8 * (long)i);
+  for (int k = 0; k < 4; ++k) gq[k] = make_double2(gx[k], gy[k]);
+  double* kq = K + 12 * (long)i;
+  for (int a = 0; a < 12; ++a) kq[a] = kk[a];
+  store4(PHI, i, phi[0], phi[1], phi[2], phi[3]);
+}
+
 // W_f = W + phi*(g_x*dx + g_y*dy) (_kernels.py:90-98, solver.py:2418-2434).
 __device__ __forceinline__ P4 reconstruct(const double* __restrict__ W,
                                           const double* __restrict__ G,
@@ -382,6 +459,26 @@ __device__ __forceinline__ P4 reconstruct(const double* __restrict__ W,
   o.v = w.v + f.v * (g2.x * dx + g2.y * dy);
   o.p = w.p + f.p * (g3.x * dx + g3.y * dy);
   return o;
+}
+
+// With curvature (solver.py:2424-2434): the quadratic increment is added to
+// the linear one as a group.
+__device__ __forceinline__ P4 reconstruct_q(const double* __restrict__ W,
+                                            const double* __restrict__ G,
+                                            const double* __restrict__ K,
+                                            const double* __restrict__ PHI, int c, double dx,
+                                            double dy) {
+  P4 w = load4(W, c);
+  const double* g = G + 8 * (long)c;
+  const double* k = K + 12 * (long)c;
+  P4 f = load4(PHI, c);
+  double q[4];
+  for (int v = 0; v < 4; ++v) {
+    double lin = g[2 * v] * dx + g[2 * v + 1] * dy;
+    double quad = 0.5 * k[3 * v] * (dx * dx) + k[3 * v + 1] * (dx * dy) + 0.5 * k[3 * v + 2] * (dy * dy);
+    q[v] = lin + quad;
+  }
+  return P4{w.r + f.r * q[0], w.u + f.u * q[1], w.v + f.v * q[2], w.p + f.p * q[3]};
 }
 
 struct FaceArrays {
@@ -401,17 +498,20 @@ struct FaceArrays {
 
 __global__ void __launch_bounds__(kBlock)
     k_flux(int nf, FaceArrays fa, const double* __restrict__ W, const double* __restrict__ G,
-           const double* __restrict__ PHI, int scheme, int muscl, double g,
-           double* __restrict__ FLUX, int* __restrict__ status) {
+           const double* __restrict__ PHI, const double* __restrict__ K, int scheme, int muscl,
+           double g, double* __restrict__ FLUX, int* __restrict__ status) {
   int k = xcd_block() * blockDim.x + threadIdx.x;
   if (k >= nf) return;
   int kind = fa.kind[k];
   double nx = fa.nx[k], ny = fa.ny[k];
-  P4 L = reconstruct(W, G, PHI, fa.left[k], fa.lx[k], fa.ly[k], muscl);
+  const bool quad = K != nullptr;
+  P4 L = quad ? reconstruct_q(W, G, K, PHI, fa.left[k], fa.lx[k], fa.ly[k])
+              : reconstruct(W, G, PHI, fa.left[k], fa.lx[k], fa.ly[k], muscl);
   P4 R;
   int err = 0;
   if (kind <= MCFD_FACE_PERIODIC) {
-    R = reconstruct(W, G, PHI, fa.right[k], fa.rx[k], fa.ry[k], muscl);
+    R = quad ? reconstruct_q(W, G, K, PHI, fa.right[k], fa.rx[k], fa.ry[k])
+             : reconstruct(W, G, PHI, fa.right[k], fa.rx[k], fa.ry[k], muscl);
   } else {
     int b = fa.bc[k];
     const double* st = fa.bc_state + 4 * (long)b;
@@ -435,8 +535,11 @@ __global__ void __launch_bounds__(kBlock)
   }
   F4 f;
   if (scheme == 1) {
-    f = hllc(L, R, nx, ny, g, kind == MCFD_FACE_INTERIOR);
-    if (kind != MCFD_FACE_INTERIOR &&
+    // interior faces take the compiled kernel's branch order unless the
+    // reconstruction is parabolic (solver.py:2479-2496)
+    bool fused_order = kind == MCFD_FACE_INTERIOR && !quad;
+    f = hllc(L, R, nx, ny, g, fused_order);
+    if (!fused_order &&
         !(finite(f.a) && finite(f.b) && finite(f.c) && finite(f.d)))
       err = MCFD_ERR_HLLC;
   } else if (scheme == 2) {
@@ -1945,6 +2048,15 @@ struct McfdMesh {
   double* gm_rp = nullptr;      // 4C
   double* gm_scalars = nullptr; // 64
   double* gm_partial = nullptr; // dot partials
+  // MUSCL parabolic
+  bool parabolic = false;
+  int n_par = 0;
+  int* par_ptr = nullptr;
+  int* par_nb = nullptr;
+  double* par_coef = nullptr;
+  int* par_full = nullptr;
+  double* K = nullptr;
+  std::vector<void*> parabolic_allocations;
   // adjoint
   bool adjoint = false;
   int* grad_rev = nullptr;
@@ -2044,7 +2156,24 @@ int alloc_adjoint(McfdMesh* m, T** dst, const T* src, size_t count) {
   return 0;
 }
 
+void free_parabolic(McfdMesh* m) {
+  for (void* p : m->parabolic_allocations) (void)hipFree(p);
+  m->parabolic_allocations.clear();
+  m->parabolic = false;
+}
+
+template <typename T>
+int alloc_parabolic(McfdMesh* m, T** dst, const T* src, size_t count) {
+  void* p = nullptr;
+  HIP_CHECK(hipMalloc(&p, sizeof(T) * (count ? count : 1)));
+  m->parabolic_allocations.push_back(p);
+  if (count && src) HIP_CHECK(hipMemcpy(p, src, sizeof(T) * count, hipMemcpyHostToDevice));
+  *dst = static_cast<T*>(p);
+  return 0;
+}
+
 void free_mesh(McfdMesh* m) {
+  free_parabolic(m);
   free_adjoint(m);
   free_implicit(m);
   free_blocks(m);
@@ -2078,7 +2207,7 @@ struct TimedLaunch {
 int check_settings(const McfdSettings* s) {
   if (!s) return fail("settings is NULL");
   if (s->flux < 0 || s->flux > 2) return fail("Unsupported convective flux code.");
-  if (s->reconstruction < 0 || s->reconstruction > 1)
+  if (s->reconstruction < 0 || s->reconstruction > 2)
     return fail("Unsupported reconstruction code for the GPU residual.");
   if (s->limiter < 0 || s->limiter > 3) return fail("Unsupported limiter code.");
   return 0;
@@ -2091,8 +2220,16 @@ int residual_front(McfdMesh* m, const McfdSettings* st, const double* u, hipStre
     hipLaunchKernelGGL(k_primitive, dim3(blocks(n)), dim3(kBlock), 0, s, n, u, m->W, st->gamma,
                        m->status);
   }
-  bool muscl = st->reconstruction == 1 && m->n_grad > 0;
-  if (muscl) {
+  bool para = st->reconstruction == 2 && m->n_grad > 0;
+  if (para && !m->parabolic) return fail("MUSCL parabolic needs mcfd_mesh_set_parabolic first");
+  bool muscl = (st->reconstruction == 1 || para) && m->n_grad > 0;
+  if (para) {
+    TimedLaunch t(m, s, MCFD_KERNEL_GRADIENT);
+    hipLaunchKernelGGL(k_gradient_parabolic, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W,
+                       m->grad_ptr, m->grad_nb, m->grad_cx, m->grad_cy, m->lim_ptr, m->lim_dx,
+                       m->lim_dy, st->limiter, m->par_ptr, m->par_nb, m->par_coef, m->par_full,
+                       m->G, m->K, m->PHI);
+  } else if (muscl) {
     TimedLaunch t(m, s, MCFD_KERNEL_GRADIENT);
     hipLaunchKernelGGL(k_gradient, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->grad_ptr,
                        m->grad_nb, m->grad_cx, m->grad_cy, m->lim_ptr, m->lim_dx, m->lim_dy,
@@ -2101,8 +2238,8 @@ int residual_front(McfdMesh* m, const McfdSettings* st, const double* u, hipStre
   if (m->n_faces) {
     TimedLaunch t(m, s, MCFD_KERNEL_FLUX);
     hipLaunchKernelGGL(k_flux, dim3(blocks(m->n_faces)), dim3(kBlock), 0, s, m->n_faces,
-                       m->faces, m->W, m->G, m->PHI, st->flux, muscl ? 1 : 0, st->gamma,
-                       m->FLUX, m->status);
+                       m->faces, m->W, m->G, m->PHI, para ? m->K : nullptr, st->flux,
+                       muscl ? 1 : 0, st->gamma, m->FLUX, m->status);
   }
   HIP_CHECK(hipGetLastError());
   return 0;
@@ -2495,7 +2632,7 @@ int mcfd_stage(McfdMesh* m, const McfdSettings* st, const McfdStage* sg, void* s
               reinterpret_cast<unsigned long long*>(sg->dt_min), st->cfl};
   if (sg->dt_next && (!m->fused || !sg->dt_min || sg->op == MCFD_OP_RESIDUAL))
     return fail("mcfd_stage: dt_next needs the fused path, dt_min and a state update");
-  if (m->fused) {
+  if (m->fused && st->reconstruction != 2) {
     bool muscl = st->reconstruction == 1 && m->n_grad > 0;
     size_t lds = fused_lds_bytes(m, muscl);
     GeomArrays ga{m->area,   m->grad_ptr, m->grad_cx,  m->grad_cy, m->lim_ptr,
@@ -2786,11 +2923,36 @@ int mcfd_mesh_set_adjoint(McfdMesh* m, const int32_t* grad_rev) {
   return 0;
 }
 
+
+int mcfd_mesh_set_parabolic(McfdMesh* m, int32_t n_entries, const int32_t* ptr, const int32_t* nb,
+                            const double* coef, const int32_t* full) {
+  if (!m) return fail("mcfd_mesh_set_parabolic: NULL mesh");
+  (void)hipDeviceSynchronize();
+  free_parabolic(m);
+  if (!ptr) return 0;
+  size_t C = m->n_cells;
+  int rc = 0;
+  rc |= alloc_parabolic(m, &m->par_ptr, ptr, C + 1);
+  rc |= alloc_parabolic(m, &m->par_nb, nb, (size_t)n_entries);
+  rc |= alloc_parabolic(m, &m->par_coef, coef, 5 * (size_t)n_entries);
+  rc |= alloc_parabolic(m, &m->par_full, full, C);
+  rc |= alloc_parabolic(m, &m->K, (const double*)nullptr, 12 * C);
+  if (rc) {
+    std::string msg = g_last_error;
+    free_parabolic(m);
+    return fail(msg);
+  }
+  m->n_par = n_entries;
+  m->parabolic = true;
+  return 0;
+}
+
 int mcfd_residual_vjp(McfdMesh* m, const McfdSettings* st, const double* u, const double* lam,
                       double* ubar, void* stream) {
   if (!m || !u || !lam || !ubar) return fail("mcfd_residual_vjp: NULL argument");
   if (!m->adjoint) return fail("mcfd_residual_vjp: call mcfd_mesh_set_adjoint first");
   if (check_settings(st)) return -1;
+  if (st->reconstruction == 2) return fail("mcfd_residual_vjp: MUSCL parabolic has no adjoint yet");
   hipStream_t s = static_cast<hipStream_t>(stream);
   int n = m->n_cells;
   bool muscl = st->reconstruction == 1 && m->n_grad > 0;

@@ -35,8 +35,6 @@ RK4_COEFFS = (
 def native_settings(s: NumericalSettings) -> native.McfdSettings:
     if s.reconstruction not in RECONSTRUCTION_CODES:
         raise SolverError(f"Unsupported reconstruction '{s.reconstruction}'.")
-    if s.reconstruction == "MUSCL parabolic":
-        raise SolverError("MUSCL parabolic reconstruction is not implemented by the GPU residual.")
     if s.limiter not in LIMITER_CODES:
         raise SolverError(f"Unsupported limiter '{s.limiter}'.")
     if s.flux not in FLUX_CODES:
@@ -92,6 +90,8 @@ class DeviceMesh:
         if fused:
             self.set_blocks(block_cells)
         self.native_settings = native_settings(self.settings)
+        if self.settings.reconstruction == "MUSCL parabolic":
+            self.set_parabolic()
         opts = dict(dtype=torch.float64, device=self.device)
         n = self.n
         self.u = torch.empty((n, 4), **opts)
@@ -154,6 +154,22 @@ class DeviceMesh:
 
     def replay(self):
         self.graph.replay()
+
+    def set_parabolic(self) -> None:
+        """Upload the two-ring stencils and precomputed quadratic-fit
+        coefficients (MUSCL parabolic)."""
+        from .parabolic import build_parabolic
+
+        new_id = np.empty(self.n, dtype=np.int64)
+        new_id[self.layout.cell_order] = np.arange(self.n)
+        p = build_parabolic(self.geometry, new_id)
+        self._parabolic = p
+        native.check(
+            self.lib.mcfd_mesh_set_parabolic(
+                self.handle, len(p["nb"]), C.c_void_p(p["ptr"].ctypes.data), C.c_void_p(p["nb"].ctypes.data),
+                C.c_void_p(p["coef"].ctypes.data), C.c_void_p(p["full"].ctypes.data),
+            )
+        )
 
     def set_blocks(self, block_cells: int | None) -> None:
         """Switch to the fused one-kernel-per-stage path with blocks of

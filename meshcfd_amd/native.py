@@ -178,6 +178,7 @@ EXPORTS = {
     ),
     "mcfd_stage": (C.c_int, [C.c_void_p, C.POINTER(McfdSettings), C.POINTER(McfdStage), C.c_void_p]),
     "mcfd_sum_squares": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mcfd_mesh_set_parabolic": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mcfd_batch_time_steps": (
         C.c_int,
         [C.c_void_p, C.POINTER(McfdSettings), C.c_void_p, C.c_int32, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,

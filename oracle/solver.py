@@ -58,6 +58,14 @@ class Oracle:
         self.g = geometry
         self.s = settings
         self.D = divergence_operator(geometry)
+        self.stencil = None
+        if settings.reconstruction == "MUSCL parabolic":
+            from .parabolic import parabolic_stencil
+
+            self.stencil = parabolic_stencil(
+                len(geometry.areas), geometry.gradient_sources, geometry.gradient_neighbours,
+                geometry.gradient_dx, geometry.gradient_dy, geometry.domain_scale,
+            )
 
     # -- reconstruction ---------------------------------------------------
     def gradients_and_factors(self, primitive):
@@ -79,19 +87,32 @@ class Oracle:
         n = len(primitive)
         return np.zeros((n, 4, 2)), np.ones((n, 4))
 
+    def parabolic(self, primitive):
+        from .parabolic import parabolic_reconstruction
+
+        return parabolic_reconstruction(primitive, self.g, self.stencil, LIMITER_CODES[self.s.limiter])
+
     # -- residual ---------------------------------------------------------
     def residual(self, conservative):
         g, s = self.g, self.s
         gamma = s.gamma
         primitive = conservative_to_primitive(conservative, gamma)
-        grads, factors = self.gradients_and_factors(primitive)
+        curvature = None
+        if s.reconstruction == "MUSCL parabolic":
+            grads, curvature, factors = self.parabolic(primitive)
+        else:
+            grads, factors = self.gradients_and_factors(primitive)
 
         def rec(cells, offsets):
+            if curvature is not None:
+                from .parabolic import reconstruct as prec
+
+                return prec(primitive, grads, factors, curvature, cells, offsets)
             return kernels.reconstruct(primitive, grads, factors, cells, offsets)
 
         deriv = np.zeros_like(conservative)
         if len(g.interior_faces):
-            if s.flux == "HLLC":
+            if s.flux == "HLLC" and curvature is None:
                 flux = kernels.reconstruct_hllc_flux(
                     primitive, grads, factors, g.interior_owners, g.interior_neighbours,
                     g.interior_owner_offsets, g.interior_neighbour_offsets, g.interior_normals, gamma,

@@ -204,3 +204,42 @@ def test_fused_block_sizes_match_unfused(cuda_device, block_cells):
         out.append(dm.state())
         dm.close()
     assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("flux", ["Rusanov", "HLLC", "Roe"])
+@pytest.mark.parametrize("kind", ["periodic", "mixed", "farfield"])
+def test_parabolic_residual_matches_oracle(cuda_device, flux, kind):
+    """MUSCL parabolic: the device uses fit coefficients factored at setup, the
+    oracle solves the normal equations per call (solver.py:2322-2372), so the
+    two agree to rounding, not bit for bit."""
+    from oracle.euler import primitive_to_conservative
+    from oracle.solver import Oracle
+
+    case = _case(kind, flux, "MUSCL parabolic", n=14)
+    dm = _device(case, cuda_device, "fused")
+    u = primitive_to_conservative(_perturbed_primitive(case, seed=4), 1.4)
+    ref = Oracle(dm.geometry, case.numerics).residual(u)
+    gpu = dm.to_mesh_order(dm.residual(dm.to_device_order(u)))
+    dm.check_status()
+    err = rel_err(gpu, ref)
+    print("parabolic residual rel err", err)
+    assert np.all(err <= 1e-11), err
+    dm.close()
+
+
+@pytest.mark.parametrize("integrator", ["Explicit RK3", "Explicit RK4"])
+def test_parabolic_steps_match_oracle(cuda_device, integrator):
+    from oracle.euler import primitive_to_conservative
+    from oracle.solver import Oracle
+
+    case = _case("farfield", "Roe", "MUSCL parabolic", n=16)
+    case.numerics.time_integration = integrator
+    dm = _device(case, cuda_device, "fused")
+    prim = _perturbed_primitive(case, seed=6)
+    dm.set_state(primitive_to_conservative(prim, 1.4))
+    dm.run(10)
+    ref, _ = Oracle(dm.geometry, case.numerics).run(prim, 10)
+    err = rel_err(dm.state(), ref)
+    print(integrator, "parabolic rel err", err)
+    assert np.all(err <= TOL)
+    dm.close()

@@ -366,3 +366,56 @@ def test_mesh_numbering_follows_first_visit_ownership():
     assert mesh.face_owners[:4].tolist() == [0, 0, 0, 0]
     assert mesh.face_neighbours[1] == 1  # shared with the next cell in the row
     assert sorted(mesh.patches[0].face_ids.tolist()) == mesh.patches[0].face_ids.tolist()
+
+
+def test_parabolic_recovers_quadratic_field():
+    """Reference tests/test_solver.py:843-873: gradient and curvature of
+    10 + x^2 + 2xy + 3y^2 at an interior cell to 1e-10, limiter inactive."""
+    from oracle import parabolic as opar
+
+    mesh = meshgen.rectangle_mesh(6, 6)
+    case = meshgen.periodic_case(mesh, NumericalSettings(reconstruction="MUSCL parabolic"))
+    g = build_geometry(case)
+    x, y = g.centers[:, 0], g.centers[:, 1]
+    prim = np.column_stack((10.0 + x * x + 2 * x * y + 3 * y * y, np.full_like(x, 0.2), np.full_like(x, -0.1),
+                            np.ones_like(x)))
+    st = opar.parabolic_stencil(g.cell_count, g.gradient_sources, g.gradient_neighbours, g.gradient_dx,
+                                g.gradient_dy, g.domain_scale)
+    grads, curv, factors = opar.parabolic_reconstruction(prim, g, st, 0)
+    c = 3 * 6 + 3
+    np.testing.assert_allclose(grads[c, 0], (2 * x[c] + 2 * y[c], 2 * x[c] + 6 * y[c]), rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(curv[c, 0], (2.0, 2.0, 6.0), rtol=1e-10, atol=1e-10)
+    assert factors[c, 0] == pytest.approx(1.0)
+
+
+def test_parabolic_stencil_vectorised_equals_bfs():
+    from meshcfd_amd.parabolic import parabolic_stencil
+    from oracle import parabolic as opar
+
+    for split in (0.0, 0.4):
+        mesh = meshgen.rectangle_mesh(7, 5, jitter=0.2, seed=3, split_fraction=split)
+        g = build_geometry(meshgen.periodic_case(mesh, NumericalSettings()))
+        want = opar.parabolic_stencil(g.cell_count, g.gradient_sources, g.gradient_neighbours, g.gradient_dx,
+                                      g.gradient_dy, g.domain_scale)
+        got = parabolic_stencil(g)
+        for a, b in zip(got, want):
+            assert np.array_equal(a, b)
+
+
+def test_precomputed_fit_equals_per_call_solve():
+    from meshcfd_amd.parabolic import fit_coefficients, parabolic_stencil
+
+    mesh = meshgen.rectangle_mesh(8, 7, jitter=0.2, seed=4, split_fraction=0.3)
+    g = build_geometry(meshgen.periodic_case(mesh, NumericalSettings()))
+    src, nb, dx, dy = parabolic_stencil(g)
+    P, full = fit_coefficients(g.cell_count, src, dx, dy)
+    rng = np.random.default_rng(0)
+    prim = rng.uniform(0.5, 1.5, (g.cell_count, 4))
+    from oracle import parabolic as opar
+
+    grads, curv, _ = opar.parabolic_reconstruction(prim, g, (src, nb, dx, dy), 0)
+    d = prim[nb] - prim[src]
+    coef = np.zeros((g.cell_count, 5, 4))
+    np.add.at(coef, src, P[:, :, None] * d[:, None, :])
+    np.testing.assert_allclose(coef[full, 0, :], grads[full, :, 0], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(coef[full, 4, :], curv[full, :, 2], rtol=1e-9, atol=1e-9)
