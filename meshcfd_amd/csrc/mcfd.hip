@@ -750,12 +750,17 @@ __device__ __forceinline__ P4 rec_lds(const double* sW, const double* sG, const 
   return o;
 }
 
+// Instantiated per flux scheme, reconstruction, far-field presence and
+// next-step dt so that each launch carries only the code it runs (the full
+// kernel is ~48 KB, most of a CU pair's instruction cache).
+template <int SCHEME, bool MUSCL, int LIM, bool FARFIELD, bool DTNEXT>
 __global__ void __launch_bounds__(kBlock)
     k_stage_fused(int block_cells, int max_local, int max_grad, int max_faces, BlockArrays ba,
-                  GeomArrays ga, FaceArrays fa, DtArrays da, int scheme, int muscl_i, int limiter,
-                  double g, double ax, double ay, StageArgs s, int* __restrict__ status) {
+                  GeomArrays ga, FaceArrays fa, DtArrays da, double g, double ax, double ay,
+                  StageArgs s, int* __restrict__ status) {
+  constexpr int limiter = LIM;
   extern __shared__ double smem[];
-  const bool muscl = muscl_i != 0;
+  constexpr bool muscl = MUSCL;
   const int b = xcd_block();
   const int* inf = ba.info + 8 * b;
   const int cs = inf[0], n_own = inf[1], n_r1 = inf[2], n_r2 = inf[3];
@@ -792,7 +797,7 @@ __global__ void __launch_bounds__(kBlock)
   __syncthreads();
 
   // P1: LSQ gradient (_kernels.py:252-262) + limiter (_kernels.py:9-67)
-  if (muscl) {
+  if constexpr (MUSCL) {
     for (int j = threadIdx.x; j < n_grad; j += blockDim.x) {
       int c = ba.cells[cs + j];
       double wv[4] = {sW[4 * j], sW[4 * j + 1], sW[4 * j + 2], sW[4 * j + 3]};
@@ -889,19 +894,19 @@ __global__ void __launch_bounds__(kBlock)
       } else if (kind == MCFD_FACE_PRESSURE_OUTLET) {
         R = L;
         R.p = st[3];
-      } else if (kind == MCFD_FACE_FAR_FIELD) {
+      } else if (FARFIELD && kind == MCFD_FACE_FAR_FIELD) {
         R = far_field(L, P4{st[0], st[1], st[2], st[3]}, nx, ny, g, &err);
       } else {
         R = L;
       }
     }
     F4 fl;
-    if (scheme == 1) {
+    if constexpr (SCHEME == 1) {
       fl = hllc(L, R, nx, ny, g, kind == MCFD_FACE_INTERIOR);
       if (kind != MCFD_FACE_INTERIOR &&
           !(finite(fl.a) && finite(fl.b) && finite(fl.c) && finite(fl.d)))
         err = MCFD_ERR_HLLC;
-    } else if (scheme == 2) {
+    } else if constexpr (SCHEME == 2) {
       fl = roe(L, R, nx, ny, g, &err);
       if (!err && !(finite(fl.a) && finite(fl.b) && finite(fl.c) && finite(fl.d)))
         err = MCFD_ERR_ROE;
@@ -993,7 +998,7 @@ __global__ void __launch_bounds__(kBlock)
       o3 = s.c0 * z.p + s.c1 * y.p + k2 * q.p + s.c3 * u.p + k4 * r3;
     }
     store4(s.out, i, o0, o1, o2, o3);
-    if (s.dt_next) {
+    if (DTNEXT && s.dt_next) {
       // next step's stable step from the updated state (euler.py:20-40 + solver.py:4132-4343)
       double rho = o0;
       double uu = o1 / rho;
@@ -1006,7 +1011,7 @@ __global__ void __launch_bounds__(kBlock)
       dt_local = fmin(dt_local, dt);
     }
   }
-  if (s.dt_next) {
+  if (DTNEXT && s.dt_next) {
     __syncthreads();  // every thread is done with the flux slots: reuse them
     double* red = sF;
     red[threadIdx.x] = dt_local;
@@ -2030,6 +2035,7 @@ struct McfdMesh {
   double* FLUX = nullptr;
   double* partial = nullptr;
   int* status = nullptr;
+  bool has_far_field = false;
   // fused-stage blocks
   bool fused = false;
   int nb = 0, block_cells = 0, max_local = 0, max_grad = 0, max_faces = 0;
@@ -2211,6 +2217,49 @@ int check_settings(const McfdSettings* s) {
     return fail("Unsupported reconstruction code for the GPU residual.");
   if (s->limiter < 0 || s->limiter > 3) return fail("Unsupported limiter code.");
   return 0;
+}
+
+
+template <int SCHEME, bool MUSCL, int LIM, bool FARFIELD, bool DTNEXT>
+void launch_fused_t(McfdMesh* m, size_t lds, hipStream_t s, const GeomArrays& ga,
+                    const McfdSettings* st, const StageArgs& a) {
+  auto kern = k_stage_fused<SCHEME, MUSCL, LIM, FARFIELD, DTNEXT>;
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024);
+  hipLaunchKernelGGL(kern, dim3(m->nb), dim3(kBlock), lds, s, m->block_cells, m->max_local,
+                     m->max_grad, m->max_faces, m->blocks, ga, m->faces, m->dts, st->gamma,
+                     st->acceleration_x, st->acceleration_y, a, m->status);
+}
+
+template <int SCHEME, bool MUSCL, int LIM>
+void launch_fused_l(McfdMesh* m, bool far, bool dtn, size_t lds, hipStream_t s,
+                    const GeomArrays& ga, const McfdSettings* st, const StageArgs& a) {
+  if (far) {
+    if (dtn) launch_fused_t<SCHEME, MUSCL, LIM, true, true>(m, lds, s, ga, st, a);
+    else launch_fused_t<SCHEME, MUSCL, LIM, true, false>(m, lds, s, ga, st, a);
+  } else {
+    if (dtn) launch_fused_t<SCHEME, MUSCL, LIM, false, true>(m, lds, s, ga, st, a);
+    else launch_fused_t<SCHEME, MUSCL, LIM, false, false>(m, lds, s, ga, st, a);
+  }
+}
+
+template <int SCHEME>
+void launch_fused_s(McfdMesh* m, bool muscl, int lim, bool far, bool dtn, size_t lds,
+                    hipStream_t s, const GeomArrays& ga, const McfdSettings* st,
+                    const StageArgs& a) {
+  if (!muscl) return launch_fused_l<SCHEME, false, 0>(m, far, dtn, lds, s, ga, st, a);
+  // Barth-Jespersen and Minmod share one limiter function
+  if (lim <= 1) return launch_fused_l<SCHEME, true, 0>(m, far, dtn, lds, s, ga, st, a);
+  if (lim == 2) return launch_fused_l<SCHEME, true, 2>(m, far, dtn, lds, s, ga, st, a);
+  return launch_fused_l<SCHEME, true, 3>(m, far, dtn, lds, s, ga, st, a);
+}
+
+void launch_fused(McfdMesh* m, int scheme, bool muscl, bool far, bool dtn, size_t lds,
+                  hipStream_t s, const GeomArrays& ga, const McfdSettings* st,
+                  const StageArgs& a) {
+  if (scheme == 1) return launch_fused_s<1>(m, muscl, st->limiter, far, dtn, lds, s, ga, st, a);
+  if (scheme == 2) return launch_fused_s<2>(m, muscl, st->limiter, far, dtn, lds, s, ga, st, a);
+  return launch_fused_s<0>(m, muscl, st->limiter, far, dtn, lds, s, ga, st, a);
 }
 
 int residual_front(McfdMesh* m, const McfdSettings* st, const double* u, hipStream_t s) {
@@ -2501,6 +2550,8 @@ int mcfd_mesh_create(const McfdMeshDesc* d, McfdMesh** out) {
   m->n_acc = d->n_acc;
   m->n_dts = d->n_dts;
   m->n_bc = d->n_bc;
+  for (int b = 0; b < d->n_bc; ++b)
+    if (d->bc_kind && d->bc_kind[b] == MCFD_FACE_FAR_FIELD) m->has_far_field = true;
   (void)hipGetDevice(&m->device);
   size_t C = d->n_cells, F = d->n_faces;
   int rc = 0;
@@ -2591,8 +2642,6 @@ int mcfd_mesh_set_blocks(McfdMesh* m, const McfdBlockDesc* d) {
     free_blocks(m);
     return fail(msg);
   }
-  HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_stage_fused),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
   m->fused = true;
   return 0;
 }
@@ -2638,10 +2687,7 @@ int mcfd_stage(McfdMesh* m, const McfdSettings* st, const McfdStage* sg, void* s
     GeomArrays ga{m->area,   m->grad_ptr, m->grad_cx,  m->grad_cy, m->lim_ptr,
                   m->lim_dx, m->lim_dy,   m->acc_ptr,  m->face_len, m->acc_coef};
     TimedLaunch t(m, s, MCFD_KERNEL_STAGE);
-    hipLaunchKernelGGL(k_stage_fused, dim3(m->nb), dim3(kBlock), lds, s, m->block_cells,
-                       m->max_local, m->max_grad, m->max_faces, m->blocks, ga, m->faces, m->dts,
-                       st->flux, muscl ? 1 : 0, st->limiter, st->gamma, st->acceleration_x,
-                       st->acceleration_y, a, m->status);
+    launch_fused(m, st->flux, muscl, m->has_far_field, a.dt_next != nullptr, lds, s, ga, st, a);
     HIP_CHECK(hipGetLastError());
     return 0;
   }
