@@ -112,7 +112,8 @@ def concat_geometries(geoms: list[Geometry]) -> tuple[Geometry, np.ndarray]:
     cell_fields = ("owners", "neighbours", "interior_owners", "interior_neighbours", "gradient_sources",
                    "gradient_neighbours", "limiter_cells")
     face_fields = ("interior_faces", "wall_faces", "symmetry_faces", "periodic_faces_a", "periodic_faces_b",
-                   "velocity_inlet_faces", "pressure_outlet_faces", "far_field_faces", "transmissive_faces")
+                   "velocity_inlet_faces", "pressure_outlet_faces", "far_field_faces", "transmissive_faces",
+                   "diffusive_boundary_faces")
     kw = {}
     for f in Geometry.__dataclass_fields__:
         if f == "domain_scale":

@@ -31,6 +31,16 @@ class SolverError(ValueError):
     """Raised when a case cannot be advanced (reference `solver.py:50-51`)."""
 
 
+BOUNDARY_CODES = {
+    "Periodic": 0,
+    "Wall": 1,
+    "Symmetry": 2,
+    "Velocity Inlet": 3,
+    "Pressure Outlet": 4,
+    "Far Field": 5,
+    "Transmissive": 6,
+}
+
 SUPPORTED_KINDS = (
     "Periodic",
     "Wall",
@@ -83,6 +93,14 @@ class Geometry:
     limiter_dx: np.ndarray
     limiter_dy: np.ndarray
     domain_scale: float
+    # viscous boundary data (solver.py:1755-1757, 1911-1946): boundary faces
+    # that are not periodic, the kind of every face's patch, and per wall face
+    # the mechanical / thermal condition
+    diffusive_boundary_faces: np.ndarray = None
+    face_kind_code: np.ndarray = None  # BOUNDARY_CODES below, -1 on interior faces
+    wall_no_slip: np.ndarray = None  # bool per face
+    wall_thermal: np.ndarray = None  # 0 none, 1 temperature, 2 heat flux, per face
+    wall_thermal_value: np.ndarray = None
 
     @property
     def cell_count(self) -> int:
@@ -378,6 +396,30 @@ def build_geometry(case: Case) -> Geometry:
         float(pts[:, 1].max() - pts[:, 1].min()),
         1e-12,
     )
+    nfaces = len(owners)
+    face_kind_code = np.full(nfaces, -1, dtype=np.int64)
+    face_kind_code[boundary] = [BOUNDARY_CODES[kind_of_patch[int(p)]] for p in patch_ids[boundary]]
+    wall_no_slip = np.zeros(nfaces, dtype=bool)
+    wall_thermal = np.zeros(nfaces, dtype=np.int64)
+    wall_value = np.zeros(nfaces)
+    if case.numerics.viscosity_model != "Inviscid":
+        for pid, bc in conditions.items():
+            if bc.kind != "Wall":
+                continue
+            faces = boundary[patch_ids[boundary] == pid]
+            if bc.mechanical_condition not in ("Slip", "No-slip"):
+                raise SolverError(f"Unsupported mechanical condition on wall '{bc.patch_name}'.")
+            if bc.thermal_condition == "Temperature":
+                value = _positive(bc.values.get("temperature", 300.0), f"Wall temperature for '{bc.patch_name}'")
+                wall_thermal[faces] = 1
+            elif bc.thermal_condition == "Heat Flux":
+                value = _finite(bc.values.get("heat_flux", 0.0), f"Wall heat flux for '{bc.patch_name}'")
+                wall_thermal[faces] = 2
+            else:
+                raise SolverError(f"Unsupported thermal condition on wall '{bc.patch_name}'.")
+            wall_no_slip[faces] = bc.mechanical_condition == "No-slip"
+            wall_value[faces] = value
+    diffusive = boundary[~is_periodic[boundary]]
     return Geometry(
         centers=centers,
         areas=areas,
@@ -418,4 +460,9 @@ def build_geometry(case: Case) -> Geometry:
         limiter_dx=lim_dx,
         limiter_dy=lim_dy,
         domain_scale=scale,
+        diffusive_boundary_faces=diffusive,
+        face_kind_code=face_kind_code,
+        wall_no_slip=wall_no_slip,
+        wall_thermal=wall_thermal,
+        wall_thermal_value=wall_value,
     )

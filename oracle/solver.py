@@ -170,6 +170,10 @@ class Oracle:
             )
         if len(getattr(g, "transmissive_faces", ())):
             owner_side(g.transmissive_faces, lambda interior, faces: interior.copy())
+        if s.viscosity_model != "Inviscid":
+            from .viscous import viscous_residual
+
+            deriv += viscous_residual(primitive, g, s, self.D)
         if s.acceleration_x != 0.0 or s.acceleration_y != 0.0:
             deriv[:, 1] += primitive[:, 0] * s.acceleration_x
             deriv[:, 2] += primitive[:, 0] * s.acceleration_y
@@ -210,6 +214,10 @@ class Oracle:
             nb = g.neighbours[inner]
             nun = primitive[nb, 1] * n[inner, 0] + primitive[nb, 2] * n[inner, 1]
             np.add.at(total, nb, (np.abs(nun) + c[nb]) * g.face_lengths[inner])
+        if s.viscosity_model != "Inviscid":
+            from .viscous import viscous_time_step_sum
+
+            total += viscous_time_step_sum(primitive, g, s)
         steps = s.cfl * g.areas / total
         if not np.isfinite(steps).all() or np.any(steps <= 0.0):
             raise OracleError("Local stable time steps must be positive and finite.")

@@ -419,3 +419,55 @@ def test_precomputed_fit_equals_per_call_solve():
     np.add.at(coef, src, P[:, :, None] * d[:, None, :])
     np.testing.assert_allclose(coef[full, 0, :], grads[full, :, 0], rtol=1e-9, atol=1e-9)
     np.testing.assert_allclose(coef[full, 4, :], curv[full, :, 2], rtol=1e-9, atol=1e-9)
+
+
+def test_sutherland_reference_air_value():
+    """Reference tests/test_solver.py:159-168: mu(273.15 K) = 1.716e-5."""
+    from oracle.viscous import dynamic_viscosity
+
+    s = NumericalSettings(viscosity_model="Sutherland")
+    assert dynamic_viscosity(np.array([273.15]), s)[0] == pytest.approx(1.716e-5)
+    assert dynamic_viscosity(np.array([400.0]), s)[0] > 1.716e-5
+
+
+def test_viscous_flux_closed_form():
+    """Reference tests/test_solver.py:187-202: Stokes stress and Fourier heat flux."""
+    from oracle.viscous import viscous_flux
+
+    grads = np.array([[[1.0, 2.0], [3.0, 4.0], [5.0, 6.0]]])
+    f = viscous_flux(grads, np.array([[0.5, -0.25]]), np.array([2.0]), np.array([3.0]), np.array([[1.0, 0.0]]))
+    div = 1.0 + 4.0
+    txx = 2.0 * (2.0 * 1.0 - 2.0 / 3.0 * div)
+    txy = 2.0 * (2.0 + 3.0)
+    np.testing.assert_allclose(f[0], (0.0, txx, txy, 0.5 * txx - 0.25 * txy + 3.0 * 5.0))
+
+
+def test_wlsq_exact_for_linear_fields():
+    from oracle.viscous import weighted_least_squares
+
+    case = _wall_case()
+    g = build_geometry(case)
+    a, b = np.array((0.3, -1.1, 2.0)), np.array((1.7, 0.4, -0.9))
+    vals = g.centers[:, :1] * a + g.centers[:, 1:2] * b
+    grads = weighted_least_squares(g.cell_count, g.gradient_sources, g.gradient_dx, g.gradient_dy,
+                                   vals[g.gradient_neighbours] - vals[g.gradient_sources])
+    np.testing.assert_allclose(grads[:, :, 0], np.broadcast_to(a, (g.cell_count, 3)), atol=1e-11)
+    np.testing.assert_allclose(grads[:, :, 1], np.broadcast_to(b, (g.cell_count, 3)), atol=1e-11)
+
+
+@pytest.mark.parametrize("model", ["Constant", "Sutherland"])
+def test_viscous_uniform_flow_and_conservation(model):
+    """Uniform periodic flow has zero viscous residual; a smooth periodic
+    field conserves mass, momentum and energy to 1e-11."""
+    mesh = meshgen.rectangle_mesh(6, 5, jitter=0.2, seed=7, split_fraction=0.3)
+    numerics = NumericalSettings(viscosity_model=model, dynamic_viscosity=0.05, flux="HLLC",
+                                 reconstruction="MUSCL linear", gas_constant=1.0)
+    g = build_geometry(meshgen.periodic_case(mesh, numerics))
+    o = Oracle(g, numerics)
+    u = euler.primitive_to_conservative(np.tile((1.0, 0.3, -0.2, 1.0), (g.cell_count, 1)), GAMMA)
+    np.testing.assert_allclose(o.residual(u), 0.0, atol=1e-12)
+    x, y = g.centers[:, 0], g.centers[:, 1]
+    prim = np.column_stack((1.0 + 0.1 * np.sin(2 * np.pi * x), 0.3 + 0.1 * np.cos(2 * np.pi * y),
+                            -0.2 + 0.05 * np.sin(2 * np.pi * (x + y)), 1.0 + 0.05 * np.cos(2 * np.pi * x)))
+    r = o.residual(euler.primitive_to_conservative(prim, GAMMA))
+    np.testing.assert_allclose((r * g.areas[:, None]).sum(0), 0.0, atol=1e-11)
