@@ -215,12 +215,13 @@ def main(argv=None) -> int:
         if tfile.exists():
             try:
                 tdata = json.loads(tfile.read_text())
-                traffic = tdata.get("kernels", {}).get(dominant)
+                # HIP kernel names: the fused stage kernel is k_stage_fused
+                traffic = tdata.get("kernels", {}).get({"stage": "stage_fused"}.get(dominant, dominant))
             except Exception:
                 traffic = None
         roofline = {
             "bound": "hbm",
-            "kernel": f"k_{dominant}",
+            "kernel": "k_stage_fused" if dominant == "stage" else f"k_{dominant}",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

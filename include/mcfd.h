@@ -56,7 +56,16 @@ enum {
   MCFD_ERR_FAR_STATE = 6,      /* "Far-field characteristics produced an invalid state." */
   MCFD_ERR_RESIDUAL = 7,       /* "Spatial residual contains non-finite values." */
   MCFD_ERR_LOCAL_DT = 8,       /* "Local stable time steps must be positive and finite." */
-  MCFD_ERR_GLOBAL_DT = 9       /* "The stable time step is non-positive or non-finite." */
+  MCFD_ERR_GLOBAL_DT = 9,      /* "The stable time step is non-positive or non-finite." */
+  MCFD_ERR_TEMPERATURE = 10,   /* "Viscous flow requires finite positive temperature." */
+  MCFD_ERR_VISCOSITY = 11      /* "Sutherland law produced an invalid viscosity." */
+};
+
+/* Molecular viscosity models (McfdSettings.viscosity), solver.py:2721-2737. */
+enum {
+  MCFD_VISCOSITY_INVISCID = 0,
+  MCFD_VISCOSITY_CONSTANT = 1,
+  MCFD_VISCOSITY_SUTHERLAND = 2
 };
 
 /*
@@ -136,11 +145,18 @@ typedef struct McfdSettings {
   int32_t flux;            /* 0 Rusanov, 1 HLLC, 2 Roe */
   int32_t reconstruction;  /* 0 first order, 1 MUSCL linear, 2 MUSCL parabolic */
   int32_t limiter;         /* 0 Barth-Jespersen, 1 Minmod, 2 Van Leer, 3 Venkatakrishnan */
-  int32_t reserved;
+  int32_t viscosity;       /* MCFD_VISCOSITY_*; non-zero needs mcfd_mesh_set_viscous */
   double gamma;
   double cfl;
   double acceleration_x;
   double acceleration_y;
+  /* laminar viscous terms (NumericalSettings fields of the same names) */
+  double gas_constant;
+  double prandtl_number;
+  double dynamic_viscosity;               /* Constant model */
+  double sutherland_reference_temperature;
+  double sutherland_reference_viscosity;
+  double sutherland_temperature;
 } McfdSettings;
 
 /* Stage update ops (McfdStage.op). h = dt (scalar dt[0], or dt[i] if dt_per_cell). */
@@ -300,6 +316,58 @@ int mcfd_norm2(McfdMesh* mesh, const double* a, double* out, void* stream);
 int mcfd_mesh_set_parabolic(McfdMesh* mesh, int32_t n_entries, const int32_t* ptr,
                             const int32_t* nb, const double* coef, const int32_t* full);
 
+/*
+ * Laminar viscous terms (host pointers, device cell and flux-face ids;
+ * copied). Replaces `_viscous_gradients`, `_viscous_residual` and the
+ * diffusive part of `_stable_time_steps` (solver.py:3615-3714, 3750-4114,
+ * 4246-4332) for the Constant and Sutherland models:
+ *
+ * - weighted least squares of (u, v, T) per cell over its LSQ entries
+ *   (grad_ptr order, weights wl_wx/wl_wy = w*dx, w*dy with
+ *   w = 1/max(dx^2+dy^2, 1e-300)) followed by one ghost entry per
+ *   non-periodic boundary face it owns (gh_* lists, the reference's
+ *   diffusive-boundary order); wl_cell = {xx, xy, yy, 1/det} per cell,
+ *   1/det = 0 where the reference's determinant test leaves the gradient 0;
+ * - per boundary record: bv_code (1 wall, 2 symmetry, 3 velocity inlet,
+ *   5 far field, 0 zero normal gradient), bv_flags (bit 0 no-slip, bits 1-2
+ *   thermal: 1 temperature, 2 heat flux), bv_geom = {dn, tx, ty, value} with
+ *   dn = (face centre - owner centroid).n and (tx, ty) = offset - dn*n;
+ * - viscous accumulation per cell: items flux_face*4 + mode, mode 0 adds
+ *   va_coef*F to the interior sum S (the divergence operator's CSR data),
+ *   mode 1 adds (F*L)/A, mode 2 adds (-(F*L))/A to 0 - S;
+ * - diffusive time-step items per cell: partner cell (-1 on boundary faces),
+ *   face length and max(distance, 1e-14), summed in the reference's order.
+ */
+typedef struct McfdViscousDesc {
+  int32_t n_ghost;
+  int32_t n_vacc;
+  int32_t n_vdt;
+  int32_t reserved;
+  const double* wl_wx;       /* [n_grad] */
+  const double* wl_wy;       /* [n_grad] */
+  const double* wl_cell;     /* [n_cells*4] */
+  const int32_t* gh_ptr;     /* [n_cells+1] */
+  const int32_t* gh_face;    /* [n_ghost] flux face */
+  const double* gh_wx;       /* [n_ghost] */
+  const double* gh_wy;       /* [n_ghost] */
+  const int32_t* bv_code;    /* [n_bc] */
+  const int32_t* bv_flags;   /* [n_bc] */
+  const double* bv_geom;     /* [n_bc*4] */
+  const int32_t* va_ptr;     /* [n_cells+1] */
+  const int32_t* va_item;    /* [n_vacc] */
+  const double* va_coef;     /* [n_vacc] */
+  const int32_t* vd_ptr;     /* [n_cells+1] */
+  const int32_t* vd_nb;      /* [n_vdt] */
+  const double* vd_len;      /* [n_vdt] */
+  const double* vd_dist;     /* [n_vdt] */
+} McfdViscousDesc;
+
+/* Upload the viscous lists and allocate their scratch (gradients, face
+ * fluxes, diffusivities); NULL frees. Afterwards residual, stage and
+ * time-step calls whose settings have viscosity != 0 add the viscous terms
+ * (stages then run the unfused kernels). */
+int mcfd_mesh_set_viscous(McfdMesh* mesh, const McfdViscousDesc* desc);
+
 /* Upload the reverse-entry map of the LSQ lists (grad_rev[e] = entry of the
  * neighbour that points back) and allocate the adjoint scratch; NULL frees. */
 int mcfd_mesh_set_adjoint(McfdMesh* mesh, const int32_t* grad_rev);
@@ -340,6 +408,7 @@ enum {
   MCFD_KERNEL_TIME_STEP = 4,
   MCFD_KERNEL_REDUCE = 5,
   MCFD_KERNEL_STAGE = 6,
+  MCFD_KERNEL_VISCOUS = 7,
   MCFD_KERNEL_KINDS = 8
 };
 

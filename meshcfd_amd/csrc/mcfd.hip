@@ -584,8 +584,8 @@ struct DtArrays {
 // faces of (|u.n| + c) L with the exterior state's radius at inlet, outlet and
 // far-field faces, then cfl * A / sum. Shared by the time-step kernel and the
 // fused final stage so both give the same bits.
-__device__ double cell_time_step(int i, const P4& w, const DtArrays& d, double g, double cfl,
-                                 int* status) {
+__device__ double cell_spectral_sum(int i, const P4& w, const DtArrays& d, double g,
+                                    int* status) {
   double c = sound(w, g);
   double sum = 0.0;
   int e0 = d.ptr[i], e1 = d.ptr[i + 1];
@@ -610,9 +610,370 @@ __device__ double cell_time_step(int i, const P4& w, const DtArrays& d, double g
     }
     sum = sum + rad * d.len[e];
   }
+  return sum;
+}
+
+__device__ double cell_time_step(int i, const P4& w, const DtArrays& d, double g, double cfl,
+                                 int* status) {
+  double sum = cell_spectral_sum(i, w, d, g, status);
   double dt = cfl * d.area[i] / sum;
   if (!finite(dt) || !(dt > 0.0)) report(status, MCFD_ERR_LOCAL_DT, i);
   return dt;
+}
+
+// ---------------------------------------------------------------------------
+// Laminar viscous terms (solver.py:2721-2737, 3615-3747, 3750-4114, 4246-4332)
+
+struct ViscParams {
+  int model;  // MCFD_VISCOSITY_*
+  double gamma, R, Pr, mu, tref, muref, S;
+};
+
+// _dynamic_viscosity (solver.py:2721-2737)
+__device__ double visc_mu(double T, const ViscParams& v, int* status, int index) {
+  if (v.model == MCFD_VISCOSITY_CONSTANT) return v.mu;
+  double mu = v.muref * pow(T / v.tref, 1.5);
+  mu = mu * ((v.tref + v.S) / (T + v.S));
+  if (!finite(mu) || !(mu > 0.0)) report(status, MCFD_ERR_VISCOSITY, index);
+  return mu;
+}
+
+// conductivity = mu * gamma * R / ((gamma - 1) * Pr)
+__device__ __forceinline__ double visc_k(double mu, const ViscParams& v) {
+  return mu * v.gamma * v.R / ((v.gamma - 1.0) * v.Pr);
+}
+
+struct ViscArrays {
+  const double* wl_wx;
+  const double* wl_wy;
+  const double* wl_cell;
+  const int* gh_ptr;
+  const int* gh_face;
+  const double* gh_wx;
+  const double* gh_wy;
+  const int* bv_code;
+  const int* bv_flags;
+  const double* bv_geom;
+};
+
+enum { kVisWall = 1, kVisSymmetry = 2, kVisInlet = 3, kVisFarField = 5 };
+
+// Exterior (u, v, T) of a velocity inlet or far-field face from the owner's
+// cell state (_viscous_inlet_states, solver.py:2740-2785).
+__device__ __forceinline__ void visc_inlet(int code, const P4& w, const double* st, double nx,
+                                           double ny, const ViscParams& v, double out[3],
+                                           int* status, int index) {
+  P4 x{st[0], st[1], st[2], st[3]};
+  if (code == kVisFarField) {
+    int err = 0;
+    x = far_field(w, x, nx, ny, v.gamma, &err);
+    if (err) report(status, err, index);
+  }
+  out[0] = x.u;
+  out[1] = x.v;
+  out[2] = x.p / (x.r * v.R);
+}
+
+// Weighted least-squares gradients of (u, v, T) with one ghost entry per
+// non-periodic boundary face (_viscous_gradients + the numba kernel
+// weighted_least_squares_gradients, _kernels.py:163-207), then the cell's
+// viscosity and conductivity. VG[i] = {du/dx, du/dy, dv/dx, dv/dy, dT/dx,
+// dT/dy, mu, k}.
+__global__ void __launch_bounds__(kBlock)
+    k_visc_gradient(int n, const double* __restrict__ W, const int* __restrict__ grad_ptr,
+                    const int* __restrict__ grad_nb, ViscArrays va, FaceArrays fa, ViscParams vp,
+                    double* __restrict__ VG, int* __restrict__ status) {
+  int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  P4 w = load4(W, i);
+  double T = w.p / (w.r * vp.R);
+  if (!finite(T) || !(T > 0.0)) report(status, MCFD_ERR_TEMPERATURE, i);
+  double val[3] = {w.u, w.v, T};
+  double rx[3] = {0.0, 0.0, 0.0}, ry[3] = {0.0, 0.0, 0.0};
+  for (int e = grad_ptr[i]; e < grad_ptr[i + 1]; ++e) {
+    P4 o = load4(W, grad_nb[e]);
+    double d[3] = {o.u - w.u, o.v - w.v, o.p / (o.r * vp.R) - T};
+    double wx = va.wl_wx[e], wy = va.wl_wy[e];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      rx[k] = rx[k] + wx * d[k];
+      ry[k] = ry[k] + wy * d[k];
+    }
+  }
+  double mu = visc_mu(T, vp, status, i);
+  double kc = visc_k(mu, vp);
+  for (int e = va.gh_ptr[i]; e < va.gh_ptr[i + 1]; ++e) {
+    int f = va.gh_face[e];
+    int b = fa.bc[f];
+    int code = va.bv_code[b];
+    double nx = fa.nx[f], ny = fa.ny[f];
+    double gh[3] = {val[0], val[1], val[2]};
+    if (code == kVisWall) {
+      int flags = va.bv_flags[b];
+      const double* bg = va.bv_geom + 4 * (long)b;
+      if (flags & 1) {
+        const double* wv = fa.bc_wall + 2 * (long)b;
+        gh[0] = 2.0 * wv[0] - gh[0];
+        gh[1] = 2.0 * wv[1] - gh[1];
+      } else {
+        double t = 2.0 * (gh[0] * nx + gh[1] * ny);
+        gh[0] = gh[0] - t * nx;
+        gh[1] = gh[1] - t * ny;
+      }
+      if (((flags >> 1) & 3) == 1)
+        gh[2] = 2.0 * bg[3] - val[2];
+      else
+        gh[2] = gh[2] + 2.0 * bg[0] * bg[3] / kc;
+    } else if (code == kVisSymmetry) {
+      double t = 2.0 * (val[0] * nx + val[1] * ny);
+      gh[0] = gh[0] - t * nx;
+      gh[1] = gh[1] - t * ny;
+    } else if (code == kVisInlet || code == kVisFarField) {
+      double x[3];
+      visc_inlet(code, w, fa.bc_state + 4 * (long)b, nx, ny, vp, x, status, i);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) gh[k] = 2.0 * x[k] - gh[k];
+    }
+    double wx = va.gh_wx[e], wy = va.gh_wy[e];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double d = gh[k] - val[k];
+      rx[k] = rx[k] + wx * d;
+      ry[k] = ry[k] + wy * d;
+    }
+  }
+  const double* m = va.wl_cell + 4 * (long)i;
+  double xx = m[0], xy = m[1], yy = m[2], inv = m[3];
+  double out[8];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    out[2 * k] = (inv != 0.0) ? (rx[k] * yy - ry[k] * xy) * inv : 0.0;
+    out[2 * k + 1] = (inv != 0.0) ? (ry[k] * xx - rx[k] * xy) * inv : 0.0;
+  }
+  out[6] = mu;
+  out[7] = kc;
+  double2* q = reinterpret_cast<double2*>(VG + 8 * (long)i);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) q[k] = make_double2(out[2 * k], out[2 * k + 1]);
+}
+
+// _viscous_flux (solver.py:3717-3747): Stokes stress and Fourier heat flux
+// through a face of unit normal n; returns {0, tx, ty, u tx + v ty + q}.
+__device__ __forceinline__ void visc_flux(const double g[6], double u, double v, double mu,
+                                          double k, double nx, double ny, double F[4]) {
+  double div = g[0] + g[3];
+  double txx = mu * (2.0 * g[0] - (2.0 / 3.0) * div);
+  double tyy = mu * (2.0 * g[3] - (2.0 / 3.0) * div);
+  double txy = mu * (g[1] + g[2]);
+  double tx = txx * nx + txy * ny;
+  double ty = txy * nx + tyy * ny;
+  double q = k * (g[4] * nx + g[5] * ny);
+  F[0] = 0.0;
+  F[1] = tx;
+  F[2] = ty;
+  F[3] = u * tx + v * ty + q;
+}
+
+// One viscous flux per flux face: face averages on interior faces and
+// periodic pairs, the reference's boundary gradient corrections on boundary
+// faces (solver.py:3822-4112).
+__global__ void __launch_bounds__(kBlock)
+    k_visc_flux(int nf, FaceArrays fa, const double* __restrict__ W,
+                const double* __restrict__ VG, ViscArrays va, ViscParams vp,
+                double* __restrict__ VF, int* __restrict__ status) {
+  int f = xcd_block() * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  int kind = fa.kind[f];
+  double nx = fa.nx[f], ny = fa.ny[f];
+  int o = fa.left[f];
+  const double* go = VG + 8 * (long)o;
+  double F[4];
+  if (kind == MCFD_FACE_INTERIOR || kind == MCFD_FACE_PERIODIC) {
+    int r = fa.right[f];
+    const double* gr = VG + 8 * (long)r;
+    double g[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) g[k] = 0.5 * (go[k] + gr[k]);
+    P4 a = load4(W, o), c = load4(W, r);
+    visc_flux(g, 0.5 * (a.u + c.u), 0.5 * (a.v + c.v), 0.5 * (go[6] + gr[6]),
+              0.5 * (go[7] + gr[7]), nx, ny, F);
+    store4(VF, f, F[0], F[1], F[2], F[3]);
+    return;
+  }
+  int b = fa.bc[f];
+  int code = va.bv_code[b];
+  const double* bg = va.bv_geom + 4 * (long)b;
+  double dn = bg[0], tx = bg[1], ty = bg[2];
+  double offx = fa.lx[f], offy = fa.ly[f];
+  P4 w = load4(W, o);
+  double T = w.p / (w.r * vp.R);
+  double g[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) g[k] = go[k];
+  double bv[3] = {w.u, w.v, T};
+  int flags = va.bv_flags[b];
+  int thermal = (flags >> 1) & 3;
+  if (code == kVisWall) {
+    if (flags & 1) {
+      const double* wv = fa.bc_wall + 2 * (long)b;
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        double nd = (wv[c] - bv[c] - (g[2 * c] * tx + g[2 * c + 1] * ty)) / dn;
+        double s = nd - (g[2 * c] * nx + g[2 * c + 1] * ny);
+        g[2 * c] = g[2 * c] + s * nx;
+        g[2 * c + 1] = g[2 * c + 1] + s * ny;
+        bv[c] = wv[c];
+      }
+    } else {
+      double un = bv[0] * nx + bv[1] * ny;
+      // boundary_values[:2] += face_gradient[:2] @ offset
+      bv[0] = bv[0] + (g[0] * offx + g[1] * offy);
+      bv[1] = bv[1] + (g[2] * offx + g[3] * offy);
+      double un2 = bv[0] * nx + bv[1] * ny;
+      bv[0] = bv[0] - un2 * nx;
+      bv[1] = bv[1] - un2 * ny;
+      // gradient of the normal velocity, n0*grad(u) + n1*grad(v)
+      double gnx = nx * g[0] + ny * g[2];
+      double gny = nx * g[1] + ny * g[3];
+      double nd = -(un + (gnx * tx + gny * ty)) / dn;
+      double vn0 = g[0] * nx + g[1] * ny;
+      double vn1 = g[2] * nx + g[3] * ny;
+      g[0] = g[0] + (nd * nx * nx - vn0 * nx);
+      g[1] = g[1] + (nd * nx * ny - vn0 * ny);
+      g[2] = g[2] + (nd * ny * nx - vn1 * nx);
+      g[3] = g[3] + (nd * ny * ny - vn1 * ny);
+    }
+    double nd;
+    if (thermal == 1) {
+      nd = (bg[3] - bv[2] - (g[4] * tx + g[5] * ty)) / dn;
+      bv[2] = bg[3];
+    } else {
+      nd = bg[3] / go[7];
+      bv[2] = bv[2] + (g[4] * offx + g[5] * offy);
+    }
+    double s = nd - (g[4] * nx + g[5] * ny);
+    g[4] = g[4] + s * nx;
+    g[5] = g[5] + s * ny;
+  } else if (code == kVisSymmetry) {
+    double un = bv[0] * nx + bv[1] * ny;
+    double dvx = -un / dn * nx, dvy = -un / dn * ny;
+    double dv[2] = {dvx, dvy};
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      double s = dv[c] - (g[2 * c] * nx + g[2 * c + 1] * ny);
+      g[2 * c] = g[2 * c] + s * nx;
+      g[2 * c + 1] = g[2 * c + 1] + s * ny;
+    }
+    double gt = g[4] * nx + g[5] * ny;
+    g[4] = g[4] - gt * nx;
+    g[5] = g[5] - gt * ny;
+    bv[0] = bv[0] - un * nx;
+    bv[1] = bv[1] - un * ny;
+  } else if (code == kVisInlet || code == kVisFarField) {
+    double x[3];
+    visc_inlet(code, w, fa.bc_state + 4 * (long)b, nx, ny, vp, x, status, o);
+    double cv[3] = {w.u, w.v, T};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      bv[c] = x[c];
+      double nd = (bv[c] - cv[c] - (g[2 * c] * tx + g[2 * c + 1] * ty)) / dn;
+      double s = nd - (g[2 * c] * nx + g[2 * c + 1] * ny);
+      g[2 * c] = g[2 * c] + s * nx;
+      g[2 * c + 1] = g[2 * c + 1] + s * ny;
+    }
+  } else {
+    // zero normal gradient: g -= outer(g @ n, n); values += g @ offset
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double gn = g[2 * c] * nx + g[2 * c + 1] * ny;
+      g[2 * c] = g[2 * c] - gn * nx;
+      g[2 * c + 1] = g[2 * c + 1] - gn * ny;
+      bv[c] = bv[c] + (g[2 * c] * offx + g[2 * c + 1] * offy);
+    }
+  }
+  double fmu = visc_mu(bv[2], vp, status, o);
+  visc_flux(g, bv[0], bv[1], fmu, visc_k(fmu, vp), nx, ny, F);
+  if (code == kVisWall && thermal == 2) F[3] = bg[3];
+  store4(VF, f, F[0], F[1], F[2], F[3]);
+}
+
+// Effective diffusivity max(4/3 mu/rho, mu/(rho min(Pr, 0.9))) per cell
+// (solver.py:4256-4259).
+__global__ void __launch_bounds__(kBlock)
+    k_visc_diffusivity(int n, const double* __restrict__ W, ViscParams vp,
+                       double* __restrict__ EFF, int* __restrict__ status) {
+  int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  P4 w = load4(W, i);
+  double T = w.p / (w.r * vp.R);
+  double mu = visc_mu(T, vp, status, i);
+  double pr = vp.Pr < 0.9 ? vp.Pr : 0.9;
+  double a = (4.0 / 3.0) * mu / w.r;
+  double b = mu / (w.r * pr);
+  EFF[i] = vmax(a, b);
+}
+
+struct ViscDt {
+  const int* ptr;
+  const int* nb;
+  const double* len;
+  const double* dist;
+  const double* eff;
+};
+
+// Diffusion sum of one cell in the reference's order (interior owners,
+// interior neighbours, periodic a, periodic b, boundary faces).
+__device__ __forceinline__ double visc_dt_sum(int i, const ViscDt& vd) {
+  double ei = vd.eff[i];
+  double sum = 0.0;
+  for (int e = vd.ptr[i]; e < vd.ptr[i + 1]; ++e) {
+    int j = vd.nb[e];
+    double fe = (j >= 0) ? vmax(ei, vd.eff[j]) : ei;
+    sum = sum + fe * vd.len[e] / vd.dist[e];
+  }
+  return sum;
+}
+
+// Viscous accumulation of one cell: 0 - S over the interior items, then the
+// periodic and boundary items (derivative -= D @ flux, np.add.at, +=).
+struct ViscAcc {
+  const int* ptr;
+  const int* item;
+  const double* coef;
+  const double* VF;
+};
+
+__device__ __forceinline__ void visc_gather(int i, const ViscAcc& v, const double* flen, double A,
+                                            double r[4]) {
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  int e = v.ptr[i], e1 = v.ptr[i + 1];
+  for (; e < e1; ++e) {
+    int item = v.item[e];
+    if ((item & 3) != 0) break;
+    P4 F = load4(v.VF, item >> 2);
+    double c = v.coef[e];
+    s[0] = s[0] + c * F.r;
+    s[1] = s[1] + c * F.u;
+    s[2] = s[2] + c * F.v;
+    s[3] = s[3] + c * F.p;
+  }
+  double d[4] = {0.0 - s[0], 0.0 - s[1], 0.0 - s[2], 0.0 - s[3]};
+  for (; e < e1; ++e) {
+    int item = v.item[e];
+    int f = item >> 2;
+    P4 F = load4(v.VF, f);
+    double L = flen[f];
+    double x[4] = {F.r * L, F.u * L, F.v * L, F.p * L};
+    if ((item & 3) == 1) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) d[k] = d[k] + x[k] / A;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) d[k] = d[k] + (-x[k]) / A;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) r[k] = r[k] + d[k];
 }
 
 // Face -> cell gather in the reference order (solver.py:2501-2704), body force
@@ -622,7 +983,7 @@ __global__ void __launch_bounds__(kBlock)
              const double* __restrict__ acoef, const double* __restrict__ FLUX,
              const double* __restrict__ flen,
              const double* __restrict__ area, const double* __restrict__ W, double ax, double ay,
-             StageArgs s, int* __restrict__ status) {
+             StageArgs s, ViscAcc vacc, int* __restrict__ status) {
   int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double A = area[i];
@@ -653,6 +1014,14 @@ __global__ void __launch_bounds__(kBlock)
       r2 = r2 + (F.v * L) / A;
       r3 = r3 + (F.p * L) / A;
     }
+  }
+  if (vacc.ptr) {  // derivative += _viscous_residual(...) (solver.py:2705-2708)
+    double r[4] = {r0, r1, r2, r3};
+    visc_gather(i, vacc, flen, A, r);
+    r0 = r[0];
+    r1 = r[1];
+    r2 = r[2];
+    r3 = r[3];
   }
   if (ax != 0.0 || ay != 0.0) {
     P4 w = load4(W, i);
@@ -1920,6 +2289,30 @@ __global__ void __launch_bounds__(kBlock)
   if (threadIdx.x == 0) partial[xcd_block()] = red[0];
 }
 
+// Viscous flow: spectral sum + 2 * diffusion sum (solver.py:4246-4332).
+__global__ void __launch_bounds__(kBlock)
+    k_time_steps_viscous(int n, const double* __restrict__ W, DtArrays d, ViscDt vd, double g,
+                         double cfl, double* __restrict__ dt_cell, double* __restrict__ partial,
+                         int* __restrict__ status) {
+  __shared__ double red[kBlock];
+  int i = xcd_block() * blockDim.x + threadIdx.x;
+  double dt = INFINITY;
+  if (i < n) {
+    double sum = cell_spectral_sum(i, load4(W, i), d, g, status);
+    sum = sum + 2.0 * visc_dt_sum(i, vd);
+    dt = cfl * d.area[i] / sum;
+    if (!finite(dt) || !(dt > 0.0)) report(status, MCFD_ERR_LOCAL_DT, i);
+    dt_cell[i] = dt;
+  }
+  red[threadIdx.x] = dt;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[xcd_block()] = red[0];
+}
+
 __global__ void __launch_bounds__(1024)
     k_min_reduce(int n, const double* __restrict__ partial, double* __restrict__ out) {
   __shared__ double red[1024];
@@ -2069,6 +2462,15 @@ struct McfdMesh {
   double* FADJ = nullptr;
   AdjArrays adj{};
   std::vector<void*> adjoint_allocations;
+  // laminar viscous terms
+  bool viscous = false;
+  ViscArrays visc{};
+  ViscAcc vacc{};
+  ViscDt vdt{};
+  double* VG = nullptr;       // [C*8] (u, v, T) gradients, mu, k
+  double* VF = nullptr;       // [n_faces*4] viscous face fluxes
+  double* EFF = nullptr;      // [C] effective diffusivity
+  std::vector<void*> viscous_allocations;
   // timing
   bool timing = false;
   std::vector<hipEvent_t> event_pool;
@@ -2178,7 +2580,49 @@ int alloc_parabolic(McfdMesh* m, T** dst, const T* src, size_t count) {
   return 0;
 }
 
+void free_viscous(McfdMesh* m) {
+  for (void* p : m->viscous_allocations) (void)hipFree(p);
+  m->viscous_allocations.clear();
+  m->viscous = false;
+  m->visc = ViscArrays{};
+  m->vacc = ViscAcc{};
+  m->vdt = ViscDt{};
+}
+
+template <typename T>
+int alloc_viscous(McfdMesh* m, const T** dst, const T* src, size_t count) {
+  void* p = nullptr;
+  HIP_CHECK(hipMalloc(&p, sizeof(T) * (count ? count : 1)));
+  m->viscous_allocations.push_back(p);
+  if (count) {
+    if (!src) return fail("mcfd_mesh_set_viscous: missing array");
+    HIP_CHECK(hipMemcpy(p, src, sizeof(T) * count, hipMemcpyHostToDevice));
+  }
+  *dst = static_cast<const T*>(p);
+  return 0;
+}
+
+int alloc_viscous_scratch(McfdMesh* m, double** dst, size_t count) {
+  void* p = nullptr;
+  HIP_CHECK(hipMalloc(&p, sizeof(double) * (count ? count : 1)));
+  m->viscous_allocations.push_back(p);
+  *dst = static_cast<double*>(p);
+  return 0;
+}
+
+ViscParams visc_params(const McfdSettings* st) {
+  return ViscParams{st->viscosity,
+                    st->gamma,
+                    st->gas_constant,
+                    st->prandtl_number,
+                    st->dynamic_viscosity,
+                    st->sutherland_reference_temperature,
+                    st->sutherland_reference_viscosity,
+                    st->sutherland_temperature};
+}
+
 void free_mesh(McfdMesh* m) {
+  free_viscous(m);
   free_parabolic(m);
   free_adjoint(m);
   free_implicit(m);
@@ -2216,6 +2660,15 @@ int check_settings(const McfdSettings* s) {
   if (s->reconstruction < 0 || s->reconstruction > 2)
     return fail("Unsupported reconstruction code for the GPU residual.");
   if (s->limiter < 0 || s->limiter > 3) return fail("Unsupported limiter code.");
+  if (s->viscosity < 0 || s->viscosity > 2) return fail("Unsupported viscosity model.");
+  if (s->viscosity && !(s->gas_constant > 0.0 && s->prandtl_number > 0.0))
+    return fail("Viscous settings need a positive gas constant and Prandtl number.");
+  return 0;
+}
+
+int check_viscous(const McfdMesh* m, const McfdSettings* s) {
+  if (s->viscosity && !m->viscous)
+    return fail("Viscous settings need mcfd_mesh_set_viscous first.");
   return 0;
 }
 
@@ -2289,6 +2742,15 @@ int residual_front(McfdMesh* m, const McfdSettings* st, const double* u, hipStre
     hipLaunchKernelGGL(k_flux, dim3(blocks(m->n_faces)), dim3(kBlock), 0, s, m->n_faces,
                        m->faces, m->W, m->G, m->PHI, para ? m->K : nullptr, st->flux,
                        muscl ? 1 : 0, st->gamma, m->FLUX, m->status);
+  }
+  if (st->viscosity) {
+    TimedLaunch t(m, s, MCFD_KERNEL_VISCOUS);
+    ViscParams vp = visc_params(st);
+    hipLaunchKernelGGL(k_visc_gradient, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W,
+                       m->grad_ptr, m->grad_nb, m->visc, m->faces, vp, m->VG, m->status);
+    if (m->n_faces)
+      hipLaunchKernelGGL(k_visc_flux, dim3(blocks(m->n_faces)), dim3(kBlock), 0, s, m->n_faces,
+                         m->faces, m->W, m->VG, m->visc, vp, m->VF, m->status);
   }
   HIP_CHECK(hipGetLastError());
   return 0;
@@ -2681,7 +3143,10 @@ int mcfd_stage(McfdMesh* m, const McfdSettings* st, const McfdStage* sg, void* s
               reinterpret_cast<unsigned long long*>(sg->dt_min), st->cfl};
   if (sg->dt_next && (!m->fused || !sg->dt_min || sg->op == MCFD_OP_RESIDUAL))
     return fail("mcfd_stage: dt_next needs the fused path, dt_min and a state update");
-  if (m->fused && st->reconstruction != 2) {
+  if (check_viscous(m, st)) return -1;
+  if (sg->dt_next && st->viscosity)
+    return fail("mcfd_stage: dt_next is not available with viscous settings");
+  if (m->fused && st->reconstruction != 2 && !st->viscosity) {
     bool muscl = st->reconstruction == 1 && m->n_grad > 0;
     size_t lds = fused_lds_bytes(m, muscl);
     GeomArrays ga{m->area,   m->grad_ptr, m->grad_cx,  m->grad_cy, m->lim_ptr,
@@ -2696,7 +3161,8 @@ int mcfd_stage(McfdMesh* m, const McfdSettings* st, const McfdStage* sg, void* s
     TimedLaunch t(m, s, MCFD_KERNEL_UPDATE);
     hipLaunchKernelGGL(k_update, dim3(blocks(m->n_cells)), dim3(kBlock), 0, s, m->n_cells,
                        m->acc_ptr, m->acc_item, m->acc_coef, m->FLUX, m->face_len, m->area, m->W,
-                       st->acceleration_x, st->acceleration_y, a, m->status);
+                       st->acceleration_x, st->acceleration_y, a,
+                       st->viscosity ? m->vacc : ViscAcc{}, m->status);
   }
   HIP_CHECK(hipGetLastError());
   return 0;
@@ -2713,7 +3179,14 @@ int mcfd_time_steps(McfdMesh* m, const McfdSettings* st, const double* u, double
     hipLaunchKernelGGL(k_primitive, dim3(blocks(n)), dim3(kBlock), 0, s, n, u, m->W, st->gamma,
                        m->status);
   }
-  {
+  if (st->viscosity) {
+    if (check_viscous(m, st)) return -1;
+    TimedLaunch t(m, s, MCFD_KERNEL_TIME_STEP);
+    hipLaunchKernelGGL(k_visc_diffusivity, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W,
+                       visc_params(st), m->EFF, m->status);
+    hipLaunchKernelGGL(k_time_steps_viscous, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W,
+                       m->dts, m->vdt, st->gamma, st->cfl, dt_cell, m->partial, m->status);
+  } else {
     TimedLaunch t(m, s, MCFD_KERNEL_TIME_STEP);
     hipLaunchKernelGGL(k_time_steps, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->dts,
                        st->gamma, st->cfl, dt_cell, m->partial, m->status);
@@ -2993,12 +3466,55 @@ int mcfd_mesh_set_parabolic(McfdMesh* m, int32_t n_entries, const int32_t* ptr, 
   return 0;
 }
 
+int mcfd_mesh_set_viscous(McfdMesh* m, const McfdViscousDesc* d) {
+  if (!m) return fail("mcfd_mesh_set_viscous: NULL mesh");
+  (void)hipDeviceSynchronize();
+  free_viscous(m);
+  if (!d) return 0;
+  if (d->n_ghost < 0 || d->n_vacc < 0 || d->n_vdt < 0)
+    return fail("mcfd_mesh_set_viscous: negative count");
+  size_t C = m->n_cells;
+  size_t B = m->n_bc;
+  ViscArrays& v = m->visc;
+  int rc = 0;
+  rc |= alloc_viscous(m, &v.wl_wx, d->wl_wx, (size_t)m->n_grad);
+  rc |= alloc_viscous(m, &v.wl_wy, d->wl_wy, (size_t)m->n_grad);
+  rc |= alloc_viscous(m, &v.wl_cell, d->wl_cell, 4 * C);
+  rc |= alloc_viscous(m, &v.gh_ptr, d->gh_ptr, C + 1);
+  rc |= alloc_viscous(m, &v.gh_face, d->gh_face, (size_t)d->n_ghost);
+  rc |= alloc_viscous(m, &v.gh_wx, d->gh_wx, (size_t)d->n_ghost);
+  rc |= alloc_viscous(m, &v.gh_wy, d->gh_wy, (size_t)d->n_ghost);
+  rc |= alloc_viscous(m, &v.bv_code, d->bv_code, B);
+  rc |= alloc_viscous(m, &v.bv_flags, d->bv_flags, B);
+  rc |= alloc_viscous(m, &v.bv_geom, d->bv_geom, 4 * B);
+  rc |= alloc_viscous(m, &m->vacc.ptr, d->va_ptr, C + 1);
+  rc |= alloc_viscous(m, &m->vacc.item, d->va_item, (size_t)d->n_vacc);
+  rc |= alloc_viscous(m, &m->vacc.coef, d->va_coef, (size_t)d->n_vacc);
+  rc |= alloc_viscous(m, &m->vdt.ptr, d->vd_ptr, C + 1);
+  rc |= alloc_viscous(m, &m->vdt.nb, d->vd_nb, (size_t)d->n_vdt);
+  rc |= alloc_viscous(m, &m->vdt.len, d->vd_len, (size_t)d->n_vdt);
+  rc |= alloc_viscous(m, &m->vdt.dist, d->vd_dist, (size_t)d->n_vdt);
+  rc |= alloc_viscous_scratch(m, &m->VG, 8 * C);
+  rc |= alloc_viscous_scratch(m, &m->VF, 4 * (size_t)m->n_faces);
+  rc |= alloc_viscous_scratch(m, &m->EFF, C);
+  if (rc) {
+    std::string msg = g_last_error;
+    free_viscous(m);
+    return fail(msg);
+  }
+  m->vacc.VF = m->VF;
+  m->vdt.eff = m->EFF;
+  m->viscous = true;
+  return 0;
+}
+
 int mcfd_residual_vjp(McfdMesh* m, const McfdSettings* st, const double* u, const double* lam,
                       double* ubar, void* stream) {
   if (!m || !u || !lam || !ubar) return fail("mcfd_residual_vjp: NULL argument");
   if (!m->adjoint) return fail("mcfd_residual_vjp: call mcfd_mesh_set_adjoint first");
   if (check_settings(st)) return -1;
   if (st->reconstruction == 2) return fail("mcfd_residual_vjp: MUSCL parabolic has no adjoint yet");
+  if (st->viscosity) return fail("mcfd_residual_vjp: viscous terms have no adjoint yet");
   hipStream_t s = static_cast<hipStream_t>(stream);
   int n = m->n_cells;
   bool muscl = st->reconstruction == 1 && m->n_grad > 0;

@@ -39,17 +39,24 @@ def native_settings(s: NumericalSettings) -> native.McfdSettings:
         raise SolverError(f"Unsupported limiter '{s.limiter}'.")
     if s.flux not in FLUX_CODES:
         raise SolverError(f"Unsupported convective flux '{s.flux}'.")
-    if s.viscosity_model != "Inviscid":
-        raise SolverError("Viscous residuals are not implemented by the GPU residual.")
+    models = {"Inviscid": 0, "Constant": 1, "Sutherland": 2}
+    if s.viscosity_model not in models:
+        raise SolverError("Unsupported viscosity model.")
     return native.McfdSettings(
         flux=FLUX_CODES[s.flux],
         reconstruction=RECONSTRUCTION_CODES[s.reconstruction],
         limiter=LIMITER_CODES[s.limiter],
-        reserved=0,
+        viscosity=models[s.viscosity_model],
         gamma=s.gamma,
         cfl=s.cfl,
         acceleration_x=s.acceleration_x,
         acceleration_y=s.acceleration_y,
+        gas_constant=s.gas_constant,
+        prandtl_number=s.prandtl_number,
+        dynamic_viscosity=s.dynamic_viscosity,
+        sutherland_reference_temperature=s.sutherland_reference_temperature,
+        sutherland_reference_viscosity=s.sutherland_reference_viscosity,
+        sutherland_temperature=s.sutherland_temperature,
     )
 
 
@@ -92,6 +99,10 @@ class DeviceMesh:
         self.native_settings = native_settings(self.settings)
         if self.settings.reconstruction == "MUSCL parabolic":
             self.set_parabolic()
+        if self.settings.viscosity_model != "Inviscid":
+            from . import viscous
+
+            viscous.upload(self.handle, self.geometry, self.layout)
         opts = dict(dtype=torch.float64, device=self.device)
         n = self.n
         self.u = torch.empty((n, 4), **opts)
@@ -342,6 +353,7 @@ class DeviceMesh:
         # the fused final stage also computes the next step's CFL steps
         fuse_dt = (
             self.fuse_time_step and self.blocks is not None and self.case_ptr is None and mode != "Global fixed"
+            and self.settings.viscosity_model == "Inviscid"
         )
         nxt = dict(dt_next=self.dt_cell, dt_min=self.dt_min) if fuse_dt else {}
         if ti == "Explicit Euler":

@@ -93,6 +93,7 @@ class DeviceLayout:
     bc_kind: np.ndarray
     bc_state: np.ndarray
     bc_wall: np.ndarray
+    mesh_face_flux: np.ndarray = None  # mesh face -> flux face (periodic b faces: their pair's)
 
     @property
     def n_faces(self) -> int:
@@ -187,6 +188,11 @@ def build_layout(g: Geometry, reorder: bool = True, order: np.ndarray | None = N
     fperm = np.argsort(new_id[left], kind="stable")  # new face -> old face
     fnew = np.empty(nf, dtype=np.int64)
     fnew[fperm] = np.arange(nf)
+
+    mesh_face_flux = np.full(len(g.owners), -1, dtype=np.int64)
+    old_mesh = np.concatenate([inter, fa] + bc_faces).astype(np.int64)
+    mesh_face_flux[old_mesh] = fnew[np.arange(nf)]
+    mesh_face_flux[fb] = fnew[n_int + np.arange(n_per)]
 
     # --- accumulation items ------------------------------------------------
     k_int = np.arange(n_int, dtype=np.int64)
@@ -290,4 +296,5 @@ def build_layout(g: Geometry, reorder: bool = True, order: np.ndarray | None = N
         bc_kind=bc_kind.astype(np.int32),
         bc_state=np.ascontiguousarray(bc_state.reshape(-1)),
         bc_wall=np.ascontiguousarray(bc_wall.reshape(-1)),
+        mesh_face_flux=mesh_face_flux,
     )

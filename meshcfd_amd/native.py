@@ -24,10 +24,12 @@ ERROR_MESSAGES = {
     7: "Spatial residual contains non-finite values.",
     8: "Local stable time steps must be positive and finite.",
     9: "The stable time step is non-positive or non-finite.",
+    10: "Viscous flow requires finite positive temperature.",
+    11: "Sutherland law produced an invalid viscosity.",
 }
 
 OP_EULER, OP_SSP, OP_RK4, OP_RK4_FINAL, OP_RESIDUAL = 0, 1, 2, 3, 4
-KERNEL_NAMES = ("primitive", "gradient", "flux", "update", "time_step", "reduce", "stage", "spare")
+KERNEL_NAMES = ("primitive", "gradient", "flux", "update", "time_step", "reduce", "stage", "viscous")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -136,11 +138,17 @@ class McfdSettings(C.Structure):
         ("flux", C.c_int32),
         ("reconstruction", C.c_int32),
         ("limiter", C.c_int32),
-        ("reserved", C.c_int32),
+        ("viscosity", C.c_int32),
         ("gamma", C.c_double),
         ("cfl", C.c_double),
         ("acceleration_x", C.c_double),
         ("acceleration_y", C.c_double),
+        ("gas_constant", C.c_double),
+        ("prandtl_number", C.c_double),
+        ("dynamic_viscosity", C.c_double),
+        ("sutherland_reference_temperature", C.c_double),
+        ("sutherland_reference_viscosity", C.c_double),
+        ("sutherland_temperature", C.c_double),
     ]
 
 

@@ -27,6 +27,7 @@ def test_header_declares_entry_points():
 def test_library_exports_every_declared_symbol():
     import meshcfd_amd.adjoint  # noqa: F401  (registers its entry points)
     import meshcfd_amd.implicit  # noqa: F401
+    import meshcfd_amd.viscous  # noqa: F401
 
     lib = native.load()
     for name in declared_symbols():
@@ -48,7 +49,10 @@ def test_ctypes_structs_match_header_layout():
 
     # 8 int32 counts + 30 pointers
     assert C.sizeof(native.McfdMeshDesc) == 8 * 4 + 30 * 8
-    assert C.sizeof(native.McfdSettings) == 4 * 4 + 4 * 8
+    assert C.sizeof(native.McfdSettings) == 4 * 4 + 10 * 8
+    from meshcfd_amd.viscous import McfdViscousDesc
+
+    assert C.sizeof(McfdViscousDesc) == 4 * 4 + 17 * 8
     assert C.sizeof(native.McfdStage) == 8 + 7 * 8 + 5 * 8 + 2 * 8
 
 
