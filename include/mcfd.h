@@ -117,9 +117,11 @@ typedef struct McfdMeshDesc {
 
 /*
  * Cell blocks for the fused stage kernel (host pointers; copied). Block b owns
- * cells [b*block_cells, min((b+1)*block_cells, n_cells)); its local cells are
- * own cells, then ring-1 and ring-2 halo cells. info[b] = {cells_start, n_own,
- * n_ring1, n_ring2, faces_start, n_faces, gnb_start, gptr_start}.
+ * the consecutive cells [own_start, own_start + n_own) (the blocks tile the
+ * mesh in order, at most block_cells cells each) and touches at most 256 flux
+ * faces; its local cells are own cells, then ring-1 and ring-2 halo cells.
+ * info[b] = {cells_start, n_own, n_ring1, n_ring2, faces_start, n_faces,
+ * gnb_start, gptr_start, own_start}.
  */
 typedef struct McfdBlockDesc {
   int32_t n_blocks;
@@ -132,7 +134,7 @@ typedef struct McfdBlockDesc {
   int32_t n_gnb;            /* length of gnb */
   int32_t n_block_faces;    /* length of faces (face_lr has twice as many) */
   int32_t reserved;
-  const int32_t* info;      /* [n_blocks*8] */
+  const int32_t* info;      /* [n_blocks*9] */
   const int32_t* cells;     /* local -> global cell */
   const int32_t* gnb_ptr;   /* per own/ring-1 local cell: first LSQ entry in gnb (block relative) */
   const uint16_t* gnb;      /* LSQ neighbours as local cell ids, reference entry order */

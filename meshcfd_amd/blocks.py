@@ -1,8 +1,9 @@
 """Cell blocks with two-ring halos for the fused stage kernel.
 
-The device mesh is cut into blocks of ``B`` consecutive cells in its
-space-filling-curve order. One workgroup owns one block and evaluates a whole
-stage for it out of LDS:
+The device mesh is cut into runs of consecutive cells in its
+space-filling-curve order, each as long as possible up to ``B`` cells while
+it touches at most 256 flux faces (one per thread of the workgroup). One
+workgroup owns one block and evaluates a whole stage for it out of LDS:
 
     local cells = [own cells (B)] + [ring 1] + [ring 2]
 
@@ -27,7 +28,51 @@ import numpy as np
 
 from .layout import DeviceLayout
 
-INFO_FIELDS = 8  # cells_start, n_own, n_ring1, n_ring2, faces_start, n_faces, gnb_start, gptr_start
+INFO_FIELDS = 9  # cells_start, n_own, n_ring1, n_ring2, faces_start, n_faces, gnb_start, gptr_start, own_start
+MAX_BLOCK_FACES = 256  # one flux face per thread of the 256-thread workgroup
+
+
+def partition(L: DeviceLayout, max_cells: int, max_faces: int = MAX_BLOCK_FACES) -> np.ndarray:
+    """Block start offsets over the device cell order: each block takes as
+    many consecutive cells as it can (at most ``max_cells``) while the flux
+    faces touching its cells stay within ``max_faces``, so the stage kernel
+    evaluates every face of a block in one pass of its threads.
+
+    A face is counted for block [s, e) when one of its cells lies in [s, e);
+    with ``key`` = its smallest cell index >= s, that is key < e, so the
+    largest admissible e is the (max_faces+1)-th smallest key."""
+    n = L.n_cells
+    fl = L.face_left.astype(np.int64)
+    fr = L.face_right.astype(np.int64)
+    nf = len(fl)
+    # incident faces per cell (CSR)
+    has_r = fr >= 0
+    ends = np.concatenate((fl, fr[has_r]))
+    fids = np.concatenate((np.arange(nf), np.flatnonzero(has_r)))
+    order = np.argsort(ends, kind="stable")
+    inc_face = fids[order]
+    inc_ptr = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(np.bincount(ends, minlength=n), out=inc_ptr[1:])
+    other = np.where(fl[inc_face] == ends[order], fr[inc_face], fl[inc_face])  # the face's other cell
+    starts = [0]
+    s = 0
+    while s < n:
+        w = min(n, s + max_cells)
+        a, b = inc_ptr[s], inc_ptr[w]
+        cell = np.repeat(np.arange(s, w), np.diff(inc_ptr[s:w + 1]))
+        oth = other[a:b]
+        # a face whose other cell lies in [s, cell) was already counted at that cell
+        first_visit = ~((oth >= s) & (oth < cell))
+        keys = cell[first_visit]
+        if len(keys) > max_faces:
+            e = int(np.sort(keys)[max_faces])
+        else:
+            e = w
+        if e <= s:
+            raise ValueError(f"cell {s} alone touches more than {max_faces} flux faces")
+        starts.append(e)
+        s = e
+    return np.asarray(starts, dtype=np.int64)
 
 
 @dataclass
@@ -59,8 +104,9 @@ def _lookup(keys_sorted: np.ndarray, values: np.ndarray, query: np.ndarray) -> n
 def build_blocks(L: DeviceLayout, block_cells: int = 128) -> BlockLayout:
     n = L.n_cells
     B = block_cells
-    nb = (n + B - 1) // B
-    blk = np.arange(n, dtype=np.int64) // B
+    bstart = partition(L, B)
+    nb = len(bstart) - 1
+    blk = np.repeat(np.arange(nb, dtype=np.int64), np.diff(bstart))
     gptr = L.grad_ptr.astype(np.int64)
     deg = np.diff(gptr)
     src = np.repeat(np.arange(n, dtype=np.int64), deg)
@@ -87,7 +133,7 @@ def build_blocks(L: DeviceLayout, block_cells: int = 128) -> BlockLayout:
     n_loc = n_own + n_r1 + n_r2
     cstart = np.concatenate(([0], np.cumsum(n_loc)[:-1]))
     # local ids: own cells 0..n_own-1, ring 1 next, ring 2 last
-    own_local = np.arange(n) - blk * B
+    own_local = np.arange(n) - bstart[blk]
     r1_rank = np.arange(len(r1)) - np.concatenate(([0], np.cumsum(n_r1)[:-1]))[r1_b]
     r2_rank = np.arange(len(r2)) - np.concatenate(([0], np.cumsum(n_r2)[:-1]))[r2_b]
     r1_local = n_own[r1_b] + r1_rank
@@ -164,6 +210,9 @@ def build_blocks(L: DeviceLayout, block_cells: int = 128) -> BlockLayout:
     info[:, 5] = nfb
     info[:, 6] = gnb_start
     info[:, 7] = gstart_cells
+    info[:, 8] = bstart[:-1]
+    if nfb.max(initial=0) > MAX_BLOCK_FACES:
+        raise ValueError("block partition left a block with too many faces")
     return BlockLayout(
         block_cells=B,
         n_blocks=nb,

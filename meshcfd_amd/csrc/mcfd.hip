@@ -42,6 +42,7 @@ int fail(const std::string& msg) {
   } while (0)
 
 constexpr int kBlock = 256;
+constexpr int kBlockInfo = 9;  // ints per fused-kernel block record
 
 struct P4 {
   double r, u, v, p;
@@ -1131,20 +1132,23 @@ __global__ void __launch_bounds__(kBlock)
   extern __shared__ double smem[];
   constexpr bool muscl = MUSCL;
   const int b = xcd_block();
-  const int* inf = ba.info + 8 * b;
+  const int* inf = ba.info + kBlockInfo * b;
   const int cs = inf[0], n_own = inf[1], n_r1 = inf[2], n_r2 = inf[3];
   const int fs = inf[4], n_faces = inf[5], gs = inf[6], gps = inf[7];
-  const int first = b * block_cells;
+  const int first = inf[8];
   const int n_loc = n_own + n_r1 + n_r2;
   const int n_grad = n_own + n_r1;
+  // LDS: W of every local cell, then one region that holds the gradients
+  // and limiter factors of own + ring-1 cells until the face fluxes are
+  // computed, and the face fluxes and lengths afterwards (and the block
+  // reduction of the next time step at the end).
   double* sW = smem;
   double* sG = sW + 4 * max_local;
-  double* sP = sG + (muscl ? 8 * max_grad : 0);
-  double* sF = sP + (muscl ? 4 * max_grad : 0);
-  // the flux slots hold at least kBlock doubles: the final stage reuses them
-  // for the block reduction of the next time step
-  double* sL = sF + (4 * max_faces > kBlock ? 4 * max_faces : kBlock);
+  double* sP = sG + 8 * max_grad;
+  double* sF = sG;
+  double* sL = sF + 4 * max_faces;
   const double* __restrict__ U = s.u;
+  (void)block_cells;
 
   // P0: primitive states (euler.py:20-40)
   for (int j = threadIdx.x; j < n_loc; j += blockDim.x) {
@@ -1236,8 +1240,13 @@ __global__ void __launch_bounds__(kBlock)
     __syncthreads();
   }
 
-  // P2: face fluxes
-  for (int f = threadIdx.x; f < n_faces; f += blockDim.x) {
+  // P2: face fluxes, one face per thread (a block touches at most kBlock
+  // faces); the results wait in registers until every thread is done with
+  // the gradient slots they overwrite.
+  F4 fl{0.0, 0.0, 0.0, 0.0};
+  double flen = 0.0;
+  const int f = threadIdx.x;
+  if (f < n_faces) {
     int k = ba.faces[fs + f];
     int l = ba.face_lr[2 * (fs + f)];
     int r = ba.face_lr[2 * (fs + f) + 1];
@@ -1269,7 +1278,6 @@ __global__ void __launch_bounds__(kBlock)
         R = L;
       }
     }
-    F4 fl;
     if constexpr (SCHEME == 1) {
       fl = hllc(L, R, nx, ny, g, kind == MCFD_FACE_INTERIOR);
       if (kind != MCFD_FACE_INTERIOR &&
@@ -1283,11 +1291,15 @@ __global__ void __launch_bounds__(kBlock)
       fl = rusanov(L, R, nx, ny, g);
     }
     if (err) report(status, err, k);
+    flen = ga.face_len[k];
+  }
+  if constexpr (MUSCL) __syncthreads();
+  if (f < n_faces) {
     sF[4 * f] = fl.a;
     sF[4 * f + 1] = fl.b;
     sF[4 * f + 2] = fl.c;
     sF[4 * f + 3] = fl.d;
-    sL[f] = ga.face_len[k];
+    sL[f] = flen;
   }
   __syncthreads();
 
@@ -2523,11 +2535,13 @@ int upload_block(McfdMesh* m, T** dst, const T* src, size_t count) {
   return 0;
 }
 
+// W of the local cells + one region shared by (gradients, limiter factors)
+// and then (face fluxes, face lengths) and the final block reduction.
 size_t fused_lds_bytes(const McfdMesh* m, bool muscl) {
-  size_t flux_slots = 4 * (size_t)m->max_faces > (size_t)kBlock ? 4 * (size_t)m->max_faces : kBlock;
-  size_t d = 4 * (size_t)m->max_local + (muscl ? 12 * (size_t)m->max_grad : 0) + flux_slots +
-             (size_t)m->max_faces;
-  return d * sizeof(double);
+  size_t region = 5 * (size_t)m->max_faces;
+  if (muscl) region = std::max(region, 12 * (size_t)m->max_grad);
+  region = std::max(region, (size_t)kBlock);
+  return (4 * (size_t)m->max_local + region) * sizeof(double);
 }
 
 void free_implicit(McfdMesh* m) {
@@ -3076,10 +3090,21 @@ int mcfd_mesh_set_blocks(McfdMesh* m, const McfdBlockDesc* d) {
   (void)hipDeviceSynchronize();
   free_blocks(m);
   if (!d) return 0;
-  if (d->n_blocks <= 0 || d->block_cells <= 0 ||
-      (long)d->n_blocks * d->block_cells < m->n_cells ||
-      (long)(d->n_blocks - 1) * d->block_cells >= m->n_cells)
-    return fail("mcfd_mesh_set_blocks: block count does not cover the mesh");
+  if (d->n_blocks <= 0 || d->block_cells <= 0 || !d->info)
+    return fail("mcfd_mesh_set_blocks: no blocks");
+  {
+    // own cells of block b are [info[b][8], info[b][8] + n_own) and tile the mesh;
+    // a block touches at most kBlock flux faces (one per thread)
+    long next = 0;
+    for (int b = 0; b < d->n_blocks; ++b) {
+      const int32_t* inf = d->info + (size_t)kBlockInfo * b;
+      if (inf[8] != next || inf[1] <= 0 || inf[5] < 0 || inf[5] > kBlock)
+        return fail("mcfd_mesh_set_blocks: blocks do not tile the mesh or touch too many faces");
+      next += inf[1];
+    }
+    if (next != m->n_cells) return fail("mcfd_mesh_set_blocks: blocks do not cover the mesh");
+    if (d->max_faces > kBlock) return fail("mcfd_mesh_set_blocks: max_faces above the block size");
+  }
   m->nb = d->n_blocks;
   m->block_cells = d->block_cells;
   m->max_local = d->max_local;
@@ -3091,7 +3116,7 @@ int mcfd_mesh_set_blocks(McfdMesh* m, const McfdBlockDesc* d) {
   }
   int rc = 0;
   BlockArrays& b = m->blocks;
-  rc |= upload_block(m, const_cast<int**>(&b.info), d->info, 8 * (size_t)d->n_blocks);
+  rc |= upload_block(m, const_cast<int**>(&b.info), d->info, kBlockInfo * (size_t)d->n_blocks);
   rc |= upload_block(m, const_cast<int**>(&b.cells), d->cells, (size_t)d->n_local);
   rc |= upload_block(m, const_cast<int**>(&b.gnb_ptr), d->gnb_ptr, (size_t)d->n_gnb_ptr);
   rc |= upload_block(m, const_cast<unsigned short**>(&b.gnb), d->gnb, (size_t)d->n_gnb);
