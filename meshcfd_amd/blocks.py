@@ -104,7 +104,9 @@ def _lookup(keys_sorted: np.ndarray, values: np.ndarray, query: np.ndarray) -> n
 def build_blocks(L: DeviceLayout, block_cells: int = 128) -> BlockLayout:
     n = L.n_cells
     B = block_cells
-    bstart = partition(L, B)
+    if B < 1:
+        raise ValueError("block_cells must be positive")
+    bstart = partition(L, min(B, MAX_BLOCK_FACES))
     nb = len(bstart) - 1
     blk = np.repeat(np.arange(nb, dtype=np.int64), np.diff(bstart))
     gptr = L.grad_ptr.astype(np.int64)
@@ -213,6 +215,8 @@ def build_blocks(L: DeviceLayout, block_cells: int = 128) -> BlockLayout:
     info[:, 8] = bstart[:-1]
     if nfb.max(initial=0) > MAX_BLOCK_FACES:
         raise ValueError("block partition left a block with too many faces")
+    if n_gcells.max(initial=0) > MAX_BLOCK_FACES:
+        raise ValueError("a block has more than 256 own + ring-1 cells; use smaller blocks")
     return BlockLayout(
         block_cells=B,
         n_blocks=nb,

@@ -1149,10 +1149,31 @@ __global__ void __launch_bounds__(kBlock)
   double* sL = sF + 4 * max_faces;
   const double* __restrict__ U = s.u;
   (void)block_cells;
+  const int t = threadIdx.x;
+
+  // Index loads of the later phases are issued now, so that their latency
+  // overlaps P0 instead of adding a round trip to each phase. Thread t works
+  // on gradient cell t in P1, face t in P2 and own cell t in P3 (blocks have
+  // at most kBlock of each).
+  int e0 = 0, e1 = 0, l0 = 0, l1 = 0, gnb0 = 0;
+  if (MUSCL && t < n_grad) {
+    int c = (t < n_own) ? first + t : ba.cells[cs + t];
+    e0 = ga.grad_ptr[c];
+    e1 = ga.grad_ptr[c + 1];
+    l0 = ga.lim_ptr[c];
+    l1 = ga.lim_ptr[c + 1];
+    gnb0 = ba.gnb_ptr[gps + t];
+  }
+  int fk = 0, f_l = 0, f_r = 0;
+  if (t < n_faces) {
+    fk = ba.faces[fs + t];
+    f_l = ba.face_lr[2 * (fs + t)];
+    f_r = ba.face_lr[2 * (fs + t) + 1];
+  }
 
   // P0: primitive states (euler.py:20-40)
-  for (int j = threadIdx.x; j < n_loc; j += blockDim.x) {
-    int c = ba.cells[cs + j];
+  for (int j = t; j < n_loc; j += kBlock) {
+    int c = (j < n_own) ? first + j : ba.cells[cs + j];
     P4 q = load4(U, c);
     double rho = q.r;
     double u = q.u / rho;
@@ -1171,16 +1192,14 @@ __global__ void __launch_bounds__(kBlock)
 
   // P1: LSQ gradient (_kernels.py:252-262) + limiter (_kernels.py:9-67)
   if constexpr (MUSCL) {
-    for (int j = threadIdx.x; j < n_grad; j += blockDim.x) {
-      int c = ba.cells[cs + j];
+    if (t < n_grad) {
+      const int j = t;
       double wv[4] = {sW[4 * j], sW[4 * j + 1], sW[4 * j + 2], sW[4 * j + 3]};
       double gx[4] = {0.0, 0.0, 0.0, 0.0};
       double gy[4] = {0.0, 0.0, 0.0, 0.0};
       double mn[4] = {wv[0], wv[1], wv[2], wv[3]};
       double mx[4] = {wv[0], wv[1], wv[2], wv[3]};
-      int e0 = ga.grad_ptr[c], e1 = ga.grad_ptr[c + 1];
-      int l0 = ga.lim_ptr[c], l1 = ga.lim_ptr[c + 1];
-      const unsigned short* nbl = ba.gnb + gs + ba.gnb_ptr[gps + j] - e0;
+      const unsigned short* nbl = ba.gnb + gs + gnb0 - e0;
       // entries in batches of 4: the batch's loads are issued together, the
       // sums still run in entry order
       for (int base = e0; base < e1; base += 4) {
@@ -1245,11 +1264,9 @@ __global__ void __launch_bounds__(kBlock)
   // the gradient slots they overwrite.
   F4 fl{0.0, 0.0, 0.0, 0.0};
   double flen = 0.0;
-  const int f = threadIdx.x;
+  const int f = t;
   if (f < n_faces) {
-    int k = ba.faces[fs + f];
-    int l = ba.face_lr[2 * (fs + f)];
-    int r = ba.face_lr[2 * (fs + f) + 1];
+    const int k = fk, l = f_l, r = f_r;
     int kind = fa.kind[k];
     double nx = fa.nx[k], ny = fa.ny[k];
     P4 L = rec_lds(sW, sG, sP, l, fa.lx[k], fa.ly[k], muscl);
@@ -1293,6 +1310,21 @@ __global__ void __launch_bounds__(kBlock)
     if (err) report(status, err, k);
     flen = ga.face_len[k];
   }
+  // P3's per-cell loads, issued before the barrier
+  int a0 = 0, a1 = 0;
+  double A = 1.0, h = 0.0;
+  P4 u{0.0, 0.0, 0.0, 0.0}, z{0.0, 0.0, 0.0, 0.0};
+  if (t < n_own) {
+    const int i = first + t;
+    a0 = ga.acc_ptr[i];
+    a1 = ga.acc_ptr[i + 1];
+    A = ga.area[i];
+    if (s.op != MCFD_OP_RESIDUAL) {
+      h = s.dt_per_cell ? s.dt[i] : s.dt[0];
+      u = load4(U, i);
+      if (s.op != MCFD_OP_EULER) z = load4(s.u0, i);
+    }
+  }
   if constexpr (MUSCL) __syncthreads();
   if (f < n_faces) {
     sF[4 * f] = fl.a;
@@ -1305,12 +1337,11 @@ __global__ void __launch_bounds__(kBlock)
 
   // P3: gather (solver.py:2501-2704) + body force + stage update
   double dt_local = INFINITY;
-  for (int j = threadIdx.x; j < n_own; j += blockDim.x) {
-    int i = first + j;
-    double A = ga.area[i];
+  if (t < n_own) {
+    const int j = t;
+    const int i = first + j;
     double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
-    int e0 = ga.acc_ptr[i], e1 = ga.acc_ptr[i + 1];
-    for (int e = e0; e < e1; ++e) {
+    for (int e = a0; e < a1; ++e) {
       int item = ba.acc_local[e];
       int slot = item >> 2;
       int mode = item & 3;
@@ -1345,10 +1376,7 @@ __global__ void __launch_bounds__(kBlock)
     if (!(finite(r0) && finite(r1) && finite(r2) && finite(r3)))
       report(status, MCFD_ERR_RESIDUAL, i);
     if (s.r_out) store4(s.r_out, i, r0, r1, r2, r3);
-    if (s.op == MCFD_OP_RESIDUAL) continue;
-    double h = s.dt_per_cell ? s.dt[i] : s.dt[0];
-    P4 u = load4(U, i);
-    double o0, o1, o2, o3;
+    double o0 = 0.0, o1 = 0.0, o2 = 0.0, o3 = 0.0;
     if (s.op == MCFD_OP_EULER) {
       double k = s.c0 * h;
       o0 = u.r + k * r0;
@@ -1356,20 +1384,17 @@ __global__ void __launch_bounds__(kBlock)
       o2 = u.v + k * r2;
       o3 = u.p + k * r3;
     } else if (s.op == MCFD_OP_SSP) {
-      P4 z = load4(s.u0, i);
       o0 = s.c0 * z.r + s.c1 * (u.r + h * r0);
       o1 = s.c0 * z.u + s.c1 * (u.u + h * r1);
       o2 = s.c0 * z.v + s.c1 * (u.v + h * r2);
       o3 = s.c0 * z.p + s.c1 * (u.p + h * r3);
     } else if (s.op == MCFD_OP_RK4) {
-      P4 z = load4(s.u0, i);
       double k = s.c2 * h;
       o0 = s.c0 * z.r + s.c1 * u.r + k * r0;
       o1 = s.c0 * z.u + s.c1 * u.u + k * r1;
       o2 = s.c0 * z.v + s.c1 * u.v + k * r2;
       o3 = s.c0 * z.p + s.c1 * u.p + k * r3;
-    } else {
-      P4 z = load4(s.u0, i);
+    } else if (s.op == MCFD_OP_RK4_FINAL) {
       P4 y = load4(s.u1, i);
       P4 q = load4(s.r1, i);
       double k2 = s.c2 * h, k4 = s.c4 * h;
@@ -1378,8 +1403,8 @@ __global__ void __launch_bounds__(kBlock)
       o2 = s.c0 * z.v + s.c1 * y.v + k2 * q.v + s.c3 * u.v + k4 * r2;
       o3 = s.c0 * z.p + s.c1 * y.p + k2 * q.p + s.c3 * u.p + k4 * r3;
     }
-    store4(s.out, i, o0, o1, o2, o3);
-    if (DTNEXT && s.dt_next) {
+    if (s.op != MCFD_OP_RESIDUAL) store4(s.out, i, o0, o1, o2, o3);
+    if (DTNEXT && s.dt_next && s.op != MCFD_OP_RESIDUAL) {
       // next step's stable step from the updated state (euler.py:20-40 + solver.py:4132-4343)
       double rho = o0;
       double uu = o1 / rho;
@@ -3094,11 +3119,13 @@ int mcfd_mesh_set_blocks(McfdMesh* m, const McfdBlockDesc* d) {
     return fail("mcfd_mesh_set_blocks: no blocks");
   {
     // own cells of block b are [info[b][8], info[b][8] + n_own) and tile the mesh;
-    // a block touches at most kBlock flux faces (one per thread)
+    // a block has at most kBlock own + ring-1 cells and touches at most kBlock
+    // flux faces (one per thread in each phase)
     long next = 0;
     for (int b = 0; b < d->n_blocks; ++b) {
       const int32_t* inf = d->info + (size_t)kBlockInfo * b;
-      if (inf[8] != next || inf[1] <= 0 || inf[5] < 0 || inf[5] > kBlock)
+      if (inf[8] != next || inf[1] <= 0 || inf[2] < 0 || inf[1] + inf[2] > kBlock || inf[5] < 0 ||
+          inf[5] > kBlock)
         return fail("mcfd_mesh_set_blocks: blocks do not tile the mesh or touch too many faces");
       next += inf[1];
     }
