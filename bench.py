@@ -123,6 +123,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--unfused", action="store_true", help="four-kernel path instead of the fused stage kernel")
     ap.add_argument("--block-cells", type=int, default=128, help="cells per workgroup of the fused stage kernel")
+    ap.add_argument("--separate-time-step", action="store_true",
+                    help="compute the CFL steps with their own kernels instead of inside the final fused stage")
     ap.add_argument(
         "--copy-probe",
         type=int,
@@ -168,7 +170,8 @@ def main(argv=None) -> int:
     print(f"[bench] setup {setup_s:.1f}s: {layout.n_cells} cells, {layout.n_faces} flux faces", file=sys.stderr, flush=True)
 
     dm = DeviceMesh(
-        case, "cuda:0", geometry=geometry, layout=layout, fused=not args.unfused, block_cells=args.block_cells
+        case, "cuda:0", geometry=geometry, layout=layout, fused=not args.unfused, block_cells=args.block_cells,
+        fuse_time_step=not args.separate_time_step,
     )
     gamma = case.numerics.gamma
     rho, u, v, p = primitive.T
@@ -255,7 +258,10 @@ def main(argv=None) -> int:
             "cells": cells,
             "flux_faces": layout.n_faces,
             "parallelism": "single GPU",
-            "path": "four-kernel" if args.unfused else f"fused stage kernel, {args.block_cells}-cell blocks",
+            "path": "four-kernel" if args.unfused else (
+                f"fused stage kernel, blocks of <= {args.block_cells} cells and <= 256 flux faces"
+                + ("" if args.separate_time_step else ", next step's CFL steps inside the final stage")
+            ),
         },
         "roofline": roofline,
         "stage": {

@@ -141,6 +141,9 @@ typedef struct McfdBlockDesc {
   const int32_t* faces;     /* local face -> flux face */
   const uint16_t* face_lr;  /* [n_block_faces*2] local left/right cell (0xFFFF: none) */
   const uint16_t* acc_local;/* [n_acc] accumulation items as local face slot*4 + mode */
+  const uint16_t* dts_local;/* [n_dts] spectral-radius items as local face slot, | 0x8000 when
+                               the item's normal/length/exterior state is read from the
+                               global item (periodic b sides, inlet/outlet/far-field faces) */
 } McfdBlockDesc;
 
 typedef struct McfdSettings {
@@ -181,9 +184,10 @@ typedef struct McfdStage {
   double* out;           /* (C,4) updated state, may be NULL for MCFD_OP_RESIDUAL */
   double* r_out;         /* (C,4) residual copy, or NULL */
   double c[5];
-  /* Fused path only: also compute the next step's local stable steps from
-   * the updated state into dt_next [C], and their minimum into dt_min[0]
-   * (device; must hold +inf or a larger value beforehand). Or NULL. */
+  /* Fused path only: with dt_min set, also compute the next step's local
+   * stable steps from the updated state, their minimum into dt_min[0]
+   * (device) and, when dt_next is set, the steps themselves into
+   * dt_next [C]. Or both NULL. */
   double* dt_next;
   double* dt_min;
 } McfdStage;

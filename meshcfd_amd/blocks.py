@@ -86,6 +86,7 @@ class BlockLayout:
     faces: np.ndarray  # int32 global flux-face ids, per block
     face_lr: np.ndarray  # uint16 (n, 2) local left / right ids (0xFFFF: boundary)
     acc_local: np.ndarray  # uint16, per global acc item: local face slot * 4 + mode
+    dts_local: np.ndarray  # uint16, per global spectral-radius item: local face slot | 0x8000 (global item)
     max_local: int
     max_faces: int
     max_grad_cells: int
@@ -202,6 +203,14 @@ def build_blocks(L: DeviceLayout, block_cells: int = 128) -> BlockLayout:
     if aslot.max(initial=0) >= (1 << 14):
         raise ValueError("too many faces per block for 14-bit slots")
     acc_local = (aslot << 2) | (item & 3)
+    # spectral-radius items -> local slots (bit 15: the item's own normal,
+    # length or exterior state is read from the global item arrays)
+    dptr = L.dts_ptr.astype(np.int64)
+    dcell = np.repeat(np.arange(n), np.diff(dptr))
+    dslot = _lookup(fkeys_s, slots_s, blk[dcell] * nf + L.dts_flux.astype(np.int64))
+    if dslot.max(initial=0) >= (1 << 15):
+        raise ValueError("too many faces per block for 15-bit slots")
+    dts_local = dslot | np.where(L.dts_own, 0, 1 << 15)
 
     info = np.zeros((nb, INFO_FIELDS), dtype=np.int64)
     info[:, 0] = cstart
@@ -227,6 +236,7 @@ def build_blocks(L: DeviceLayout, block_cells: int = 128) -> BlockLayout:
         faces=pf_k.astype(np.int32),
         face_lr=lr.astype(np.uint16),
         acc_local=acc_local.astype(np.uint16),
+        dts_local=dts_local.astype(np.uint16),
         max_local=int(n_loc.max()),
         max_faces=int(nfb.max()),
         max_grad_cells=int(n_gcells.max()),

@@ -565,7 +565,7 @@ struct StageArgs {
   double* r_out;
   double c0, c1, c2, c3, c4;
   double* dt_next;                 // per-cell stable step of the updated state, or NULL
-  unsigned long long* dt_min_bits; // running minimum of dt_next (positive doubles order as integers)
+  double* dt_partial;              // per-block minimum of the next step's dt (fused final stage), or NULL
   double cfl;
 };
 
@@ -977,6 +977,52 @@ __device__ __forceinline__ void visc_gather(int i, const ViscAcc& v, const doubl
   for (int k = 0; k < 4; ++k) r[k] = r[k] + d[k];
 }
 
+// cell_time_step for the fused final stage: the normals and lengths of the
+// cell's faces come from the block's LDS slots; items flagged 0x8000 (periodic
+// b sides and faces with an exterior state) use the global item as above.
+__device__ double block_time_step(int i, const P4& w, int e0, int e1,
+                                  const unsigned short* __restrict__ dloc,
+                                  const double* sN, const double* sL, const DtArrays& d, double g,
+                                  double cfl, int* status) {
+  double c = sound(w, g);
+  double sum = 0.0;
+  for (int e = e0; e < e1; ++e) {
+    int item = dloc[e];
+    double nx, ny, len;
+    int b = -1;
+    if (item & 0x8000) {
+      nx = d.nx[e];
+      ny = d.ny[e];
+      len = d.len[e];
+      b = d.bc[e];
+    } else {
+      nx = sN[2 * item];
+      ny = sN[2 * item + 1];
+      len = sL[item];
+    }
+    double rad = fabs(w.u * nx + w.v * ny) + c;
+    if (b >= 0) {
+      int kind = d.bc_kind[b];
+      const double* st = d.bc_state + 4 * (long)b;
+      P4 x = w;
+      int err = 0;
+      if (kind == MCFD_FACE_VELOCITY_INLET)
+        x = P4{st[0], st[1], st[2], st[3]};
+      else if (kind == MCFD_FACE_PRESSURE_OUTLET)
+        x.p = st[3];
+      else if (kind == MCFD_FACE_FAR_FIELD)
+        x = far_field(w, P4{st[0], st[1], st[2], st[3]}, nx, ny, g, &err);
+      if (err) report(status, err, i);
+      double xr = fabs(x.u * nx + x.v * ny) + sound(x, g);
+      rad = vmax(rad, xr);
+    }
+    sum = sum + rad * len;
+  }
+  double dt = cfl * d.area[i] / sum;
+  if (!finite(dt) || !(dt > 0.0)) report(status, MCFD_ERR_LOCAL_DT, i);
+  return dt;
+}
+
 // Face -> cell gather in the reference order (solver.py:2501-2704), body force
 // (solver.py:2709-2715), finiteness check (:2716-2717), then the stage update.
 __global__ void __launch_bounds__(kBlock)
@@ -1087,6 +1133,7 @@ struct BlockArrays {
   const int* faces;         // local face -> global flux face
   const unsigned short* face_lr;
   const unsigned short* acc_local;
+  const unsigned short* dts_local;
 };
 
 struct GeomArrays {
@@ -1147,6 +1194,7 @@ __global__ void __launch_bounds__(kBlock)
   double* sP = sG + 8 * max_grad;
   double* sF = sG;
   double* sL = sF + 4 * max_faces;
+  double* sN = sL + max_faces;  // face normals (final stage with the next step's dt)
   const double* __restrict__ U = s.u;
   (void)block_cells;
   const int t = threadIdx.x;
@@ -1311,7 +1359,7 @@ __global__ void __launch_bounds__(kBlock)
     flen = ga.face_len[k];
   }
   // P3's per-cell loads, issued before the barrier
-  int a0 = 0, a1 = 0;
+  int a0 = 0, a1 = 0, d0 = 0, d1 = 0;
   double A = 1.0, h = 0.0;
   P4 u{0.0, 0.0, 0.0, 0.0}, z{0.0, 0.0, 0.0, 0.0};
   if (t < n_own) {
@@ -1319,6 +1367,10 @@ __global__ void __launch_bounds__(kBlock)
     a0 = ga.acc_ptr[i];
     a1 = ga.acc_ptr[i + 1];
     A = ga.area[i];
+    if (DTNEXT) {
+      d0 = da.ptr[i];
+      d1 = da.ptr[i + 1];
+    }
     if (s.op != MCFD_OP_RESIDUAL) {
       h = s.dt_per_cell ? s.dt[i] : s.dt[0];
       u = load4(U, i);
@@ -1332,6 +1384,10 @@ __global__ void __launch_bounds__(kBlock)
     sF[4 * f + 2] = fl.c;
     sF[4 * f + 3] = fl.d;
     sL[f] = flen;
+    if (DTNEXT) {
+      sN[2 * f] = fa.nx[fk];
+      sN[2 * f + 1] = fa.ny[fk];
+    }
   }
   __syncthreads();
 
@@ -1404,7 +1460,7 @@ __global__ void __launch_bounds__(kBlock)
       o3 = s.c0 * z.p + s.c1 * y.p + k2 * q.p + s.c3 * u.p + k4 * r3;
     }
     if (s.op != MCFD_OP_RESIDUAL) store4(s.out, i, o0, o1, o2, o3);
-    if (DTNEXT && s.dt_next && s.op != MCFD_OP_RESIDUAL) {
+    if (DTNEXT && s.op != MCFD_OP_RESIDUAL) {
       // next step's stable step from the updated state (euler.py:20-40 + solver.py:4132-4343)
       double rho = o0;
       double uu = o1 / rho;
@@ -1412,22 +1468,27 @@ __global__ void __launch_bounds__(kBlock)
       double p = (g - 1.0) * (o3 - 0.5 * (o1 * o1 + o2 * o2) / rho);
       if (!finite(rho) || !finite(uu) || !finite(vv) || !finite(p) || !(rho > 0.0) || !(p > 0.0))
         report(status, MCFD_ERR_STATE, i);
-      double dt = cell_time_step(i, P4{rho, uu, vv, p}, da, g, s.cfl, status);
-      s.dt_next[i] = dt;
+      double dt = block_time_step(i, P4{rho, uu, vv, p}, d0, d1, ba.dts_local, sN, sL, da, g,
+                                  s.cfl, status);
+      if (s.dt_next) s.dt_next[i] = dt;
       dt_local = fmin(dt_local, dt);
     }
   }
-  if (DTNEXT && s.dt_next) {
-    __syncthreads();  // every thread is done with the flux slots: reuse them
-    double* red = sF;
-    red[threadIdx.x] = dt_local;
+  if (DTNEXT) {
+    // block minimum (wave minimum by lane exchanges, then over the four
+    // waves) into the per-block partial; k_min_reduce folds the partials
+    __shared__ double wave_min[kBlock / 64];
+    double v = dt_local;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
+    if ((t & 63) == 0) wave_min[t >> 6] = v;
     __syncthreads();
-    for (int k = blockDim.x / 2; k > 0; k >>= 1) {
-      if (threadIdx.x < k) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + k]);
-      __syncthreads();
+    if (t == 0) {
+      double m = wave_min[0];
+#pragma unroll
+      for (int w = 1; w < kBlock / 64; ++w) m = fmin(m, wave_min[w]);
+      s.dt_partial[b] = m;
     }
-    if (threadIdx.x == 0 && red[0] > 0.0 && red[0] < INFINITY)
-      atomicMin(s.dt_min_bits, (unsigned long long)__double_as_longlong(red[0]));
   }
 }
 
@@ -2469,6 +2530,7 @@ struct McfdMesh {
   // fused-stage blocks
   bool fused = false;
   int nb = 0, block_cells = 0, max_local = 0, max_grad = 0, max_faces = 0;
+  double* block_partial = nullptr;  // [nb] per-block minima of the fused final stage
   BlockArrays blocks{};
   std::vector<void*> block_allocations;
   // implicit
@@ -2545,6 +2607,7 @@ void free_blocks(McfdMesh* m) {
   m->block_allocations.clear();
   m->fused = false;
   m->blocks = BlockArrays{};
+  m->block_partial = nullptr;
 }
 
 template <typename T>
@@ -2561,9 +2624,9 @@ int upload_block(McfdMesh* m, T** dst, const T* src, size_t count) {
 }
 
 // W of the local cells + one region shared by (gradients, limiter factors)
-// and then (face fluxes, face lengths) and the final block reduction.
+// and then (face fluxes, lengths, normals) and the final block reduction.
 size_t fused_lds_bytes(const McfdMesh* m, bool muscl) {
-  size_t region = 5 * (size_t)m->max_faces;
+  size_t region = 7 * (size_t)m->max_faces;
   if (muscl) region = std::max(region, 12 * (size_t)m->max_grad);
   region = std::max(region, (size_t)kBlock);
   return (4 * (size_t)m->max_local + region) * sizeof(double);
@@ -3151,6 +3214,16 @@ int mcfd_mesh_set_blocks(McfdMesh* m, const McfdBlockDesc* d) {
   rc |= upload_block(m, const_cast<unsigned short**>(&b.face_lr), d->face_lr,
                      2 * (size_t)d->n_block_faces);
   rc |= upload_block(m, const_cast<unsigned short**>(&b.acc_local), d->acc_local, (size_t)m->n_acc);
+  rc |= upload_block(m, const_cast<unsigned short**>(&b.dts_local), d->dts_local, (size_t)m->n_dts);
+  if (!rc) {
+    void* p = nullptr;
+    if (hipMalloc(&p, sizeof(double) * (size_t)d->n_blocks) != hipSuccess) {
+      rc = fail("mcfd_mesh_set_blocks: out of device memory");
+    } else {
+      m->block_allocations.push_back(p);
+      m->block_partial = static_cast<double*>(p);
+    }
+  }
   if (rc) {
     std::string msg = g_last_error;
     free_blocks(m);
@@ -3192,19 +3265,25 @@ int mcfd_stage(McfdMesh* m, const McfdSettings* st, const McfdStage* sg, void* s
   StageArgs a{sg->op,   sg->dt_per_cell, sg->u,    sg->u0,   sg->u1,   sg->r1,
               sg->dt,   sg->out,         sg->r_out, sg->c[0], sg->c[1], sg->c[2],
               sg->c[3], sg->c[4],        sg->dt_next,
-              reinterpret_cast<unsigned long long*>(sg->dt_min), st->cfl};
-  if (sg->dt_next && (!m->fused || !sg->dt_min || sg->op == MCFD_OP_RESIDUAL))
+              sg->dt_min ? m->block_partial : nullptr, st->cfl};
+  if ((sg->dt_next || sg->dt_min) && (!m->fused || !sg->dt_min || sg->op == MCFD_OP_RESIDUAL))
     return fail("mcfd_stage: dt_next needs the fused path, dt_min and a state update");
   if (check_viscous(m, st)) return -1;
-  if (sg->dt_next && st->viscosity)
-    return fail("mcfd_stage: dt_next is not available with viscous settings");
+  if (sg->dt_min && (st->viscosity || st->reconstruction == 2))
+    return fail("mcfd_stage: the next step's dt needs the fused stage (inviscid, no MUSCL parabolic)");
   if (m->fused && st->reconstruction != 2 && !st->viscosity) {
     bool muscl = st->reconstruction == 1 && m->n_grad > 0;
     size_t lds = fused_lds_bytes(m, muscl);
     GeomArrays ga{m->area,   m->grad_ptr, m->grad_cx,  m->grad_cy, m->lim_ptr,
                   m->lim_dx, m->lim_dy,   m->acc_ptr,  m->face_len, m->acc_coef};
-    TimedLaunch t(m, s, MCFD_KERNEL_STAGE);
-    launch_fused(m, st->flux, muscl, m->has_far_field, a.dt_next != nullptr, lds, s, ga, st, a);
+    {
+      TimedLaunch t(m, s, MCFD_KERNEL_STAGE);
+      launch_fused(m, st->flux, muscl, m->has_far_field, a.dt_partial != nullptr, lds, s, ga, st, a);
+    }
+    if (sg->dt_min) {
+      TimedLaunch t(m, s, MCFD_KERNEL_REDUCE);
+      hipLaunchKernelGGL(k_min_reduce, dim3(1), dim3(1024), 0, s, m->nb, m->block_partial, sg->dt_min);
+    }
     HIP_CHECK(hipGetLastError());
     return 0;
   }

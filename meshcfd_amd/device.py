@@ -75,7 +75,7 @@ class DeviceMesh:
         layout=None,
         fused: bool = True,
         block_cells: int = 128,
-        fuse_time_step: bool = False,
+        fuse_time_step: bool = True,
     ):
         if not torch.cuda.is_available():
             raise native.NativeError("No ROCm device is visible: the GPU residual cannot run.")
@@ -353,9 +353,11 @@ class DeviceMesh:
         # the fused final stage also computes the next step's CFL steps
         fuse_dt = (
             self.fuse_time_step and self.blocks is not None and self.case_ptr is None and mode != "Global fixed"
-            and self.settings.viscosity_model == "Inviscid"
+            and self.settings.viscosity_model == "Inviscid" and self.settings.reconstruction != "MUSCL parabolic"
         )
-        nxt = dict(dt_next=self.dt_cell, dt_min=self.dt_min) if fuse_dt else {}
+        nxt = {}
+        if fuse_dt:  # the global modes need only the minimum
+            nxt = dict(dt_next=self.dt_cell, dt_min=self.dt_min) if local else dict(dt_min=self.dt_min)
         if ti == "Explicit Euler":
             self._stage(native.OP_EULER, u, out=self.s1, dt=dt, per_cell=per_cell, coeffs=(1.0,), r_out=residual_out,
                         **nxt)

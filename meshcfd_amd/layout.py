@@ -94,6 +94,8 @@ class DeviceLayout:
     bc_state: np.ndarray
     bc_wall: np.ndarray
     mesh_face_flux: np.ndarray = None  # mesh face -> flux face (periodic b faces: their pair's)
+    dts_flux: np.ndarray = None  # flux face of each spectral-radius item
+    dts_own: np.ndarray = None  # item uses its flux face's normal and length (not a periodic b side, no exterior state)
 
     @property
     def n_faces(self) -> int:
@@ -253,6 +255,7 @@ def build_layout(g: Geometry, reorder: bool = True, order: np.ndarray | None = N
     d_bc = np.concatenate((mesh_bc, np.full(len(inter), -1, np.int64)))
     dts_ptr, order = _csr(new_id[d_cells], n)
     d_faces = d_faces[order]
+    d_bc_sorted = d_bc[order]
     right_new = np.where(right >= 0, new_id[np.maximum(right, 0)], -1)
     left = new_id[left][fperm].astype(np.int32)
     right = right_new[fperm].astype(np.int32)
@@ -297,4 +300,6 @@ def build_layout(g: Geometry, reorder: bool = True, order: np.ndarray | None = N
         bc_state=np.ascontiguousarray(bc_state.reshape(-1)),
         bc_wall=np.ascontiguousarray(bc_wall.reshape(-1)),
         mesh_face_flux=mesh_face_flux,
+        dts_flux=mesh_face_flux[d_faces].astype(np.int32),
+        dts_own=(~np.isin(d_faces, fb)) & (d_bc_sorted < 0),
     )

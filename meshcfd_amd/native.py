@@ -100,6 +100,7 @@ class McfdBlockDesc(C.Structure):
         ("faces", _ip),
         ("face_lr", _hp),
         ("acc_local", _hp),
+        ("dts_local", _hp),
     ]
 
 
@@ -129,6 +130,7 @@ def block_desc(bl) -> tuple[McfdBlockDesc, list]:
         faces=arr(bl.faces, np.int32, C.c_int32),
         face_lr=arr(bl.face_lr.reshape(-1), np.uint16, C.c_uint16),
         acc_local=arr(bl.acc_local, np.uint16, C.c_uint16),
+        dts_local=arr(bl.dts_local, np.uint16, C.c_uint16),
     )
     return desc, keep
 

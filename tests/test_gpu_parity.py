@@ -243,3 +243,34 @@ def test_parabolic_steps_match_oracle(cuda_device, integrator):
     print(integrator, "parabolic rel err", err)
     assert np.all(err <= TOL)
     dm.close()
+
+
+@pytest.mark.parametrize("integrator", ["Explicit Euler", "Explicit RK2", "Explicit RK3", "Explicit RK4"])
+@pytest.mark.parametrize("mode", ["Global adaptive", "Local (steady)"])
+@pytest.mark.parametrize("kind", ["periodic", "mixed", "farfield"])
+def test_fused_next_step_dt_matches_oracle(cuda_device, integrator, mode, kind):
+    """The final fused stage computes the next step's CFL steps from its
+    LDS face normals (periodic b sides and exterior-state faces from the
+    global items): 8 steps equal the oracle bit for bit."""
+    from meshcfd_amd.device import DeviceMesh
+    from oracle.euler import primitive_to_conservative
+    from oracle.solver import Oracle
+
+    case = _case(kind, "HLLC", "MUSCL linear")
+    case.numerics.time_integration = integrator
+    case.numerics.time_step_mode = mode
+    print(f"case: {kind} {integrator} {mode} fused next-step dt", flush=True)
+    dm = DeviceMesh(case, cuda_device, fused=True, block_cells=64, fuse_time_step=True)
+    prim = _perturbed_primitive(case)
+    dm.set_state(primitive_to_conservative(prim, 1.4))
+    t_gpu = dm.run(8)
+    gpu = dm.state()
+    ref, t_ref = Oracle(dm.geometry, case.numerics).run(prim, 8)
+    err = rel_err(gpu, ref)
+    print("rel err", err)
+    assert np.all(err <= 1e-13)
+    if kind != "farfield":
+        assert np.array_equal(gpu, ref)
+        if mode == "Global adaptive":
+            assert t_gpu == t_ref
+    dm.close()
