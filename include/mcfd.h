@@ -118,10 +118,18 @@ typedef struct McfdMeshDesc {
 /*
  * Cell blocks for the fused stage kernel (host pointers; copied). Block b owns
  * the consecutive cells [own_start, own_start + n_own) (the blocks tile the
- * mesh in order, at most block_cells cells each) and touches at most 256 flux
- * faces; its local cells are own cells, then ring-1 and ring-2 halo cells.
- * info[b] = {cells_start, n_own, n_ring1, n_ring2, faces_start, n_faces,
- * gnb_start, gptr_start, own_start}.
+ * mesh in order, at most block_cells cells each), has at most 256 own +
+ * ring-1 cells and touches at most 256 flux faces; its local cells are own
+ * cells, then ring-1 and ring-2 halo cells. Everything a block reads besides
+ * cell states is packed block by block, so each phase of the kernel reads it
+ * with one coalesced load per item:
+ *   info[b*12] = {cells_start, n_own, n_ring1, n_ring2, faces_start, n_faces,
+ *                 gnb_start, gptr_start, own_start, lent_start, 0, 0};
+ *   gptr / lptr: per block n_own + n_ring1 + 1 block-relative offsets of each
+ *     own / ring-1 cell's LSQ entries (gnb, gent) and limiter entries (lent);
+ *   frec: one 64-byte record per block face,
+ *     {nx, ny, lx, ly, rx, ry, length, kind | bc << 4, left | right << 16}
+ *     (local left/right cell ids, 0xFFFF: none; bc -1: none).
  */
 typedef struct McfdBlockDesc {
   int32_t n_blocks;
@@ -130,16 +138,19 @@ typedef struct McfdBlockDesc {
   int32_t max_grad;         /* max own + ring-1 cells of a block */
   int32_t max_faces;        /* max faces of a block */
   int32_t n_local;          /* length of cells */
-  int32_t n_gnb_ptr;        /* length of gnb_ptr */
-  int32_t n_gnb;            /* length of gnb */
-  int32_t n_block_faces;    /* length of faces (face_lr has twice as many) */
-  int32_t reserved;
-  const int32_t* info;      /* [n_blocks*9] */
+  int32_t n_gptr;           /* length of gptr and lptr */
+  int32_t n_gnb;            /* length of gnb (gent has twice as many doubles) */
+  int32_t n_lent;           /* limiter entries (lent has twice as many doubles) */
+  int32_t n_block_faces;    /* length of faces and frec */
+  const int32_t* info;      /* [n_blocks*12] */
   const int32_t* cells;     /* local -> global cell */
-  const int32_t* gnb_ptr;   /* per own/ring-1 local cell: first LSQ entry in gnb (block relative) */
+  const int32_t* gptr;
   const uint16_t* gnb;      /* LSQ neighbours as local cell ids, reference entry order */
+  const double* gent;       /* [n_gnb*2] LSQ coefficients (cx, cy) */
+  const int32_t* lptr;
+  const double* lent;       /* [n_lent*2] limiter offsets (dx, dy), reference entry order */
   const int32_t* faces;     /* local face -> flux face */
-  const uint16_t* face_lr;  /* [n_block_faces*2] local left/right cell (0xFFFF: none) */
+  const void* frec;         /* [n_block_faces] 64-byte face records */
   const uint16_t* acc_local;/* [n_acc] accumulation items as local face slot*4 + mode */
   const uint16_t* dts_local;/* [n_dts] spectral-radius items as local face slot, | 0x8000 when
                                the item's normal/length/exterior state is read from the

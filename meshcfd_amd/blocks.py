@@ -28,7 +28,10 @@ import numpy as np
 
 from .layout import DeviceLayout
 
-INFO_FIELDS = 9  # cells_start, n_own, n_ring1, n_ring2, faces_start, n_faces, gnb_start, gptr_start, own_start
+# cells_start, n_own, n_ring1, n_ring2, faces_start, n_faces, gnb_start,
+# gptr_start, own_start, lent_start, (2 spare)
+INFO_FIELDS = 12
+FACE_RECORD = np.dtype([("geom", np.float64, 7), ("kind_bc", np.int32), ("lr", np.uint32)])  # 64 bytes
 MAX_BLOCK_FACES = 256  # one flux face per thread of the 256-thread workgroup
 
 
@@ -81,10 +84,13 @@ class BlockLayout:
     n_blocks: int
     info: np.ndarray  # (n_blocks, 8) int32
     cells: np.ndarray  # int32 global ids of local cells, per block
-    gnb_ptr: np.ndarray  # int32, per local cell of own+ring1: start into gnb (relative to block gnb_start)
+    gnb_ptr: np.ndarray  # int32, per block n_own + n_ring1 + 1 offsets into the block's LSQ entries
     gnb: np.ndarray  # uint16 local neighbour ids of the LSQ entries
+    gent: np.ndarray  # float64 (n, 2) LSQ coefficients (cx, cy) of the entries, same order as gnb
+    lptr: np.ndarray  # int32, per block n_own + n_ring1 + 1 offsets into the block's limiter entries
+    lent: np.ndarray  # float64 (n, 2) limiter offsets (dx, dy) of own + ring-1 cells, per block
     faces: np.ndarray  # int32 global flux-face ids, per block
-    face_lr: np.ndarray  # uint16 (n, 2) local left / right ids (0xFFFF: boundary)
+    frec: np.ndarray  # FACE_RECORD per block face: nx, ny, lx, ly, rx, ry, length; kind | bc << 4; l | r << 16
     acc_local: np.ndarray  # uint16, per global acc item: local face slot * 4 + mode
     dts_local: np.ndarray  # uint16, per global spectral-radius item: local face slot | 0x8000 (global item)
     max_local: int
@@ -164,12 +170,32 @@ def build_blocks(L: DeviceLayout, block_cells: int = 128) -> BlockLayout:
     gnb = _lookup(keys, vals, g_b * n + g_nb)
     if gnb.max(initial=0) >= 0xFFFF:
         raise ValueError("block too large for 16-bit local ids")
+    gent = np.column_stack((L.grad_cx[np.repeat(gptr[gc_c], gcnt) + g_off],
+                            L.grad_cy[np.repeat(gptr[gc_c], gcnt) + g_off]))
     n_gcells = n_own + n_r1
-    gstart_cells = np.concatenate(([0], np.cumsum(n_gcells)[:-1]))
-    ecount = np.bincount(g_b, weights=None, minlength=nb) if len(g_b) else np.zeros(nb, np.int64)
-    gnb_start = np.concatenate(([0], np.cumsum(ecount)[:-1])).astype(np.int64)
-    cell_first = np.concatenate(([0], np.cumsum(gcnt)[:-1]))
-    gnb_ptr = cell_first - gnb_start[gc_b]  # relative to the block's gnb_start
+    # per block n_gcells + 1 offsets (block relative) into the entry lists
+    gstart_cells = np.concatenate(([0], np.cumsum(n_gcells + 1)[:-1]))
+
+    def block_offsets(counts):
+        """Per gradient cell (sorted by block, local id) entry counts ->
+        per block [0, c0, c0+c1, ...] and each block's first entry."""
+        total = np.bincount(gc_b, weights=counts, minlength=nb).astype(np.int64)
+        start = np.concatenate(([0], np.cumsum(total)[:-1]))
+        cum = np.cumsum(counts) - counts  # global exclusive prefix
+        rel = cum - start[gc_b]
+        ptr = np.empty(int((n_gcells + 1).sum()), dtype=np.int64)
+        slot = gstart_cells[gc_b] + (np.arange(len(gc_b)) - np.concatenate(([0], np.cumsum(n_gcells)[:-1]))[gc_b])
+        ptr[slot] = rel
+        ptr[gstart_cells + n_gcells] = total
+        return ptr, start
+
+    gnb_ptr, gnb_start = block_offsets(gcnt)
+    lp = L.lim_ptr.astype(np.int64)
+    lcnt = lp[gc_c + 1] - lp[gc_c]
+    l_off = np.arange(lcnt.sum()) - np.repeat(np.cumsum(lcnt) - lcnt, lcnt)
+    l_idx = np.repeat(lp[gc_c], lcnt) + l_off
+    lent = np.column_stack((L.lim_dx[l_idx], L.lim_dy[l_idx]))
+    lptr, lent_start = block_offsets(lcnt)
 
     # flux faces per block: the left cell's block, plus the right cell's block if different
     fl = L.face_left.astype(np.int64)
@@ -222,6 +248,12 @@ def build_blocks(L: DeviceLayout, block_cells: int = 128) -> BlockLayout:
     info[:, 6] = gnb_start
     info[:, 7] = gstart_cells
     info[:, 8] = bstart[:-1]
+    info[:, 9] = lent_start
+    frec = np.zeros(len(pf_k), dtype=FACE_RECORD)
+    frec["geom"] = np.column_stack((L.face_nx[pf_k], L.face_ny[pf_k], L.face_lx[pf_k], L.face_ly[pf_k],
+                                    L.face_rx[pf_k], L.face_ry[pf_k], L.face_len[pf_k]))
+    frec["kind_bc"] = (L.face_kind[pf_k].astype(np.int64) + 16 * L.face_bc[pf_k].astype(np.int64)).astype(np.int32)
+    frec["lr"] = (lr[:, 0] | (lr[:, 1] << 16)).astype(np.uint32)
     if nfb.max(initial=0) > MAX_BLOCK_FACES:
         raise ValueError("block partition left a block with too many faces")
     if n_gcells.max(initial=0) > MAX_BLOCK_FACES:
@@ -233,8 +265,11 @@ def build_blocks(L: DeviceLayout, block_cells: int = 128) -> BlockLayout:
         cells=cells.astype(np.int32),
         gnb_ptr=gnb_ptr.astype(np.int32),
         gnb=gnb.astype(np.uint16),
+        gent=np.ascontiguousarray(gent),
+        lptr=lptr.astype(np.int32),
+        lent=np.ascontiguousarray(lent),
         faces=pf_k.astype(np.int32),
-        face_lr=lr.astype(np.uint16),
+        frec=frec,
         acc_local=acc_local.astype(np.uint16),
         dts_local=dts_local.astype(np.uint16),
         max_local=int(n_loc.max()),

@@ -42,7 +42,7 @@ int fail(const std::string& msg) {
   } while (0)
 
 constexpr int kBlock = 256;
-constexpr int kBlockInfo = 9;  // ints per fused-kernel block record
+constexpr int kBlockInfo = 12;  // ints per fused-kernel block record
 
 struct P4 {
   double r, u, v, p;
@@ -1125,13 +1125,25 @@ __global__ void __launch_bounds__(kBlock)
 // Arithmetic is the same as the unfused kernels (same helpers, same order).
 // ---------------------------------------------------------------------------
 
+// One block face: geometry, kind | bc << 4 (bc -1: none), left | right << 16
+// (local cell ids, 0xFFFF: none). 64 bytes, read with four 16-byte loads.
+struct alignas(16) FaceRec {
+  double nx, ny, lx, ly, rx, ry, len;
+  int kind_bc;
+  unsigned lr;
+};
+static_assert(sizeof(FaceRec) == 64, "face record is 64 bytes");
+
 struct BlockArrays {
-  const int* info;          // [nb][8]
+  const int* info;          // [nb][kBlockInfo]
   const int* cells;         // local -> global cell
-  const int* gnb_ptr;       // per gradient cell, start in gnb (block relative)
+  const int* gptr;          // per block, n_grad + 1 offsets into gnb / gent (block relative)
   const unsigned short* gnb;
+  const double2* gent;      // LSQ coefficients (cx, cy)
+  const int* lptr;          // per block, n_grad + 1 offsets into lent (block relative)
+  const double2* lent;      // limiter offsets (dx, dy)
   const int* faces;         // local face -> global flux face
-  const unsigned short* face_lr;
+  const FaceRec* frec;
   const unsigned short* acc_local;
   const unsigned short* dts_local;
 };
@@ -1182,7 +1194,7 @@ __global__ void __launch_bounds__(kBlock)
   const int* inf = ba.info + kBlockInfo * b;
   const int cs = inf[0], n_own = inf[1], n_r1 = inf[2], n_r2 = inf[3];
   const int fs = inf[4], n_faces = inf[5], gs = inf[6], gps = inf[7];
-  const int first = inf[8];
+  const int first = inf[8], ls = inf[9];
   const int n_loc = n_own + n_r1 + n_r2;
   const int n_grad = n_own + n_r1;
   // LDS: W of every local cell, then one region that holds the gradients
@@ -1203,20 +1215,12 @@ __global__ void __launch_bounds__(kBlock)
   // overlaps P0 instead of adding a round trip to each phase. Thread t works
   // on gradient cell t in P1, face t in P2 and own cell t in P3 (blocks have
   // at most kBlock of each).
-  int e0 = 0, e1 = 0, l0 = 0, l1 = 0, gnb0 = 0;
+  int e0 = 0, e1 = 0, l0 = 0, l1 = 0;
   if (MUSCL && t < n_grad) {
-    int c = (t < n_own) ? first + t : ba.cells[cs + t];
-    e0 = ga.grad_ptr[c];
-    e1 = ga.grad_ptr[c + 1];
-    l0 = ga.lim_ptr[c];
-    l1 = ga.lim_ptr[c + 1];
-    gnb0 = ba.gnb_ptr[gps + t];
-  }
-  int fk = 0, f_l = 0, f_r = 0;
-  if (t < n_faces) {
-    fk = ba.faces[fs + t];
-    f_l = ba.face_lr[2 * (fs + t)];
-    f_r = ba.face_lr[2 * (fs + t) + 1];
+    e0 = ba.gptr[gps + t];
+    e1 = ba.gptr[gps + t + 1];
+    l0 = ba.lptr[gps + t];
+    l1 = ba.lptr[gps + t + 1];
   }
 
   // P0: primitive states (euler.py:20-40)
@@ -1247,7 +1251,8 @@ __global__ void __launch_bounds__(kBlock)
       double gy[4] = {0.0, 0.0, 0.0, 0.0};
       double mn[4] = {wv[0], wv[1], wv[2], wv[3]};
       double mx[4] = {wv[0], wv[1], wv[2], wv[3]};
-      const unsigned short* nbl = ba.gnb + gs + gnb0 - e0;
+      const unsigned short* nbl = ba.gnb + gs;
+      const double2* cxy = ba.gent + gs;
       // entries in batches of 4: the batch's loads are issued together, the
       // sums still run in entry order
       for (int base = e0; base < e1; base += 4) {
@@ -1257,9 +1262,10 @@ __global__ void __launch_bounds__(kBlock)
         for (int q = 0; q < 4; ++q) {
           int e = base + q;
           bool ok = e < e1;
+          double2 c2 = ok ? cxy[e] : make_double2(0.0, 0.0);
           ob[q] = ok ? (int)nbl[e] : 0;
-          cxb[q] = ok ? ga.grad_cx[e] : 0.0;
-          cyb[q] = ok ? ga.grad_cy[e] : 0.0;
+          cxb[q] = c2.x;
+          cyb[q] = c2.y;
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -1278,14 +1284,16 @@ __global__ void __launch_bounds__(kBlock)
         }
       }
       double phi[4] = {1.0, 1.0, 1.0, 1.0};
+      const double2* lxy = ba.lent + ls;
       for (int base = l0; base < l1; base += 4) {
         double dxb[4], dyb[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           int e = base + q;
           bool ok = e < l1;
-          dxb[q] = ok ? ga.lim_dx[e] : 0.0;
-          dyb[q] = ok ? ga.lim_dy[e] : 0.0;
+          double2 d2 = ok ? lxy[e] : make_double2(0.0, 0.0);
+          dxb[q] = d2.x;
+          dyb[q] = d2.y;
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -1311,19 +1319,22 @@ __global__ void __launch_bounds__(kBlock)
   // faces); the results wait in registers until every thread is done with
   // the gradient slots they overwrite.
   F4 fl{0.0, 0.0, 0.0, 0.0};
-  double flen = 0.0;
+  double flen = 0.0, fnx = 0.0, fny = 0.0;
   const int f = t;
   if (f < n_faces) {
-    const int k = fk, l = f_l, r = f_r;
-    int kind = fa.kind[k];
-    double nx = fa.nx[k], ny = fa.ny[k];
-    P4 L = rec_lds(sW, sG, sP, l, fa.lx[k], fa.ly[k], muscl);
+    const FaceRec fr = ba.frec[fs + f];
+    const int kind = fr.kind_bc & 15;
+    const int l = fr.lr & 0xFFFF, r = fr.lr >> 16;
+    const double nx = fr.nx, ny = fr.ny;
+    fnx = nx;
+    fny = ny;
+    P4 L = rec_lds(sW, sG, sP, l, fr.lx, fr.ly, muscl);
     P4 R;
     int err = 0;
     if (kind <= MCFD_FACE_PERIODIC) {
-      R = rec_lds(sW, sG, sP, r, fa.rx[k], fa.ry[k], muscl);
+      R = rec_lds(sW, sG, sP, r, fr.rx, fr.ry, muscl);
     } else {
-      int bi = fa.bc[k];
+      int bi = fr.kind_bc >> 4;
       const double* st = fa.bc_state + 4 * (long)bi;
       if (kind == MCFD_FACE_WALL) {
         double rx = L.u - fa.bc_wall[2 * (long)bi];
@@ -1355,8 +1366,8 @@ __global__ void __launch_bounds__(kBlock)
     } else {
       fl = rusanov(L, R, nx, ny, g);
     }
-    if (err) report(status, err, k);
-    flen = ga.face_len[k];
+    if (err) report(status, err, ba.faces[fs + f]);
+    flen = fr.len;
   }
   // P3's per-cell loads, issued before the barrier
   int a0 = 0, a1 = 0, d0 = 0, d1 = 0;
@@ -1385,8 +1396,8 @@ __global__ void __launch_bounds__(kBlock)
     sF[4 * f + 3] = fl.d;
     sL[f] = flen;
     if (DTNEXT) {
-      sN[2 * f] = fa.nx[fk];
-      sN[2 * f + 1] = fa.ny[fk];
+      sN[2 * f] = fnx;
+      sN[2 * f + 1] = fny;
     }
   }
   __syncthreads();
@@ -3208,11 +3219,16 @@ int mcfd_mesh_set_blocks(McfdMesh* m, const McfdBlockDesc* d) {
   BlockArrays& b = m->blocks;
   rc |= upload_block(m, const_cast<int**>(&b.info), d->info, kBlockInfo * (size_t)d->n_blocks);
   rc |= upload_block(m, const_cast<int**>(&b.cells), d->cells, (size_t)d->n_local);
-  rc |= upload_block(m, const_cast<int**>(&b.gnb_ptr), d->gnb_ptr, (size_t)d->n_gnb_ptr);
+  rc |= upload_block(m, const_cast<int**>(&b.gptr), d->gptr, (size_t)d->n_gptr);
   rc |= upload_block(m, const_cast<unsigned short**>(&b.gnb), d->gnb, (size_t)d->n_gnb);
+  rc |= upload_block(m, reinterpret_cast<double**>(const_cast<double2**>(&b.gent)), d->gent,
+                     2 * (size_t)d->n_gnb);
+  rc |= upload_block(m, const_cast<int**>(&b.lptr), d->lptr, (size_t)d->n_gptr);
+  rc |= upload_block(m, reinterpret_cast<double**>(const_cast<double2**>(&b.lent)), d->lent,
+                     2 * (size_t)d->n_lent);
   rc |= upload_block(m, const_cast<int**>(&b.faces), d->faces, (size_t)d->n_block_faces);
-  rc |= upload_block(m, const_cast<unsigned short**>(&b.face_lr), d->face_lr,
-                     2 * (size_t)d->n_block_faces);
+  rc |= upload_block(m, reinterpret_cast<double**>(const_cast<FaceRec**>(&b.frec)),
+                     static_cast<const double*>(d->frec), 8 * (size_t)d->n_block_faces);
   rc |= upload_block(m, const_cast<unsigned short**>(&b.acc_local), d->acc_local, (size_t)m->n_acc);
   rc |= upload_block(m, const_cast<unsigned short**>(&b.dts_local), d->dts_local, (size_t)m->n_dts);
   if (!rc) {

@@ -89,16 +89,19 @@ class McfdBlockDesc(C.Structure):
         ("max_grad", C.c_int32),
         ("max_faces", C.c_int32),
         ("n_local", C.c_int32),
-        ("n_gnb_ptr", C.c_int32),
+        ("n_gptr", C.c_int32),
         ("n_gnb", C.c_int32),
+        ("n_lent", C.c_int32),
         ("n_block_faces", C.c_int32),
-        ("reserved", C.c_int32),
         ("info", _ip),
         ("cells", _ip),
-        ("gnb_ptr", _ip),
+        ("gptr", _ip),
         ("gnb", _hp),
+        ("gent", _dp),
+        ("lptr", _ip),
+        ("lent", _dp),
         ("faces", _ip),
-        ("face_lr", _hp),
+        ("frec", C.c_void_p),
         ("acc_local", _hp),
         ("dts_local", _hp),
     ]
@@ -112,6 +115,8 @@ def block_desc(bl) -> tuple[McfdBlockDesc, list]:
         keep.append(a)
         return as_ptr(a, ctype)
 
+    frec = np.ascontiguousarray(bl.frec)
+    keep.append(frec)
     desc = McfdBlockDesc(
         n_blocks=bl.n_blocks,
         block_cells=bl.block_cells,
@@ -119,16 +124,19 @@ def block_desc(bl) -> tuple[McfdBlockDesc, list]:
         max_grad=bl.max_grad_cells,
         max_faces=bl.max_faces,
         n_local=len(bl.cells),
-        n_gnb_ptr=len(bl.gnb_ptr),
+        n_gptr=len(bl.gnb_ptr),
         n_gnb=len(bl.gnb),
+        n_lent=len(bl.lent),
         n_block_faces=len(bl.faces),
-        reserved=0,
         info=arr(bl.info.reshape(-1), np.int32, C.c_int32),
         cells=arr(bl.cells, np.int32, C.c_int32),
-        gnb_ptr=arr(bl.gnb_ptr, np.int32, C.c_int32),
+        gptr=arr(bl.gnb_ptr, np.int32, C.c_int32),
         gnb=arr(bl.gnb, np.uint16, C.c_uint16),
+        gent=arr(bl.gent.reshape(-1), np.float64, C.c_double),
+        lptr=arr(bl.lptr, np.int32, C.c_int32),
+        lent=arr(bl.lent.reshape(-1), np.float64, C.c_double),
         faces=arr(bl.faces, np.int32, C.c_int32),
-        face_lr=arr(bl.face_lr.reshape(-1), np.uint16, C.c_uint16),
+        frec=frec.ctypes.data,
         acc_local=arr(bl.acc_local, np.uint16, C.c_uint16),
         dts_local=arr(bl.dts_local, np.uint16, C.c_uint16),
     )
