@@ -2422,18 +2422,31 @@ __global__ void __launch_bounds__(kBlock)
   if (threadIdx.x == 0) partial[xcd_block()] = red[0];
 }
 
+// out[0] = min(partial[0:n]); one block of 1024 threads: four independent
+// loads in flight per thread, then lane exchanges and one LDS step.
 __global__ void __launch_bounds__(1024)
     k_min_reduce(int n, const double* __restrict__ partial, double* __restrict__ out) {
-  __shared__ double red[1024];
-  double m = INFINITY;
-  for (int j = threadIdx.x; j < n; j += blockDim.x) m = fmin(m, partial[j]);
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
-    __syncthreads();
+  __shared__ double wave_min[16];
+  const int t = threadIdx.x;
+  double m0 = INFINITY, m1 = INFINITY, m2 = INFINITY, m3 = INFINITY;
+  int j = t;
+  for (; j + 3 * 1024 < n; j += 4 * 1024) {
+    m0 = fmin(m0, partial[j]);
+    m1 = fmin(m1, partial[j + 1024]);
+    m2 = fmin(m2, partial[j + 2048]);
+    m3 = fmin(m3, partial[j + 3072]);
   }
-  if (threadIdx.x == 0) out[0] = red[0];
+  for (; j < n; j += 1024) m0 = fmin(m0, partial[j]);
+  double m = fmin(fmin(m0, m1), fmin(m2, m3));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off));
+  if ((t & 63) == 0) wave_min[t >> 6] = m;
+  __syncthreads();
+  if (t == 0) {
+    double r = wave_min[0];
+    for (int w = 1; w < 16; ++w) r = fmin(r, wave_min[w]);
+    out[0] = r;
+  }
 }
 
 __global__ void k_clock(double* __restrict__ src, double fixed_dt, double end_time,
