@@ -123,6 +123,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--unfused", action="store_true", help="four-kernel path instead of the fused stage kernel")
     ap.add_argument("--block-cells", type=int, default=128, help="cells per workgroup of the fused stage kernel")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use --block-cells as given instead of timing candidate block sizes first")
     ap.add_argument("--separate-time-step", action="store_true",
                     help="compute the CFL steps with their own kernels instead of inside the final fused stage")
     ap.add_argument(
@@ -178,6 +180,12 @@ def main(argv=None) -> int:
     cons = np.column_stack((rho, rho * u, rho * v, p / (gamma - 1.0) + 0.5 * rho * (u**2 + v**2)))
     dm.set_state(cons)
     dm.clock.zero_()
+    tuned = None
+    if not args.unfused and not args.no_autotune:
+        block = dm.autotune(candidates=(96, 112, args.block_cells))
+        tuned = {"chosen": block, "ms_per_step": {str(k): round(v, 5) for k, v in dm.timings.items()}}
+        args.block_cells = block
+        print(f"[bench] autotuned block size {block}: {tuned['ms_per_step']}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         dm.step()
     dm.check_status("warm-up: ")
@@ -262,6 +270,7 @@ def main(argv=None) -> int:
                 f"fused stage kernel, blocks of <= {args.block_cells} cells and <= 256 flux faces"
                 + ("" if args.separate_time_step else ", next step's CFL steps inside the final stage")
             ),
+            "autotune": tuned,
         },
         "roofline": roofline,
         "stage": {

@@ -229,6 +229,8 @@ typedef struct McfdImplicitDesc {
   const int32_t* off_item;
   const int32_t* off_nb;       /* device cell of the column */
   const int32_t* natural_order;/* [n_cells] device cell at each natural (mesh) position */
+  const int32_t* off_t;        /* per off-diagonal item: the column cell's item for the same
+                                  record (its coefficient is the transposed entry) */
 } McfdImplicitDesc;
 
 /* Matrix-free implicit operator and GMRES arguments (device pointers). */
@@ -391,10 +393,32 @@ int mcfd_mesh_set_adjoint(McfdMesh* mesh, const int32_t* grad_rev);
 
 /* ubar = lambda^T dR/dU at state u: the hand-written reverse of `_residual`
  * (solver.py:2437-2718) through fluxes, ghost states, reconstruction,
- * limiter, LSQ gradient and primitive conversion. No counterpart in the
- * reference (which has no adjoint). */
+ * limiter, LSQ gradient and primitive conversion. If sbar is not NULL it
+ * receives per boundary record [n_bc*4] lambda^T dR/d(exterior state) for
+ * velocity-inlet and far-field records (0 for the others), for gradients
+ * with respect to inflow conditions. No counterpart in the reference (which
+ * has no adjoint). */
 int mcfd_residual_vjp(McfdMesh* mesh, const McfdSettings* settings, const double* u,
-                      const double* lambda, double* ubar, void* stream);
+                      const double* lambda, double* ubar, double* sbar, void* stream);
+
+/* Transposed LU-SGS: out = M^T v with M = U^-1 D L^-1 as assembled by
+ * mcfd_lusgs_assemble (forward sweep on U^T, backward on L^T). The
+ * preconditioner of the adjoint GMRES; no counterpart in the reference. */
+int mcfd_lusgs_apply_transpose(McfdMesh* mesh, const double* v, double* out, void* stream);
+
+/* Adjoint pseudo-time operator out = A^T v = v - J^T (dt v) at g->state,
+ * with J the exact Jacobian of the residual for `settings` (through
+ * mcfd_residual_vjp) and dt from g->dt / g->dt_per_cell. No counterpart in the
+ * reference. */
+int mcfd_adjoint_apply(McfdMesh* mesh, const McfdSettings* settings, McfdGmres* g, const double* v,
+                       double* out, void* stream);
+
+/* Adjoint GMRES: solves A^T x = g->rhs (A^T as in mcfd_adjoint_apply) with
+ * the algorithm of mcfd_gmres, left-preconditioned by the transposed
+ * LU-SGS of the matrix last assembled by mcfd_lusgs_assemble. g->rate,
+ * sigma and state_norm are unused. Needs mcfd_mesh_set_implicit and
+ * mcfd_mesh_set_adjoint. No counterpart in the reference. */
+int mcfd_adjoint_gmres(McfdMesh* mesh, const McfdSettings* settings, McfdGmres* g, void* stream);
 
 /* Batched cases (cells of case c in [case_ptr[c], case_ptr[c+1]), device):
  * per-cell stable steps, then per case dt = min(case minimum, end_time -

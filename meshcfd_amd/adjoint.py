@@ -25,7 +25,8 @@ native.EXPORTS.update(
         "mcfd_mesh_set_adjoint": (C.c_int, [C.c_void_p, C.c_void_p]),
         "mcfd_residual_vjp": (
             C.c_int,
-            [C.c_void_p, C.POINTER(native.McfdSettings), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+            [C.c_void_p, C.POINTER(native.McfdSettings), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+             C.c_void_p],
         ),
     }
 )
@@ -49,14 +50,18 @@ def enable(dm) -> None:
     dm._adjoint_ready = True
 
 
-def residual_vjp(dm, u: torch.Tensor, lam: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
-    """lam^T dR/dU at u (device order, (C,4))."""
+def residual_vjp(dm, u: torch.Tensor, lam: torch.Tensor, out: torch.Tensor | None = None,
+                 state_bar: torch.Tensor | None = None) -> torch.Tensor:
+    """lam^T dR/dU at u (device order, (C,4)). With ``state_bar`` ((n_bc, 4)
+    device tensor) also lam^T dR/d(exterior state) per boundary record
+    (velocity inlets and far field; 0 for the other records)."""
     enable(dm)
     u = u.contiguous()
     lam = lam.contiguous()
     out = torch.empty_like(u) if out is None else out
+    sbar = None if state_bar is None else C.c_void_p(state_bar.data_ptr())
     native.check(
-        dm.lib.mcfd_residual_vjp(dm.handle, C.byref(dm.native_settings), _p(u), _p(lam), _p(out), dm.stream)
+        dm.lib.mcfd_residual_vjp(dm.handle, C.byref(dm.native_settings), _p(u), _p(lam), _p(out), sbar, dm.stream)
     )
     return out
 

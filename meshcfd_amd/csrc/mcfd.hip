@@ -1524,6 +1524,7 @@ struct LusgsArrays {
   const int* off_split;  // per cell: first upper item
   const int* off_item;   // record*2 + side
   const int* off_nb;     // device cell of the off-diagonal column
+  const int* off_t;      // the column cell's item for the same record (transposed coefficient)
   int n_levels_f, n_levels_b;
   const int* lvl_ptr_f;
   const int* lvl_cells_f;
@@ -1578,15 +1579,20 @@ __global__ void __launch_bounds__(kBlock)
 // depends only on cells of earlier levels, so the sweep equals the
 // sequential natural-order substitution. Forward: y = L^-1 v. Backward (in
 // place): x = U^-1 (max(diag, 1e-14) * y).
+// With te (transposed application M^T = L^-T D U^-T): the forward sweep
+// solves U^T x = v and the backward sweep L^T z = D x, over the same lower /
+// upper items with the coefficient of the column cell's item for the same
+// record, S[j][i] = off_val[te[e]]. Same levels: S is structurally symmetric.
 __global__ void __launch_bounds__(1024)
-    k_lusgs_forward(LusgsArrays la, const double* __restrict__ v, double* __restrict__ y) {
+    k_lusgs_forward(LusgsArrays la, const double* __restrict__ v, double* __restrict__ y,
+                    const int* __restrict__ te) {
   for (int lev = 0; lev < la.n_levels_f; ++lev) {
     int a = la.lvl_ptr_f[lev], b = la.lvl_ptr_f[lev + 1];
     for (int idx = a + threadIdx.x; idx < b; idx += blockDim.x) {
       int i = la.lvl_cells_f[idx];
       P4 s = load4(v, i);
       for (int e = la.off_ptr[i]; e < la.off_split[i]; ++e) {
-        double c = la.off_val[e];
+        double c = te ? la.off_val[te[e]] : la.off_val[e];
         P4 o = load4(y, la.off_nb[e]);
         s.r = s.r - c * o.r;
         s.u = s.u - c * o.u;
@@ -1601,7 +1607,8 @@ __global__ void __launch_bounds__(1024)
   }
 }
 
-__global__ void __launch_bounds__(1024) k_lusgs_backward(LusgsArrays la, double* __restrict__ x) {
+__global__ void __launch_bounds__(1024)
+    k_lusgs_backward(LusgsArrays la, double* __restrict__ x, const int* __restrict__ te) {
   for (int lev = 0; lev < la.n_levels_b; ++lev) {
     int a = la.lvl_ptr_b[lev], b = la.lvl_ptr_b[lev + 1];
     for (int idx = a + threadIdx.x; idx < b; idx += blockDim.x) {
@@ -1611,7 +1618,7 @@ __global__ void __launch_bounds__(1024) k_lusgs_backward(LusgsArrays la, double*
       P4 y = load4(x, i);
       P4 s{dm * y.r, dm * y.u, dm * y.v, dm * y.p};
       for (int e = la.off_split[i]; e < la.off_ptr[i + 1]; ++e) {
-        double c = la.off_val[e];
+        double c = te ? la.off_val[te[e]] : la.off_val[e];
         P4 o = load4(x, la.off_nb[e]);
         s.r = s.r - c * o.r;
         s.u = s.u - c * o.u;
@@ -2023,19 +2030,21 @@ __device__ F4D roe_d(const P4D& L, const P4D& R, double nx, double ny, double g)
   return f;
 }
 
-__device__ P4D far_field_d(const P4D& in, const P4& out, double nx, double ny, double g) {
+// Far-field characteristics in duals of both the interior state and the
+// free-stream state (the latter seeded for inflow-parameter gradients).
+__device__ P4D far_field_d(const P4D& in, const P4D& out, double nx, double ny, double g) {
   D8 un_i = in.u * nx + in.v * ny;
-  double un_f = out.u * nx + out.v * ny;
+  D8 un_f = out.u * nx + out.v * ny;
   D8 c_i = sound_d(in, g);
-  double c_f = sound(out, g);
-  if (un_i.v <= -c_i.v) return P4D{dconst(out.r), dconst(out.u), dconst(out.v), dconst(out.p)};
+  D8 c_f = sound_d(out, g);
+  if (un_i.v <= -c_i.v) return out;
   if (un_i.v >= c_i.v) return in;
   D8 r_out = un_i + 2.0 * c_i / (g - 1.0);
-  double r_in = un_f - 2.0 * c_f / (g - 1.0);
+  D8 r_in = un_f - 2.0 * c_f / (g - 1.0);
   D8 vb = 0.5 * (r_out + r_in);
   D8 cb = 0.25 * (g - 1.0) * (r_out - r_in);
   bool inflow = vb.v < 0.0;
-  P4D src = inflow ? P4D{dconst(out.r), dconst(out.u), dconst(out.v), dconst(out.p)} : in;
+  P4D src = inflow ? out : in;
   D8 entropy = src.p / dpow(src.r, g);
   D8 rho = dpow(cb * cb / (g * entropy), 1.0 / (g - 1.0));
   D8 p = entropy * dpow(rho, g);
@@ -2060,7 +2069,7 @@ __global__ void __launch_bounds__(kBlock)
                    const double* __restrict__ G, const double* __restrict__ PHI,
                    const double* __restrict__ flen, const double* __restrict__ area,
                    const double* __restrict__ lam, int scheme, int muscl, double g,
-                   double* __restrict__ FADJ) {
+                   double* __restrict__ FADJ, double* __restrict__ SBAR) {
   int k = xcd_block() * blockDim.x + threadIdx.x;
   if (k >= nf) return;
   int kind = fa.kind[k];
@@ -2097,12 +2106,13 @@ __global__ void __launch_bounds__(kBlock)
       DR.u = DL.u - 2.0 * un * nx;
       DR.v = DL.v - 2.0 * un * ny;
     } else if (kind == MCFD_FACE_VELOCITY_INLET) {
-      DR = P4D{dconst(st[0]), dconst(st[1]), dconst(st[2]), dconst(st[3])};
+      // directions 4-7 carry the record's exterior state (inflow gradients)
+      DR = seed(P4{st[0], st[1], st[2], st[3]}, 4);
     } else if (kind == MCFD_FACE_PRESSURE_OUTLET) {
       DR = DL;
       DR.p = dconst(st[3]);
     } else if (kind == MCFD_FACE_FAR_FIELD) {
-      DR = far_field_d(DL, P4{st[0], st[1], st[2], st[3]}, nx, ny, g);
+      DR = far_field_d(DL, seed(P4{st[0], st[1], st[2], st[3]}, 4), nx, ny, g);
     } else {
       DR = DL;
     }
@@ -2123,6 +2133,12 @@ __global__ void __launch_bounds__(kBlock)
   q[1] = make_double2(out[2], out[3]);
   q[2] = make_double2(out[4], out[5]);
   q[3] = make_double2(out[6], out[7]);
+  if (SBAR && kind > MCFD_FACE_PERIODIC) {
+    // lambda^T dR/d(exterior state) of the face's boundary record
+    bool ext = kind == MCFD_FACE_VELOCITY_INLET || kind == MCFD_FACE_FAR_FIELD;
+    store4(SBAR, fa.bc[k], ext ? out[4] : 0.0, ext ? out[5] : 0.0, ext ? out[6] : 0.0,
+           ext ? out[7] : 0.0);
+  }
 }
 
 // Cell pass 1: gather face adjoints (reconstruction backward), limiter
@@ -2906,9 +2922,10 @@ int read_scalars(double* host, const double* dev, int count, hipStream_t s) {
 
 int lblocks(long n) { return (int)((n + kBlock - 1) / kBlock); }
 
-int lusgs_apply(McfdMesh* m, const double* v, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_lusgs_forward, dim3(1), dim3(1024), 0, s, m->lusgs, v, out);
-  hipLaunchKernelGGL(k_lusgs_backward, dim3(1), dim3(1024), 0, s, m->lusgs, out);
+int lusgs_apply(McfdMesh* m, const double* v, double* out, hipStream_t s, bool transpose = false) {
+  const int* te = transpose ? m->lusgs.off_t : nullptr;
+  hipLaunchKernelGGL(k_lusgs_forward, dim3(1), dim3(1024), 0, s, m->lusgs, v, out, te);
+  hipLaunchKernelGGL(k_lusgs_backward, dim3(1), dim3(1024), 0, s, m->lusgs, out, te);
   HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -2943,7 +2960,46 @@ struct JacobianOp {
   int per_cell;
   double sigma[4];
   double base;  // ||state / sigma||
+  // adjoint: A^T v = v - J^T (dt v), J the exact Jacobian of the residual
+  // with settings st (reverse kernels); preconditioner the transposed LU-SGS
+  bool adjoint = false;
 };
+
+int mcfd_residual_vjp_impl(McfdMesh* m, const McfdSettings* st, const double* u,
+                           const double* lam, double* ubar, double* sbar, hipStream_t s);
+
+__global__ void __launch_bounds__(kBlock)
+    k_scale_dt(int n, const double* __restrict__ v, const double* __restrict__ dt, int per_cell,
+               double* __restrict__ w) {
+  int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double h = per_cell ? dt[i] : dt[0];
+  P4 a = load4(v, i);
+  store4(w, i, h * a.r, h * a.u, h * a.v, h * a.p);
+}
+
+// out = A^T v = v - J^T (dt v)
+int adjoint_apply(JacobianOp& op, const double* v, double* out, hipStream_t s) {
+  McfdMesh* m = op.m;
+  long N = 4L * m->n_cells;
+  hipLaunchKernelGGL(k_scale_dt, dim3(blocks(m->n_cells)), dim3(kBlock), 0, s, m->n_cells, v, op.dt,
+                     op.per_cell, m->gm_pert);
+  if (mcfd_residual_vjp_impl(m, op.st, op.state, m->gm_pert, m->gm_rp, nullptr, s)) return -1;
+  HIP_CHECK(hipMemcpyAsync(out, v, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(k_axpby, dim3(lblocks(N)), dim3(kBlock), 0, s, N, -1.0, m->gm_rp, 1.0, out);
+  HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int jacobian_apply(JacobianOp& op, const double* v, double* out, hipStream_t s);
+
+int op_apply(JacobianOp& op, const double* v, double* out, hipStream_t s) {
+  return op.adjoint ? adjoint_apply(op, v, out, s) : jacobian_apply(op, v, out, s);
+}
+
+int op_precond(JacobianOp& op, const double* v, double* out, hipStream_t s) {
+  return lusgs_apply(op.m, v, out, s, op.adjoint);
+}
 
 // out = v - dt * J v / sigma with J v = (R1(s + eps*sigma*v) - R1(s)) / eps
 int jacobian_apply(JacobianOp& op, const double* v, double* out, hipStream_t s) {
@@ -3010,7 +3066,7 @@ int gmres_cycle(JacobianOp& op, const double* b, double* x, double rtol, int res
   }
   double atol = rtol * bnrm2;
   // ptol = ||M b|| * min(1, atol / bnrm2)
-  if (lusgs_apply(m, b, m->gm_w, s)) return -1;
+  if (op_precond(op, b, m->gm_w, s)) return -1;
   if (dot_dev(m, N, m->gm_w, m->gm_w, sc + 1, true, s)) return -1;
   double mb = 0.0;
   if (read_scalars(&mb, sc + 1, 1, s)) return -1;
@@ -3019,7 +3075,7 @@ int gmres_cycle(JacobianOp& op, const double* b, double* x, double rtol, int res
   const double eps = 2.220446049250313e-16;
   // r = b - A x (or b when x == 0)
   if (x_nonzero) {
-    if (jacobian_apply(op, x, m->gm_av, s)) return -1;
+    if (op_apply(op, x, m->gm_av, s)) return -1;
     HIP_CHECK(hipMemcpyAsync(m->gm_r, b, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(k_axpby, dim3(nbl), dim3(kBlock), 0, s, N, -1.0, m->gm_av, 1.0, m->gm_r);
     if (dot_dev(m, N, m->gm_r, m->gm_r, sc + 2, true, s)) return -1;
@@ -3034,7 +3090,7 @@ int gmres_cycle(JacobianOp& op, const double* b, double* x, double rtol, int res
     HIP_CHECK(hipMemcpyAsync(m->gm_r, b, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
   }
   // v0 = M r / ||M r||
-  if (lusgs_apply(m, m->gm_r, V, s)) return -1;
+  if (op_precond(op, m->gm_r, V, s)) return -1;
   if (dot_dev(m, N, V, V, sc + 3, true, s)) return -1;
   double t0 = 0.0;
   if (read_scalars(&t0, sc + 3, 1, s)) return -1;
@@ -3048,8 +3104,8 @@ int gmres_cycle(JacobianOp& op, const double* b, double* x, double rtol, int res
   for (col = 0; col < restart; ++col) {
     double* vc = V + (long)col * N;
     double* vn = V + (long)(col + 1) * N;
-    if (jacobian_apply(op, vc, m->gm_av, s)) return -1;
-    if (lusgs_apply(m, m->gm_av, m->gm_w, s)) return -1;
+    if (op_apply(op, vc, m->gm_av, s)) return -1;
+    if (op_precond(op, m->gm_av, m->gm_w, s)) return -1;
     // h0 = ||w||, then modified Gram-Schmidt with device coefficients
     if (dot_dev(m, N, m->gm_w, m->gm_w, sc + 4, true, s)) return -1;
     for (int k = 0; k <= col; ++k) {
@@ -3105,12 +3161,44 @@ int gmres_cycle(JacobianOp& op, const double* b, double* x, double rtol, int res
   if (y.y[0] != 0.0) y.y[0] /= h[0];
   hipLaunchKernelGGL(k_combine, dim3(nbl), dim3(kBlock), 0, s, N, col + 1, V, y, x);
   // true residual
-  if (jacobian_apply(op, x, m->gm_av, s)) return -1;
+  if (op_apply(op, x, m->gm_av, s)) return -1;
   HIP_CHECK(hipMemcpyAsync(m->gm_r, b, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL(k_axpby, dim3(nbl), dim3(kBlock), 0, s, N, -1.0, m->gm_av, 1.0, m->gm_r);
   if (dot_dev(m, N, m->gm_r, m->gm_r, sc + 2, true, s)) return -1;
   if (read_scalars(&rnorm, sc + 2, 1, s)) return -1;
   info = (rnorm <= atol) ? 0 : 1;
+  HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+namespace {
+
+int mcfd_residual_vjp_impl(McfdMesh* m, const McfdSettings* st, const double* u,
+                           const double* lam, double* ubar, double* sbar, hipStream_t s) {
+  if (!m || !u || !lam || !ubar) return fail("mcfd_residual_vjp: NULL argument");
+  if (!m->adjoint) return fail("mcfd_residual_vjp: call mcfd_mesh_set_adjoint first");
+  if (check_settings(st)) return -1;
+  if (st->reconstruction == 2) return fail("mcfd_residual_vjp: MUSCL parabolic has no adjoint yet");
+  if (st->viscosity) return fail("mcfd_residual_vjp: viscous terms have no adjoint yet");
+  int n = m->n_cells;
+  bool muscl = st->reconstruction == 1 && m->n_grad > 0;
+  hipLaunchKernelGGL(k_primitive, dim3(blocks(n)), dim3(kBlock), 0, s, n, u, m->W, st->gamma,
+                     m->status);
+  if (muscl)
+    hipLaunchKernelGGL(k_gradient, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->grad_ptr,
+                       m->grad_nb, m->grad_cx, m->grad_cy, m->lim_ptr, m->lim_dx, m->lim_dy,
+                       st->limiter, m->G, m->PHI);
+  if (m->n_faces)
+    hipLaunchKernelGGL(k_flux_adjoint, dim3(blocks(m->n_faces)), dim3(kBlock), 0, s, m->n_faces,
+                       m->faces, m->W, m->G, m->PHI, m->face_len, m->area, lam, st->flux,
+                       muscl ? 1 : 0, st->gamma, m->FADJ, sbar);
+  hipLaunchKernelGGL(k_cell_adjoint1, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->G, m->PHI,
+                     m->FADJ, m->adj, muscl ? 1 : 0, st->limiter);
+  hipLaunchKernelGGL(k_cell_adjoint2, dim3(blocks(n)), dim3(kBlock), 0, s, n, u, m->W, lam,
+                     m->adj, muscl ? 1 : 0, st->gamma, st->acceleration_x, st->acceleration_y,
+                     ubar);
   HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -3396,6 +3484,7 @@ int mcfd_mesh_set_implicit(McfdMesh* m, const McfdImplicitDesc* d) {
   (void)hipDeviceSynchronize();
   free_implicit(m);
   if (!d) return 0;
+  if (!d->off_t) return fail("mcfd_mesh_set_implicit: off_t is required");
   int n = m->n_cells;
   // Level schedules of the natural-order triangular solves: a cell's level
   // is one more than the largest level among the cells it depends on.
@@ -3448,6 +3537,7 @@ int mcfd_mesh_set_implicit(McfdMesh* m, const McfdImplicitDesc* d) {
   rc |= upload_implicit(m, const_cast<int**>(&la.off_split), d->off_split, (size_t)n);
   rc |= upload_implicit(m, const_cast<int**>(&la.off_item), d->off_item, NO);
   rc |= upload_implicit(m, const_cast<int**>(&la.off_nb), d->off_nb, NO);
+  rc |= upload_implicit(m, const_cast<int**>(&la.off_t), d->off_t, NO);
   rc |= upload_implicit(m, const_cast<int**>(&la.lvl_ptr_f), pf.data(), pf.size());
   rc |= upload_implicit(m, const_cast<int**>(&la.lvl_cells_f), cf.data(), cf.size());
   rc |= upload_implicit(m, const_cast<int**>(&la.lvl_ptr_b), pb.data(), pb.size());
@@ -3669,31 +3759,61 @@ int mcfd_mesh_set_viscous(McfdMesh* m, const McfdViscousDesc* d) {
 }
 
 int mcfd_residual_vjp(McfdMesh* m, const McfdSettings* st, const double* u, const double* lam,
-                      double* ubar, void* stream) {
-  if (!m || !u || !lam || !ubar) return fail("mcfd_residual_vjp: NULL argument");
-  if (!m->adjoint) return fail("mcfd_residual_vjp: call mcfd_mesh_set_adjoint first");
+                      double* ubar, double* sbar, void* stream) {
+  return mcfd_residual_vjp_impl(m, st, u, lam, ubar, sbar, static_cast<hipStream_t>(stream));
+}
+
+int mcfd_lusgs_apply_transpose(McfdMesh* m, const double* v, double* out, void* stream) {
+  if (!m || !v || !out) return fail("mcfd_lusgs_apply_transpose: NULL argument");
+  if (!m->implicit) return fail("mcfd_lusgs_apply_transpose: call mcfd_mesh_set_implicit first");
+  return lusgs_apply(m, v, out, static_cast<hipStream_t>(stream), true);
+}
+
+int mcfd_adjoint_apply(McfdMesh* m, const McfdSettings* st, McfdGmres* g, const double* v, double* out,
+                       void* stream) {
+  if (!m || !g || !g->state || !g->dt || !v || !out) return fail("mcfd_adjoint_apply: NULL argument");
+  if (!m->adjoint) return fail("mcfd_adjoint_apply: call mcfd_mesh_set_adjoint first");
   if (check_settings(st)) return -1;
-  if (st->reconstruction == 2) return fail("mcfd_residual_vjp: MUSCL parabolic has no adjoint yet");
-  if (st->viscosity) return fail("mcfd_residual_vjp: viscous terms have no adjoint yet");
+  if (ensure_gmres(m, 1)) return -1;
+  JacobianOp op{m, st, g->state, nullptr, g->dt, g->dt_per_cell, {1.0, 1.0, 1.0, 1.0}, 0.0};
+  op.adjoint = true;
+  return adjoint_apply(op, v, out, static_cast<hipStream_t>(stream));
+}
+
+int mcfd_adjoint_gmres(McfdMesh* m, const McfdSettings* st, McfdGmres* g, void* stream) {
+  if (!m || !g || !g->state || !g->rhs || !g->x || !g->dt)
+    return fail("mcfd_adjoint_gmres: NULL argument");
+  if (!m->implicit) return fail("mcfd_adjoint_gmres: call mcfd_mesh_set_implicit first");
+  if (!m->adjoint) return fail("mcfd_adjoint_gmres: call mcfd_mesh_set_adjoint first");
+  if (check_settings(st)) return -1;
+  if (g->krylov <= 0 || g->max_iterations <= 0) return fail("mcfd_adjoint_gmres: bad iteration limits");
+  if (ensure_gmres(m, g->krylov)) return -1;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  int n = m->n_cells;
-  bool muscl = st->reconstruction == 1 && m->n_grad > 0;
-  hipLaunchKernelGGL(k_primitive, dim3(blocks(n)), dim3(kBlock), 0, s, n, u, m->W, st->gamma,
-                     m->status);
-  if (muscl)
-    hipLaunchKernelGGL(k_gradient, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->grad_ptr,
-                       m->grad_nb, m->grad_cx, m->grad_cy, m->lim_ptr, m->lim_dx, m->lim_dy,
-                       st->limiter, m->G, m->PHI);
-  if (m->n_faces)
-    hipLaunchKernelGGL(k_flux_adjoint, dim3(blocks(m->n_faces)), dim3(kBlock), 0, s, m->n_faces,
-                       m->faces, m->W, m->G, m->PHI, m->face_len, m->area, lam, st->flux,
-                       muscl ? 1 : 0, st->gamma, m->FADJ);
-  hipLaunchKernelGGL(k_cell_adjoint1, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->G, m->PHI,
-                     m->FADJ, m->adj, muscl ? 1 : 0, st->limiter);
-  hipLaunchKernelGGL(k_cell_adjoint2, dim3(blocks(n)), dim3(kBlock), 0, s, n, u, m->W, lam,
-                     m->adj, muscl ? 1 : 0, st->gamma, st->acceleration_x, st->acceleration_y,
-                     ubar);
-  HIP_CHECK(hipGetLastError());
+  long N = 4L * m->n_cells;
+  HIP_CHECK(hipMemsetAsync(g->x, 0, sizeof(double) * N, s));
+  JacobianOp op{m, st, g->state, nullptr, g->dt, g->dt_per_cell, {1.0, 1.0, 1.0, 1.0}, 0.0};
+  op.adjoint = true;
+  int used = 0;
+  bool nonzero = false;
+  g->info = 1;
+  g->iterations = 0;
+  while (used < g->max_iterations) {
+    int restart = std::min(g->krylov, g->max_iterations - used);
+    int inner = 0, info = 1;
+    double rnorm = 0.0;
+    if (gmres_cycle(op, g->rhs, g->x, g->tolerance, restart, nonzero, inner, info, rnorm, s))
+      return -1;
+    nonzero = true;
+    used += inner;
+    g->iterations = used;
+    g->residual = rnorm;
+    if (info == 0) {
+      g->info = 0;
+      return 0;
+    }
+    if (inner == 0) break;
+  }
+  g->info = 1;
   return 0;
 }
 

@@ -14,6 +14,7 @@ arithmetic is in libmcfdhip.so.
 from __future__ import annotations
 
 import ctypes as C
+import math
 
 import numpy as np
 import torch
@@ -195,6 +196,46 @@ class DeviceMesh:
         desc, keep = native.block_desc(self.blocks)
         native.check(self.lib.mcfd_mesh_set_blocks(self.handle, C.byref(desc)))
         del keep
+
+    def autotune(self, candidates=(64, 96, 112, 128), steps: int = 3, repeats: int = 2) -> int:
+        """Pick the fused kernel's block size: for each candidate (cells per
+        block, each block also limited to 256 faces) time ``steps`` explicit
+        steps with HIP events, ``repeats`` times, from the same state; keep
+        the fastest and restore the state. Returns the chosen size."""
+        saved = (self.u.clone(), self.clock.clone(), self.dt_fresh)
+        stream = torch.cuda.current_stream(self.device)
+        best, best_ms = None, math.inf
+        self.timings = {}
+        for b in candidates:
+            try:
+                self.set_blocks(b)
+            except (ValueError, native.NativeError):
+                continue
+            times = []
+            for _ in range(repeats):
+                self.u.copy_(saved[0])
+                self.clock.copy_(saved[1])
+                self.dt_fresh = False
+                self.step()  # warm-up, first-call set-up
+                start = torch.cuda.Event(enable_timing=True)
+                stop = torch.cuda.Event(enable_timing=True)
+                start.record(stream)
+                for _ in range(steps):
+                    self.step()
+                stop.record(stream)
+                stop.synchronize()
+                times.append(start.elapsed_time(stop) / steps)
+            self.timings[b] = min(times)
+            if self.timings[b] < best_ms:
+                best, best_ms = b, self.timings[b]
+        if best is None:
+            raise SolverError("No block size fits this mesh.")
+        self.set_blocks(best)
+        self.u.copy_(saved[0])
+        self.clock.copy_(saved[1])
+        self.dt_fresh = saved[2]
+        self.check_status("autotune: ")
+        return best
 
     def close(self):
         if getattr(self, "handle", None):

@@ -63,6 +63,10 @@ def implicit_desc(g: Geometry, L: DeviceLayout):
     lower = off_cols < off_cells  # natural (mesh) order
     n_lower = np.bincount(new_id[off_cells[lower]], minlength=n)
     off_split = off_ptr[:-1] + n_lower
+    # item of (record, side) -> position; the transposed entry is the other side's item
+    pos = np.empty(2 * (ni + npair), dtype=np.int64)
+    pos[off_items] = np.arange(len(off_items))
+    off_t = pos[off_items ^ 1]
     return dict(
         n_records=len(rec_o),
         rec_o=new_id[rec_o],
@@ -79,6 +83,7 @@ def implicit_desc(g: Geometry, L: DeviceLayout):
         off_item=off_items,
         off_nb=new_id[off_cols],
         natural_order=new_id,
+        off_t=off_t,
     )
 
 
@@ -100,6 +105,7 @@ class McfdImplicitDesc(C.Structure):
         ("off_item", C.c_void_p),
         ("off_nb", C.c_void_p),
         ("natural_order", C.c_void_p),
+        ("off_t", C.c_void_p),
     ]
 
 
@@ -146,6 +152,14 @@ native.EXPORTS.update(
             [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.POINTER(C.c_double), C.c_void_p, C.c_void_p],
         ),
         "mcfd_norm2": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "mcfd_lusgs_apply_transpose": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "mcfd_adjoint_apply": (
+            C.c_int,
+            [C.c_void_p, C.POINTER(native.McfdSettings), C.POINTER(McfdGmres), C.c_void_p, C.c_void_p, C.c_void_p],
+        ),
+        "mcfd_adjoint_gmres": (
+            C.c_int, [C.c_void_p, C.POINTER(native.McfdSettings), C.POINTER(McfdGmres), C.c_void_p]
+        ),
     }
 )
 
@@ -161,7 +175,8 @@ class ImplicitStepper:
         self.dm = dm
         self.lib = native.load()
         for name in ("mcfd_mesh_set_implicit", "mcfd_lusgs_assemble", "mcfd_lusgs_apply", "mcfd_jacobian_apply",
-                     "mcfd_gmres", "mcfd_newton_equation", "mcfd_newton_candidate", "mcfd_norm2"):
+                     "mcfd_gmres", "mcfd_newton_equation", "mcfd_newton_candidate", "mcfd_norm2",
+                     "mcfd_lusgs_apply_transpose", "mcfd_adjoint_apply", "mcfd_adjoint_gmres"):
             restype, argtypes = native.EXPORTS[name]
             fn = getattr(self.lib, name)
             fn.restype, fn.argtypes = restype, argtypes
@@ -190,6 +205,7 @@ class ImplicitStepper:
             off_item=arr(d["off_item"], np.int32),
             off_nb=arr(d["off_nb"], np.int32),
             natural_order=arr(d["natural_order"], np.int32),
+            off_t=arr(d["off_t"], np.int32),
         )
         native.check(self.lib.mcfd_mesh_set_implicit(dm.handle, C.byref(desc)))
         del keep

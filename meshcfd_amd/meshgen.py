@@ -320,3 +320,154 @@ def riemann_workload(n: int = 1000, steps: int = 200, split_fraction: float = 0.
     case = Case(mesh=mesh, boundary_conditions=bcs, numerics=numerics)
     primitive = four_quadrant(cell_centroids(mesh))
     return case, primitive
+
+
+def geometric_radii(n_r: int, r_inner: float, r_outer: float, first_spacing: float) -> np.ndarray:
+    """n_r + 1 radii from r_inner to r_outer whose spacings grow by a constant
+    ratio q from ``first_spacing`` (q found by bisection; uniform if the first
+    spacing is already at least the mean spacing)."""
+    span = r_outer - r_inner
+    if first_spacing * n_r >= span:
+        return np.linspace(r_inner, r_outer, n_r + 1)
+
+    def total(q):
+        with np.errstate(over="ignore"):
+            return first_spacing * (float(np.power(np.float64(q), n_r)) - 1.0) / (q - 1.0)
+
+    lo, hi = 1.0 + 1e-15, 2.0
+    while total(hi) < span:
+        hi *= 2.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if total(mid) < span:
+            lo = mid
+        else:
+            hi = mid
+    q = 0.5 * (lo + hi)
+    r = r_inner + first_spacing * (q ** np.arange(n_r + 1) - 1.0) / (q - 1.0)
+    r[-1] = r_outer
+    return r
+
+
+def cylinder_ogrid(
+    n_theta: int,
+    n_r: int,
+    r_inner: float = 0.5,
+    r_outer: float = 30.0,
+    first_spacing: float = 1e-3,
+    jitter: float = 0.05,
+    seed: int | None = 0,
+) -> FiniteVolumeMesh:
+    """O-grid of quads around a circle: ``n_theta`` cells around, ``n_r``
+    rings outward with geometric radial stretching (first spacing
+    ``first_spacing``). The angular wrap shares nodes, so it is an ordinary
+    interior seam (no periodic pair). Nodes of the interior rings are
+    jittered by U(-jitter, jitter) times the local radial spacing (the smaller
+    of the two adjacent ones) and the angular spacing; the wall and
+    far-field rings are exact circles. Cells are numbered ring by ring from
+    the wall, CCW. Patches: 0 "wall" (inner circle), 1 "farfield"."""
+    if n_theta < 3 or n_r < 1:
+        raise ValueError("O-grid needs n_theta >= 3 and n_r >= 1")
+    r = geometric_radii(n_r, r_inner, r_outer, first_spacing)
+    dth = 2.0 * math.pi / n_theta
+    theta = dth * np.arange(n_theta)
+    rr = np.broadcast_to(r[:, None], (n_r + 1, n_theta)).copy()
+    tt = np.broadcast_to(theta[None, :], (n_r + 1, n_theta)).copy()
+    if jitter > 0.0 and n_r > 1:
+        rng = np.random.default_rng(seed)
+        dr = np.diff(r)
+        h = np.minimum(dr[:-1], dr[1:])  # interior rings 1..n_r-1
+        off = rng.uniform(-jitter, jitter, size=(n_r - 1, n_theta, 2))
+        rr[1:-1] += off[..., 0] * h[:, None]
+        tt[1:-1] += off[..., 1] * dth
+    points = np.column_stack(((rr * np.cos(tt)).ravel(), (rr * np.sin(tt)).ravel()))
+    k, j = np.meshgrid(np.arange(n_r), np.arange(n_theta), indexing="ij")
+    jn = (j + 1) % n_theta
+    a = (k * n_theta + j).ravel()  # ring k, angle j
+    b = ((k + 1) * n_theta + j).ravel()  # ring k+1, angle j
+    c = ((k + 1) * n_theta + jn).ravel()
+    d = (k * n_theta + jn).ravel()
+    # (theta, r) -> (x, y) reverses orientation: going outward first is CCW
+    cells = np.column_stack((a, b, c, d)).astype(np.int64)
+    sizes = np.full(len(cells), 4, dtype=np.int64)
+    faces, owners, neighbours = _faces_from_cells(cells, sizes)
+    ring = np.repeat(np.arange(n_r + 1), n_theta)
+    f0, f1 = faces[:, 0], faces[:, 1]
+    boundary = neighbours < 0
+    patch_ids = np.full(len(faces), -1, dtype=np.int64)
+    patch_ids[boundary & (ring[f0] == 0) & (ring[f1] == 0)] = 0
+    patch_ids[boundary & (ring[f0] == n_r) & (ring[f1] == n_r)] = 1
+    if np.any(patch_ids[boundary] < 0):
+        raise ValueError("Boundary face off the O-grid's circles.")
+    patches = tuple(MeshPatch(pid, name, np.flatnonzero(patch_ids == pid)) for pid, name in enumerate(("wall", "farfield")))
+    return FiniteVolumeMesh(
+        points=points,
+        cells=cells,
+        cell_sizes=sizes,
+        faces=faces,
+        face_owners=owners,
+        face_neighbours=neighbours,
+        face_patch_ids=patch_ids,
+        patches=patches,
+    )
+
+
+def free_stream(mach: float, alpha_deg: float, gamma: float = 1.4, density: float = 1.0,
+                pressure: float | None = None):
+    """(rho, u, v, p) of a free stream at Mach ``mach`` and angle ``alpha_deg``;
+    by default p = 1/gamma so that the sound speed is 1."""
+    p = 1.0 / gamma if pressure is None else pressure
+    c = math.sqrt(gamma * p / density)
+    a = math.radians(alpha_deg)
+    return (density, mach * c * math.cos(a), mach * c * math.sin(a), p)
+
+
+def cylinder_case(
+    n_theta: int = 2048,
+    n_r: int = 2048,
+    mach: float = 0.2,
+    alpha_deg: float = 0.0,
+    reynolds: float | None = 40.0,
+    r_outer: float = 30.0,
+    first_spacing: float = 1e-3,
+    jitter: float = 0.05,
+    seed: int = 0,
+    numerics: NumericalSettings | None = None,
+):
+    """Flow past the unit-diameter cylinder on `cylinder_ogrid`.
+
+    Laminar (``reynolds`` given): air with Sutherland viscosity at
+    T = 288.15 K, the density chosen so that Re = rho U D / mu, a no-slip wall
+    at the free-stream temperature, far field outside; Roe + MUSCL parabolic,
+    implicit Euler, local steps, CFL 100, GMRES(30) rtol 1e-3. Inviscid
+    (``reynolds=None``): unit free stream (rho = 1, c = 1), slip wall.
+    Returns (case, initial primitive state = free stream everywhere)."""
+    mesh = cylinder_ogrid(n_theta, n_r, 0.5, r_outer, first_spacing, jitter, seed)
+    gamma = 1.4
+    if numerics is None:
+        numerics = NumericalSettings(
+            flux="Roe", reconstruction="MUSCL parabolic", time_integration="Implicit Euler",
+            time_step_mode="Local (steady)", cfl=100.0, gmres_krylov_dimension=30, gmres_tolerance=1e-3,
+            residual_tolerance=1e-10, end_time=1e30,
+        )
+    if reynolds is not None:
+        numerics.viscosity_model = "Sutherland"
+        numerics.prandtl_number = 0.72
+        R = numerics.gas_constant
+        T = 288.15
+        tr = numerics.sutherland_reference_temperature
+        mu = numerics.sutherland_reference_viscosity * (T / tr) ** 1.5 * (tr + numerics.sutherland_temperature) / (
+            T + numerics.sutherland_temperature)
+        c = math.sqrt(gamma * R * T)
+        speed = mach * c
+        rho = reynolds * mu / (speed * 1.0)
+        fs = free_stream(mach, alpha_deg, gamma, density=rho, pressure=rho * R * T)
+        wall = BoundaryCondition(0, "wall", "Wall", {"temperature": T}, "Temperature", "No-slip")
+    else:
+        fs = free_stream(mach, alpha_deg, gamma)
+        wall = BoundaryCondition(0, "wall", "Wall", {}, "Temperature", "Slip")
+    far = BoundaryCondition(1, "farfield", "Far Field",
+                            {"density": fs[0], "velocity_x": fs[1], "velocity_y": fs[2], "pressure": fs[3]})
+    case = Case(mesh=mesh, boundary_conditions=[wall, far], numerics=numerics, name="cylinder")
+    prim = np.tile(np.asarray(fs, dtype=float), (mesh.cell_count, 1))
+    return case, prim

@@ -190,3 +190,14 @@ def implicit_euler_step(g, settings, initial, time_step, allow_reduction=True):
     if not allow_reduction:
         raise OracleError(f"Implicit Euler could not converge at the configured fixed time step: {last}")
     raise OracleError(f"Implicit Euler could not converge after reducing the time step 12 times; last failure: {last}")
+
+
+def lusgs_apply_transpose(S, vector, variables=4):
+    """M^T v = L^-T (D U^-T v) for the M of `lusgs_apply` (test reference of
+    the adjoint preconditioner; no counterpart in the reference)."""
+    block = sp.kron(S, sp.eye(variables, format="csc"), format="csc")
+    lower = sp.tril(block, format="csr")
+    upper = sp.triu(block, format="csr")
+    d = np.maximum(np.repeat(S.diagonal(), variables), 1e-14)
+    first = spsolve_triangular(upper.T.tocsr(), vector, lower=True)
+    return spsolve_triangular(lower.T.tocsr(), d * first, lower=False)

@@ -28,6 +28,7 @@ def test_library_exports_every_declared_symbol():
     import meshcfd_amd.adjoint  # noqa: F401  (registers its entry points)
     import meshcfd_amd.implicit  # noqa: F401
     import meshcfd_amd.viscous  # noqa: F401
+    import meshcfd_amd.steady_adjoint  # noqa: F401
 
     lib = native.load()
     for name in declared_symbols():

@@ -1,0 +1,240 @@
+"""Steady adjoint: O-grid generator, transposed LU-SGS, adjoint operator,
+adjoint GMRES and inflow gradients of wall forces.
+
+The adjoint has no counterpart in the reference. It is pinned by
+identities: the transposed LU-SGS against the oracle's dense-factor
+transpose and the dot-product test <M^T a, b> = <a, M b>; the adjoint
+operator by <A^T v, w> = <v, A w> with A w from central differences of the
+device residual; the adjoint GMRES by its true residual; and the inflow
+Mach/angle gradients of the wall force against central finite differences
+of re-converged steady states.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from meshcfd_amd import meshgen
+from meshcfd_amd.geometry import build_geometry
+from meshcfd_amd.model import NumericalSettings
+from oracle import implicit as oimp
+from oracle.euler import conservative_to_primitive, primitive_to_conservative
+
+
+def test_geometric_radii_ratio_and_ends():
+    r = meshgen.geometric_radii(64, 0.5, 30.0, 1e-3)
+    d = np.diff(r)
+    assert r[0] == 0.5 and r[-1] == 30.0
+    assert d[0] == pytest.approx(1e-3, rel=1e-12)
+    np.testing.assert_allclose(d[1:-1] / d[:-2], d[1] / d[0], rtol=1e-9)
+    np.testing.assert_allclose(meshgen.geometric_radii(4, 0.0, 1.0, 0.5), [0, 0.25, 0.5, 0.75, 1.0])
+
+
+def test_ogrid_covers_the_annulus_polygon():
+    nt, nr = 40, 12
+    m = meshgen.cylinder_ogrid(nt, nr, 0.5, 10.0, 0.02, 0.05, 3)
+    case, _ = meshgen.cylinder_case(nt, nr, reynolds=None, r_outer=10.0, first_spacing=0.02, seed=3)
+    g = build_geometry(case)
+    assert m.cell_count == nt * nr
+    assert np.all(g.areas > 0.0)
+    ring = 0.5 * nt * math.sin(2 * math.pi / nt) * (10.0**2 - 0.5**2)  # polygon annulus
+    assert g.areas.sum() == pytest.approx(ring, rel=1e-12)
+    assert len(g.wall_faces) == nt and len(g.far_field_faces) == nt
+    assert len(g.interior_faces) == 2 * nt * nr - nt
+    # wall normals point into the cylinder, far-field normals outward
+    w = g.wall_faces
+    assert np.all(np.sum(g.face_normals[w] * g.face_centers[w], axis=1) < 0)
+    f = g.far_field_faces
+    assert np.all(np.sum(g.face_normals[f] * g.face_centers[f], axis=1) > 0)
+
+
+def test_oracle_lusgs_transpose_is_the_adjoint():
+    case, prim = meshgen.cylinder_case(16, 6, reynolds=None, r_outer=5.0, first_spacing=0.05, seed=1)
+    g = build_geometry(case)
+    rng = np.random.default_rng(0)
+    prim = prim * (1.0 + 0.05 * rng.random(prim.shape))
+    h = 0.02 + 0.01 * rng.random(g.cell_count)
+    S, _ = oimp.lusgs_matrix(prim, g, 1.4, h)
+    a = rng.normal(size=4 * g.cell_count)
+    b = rng.normal(size=4 * g.cell_count)
+    lhs = np.dot(oimp.lusgs_apply_transpose(S, a), b)
+    rhs = np.dot(a, oimp.lusgs_apply(S, b))
+    assert lhs == pytest.approx(rhs, rel=1e-12)
+
+
+def test_free_stream_derivatives_by_difference():
+    from meshcfd_amd.steady_adjoint import free_stream_derivatives
+
+    dm_, da_ = free_stream_derivatives(0.3, 7.0, 1.4, 1.0, 1.0 / 1.4)
+    h = 1e-6
+    fd_m = (np.array(meshgen.free_stream(0.3 + h, 7.0)) - np.array(meshgen.free_stream(0.3 - h, 7.0))) / (2 * h)
+    hd = math.degrees(h)
+    fd_a = (np.array(meshgen.free_stream(0.3, 7.0 + hd)) - np.array(meshgen.free_stream(0.3, 7.0 - hd))) / (2 * h)
+    np.testing.assert_allclose(dm_, fd_m, atol=1e-8)
+    np.testing.assert_allclose(da_, fd_a, atol=1e-8)
+
+
+def _inviscid_cylinder(n_theta=48, n_r=24, mach=0.3, alpha=5.0, reconstruction="First order",
+                       limiter="Barth-Jespersen"):
+    numerics = NumericalSettings(
+        flux="HLLC", reconstruction=reconstruction, limiter=limiter, time_integration="Implicit Euler",
+        time_step_mode="Local (steady)", cfl=50.0, gmres_tolerance=1e-6, gmres_krylov_dimension=30,
+        residual_tolerance=1e-12, max_iterations=400, end_time=1e30,
+    )
+    return meshgen.cylinder_case(n_theta, n_r, mach=mach, alpha_deg=alpha, reynolds=None, r_outer=10.0,
+                                 first_spacing=0.02, seed=5, numerics=numerics)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("per_cell", [False, True])
+def test_gpu_lusgs_transpose_matches_oracle(cuda_device, per_cell):
+    import torch
+
+    from meshcfd_amd.device import DeviceMesh
+    from meshcfd_amd.implicit import ImplicitStepper
+
+    case, prim = _inviscid_cylinder(24, 10)
+    print("case: cylinder 24x10 transposed LU-SGS", flush=True)
+    dm = DeviceMesh(case, cuda_device, fused=True, block_cells=64)
+    st = ImplicitStepper(dm)
+    rng = np.random.default_rng(4)
+    prim = prim * (1.0 + 0.05 * rng.random(prim.shape))
+    u = primitive_to_conservative(prim, 1.4)
+    n = dm.n
+    h = 0.01 + 0.04 * rng.random(n) if per_cell else 0.03
+    S, _ = oimp.lusgs_matrix(prim, dm.geometry, 1.4, h)
+    v = rng.normal(size=u.shape)
+    want = oimp.lusgs_apply_transpose(S, v.ravel()).reshape(v.shape)
+    dt = dm.to_device_order(h) if per_cell else torch.full((1,), h, dtype=torch.float64, device=cuda_device)
+    st.assemble(dm.to_device_order(u), dt, per_cell)
+    out = torch.empty((n, 4), dtype=torch.float64, device=cuda_device)
+    import ctypes as C
+
+    dm.lib.mcfd_lusgs_apply_transpose(dm.handle, C.c_void_p(dm.to_device_order(v).data_ptr()),
+                                      C.c_void_p(out.data_ptr()), dm.stream)
+    dm.check_status()
+    np.testing.assert_allclose(dm.to_mesh_order(out), want, rtol=1e-12, atol=1e-12)
+    dm.close()
+
+
+def _device_state(reconstruction="MUSCL linear", limiter="Venkatakrishnan"):
+    from meshcfd_amd.device import DeviceMesh
+
+    case, prim = _inviscid_cylinder(32, 12, reconstruction=reconstruction, limiter=limiter)
+    rng = np.random.default_rng(7)
+    prim = prim * (1.0 + 0.03 * rng.random(prim.shape))
+    dm = DeviceMesh(case, "cuda:0", fused=True, block_cells=64)
+    dm.set_state(primitive_to_conservative(prim, 1.4))
+    return dm
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("reconstruction", ["First order", "MUSCL linear"])
+def test_gpu_adjoint_operator_dot_product(cuda_device, reconstruction):
+    """<A^T v, w> = <v, A w>, A w = w - D (R(u + e w) - R(u - e w)) / 2e.
+    The limited MUSCL residual is only piecewise smooth (min/max and limiter
+    branches), so the difference step is small enough (1e-8 relative) not to
+    cross a branch on this noisy state."""
+    import torch
+
+    from meshcfd_amd.steady_adjoint import SteadyAdjoint
+
+    print(f"case: adjoint operator dot product {reconstruction}", flush=True)
+    dm = _device_state(reconstruction)
+    sa = SteadyAdjoint(dm, cfl=20.0)
+    u = dm.u.clone()
+    gen = torch.Generator(device="cpu").manual_seed(3)
+    v = torch.randn((dm.n, 4), generator=gen, dtype=torch.float64).to(cuda_device)
+    w = torch.randn((dm.n, 4), generator=gen, dtype=torch.float64).to(cuda_device)
+    at_v = sa.transpose_apply(u, v)
+    dt = sa.dt.clone()
+    eps = 1e-8 * float(torch.linalg.norm(u).item()) / float(torch.linalg.norm(w).item())
+    jw = (dm.residual(u + eps * w).clone() - dm.residual(u - eps * w).clone()) / (2 * eps)
+    a_w = w - dt[:, None] * jw
+    lhs = float(torch.sum(at_v * w).item())
+    rhs = float(torch.sum(v * a_w).item())
+    print("dot product", lhs, rhs)
+    assert lhs == pytest.approx(rhs, rel=1e-6)
+    dm.close()
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_gmres_true_residual(cuda_device):
+    import torch
+
+    from meshcfd_amd.steady_adjoint import SteadyAdjoint
+
+    print("case: adjoint GMRES true residual", flush=True)
+    dm = _device_state()
+    sa = SteadyAdjoint(dm, cfl=20.0, rtol=1e-8)
+    u = dm.u.clone()
+    gen = torch.Generator(device="cpu").manual_seed(5)
+    b = torch.randn((dm.n, 4), generator=gen, dtype=torch.float64).to(cuda_device)
+    sa._time_steps(u)
+    sa.stepper.assemble(u, sa.dt, True)
+    import ctypes as C
+
+    from meshcfd_amd import native
+    from meshcfd_amd.implicit import McfdGmres
+
+    x = torch.empty_like(b)
+    g = McfdGmres()
+    g.state, g.rhs, g.x, g.dt = (C.c_void_p(t.data_ptr()) for t in (u, b, x, sa.dt))
+    g.dt_per_cell, g.krylov, g.max_iterations, g.tolerance = 1, 30, 600, 1e-8
+    native.check(dm.lib.mcfd_adjoint_gmres(dm.handle, C.byref(sa.settings), C.byref(g), dm.stream))
+    assert g.info == 0, (g.iterations, g.residual)
+    r = sa.transpose_apply(u, x) - b
+    rel = float(torch.linalg.norm(r).item() / torch.linalg.norm(b).item())
+    print("adjoint gmres", g.iterations, rel)
+    assert rel <= 1.5e-8
+    dm.close()
+
+
+def _steady_force(mach, alpha, warm=None, lift=False):
+    from meshcfd_amd.device import DeviceMesh
+    from meshcfd_amd.steady_adjoint import WallForce, solve_steady
+
+    case, prim = _inviscid_cylinder(mach=mach, alpha=alpha)
+    dm = DeviceMesh(case, "cuda:0", fused=True, block_cells=64)
+    dm.set_state(primitive_to_conservative(prim, 1.4) if warm is None else warm)
+    hist = solve_steady(dm, tol=1e-13, max_steps=400, raise_cfl=100.0)
+    assert hist[-1] <= 1e-13, hist[-5:]
+    force = WallForce(dm, alpha, lift=lift)
+    return dm, force, float(force(dm.u).item()), hist
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lift", [False, True])
+def test_gpu_wall_force_inflow_gradients_match_fd(cuda_device, lift):
+    """Inviscid cylinder, HLLC first order, M 0.3, alpha 5 deg, 48x24 O-grid:
+    d(force)/dM and d(force)/d(alpha) from one adjoint solve equal central
+    differences of re-converged steady states (step 1e-4) to 1e-5."""
+    from meshcfd_amd.steady_adjoint import SteadyAdjoint, free_stream_derivatives, objective_gradient
+
+    print(f"case: cylinder wall force gradients lift={lift}", flush=True)
+    mach, alpha = 0.3, 5.0
+    dm, force, j0, hist = _steady_force(mach, alpha, lift=lift)
+    print("steady", len(hist), "steps, rms", hist[-1], "force", j0)
+    u_star = dm.state()
+    sa = SteadyAdjoint(dm, cfl=1e3, rtol=1e-4)
+    _, dj_du = objective_gradient(force, dm.u)
+    lam, ahist = sa.solve(dj_du, tol=1e-12, max_iterations=60)
+    assert ahist[-1] <= 1e-12, ahist[-5:]
+    sbar = sa.inflow_gradient(lam)
+    d_mach, d_alpha = free_stream_derivatives(mach, alpha, 1.4, 1.0, 1.0 / 1.4)
+    g_mach = float(sbar @ d_mach)
+    g_alpha = float(sbar @ d_alpha) + force.d_alpha(dm.u)
+    print("adjoint", g_mach, g_alpha, "iterations", len(ahist), "gmres", sa.gmres_iterations)
+    dm.close()
+    h = 1e-4
+    fd = []
+    for dmach, dalpha in ((h, 0.0), (-h, 0.0), (0.0, math.degrees(h)), (0.0, -math.degrees(h))):
+        d2, _, j, _ = _steady_force(mach + dmach, alpha + dalpha, warm=u_star, lift=lift)
+        d2.close()
+        fd.append(j)
+    fd_mach = (fd[0] - fd[1]) / (2 * h)
+    fd_alpha = (fd[2] - fd[3]) / (2 * h)
+    print("fd", fd_mach, fd_alpha)
+    assert g_mach == pytest.approx(fd_mach, rel=1e-5, abs=1e-9)
+    assert g_alpha == pytest.approx(fd_alpha, rel=1e-5, abs=1e-9)
