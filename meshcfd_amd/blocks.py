@@ -33,6 +33,7 @@ from .layout import DeviceLayout
 INFO_FIELDS = 12
 FACE_RECORD = np.dtype([("geom", np.float64, 7), ("kind_bc", np.int32), ("lr", np.uint32)])  # 64 bytes
 MAX_BLOCK_FACES = 256  # one flux face per thread of the 256-thread workgroup
+MAX_BLOCK_CELLS = 128  # two threads per own cell in the gather/update phase
 
 
 def partition(L: DeviceLayout, max_cells: int, max_faces: int = MAX_BLOCK_FACES) -> np.ndarray:
@@ -113,7 +114,7 @@ def build_blocks(L: DeviceLayout, block_cells: int = 128) -> BlockLayout:
     B = block_cells
     if B < 1:
         raise ValueError("block_cells must be positive")
-    bstart = partition(L, min(B, MAX_BLOCK_FACES))
+    bstart = partition(L, min(B, MAX_BLOCK_CELLS))
     nb = len(bstart) - 1
     blk = np.repeat(np.arange(nb, dtype=np.int64), np.diff(bstart))
     gptr = L.grad_ptr.astype(np.int64)

@@ -1369,23 +1369,26 @@ __global__ void __launch_bounds__(kBlock)
     if (err) report(status, err, ba.faces[fs + f]);
     flen = fr.len;
   }
-  // P3's per-cell loads, issued before the barrier
+  // P3's per-cell loads, issued before the barrier. P3 runs two threads per
+  // own cell: thread t handles components 2*(t&1) and 2*(t&1)+1 of own cell
+  // t>>1 (blocks have at most kBlock/2 own cells).
+  const int j3 = t >> 1, hf = t & 1;
   int a0 = 0, a1 = 0, d0 = 0, d1 = 0;
   double A = 1.0, h = 0.0;
-  P4 u{0.0, 0.0, 0.0, 0.0}, z{0.0, 0.0, 0.0, 0.0};
-  if (t < n_own) {
-    const int i = first + t;
+  double2 u2 = make_double2(0.0, 0.0), z2 = make_double2(0.0, 0.0);
+  if (j3 < n_own) {
+    const int i = first + j3;
     a0 = ga.acc_ptr[i];
     a1 = ga.acc_ptr[i + 1];
     A = ga.area[i];
-    if (DTNEXT) {
+    if (DTNEXT && hf == 0) {
       d0 = da.ptr[i];
       d1 = da.ptr[i + 1];
     }
     if (s.op != MCFD_OP_RESIDUAL) {
       h = s.dt_per_cell ? s.dt[i] : s.dt[0];
-      u = load4(U, i);
-      if (s.op != MCFD_OP_EULER) z = load4(s.u0, i);
+      u2 = reinterpret_cast<const double2*>(U + 4 * (long)i)[hf];
+      if (s.op != MCFD_OP_EULER) z2 = reinterpret_cast<const double2*>(s.u0 + 4 * (long)i)[hf];
     }
   }
   if constexpr (MUSCL) __syncthreads();
@@ -1402,87 +1405,81 @@ __global__ void __launch_bounds__(kBlock)
   }
   __syncthreads();
 
-  // P3: gather (solver.py:2501-2704) + body force + stage update
+  // P3: gather (solver.py:2501-2704) + body force + stage update, two
+  // components per thread in the reference order of each component
   double dt_local = INFINITY;
-  if (t < n_own) {
-    const int j = t;
-    const int i = first + j;
-    double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+  if (j3 < n_own) {
+    const int i = first + j3;
+    double ra = 0.0, rb = 0.0;
     for (int e = a0; e < a1; ++e) {
       int item = ba.acc_local[e];
       int slot = item >> 2;
       int mode = item & 3;
-      P4 F = lds4(sF, slot);
+      double2 F = reinterpret_cast<const double2*>(sF + 4 * slot)[hf];
       if (mode <= MCFD_ACC_NEIGHBOUR) {
-        double c = ga.acc_coef[e];
-        r0 = r0 + c * F.r;
-        r1 = r1 + c * F.u;
-        r2 = r2 + c * F.v;
-        r3 = r3 + c * F.p;
+        double c = ga.acc_coef[e];  // (-L)/A or L/A, precomputed like the reference's CSR data
+        ra = ra + c * F.x;
+        rb = rb + c * F.y;
         continue;
       }
       double L = sL[slot];
       if (mode == MCFD_ACC_BOUNDARY) {
-        r0 = r0 + (-(F.r * L)) / A;
-        r1 = r1 + (-(F.u * L)) / A;
-        r2 = r2 + (-(F.v * L)) / A;
-        r3 = r3 + (-(F.p * L)) / A;
+        ra = ra + (-(F.x * L)) / A;
+        rb = rb + (-(F.y * L)) / A;
       } else {
-        r0 = r0 + (F.r * L) / A;
-        r1 = r1 + (F.u * L) / A;
-        r2 = r2 + (F.v * L) / A;
-        r3 = r3 + (F.p * L) / A;
+        ra = ra + (F.x * L) / A;
+        rb = rb + (F.y * L) / A;
       }
     }
     if (ax != 0.0 || ay != 0.0) {
-      P4 w = lds4(sW, j);
-      r1 = r1 + w.r * ax;
-      r2 = r2 + w.r * ay;
-      r3 = r3 + w.r * (ax * w.u + ay * w.v);
+      P4 w = lds4(sW, j3);
+      if (hf == 0) {
+        rb = rb + w.r * ax;
+      } else {
+        ra = ra + w.r * ay;
+        rb = rb + w.r * (ax * w.u + ay * w.v);
+      }
     }
-    if (!(finite(r0) && finite(r1) && finite(r2) && finite(r3)))
-      report(status, MCFD_ERR_RESIDUAL, i);
-    if (s.r_out) store4(s.r_out, i, r0, r1, r2, r3);
-    double o0 = 0.0, o1 = 0.0, o2 = 0.0, o3 = 0.0;
+    if (!(finite(ra) && finite(rb))) report(status, MCFD_ERR_RESIDUAL, i);
+    if (s.r_out) reinterpret_cast<double2*>(s.r_out + 4 * (long)i)[hf] = make_double2(ra, rb);
+    double oa = 0.0, ob = 0.0;
     if (s.op == MCFD_OP_EULER) {
       double k = s.c0 * h;
-      o0 = u.r + k * r0;
-      o1 = u.u + k * r1;
-      o2 = u.v + k * r2;
-      o3 = u.p + k * r3;
+      oa = u2.x + k * ra;
+      ob = u2.y + k * rb;
     } else if (s.op == MCFD_OP_SSP) {
-      o0 = s.c0 * z.r + s.c1 * (u.r + h * r0);
-      o1 = s.c0 * z.u + s.c1 * (u.u + h * r1);
-      o2 = s.c0 * z.v + s.c1 * (u.v + h * r2);
-      o3 = s.c0 * z.p + s.c1 * (u.p + h * r3);
+      oa = s.c0 * z2.x + s.c1 * (u2.x + h * ra);
+      ob = s.c0 * z2.y + s.c1 * (u2.y + h * rb);
     } else if (s.op == MCFD_OP_RK4) {
       double k = s.c2 * h;
-      o0 = s.c0 * z.r + s.c1 * u.r + k * r0;
-      o1 = s.c0 * z.u + s.c1 * u.u + k * r1;
-      o2 = s.c0 * z.v + s.c1 * u.v + k * r2;
-      o3 = s.c0 * z.p + s.c1 * u.p + k * r3;
+      oa = s.c0 * z2.x + s.c1 * u2.x + k * ra;
+      ob = s.c0 * z2.y + s.c1 * u2.y + k * rb;
     } else if (s.op == MCFD_OP_RK4_FINAL) {
-      P4 y = load4(s.u1, i);
-      P4 q = load4(s.r1, i);
+      double2 y = reinterpret_cast<const double2*>(s.u1 + 4 * (long)i)[hf];
+      double2 q = reinterpret_cast<const double2*>(s.r1 + 4 * (long)i)[hf];
       double k2 = s.c2 * h, k4 = s.c4 * h;
-      o0 = s.c0 * z.r + s.c1 * y.r + k2 * q.r + s.c3 * u.r + k4 * r0;
-      o1 = s.c0 * z.u + s.c1 * y.u + k2 * q.u + s.c3 * u.u + k4 * r1;
-      o2 = s.c0 * z.v + s.c1 * y.v + k2 * q.v + s.c3 * u.v + k4 * r2;
-      o3 = s.c0 * z.p + s.c1 * y.p + k2 * q.p + s.c3 * u.p + k4 * r3;
+      oa = s.c0 * z2.x + s.c1 * y.x + k2 * q.x + s.c3 * u2.x + k4 * ra;
+      ob = s.c0 * z2.y + s.c1 * y.y + k2 * q.y + s.c3 * u2.y + k4 * rb;
     }
-    if (s.op != MCFD_OP_RESIDUAL) store4(s.out, i, o0, o1, o2, o3);
+    if (s.op != MCFD_OP_RESIDUAL)
+      reinterpret_cast<double2*>(s.out + 4 * (long)i)[hf] = make_double2(oa, ob);
     if (DTNEXT && s.op != MCFD_OP_RESIDUAL) {
-      // next step's stable step from the updated state (euler.py:20-40 + solver.py:4132-4343)
-      double rho = o0;
-      double uu = o1 / rho;
-      double vv = o2 / rho;
-      double p = (g - 1.0) * (o3 - 0.5 * (o1 * o1 + o2 * o2) / rho);
-      if (!finite(rho) || !finite(uu) || !finite(vv) || !finite(p) || !(rho > 0.0) || !(p > 0.0))
-        report(status, MCFD_ERR_STATE, i);
-      double dt = block_time_step(i, P4{rho, uu, vv, p}, d0, d1, ba.dts_local, sN, sL, da, g,
-                                  s.cfl, status);
-      if (s.dt_next) s.dt_next[i] = dt;
-      dt_local = fmin(dt_local, dt);
+      // the pair's other half of the updated state
+      double pa = __shfl_xor(oa, 1), pb = __shfl_xor(ob, 1);
+      if (hf == 0) {
+        // next step's stable step from the updated state (euler.py:20-40 + solver.py:4132-4343)
+        double o0 = oa, o1 = ob, o2 = pa, o3 = pb;
+        double rho = o0;
+        double uu = o1 / rho;
+        double vv = o2 / rho;
+        double p = (g - 1.0) * (o3 - 0.5 * (o1 * o1 + o2 * o2) / rho);
+        if (!finite(rho) || !finite(uu) || !finite(vv) || !finite(p) || !(rho > 0.0) || !(p > 0.0))
+          report(status, MCFD_ERR_STATE, i);
+        double dt = block_time_step(i, P4{rho, uu, vv, p}, d0, d1, ba.dts_local, sN, sL, da, g,
+                                    s.cfl, status);
+        if (s.dt_next) s.dt_next[i] = dt;
+        dt_local = dt;
+      }
     }
   }
   if (DTNEXT) {
@@ -3294,13 +3291,13 @@ int mcfd_mesh_set_blocks(McfdMesh* m, const McfdBlockDesc* d) {
     return fail("mcfd_mesh_set_blocks: no blocks");
   {
     // own cells of block b are [info[b][8], info[b][8] + n_own) and tile the mesh;
-    // a block has at most kBlock own + ring-1 cells and touches at most kBlock
-    // flux faces (one per thread in each phase)
+    // a block has at most kBlock/2 own cells, kBlock own + ring-1 cells and
+    // touches at most kBlock flux faces (one thread per item in each phase)
     long next = 0;
     for (int b = 0; b < d->n_blocks; ++b) {
       const int32_t* inf = d->info + (size_t)kBlockInfo * b;
-      if (inf[8] != next || inf[1] <= 0 || inf[2] < 0 || inf[1] + inf[2] > kBlock || inf[5] < 0 ||
-          inf[5] > kBlock)
+      if (inf[8] != next || inf[1] <= 0 || inf[1] > kBlock / 2 || inf[2] < 0 ||
+          inf[1] + inf[2] > kBlock || inf[5] < 0 || inf[5] > kBlock)
         return fail("mcfd_mesh_set_blocks: blocks do not tile the mesh or touch too many faces");
       next += inf[1];
     }

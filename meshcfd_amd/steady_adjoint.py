@@ -45,7 +45,7 @@ def residual_rms(dm, u=None) -> float:
 
 
 def solve_steady(dm, tol: float | None = None, max_steps: int | None = None, raise_cfl: float = 0.0,
-                 max_cfl: float = 1e8):
+                 max_cfl: float = 1e8, progress=None):
     """Implicit-Euler pseudo-time steps (`DeviceMesh.step_implicit`, local or
     global steps as the case says) until the RMS residual is at most ``tol``
     (default: the case's residual tolerance). Returns the RMS history.
@@ -54,7 +54,8 @@ def solve_steady(dm, tol: float | None = None, max_steps: int | None = None, rai
     (`solver.py:1418-1425`), so at a given CFL the residual stalls near
     1e-9/dt. With ``raise_cfl`` > 1 the CFL number is multiplied by it
     whenever a step leaves the residual unchanged (up to ``max_cfl``), which
-    turns the last steps into Newton steps on R = 0."""
+    turns the last steps into Newton steps on R = 0. ``progress(step, rms,
+    cfl)`` is called after every step."""
     from .device import native_settings
 
     s = dm.settings
@@ -69,6 +70,8 @@ def solve_steady(dm, tol: float | None = None, max_steps: int | None = None, rai
                 break
             t = dm.step_implicit(t)
             history.append(residual_rms(dm))
+            if progress is not None:
+                progress(len(history) - 1, history[-1], s.cfl)
             if raise_cfl > 1.0 and history[-1] >= history[-2] and s.cfl < max_cfl:
                 s.cfl = min(s.cfl * raise_cfl, max_cfl)
                 dm.native_settings = native_settings(s)
@@ -198,7 +201,7 @@ class SteadyAdjoint:
         native.check(dm.lib.mcfd_time_steps(dm.handle, C.byref(self.dt_settings), _p(u), _p(self.dt),
                                             _p(self.dt_min), dm.stream))
 
-    def solve(self, dj_du, u=None, tol: float = 1e-10, max_iterations: int = 100):
+    def solve(self, dj_du, u=None, tol: float = 1e-10, max_iterations: int = 100, progress=None):
         """lambda with (dR/dU)^T lambda = -dJ/dU at u (default: the mesh state).
         Returns (lambda, history of ||(dR/dU)^T lambda + dJ/dU|| / ||dJ/dU||)."""
         dm = self.dm
@@ -216,6 +219,8 @@ class SteadyAdjoint:
             self.r.add_(dj_du)
             rel = float(torch.linalg.norm(self.r).item()) / g_norm
             history.append(rel)
+            if progress is not None:
+                progress(len(history) - 1, rel)
             if rel <= tol:
                 break
             g = McfdGmres()
