@@ -391,3 +391,30 @@ def test_gpu_viscous_implicit_step_matches_oracle(cuda_device, kind):
     print(kind, "implicit rel err", err)
     assert np.all(err <= 1e-8), err
     dm.close()
+
+
+@pytest.mark.gpu
+def test_gpu_laminar_cylinder_implicit_step_matches_oracle(cuda_device):
+    """Config-3 physics on a small O-grid (24 x 10, r to 5): Roe + MUSCL
+    parabolic + Sutherland, no-slip isothermal wall, far field, implicit
+    Euler with local steps: one step equals the oracle's to 1e-8."""
+    from oracle import implicit as oimp
+
+    case, prim = meshgen.cylinder_case(24, 10, mach=0.2, alpha_deg=0.0, reynolds=40.0, r_outer=5.0,
+                                       first_spacing=0.05, seed=2)
+    case.numerics.cfl = 20.0
+    case.numerics.gmres_tolerance = 1e-6
+    rng = np.random.default_rng(3)
+    prim = prim * (1.0 + 0.02 * rng.random(prim.shape))
+    dm = _device(case, cuda_device)
+    u0 = euler.primitive_to_conservative(prim, case.numerics.gamma)
+    dm.set_state(u0)
+    dm.step_implicit(0.0)
+    dm.check_status()
+    gpu = dm.state()
+    o = Oracle(dm.geometry, case.numerics)
+    ref, _ = oimp.implicit_euler_step(dm.geometry, case.numerics, u0, o.stable_time_steps(prim))
+    err = _rel(gpu, ref)
+    print("laminar cylinder implicit rel err", err)
+    assert np.all(err <= 1e-8), err
+    dm.close()
