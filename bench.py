@@ -68,6 +68,43 @@ def algorithmic_bytes(layout, reconstruction: str, integrator: str):
     return b_alg, kernels
 
 
+def adjoint_rate(dm, torch, repeats: int = 20):
+    """Adjoint throughput on the benchmark state: lambda^T dR/dU
+    (`mcfd_residual_vjp`, one full reverse residual per gradient) against one
+    forward residual (`mcfd_residual`), HIP events on the current stream."""
+    from meshcfd_amd import adjoint
+
+    adjoint.enable(dm)
+    u = dm.u.clone()
+    gen = torch.Generator(device="cpu").manual_seed(0)
+    lam = torch.randn(u.shape, generator=gen, dtype=torch.float64).to(u.device)
+    out = torch.empty_like(u)
+    r = torch.empty_like(u)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        fn()
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(repeats):
+            fn()
+        stop.record(stream)
+        stop.synchronize()
+        return start.elapsed_time(stop) / repeats
+
+    vjp_ms = timed(lambda: adjoint.residual_vjp(dm, u, lam, out=out))
+    res_ms = timed(lambda: dm.residual(u, out=r))
+    dm.check_status("adjoint timing: ")
+    return {
+        "grads_per_s": round(1e3 / vjp_ms, 1),
+        "vjp_ms": round(vjp_ms, 4),
+        "residual_ms": round(res_ms, 4),
+        "vjp_over_residual": round(vjp_ms / res_ms, 3),
+        "what": "lambda^T dR/dU of the full HLLC + MUSCL-linear residual at the benchmark state",
+    }
+
+
 def cpu_baseline(case, geometry, primitive, steps: int):
     """Time the reference-equivalent CPU path on one core: the NumPy/SciPy
     restatement of the solver with single-threaded C stand-ins for the
@@ -113,6 +150,67 @@ def cpu_baseline(case, geometry, primitive, steps: int):
     }
 
 
+def cylinder_bench(args) -> int:
+    """Implicit-Euler steps of the laminar cylinder (Mach 0.2, Re 40; Roe +
+    MUSCL parabolic + Sutherland, local steps at CFL 100, GMRES(30) rtol 1e-3
+    + LU-SGS) from the free stream on an n_theta x n_r O-grid: prints one
+    JSON line with implicit cell-updates per second (C x steps / t), the
+    Newton and GMRES iteration counts and the residual drop."""
+    import torch
+
+    from meshcfd_amd import meshgen
+    from meshcfd_amd.device import DeviceMesh
+    from meshcfd_amd.steady_adjoint import residual_rms
+
+    def note(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    case, prim = meshgen.cylinder_case(args.n_theta, args.n_r)
+    note(f"mesh {case.mesh.cell_count} cells in {time.perf_counter() - t0:.1f}s; building device data ...")
+    dm = DeviceMesh(case, "cuda:0", fused=False)  # parabolic + viscous stages use the four-kernel path
+    setup_s = time.perf_counter() - t0
+    note(f"setup {setup_s:.1f}s")
+    gamma = case.numerics.gamma
+    rho, u, v, p = prim.T
+    dm.set_state(np.column_stack((rho, rho * u, rho * v, p / (gamma - 1.0) + 0.5 * rho * (u**2 + v**2))))
+    # first step: warm-up (kernel loading, GMRES buffers)
+    t = dm.step_implicit(0.0)
+    r0 = residual_rms(dm)
+    note(f"warm-up step done, rms {r0:.3e}")
+    st = dm._implicit
+    n0, g0 = st.newton_iterations, st.gmres_iterations
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    for k in range(args.steps):
+        t = dm.step_implicit(t)
+        note(f"step {k + 1}/{args.steps}: {time.perf_counter() - start:.1f}s")
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - start
+    r1 = residual_rms(dm)
+    cells = dm.n
+    line = {
+        "metric": "implicit Mcell-updates/s (laminar cylinder, Roe + MUSCL parabolic + Sutherland, FP64)",
+        "value": round(cells * args.steps / secs / 1e6, 4),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "ms_per_step": round(secs / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "dtype": "f64",
+        "data": "synthetic (seeded O-grid, free-stream start)",
+        "config": {"workload": f"laminar cylinder O-grid {args.n_theta}x{args.n_r}, r 0.5..30, first spacing 1e-3, "
+                               "Mach 0.2, Re 40, implicit Euler local CFL 100, GMRES(30) rtol 1e-3, LU-SGS",
+                   "cells": cells},
+        "newton_per_step": round((st.newton_iterations - n0) / args.steps, 2),
+        "gmres_per_step": round((st.gmres_iterations - g0) / args.steps, 1),
+        "residual_rms": [r0, r1],
+        "setup_s": round(setup_s, 1),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +221,12 @@ def main(argv=None) -> int:
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--unfused", action="store_true", help="four-kernel path instead of the fused stage kernel")
     ap.add_argument("--block-cells", type=int, default=128, help="cells per workgroup of the fused stage kernel")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip timing the adjoint residual (VJP)")
+    ap.add_argument("--workload", choices=("riemann", "cylinder"), default="riemann",
+                    help="riemann: the explicit headline benchmark; cylinder: implicit steps of the laminar "
+                    "O-grid cylinder (Roe + MUSCL parabolic + Sutherland, LU-SGS + GMRES)")
+    ap.add_argument("--n-theta", type=int, default=2048, help="cylinder: cells around")
+    ap.add_argument("--n-r", type=int, default=2048, help="cylinder: cells outward")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use --block-cells as given instead of timing candidate block sizes first")
     ap.add_argument("--separate-time-step", action="store_true",
@@ -141,6 +245,8 @@ def main(argv=None) -> int:
             flush=True,
         )
         return 2
+    if args.workload == "cylinder":
+        return cylinder_bench(args)
 
     import torch
 
@@ -242,6 +348,11 @@ def main(argv=None) -> int:
             "avg_launch_ms": round(avg_ms, 5),
         }
 
+    adj = None
+    if not args.no_adjoint:
+        adj = adjoint_rate(dm, torch)
+        print(f"[bench] adjoint {adj}", file=sys.stderr, flush=True)
+
     cpu = None
     if args.cpu_steps > 0:
         print("[bench] CPU baseline ...", file=sys.stderr, flush=True)
@@ -280,6 +391,7 @@ def main(argv=None) -> int:
         },
         "kernel_ms": {k: round(t / c, 5) for k, (t, c) in kernel_times.items()},
         "cpu_baseline": cpu,
+        "adjoint": adj,
         "setup_s": round(setup_s, 2),
     }
     print(json.dumps(line), flush=True)
