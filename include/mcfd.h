@@ -35,7 +35,10 @@ enum {
   MCFD_FACE_VELOCITY_INLET = 3,
   MCFD_FACE_PRESSURE_OUTLET = 4,
   MCFD_FACE_FAR_FIELD = 5,
-  MCFD_FACE_TRANSMISSIVE = 6
+  MCFD_FACE_TRANSMISSIVE = 6,
+  MCFD_FACE_PRESSURE_INLET = 7,   /* these three need mcfd_mesh_set_inflow */
+  MCFD_FACE_MASS_FLOW_INLET = 8,
+  MCFD_FACE_MASS_FLOW_OUTLET = 9
 };
 
 /* Accumulation modes (low 2 bits of acc_item; face index in the high bits). */
@@ -58,7 +61,14 @@ enum {
   MCFD_ERR_LOCAL_DT = 8,       /* "Local stable time steps must be positive and finite." */
   MCFD_ERR_GLOBAL_DT = 9,      /* "The stable time step is non-positive or non-finite." */
   MCFD_ERR_TEMPERATURE = 10,   /* "Viscous flow requires finite positive temperature." */
-  MCFD_ERR_VISCOSITY = 11      /* "Sutherland law produced an invalid viscosity." */
+  MCFD_ERR_VISCOSITY = 11,     /* "Sutherland law produced an invalid viscosity." */
+  MCFD_ERR_INLET_OUTFLOW = 12, /* "Pressure inlet cannot prescribe a supersonic outflow." */
+  MCFD_ERR_INLET_SUPERSONIC = 13, /* "Pressure inlet needs Mach number to define a supersonic inflow." */
+  MCFD_ERR_INLET_TOTALS = 14,  /* "Pressure inlet total conditions are incompatible with the interior characteristic state." */
+  MCFD_ERR_INLET_STATE = 15,   /* "Pressure inlet produced an invalid state." */
+  MCFD_ERR_MASS_PRESSURE = 16, /* "Mass-flow inlet has invalid interior pressure." */
+  MCFD_ERR_MASS_FLOW = 17,     /* "Mass-flow inlet cannot satisfy the configured flow rate." */
+  MCFD_ERR_OUTLET_DENSITY = 18 /* "Mass-flow outlet has invalid interior density." */
 };
 
 /* Molecular viscosity models (McfdSettings.viscosity), solver.py:2721-2737. */
@@ -386,6 +396,54 @@ typedef struct McfdViscousDesc {
  * time-step calls whose settings have viscosity != 0 add the viscous terms
  * (stages then run the unfused kernels). */
 int mcfd_mesh_set_viscous(McfdMesh* mesh, const McfdViscousDesc* desc);
+
+/*
+ * Pressure-inlet, mass-flow-inlet and mass-flow-outlet records (host
+ * pointers; copied). Replaces `_pressure_inlet_boundary_states`,
+ * `_mass_flow_inlet_boundary_states` and `_mass_flow_outlet_boundary_states`
+ * (solver.py:4412-4577) at their residual and time-step call sites
+ * (solver.py:2626-2705, 4155-4193). The records are the n_records boundary
+ * records from first_record on (kinds 7, 8, 9); per record:
+ *
+ * - face: its flux face;
+ * - params: pressure inlet {total pressure, total temperature, dx, dy},
+ *   mass-flow inlet {mass flow, total temperature, dx, dy} (d the unit flow
+ *   direction), mass-flow outlet {mass flow, summed face length of the
+ *   patch, 0, 0};
+ * - projection: n.d (pressure inlet), -(n.d) (mass-flow inlet), 0 otherwise,
+ *   as the reference evaluates them (normals @ direction);
+ *
+ * and per mass-flow-inlet patch p (records patch_rec[p] .. patch_rec[p+1]-1,
+ * offsets from first_record): the leaves {start, size} (offsets in the
+ * patch) leaf[2*(leaf_ptr[p] .. leaf_ptr[p+1]-1)] and the token program
+ * program[prog_ptr[p] .. prog_ptr[p+1]-1] (t >= 0: push the sum of leaf t of
+ * the patch; -1: pop b, pop a, push a + b) that evaluate the patch's flow
+ * sum in NumPy's pairwise order (np.sum over a contiguous array: blocks of
+ * 8192 added in turn, each split in halves rounded down to a multiple of 8
+ * down to leaves of at most 128 values summed with 8 accumulators). A patch
+ * may hold at most 4096 records.
+ */
+typedef struct McfdInflowDesc {
+  int32_t first_record;
+  int32_t n_records;
+  int32_t n_patches;
+  int32_t n_leaves;
+  int32_t n_program;
+  int32_t reserved;
+  const int32_t* face;        /* [n_records] */
+  const double* params;       /* [n_records*4] */
+  const double* projection;   /* [n_records] */
+  const int32_t* patch_rec;   /* [n_patches+1] */
+  const int32_t* leaf_ptr;    /* [n_patches+1] */
+  const int32_t* leaf;        /* [n_leaves*2] */
+  const int32_t* prog_ptr;    /* [n_patches+1] */
+  const int32_t* program;     /* [n_program] */
+} McfdInflowDesc;
+
+/* Upload the inflow records; NULL frees. A mesh with records of kinds 7-9
+ * needs this before residual, stage and time-step calls, which then run the
+ * unfused kernels (inviscid only; no adjoint). */
+int mcfd_mesh_set_inflow(McfdMesh* mesh, const McfdInflowDesc* desc);
 
 /* Upload the reverse-entry map of the LSQ lists (grad_rev[e] = entry of the
  * neighbour that points back) and allocate the adjoint scratch; NULL frees. */

@@ -113,11 +113,20 @@ def concat_geometries(geoms: list[Geometry]) -> tuple[Geometry, np.ndarray]:
                    "gradient_neighbours", "limiter_cells")
     face_fields = ("interior_faces", "wall_faces", "symmetry_faces", "periodic_faces_a", "periodic_faces_b",
                    "velocity_inlet_faces", "pressure_outlet_faces", "far_field_faces", "transmissive_faces",
-                   "diffusive_boundary_faces")
+                   "diffusive_boundary_faces", "pressure_inlet_faces", "mass_flow_inlet_faces",
+                   "mass_flow_outlet_faces")
     kw = {}
     for f in Geometry.__dataclass_fields__:
         if f == "domain_scale":
             kw[f] = max(g.domain_scale for g in geoms)
+        elif f == "inflow_patch_names":
+            kw[f] = tuple(sum((tuple(g.inflow_patch_names[k]) if g.inflow_patch_names else () for g in geoms), ())
+                          for k in range(3))
+        elif f in ("pressure_inlet_patch", "mass_flow_inlet_patch", "mass_flow_outlet_patch"):
+            # patch ordinals continue across instances
+            shift = np.concatenate(([0], np.cumsum([int(getattr(g, f).max()) + 1 if len(getattr(g, f)) else 0
+                                                    for g in geoms])))
+            kw[f] = np.concatenate([getattr(g, f) + shift[k] for k, g in enumerate(geoms)]).astype(np.int64)
         elif f in cell_fields:
             kw[f] = cat(f, "cell")
         elif f in face_fields:

@@ -495,6 +495,8 @@ struct FaceArrays {
   const double* ry;
   const double* bc_state;
   const double* bc_wall;
+  const double* inflow;  // exterior states of the inflow records (kinds 7-9), or NULL
+  int inflow_first;      // boundary record of inflow[0]
 };
 
 __global__ void __launch_bounds__(kBlock)
@@ -530,6 +532,8 @@ __global__ void __launch_bounds__(kBlock)
       R.p = st[3];
     } else if (kind == MCFD_FACE_FAR_FIELD) {
       R = far_field(L, P4{st[0], st[1], st[2], st[3]}, nx, ny, g, &err);
+    } else if (kind >= MCFD_FACE_PRESSURE_INLET) {
+      R = load4(fa.inflow, b - fa.inflow_first);
     } else {
       R = L;
     }
@@ -579,6 +583,8 @@ struct DtArrays {
   const int* bc_kind;
   const double* bc_state;
   const double* area;
+  const double* inflow;  // exterior states of the inflow records (from cell states), or NULL
+  int inflow_first;
 };
 
 // Local stable step of one cell from its primitive state: sum over its
@@ -605,6 +611,8 @@ __device__ double cell_spectral_sum(int i, const P4& w, const DtArrays& d, doubl
         x.p = st[3];
       else if (kind == MCFD_FACE_FAR_FIELD)
         x = far_field(w, P4{st[0], st[1], st[2], st[3]}, nx, ny, g, &err);
+      else if (kind >= MCFD_FACE_PRESSURE_INLET)
+        x = load4(d.inflow, b - d.inflow_first);
       if (err) report(status, err, i);
       double xr = fabs(x.u * nx + x.v * ny) + sound(x, g);
       rad = vmax(rad, xr);
@@ -620,6 +628,223 @@ __device__ double cell_time_step(int i, const P4& w, const DtArrays& d, double g
   double dt = cfl * d.area[i] / sum;
   if (!finite(dt) || !(dt > 0.0)) report(status, MCFD_ERR_LOCAL_DT, i);
   return dt;
+}
+
+// ---------------------------------------------------------------------------
+// Pressure-inlet, mass-flow-inlet and mass-flow-outlet states
+// (solver.py:4412-4577). One pass over the inflow records computes the
+// pressure-inlet and mass-flow-outlet states and the mass-flow inlets'
+// interior pressures from the reconstructed face states (residual) or the
+// owner cells' states (time steps); one workgroup per mass-flow-inlet patch
+// then bisects the patch's inflow speed. The flux and time-step kernels read
+// the states from `ext`.
+
+struct InflowArrays {
+  int first;              // boundary record of record 0
+  int n;                  // records
+  const int* face;        // [n] flux face
+  const double* params;   // [4n]
+  const double* proj;     // [n] n.d (pressure inlet), -(n.d) (mass-flow inlet)
+  const int* patch_rec;   // [patches+1]
+  const int* leaf_ptr;    // [patches+1]
+  const int* leaf;        // [2 leaves] {start, size} within the patch
+  const int* prog_ptr;    // [patches+1]
+  const int* program;     // summation tokens
+  double* ext;            // [4n] exterior states
+};
+
+// _pressure_inlet_boundary_states for one face (solver.py:4431-4496): the
+// inflow Mach number M in [0, 1] whose incoming invariant
+// c0/sqrt(1 + (g-1)/2 M^2) (d.n M + 2/(g-1)) equals the interior's outgoing
+// one, by 64 bisection steps; then the isentropic static state along d.
+__device__ P4 pressure_inlet_state(const P4& in, double nx, double ny, const double* par, double dn,
+                                   double g, double R, int* err) {
+  double c_in = sqrt(g * in.p / in.r);
+  double un = in.u * nx + in.v * ny;
+  double m_in = un / c_in;
+  if (m_in >= 1.0) {
+    *err = MCFD_ERR_INLET_OUTFLOW;
+    return in;
+  }
+  if (m_in <= -1.0) {
+    *err = MCFD_ERR_INLET_SUPERSONIC;
+    return in;
+  }
+  const double p0 = par[0], t0 = par[1];
+  const double c0 = sqrt(g * R * t0);
+  const double half_gm1 = 0.5 * (g - 1.0);
+  const double two_gm1 = 2.0 / (g - 1.0);
+  const double outgoing = un + 2.0 * c_in / (g - 1.0);
+  auto invariant = [&](double m) {
+    double ratio = 1.0 + half_gm1 * (m * m);
+    double c = c0 / sqrt(ratio);
+    return c * (dn * m + two_gm1);
+  };
+  if (!(invariant(1.0) <= outgoing && outgoing <= invariant(0.0))) {
+    *err = MCFD_ERR_INLET_TOTALS;
+    return in;
+  }
+  double low = 0.0, high = 1.0;
+  for (int it = 0; it < 64; ++it) {
+    double mid = 0.5 * (low + high);
+    if (invariant(mid) > outgoing)
+      low = mid;
+    else
+      high = mid;
+  }
+  double mach = 0.5 * (low + high);
+  double T = t0 / (1.0 + half_gm1 * (mach * mach));
+  double p = p0 * pow(T / t0, g / (g - 1.0));
+  double rho = p / (R * T);
+  double speed = mach * sqrt(g * R * T);
+  P4 e{rho, speed * par[2], speed * par[3], p};
+  if (!finite(e.r) || !finite(e.u) || !finite(e.v) || !finite(e.p) || !(e.r > 0.0) || !(e.p > 0.0))
+    *err = MCFD_ERR_INLET_STATE;
+  return e;
+}
+
+// _mass_flow_outlet_boundary_states (solver.py:4557-4577): normal velocity
+// mdot / (sum L * rho), tangential velocity, density and pressure kept.
+__device__ P4 mass_flow_outlet_state(const P4& in, double nx, double ny, const double* par, int* err) {
+  if (!finite(in.r) || !(in.r > 0.0)) *err = MCFD_ERR_OUTLET_DENSITY;
+  double un = par[0] / (par[1] * in.r);
+  double tx = -ny, ty = nx;
+  double ut = in.u * tx + in.v * ty;
+  return P4{in.r, un * nx + ut * tx, un * ny + ut * ty, in.p};
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_inflow_records(InflowArrays ia, FaceArrays fa, const double* __restrict__ W,
+                     const double* __restrict__ G, const double* __restrict__ PHI,
+                     const double* __restrict__ K, int muscl, int reconstructed, double g, double R,
+                     int* __restrict__ status) {
+  int r = xcd_block() * blockDim.x + threadIdx.x;
+  if (r >= ia.n) return;
+  int k = ia.face[r];
+  int kind = fa.kind[k];
+  double nx = fa.nx[k], ny = fa.ny[k];
+  P4 in;
+  if (!reconstructed)
+    in = load4(W, fa.left[k]);
+  else if (K)
+    in = reconstruct_q(W, G, K, PHI, fa.left[k], fa.lx[k], fa.ly[k]);
+  else
+    in = reconstruct(W, G, PHI, fa.left[k], fa.lx[k], fa.ly[k], muscl);
+  const double* par = ia.params + 4 * (long)r;
+  int err = 0;
+  P4 e;
+  if (kind == MCFD_FACE_PRESSURE_INLET) {
+    e = pressure_inlet_state(in, nx, ny, par, ia.proj[r], g, R, &err);
+  } else if (kind == MCFD_FACE_MASS_FLOW_INLET) {
+    if (!finite(in.p) || !(in.p > 0.0)) err = MCFD_ERR_MASS_PRESSURE;
+    e = P4{0.0, 0.0, 0.0, in.p};  // the patch kernel completes the state
+  } else {
+    e = mass_flow_outlet_state(in, nx, ny, par, &err);
+  }
+  if (err) report(status, err, r);
+  store4(ia.ext, r, e.r, e.u, e.v, e.p);
+}
+
+// Sum over a mass-flow-inlet patch of (p_i / (R T)) * speed * proj_i * L_i
+// (the reference's flow_at, solver.py:4521-4535) in NumPy's pairwise order:
+// leaves of < 8 values are summed in turn from 0; larger leaves with eight
+// strided accumulators combined as ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7)) plus
+// the tail in turn; the leaves combined by the patch's token program. Every
+// thread of the workgroup calls it and gets the sum.
+__device__ double patch_flow(const InflowArrays& ia, const double* __restrict__ face_len, int p, int a,
+                             int n, int la, int nl, double speed, double t0, double cp, double R,
+                             double* terms, double* rr, double* leafsum, double* stack, double* bcast) {
+  double RT = R * (t0 - speed * speed / (2.0 * cp));
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    int r = a + i;
+    terms[i] = ia.ext[4 * (long)r + 3] / RT * speed * ia.proj[r] * face_len[ia.face[r]];
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < 8 * nl; t += blockDim.x) {
+    int l = t >> 3, j = t & 7;
+    int s = ia.leaf[2 * (la + l)], m = ia.leaf[2 * (la + l) + 1];
+    if (m < 8) {
+      if (j == 0) {
+        double res = 0.0;
+        for (int i = 0; i < m; ++i) res += terms[s + i];
+        leafsum[l] = res;
+      }
+    } else {
+      double acc = terms[s + j];
+      int end = m - (m & 7);
+      for (int i = 8; i < end; i += 8) acc += terms[s + i + j];
+      rr[t] = acc;
+    }
+  }
+  __syncthreads();
+  for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+    int s = ia.leaf[2 * (la + l)], m = ia.leaf[2 * (la + l) + 1];
+    if (m >= 8) {
+      const double* q = rr + 8 * l;
+      double res = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7]));
+      for (int i = m - (m & 7); i < m; ++i) res += terms[s + i];
+      leafsum[l] = res;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int sp = 0;
+    for (int k = ia.prog_ptr[p]; k < ia.prog_ptr[p + 1]; ++k) {
+      int tok = ia.program[k];
+      if (tok >= 0) {
+        stack[sp++] = leafsum[tok];
+      } else {
+        double rhs = stack[--sp];
+        double lhs = stack[--sp];
+        stack[sp++] = lhs + rhs;
+      }
+    }
+    *bcast = stack[0];
+  }
+  __syncthreads();
+  return *bcast;
+}
+
+// _mass_flow_inlet_boundary_states (solver.py:4499-4554): one workgroup per
+// patch; 80 bisection steps on the inflow speed in [0, vmax] against the
+// target flow, then rho = p / (R T), velocity = speed * d, pressure kept.
+// LDS: terms [max_n], accumulators [8 max_leaves], leaf sums [max_leaves],
+// token stack [64], broadcast [1]; the host checks the patch sizes.
+__global__ void __launch_bounds__(256)
+    k_mass_flow_inlet(InflowArrays ia, const double* __restrict__ face_len, int max_n, int max_leaves,
+                      double g, double R, int* __restrict__ status) {
+  extern __shared__ double sh[];
+  double* terms = sh;
+  double* rr = terms + max_n;
+  double* leafsum = rr + 8 * max_leaves;
+  double* stack = leafsum + max_leaves;
+  double* bcast = stack + 64;
+  const int p = blockIdx.x;
+  const int a = ia.patch_rec[p], n = ia.patch_rec[p + 1] - a;
+  const int la = ia.leaf_ptr[p], nl = ia.leaf_ptr[p + 1] - la;
+  const double* par = ia.params + 4 * (long)a;
+  const double mdot = par[0], t0 = par[1];
+  const double cp = g * R / (g - 1.0);
+  const double vmax = sqrt(2.0 * cp * t0) * (1.0 - 1e-12);
+  double low = 0.0, high = vmax;
+  double f = patch_flow(ia, face_len, p, a, n, la, nl, high, t0, cp, R, terms, rr, leafsum, stack, bcast);
+  if (f < mdot && threadIdx.x == 0) report(status, MCFD_ERR_MASS_FLOW, a);
+  for (int it = 0; it < 80; ++it) {
+    double mid = 0.5 * (low + high);
+    f = patch_flow(ia, face_len, p, a, n, la, nl, mid, t0, cp, R, terms, rr, leafsum, stack, bcast);
+    if (f < mdot)
+      low = mid;
+    else
+      high = mid;
+  }
+  const double speed = 0.5 * (low + high);
+  const double RT = R * (t0 - speed * speed / (2.0 * cp));
+  const double u = speed * par[2], v = speed * par[3];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    int r = a + i;
+    double pr = ia.ext[4 * (long)r + 3];
+    store4(ia.ext, r, pr / RT, u, v, pr);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2607,6 +2832,14 @@ struct McfdMesh {
   double* VF = nullptr;       // [n_faces*4] viscous face fluxes
   double* EFF = nullptr;      // [C] effective diffusivity
   std::vector<void*> viscous_allocations;
+  // pressure-inlet and mass-flow records
+  bool needs_inflow = false;  // the mesh has boundary records of kinds 7-9
+  bool inflow = false;        // mcfd_mesh_set_inflow done
+  InflowArrays inf{};
+  int inflow_patches = 0, inflow_max_n = 0, inflow_max_leaves = 0;
+  std::vector<void*> inflow_allocations;
+  std::vector<int> host_face_bc;  // [n_faces] for validating the inflow records
+  std::vector<int> host_bc_kind;  // [n_bc]
   // timing
   bool timing = false;
   std::vector<hipEvent_t> event_pool;
@@ -2760,7 +2993,28 @@ ViscParams visc_params(const McfdSettings* st) {
                     st->sutherland_temperature};
 }
 
+void free_inflow(McfdMesh* m) {
+  for (void* p : m->inflow_allocations) (void)hipFree(p);
+  m->inflow_allocations.clear();
+  m->inflow = false;
+  m->inf = InflowArrays{};
+  m->faces.inflow = nullptr;
+  m->dts.inflow = nullptr;
+  m->inflow_patches = m->inflow_max_n = m->inflow_max_leaves = 0;
+}
+
+template <typename T>
+int alloc_inflow(McfdMesh* m, T** dst, const T* src, size_t count) {
+  void* p = nullptr;
+  HIP_CHECK(hipMalloc(&p, sizeof(T) * (count ? count : 1)));
+  m->inflow_allocations.push_back(p);
+  if (count && src) HIP_CHECK(hipMemcpy(p, src, sizeof(T) * count, hipMemcpyHostToDevice));
+  *dst = static_cast<T*>(p);
+  return 0;
+}
+
 void free_mesh(McfdMesh* m) {
+  free_inflow(m);
   free_viscous(m);
   free_parabolic(m);
   free_adjoint(m);
@@ -2854,8 +3108,37 @@ void launch_fused(McfdMesh* m, int scheme, bool muscl, bool far, bool dtn, size_
   return launch_fused_s<0>(m, muscl, st->limiter, far, dtn, lds, s, ga, st, a);
 }
 
+int check_inflow(const McfdMesh* m, const McfdSettings* st) {
+  if (!m->needs_inflow) return 0;
+  if (!m->inflow) return fail("Pressure-inlet and mass-flow boundaries need mcfd_mesh_set_inflow first.");
+  if (st->viscosity)
+    return fail("Viscous terms with pressure-inlet or mass-flow boundaries are not supported on the GPU.");
+  if (!(st->gas_constant > 0.0)) return fail("Inflow boundaries need a positive gas constant.");
+  return 0;
+}
+
+// Exterior states of the inflow records from the reconstructed face states
+// (residual) or the owner cells' primitive states in m->W (time steps).
+int inflow_states(McfdMesh* m, const McfdSettings* st, hipStream_t s, bool reconstructed, bool para,
+                  bool muscl) {
+  if (!m->inflow || m->inf.n == 0) return 0;
+  hipLaunchKernelGGL(k_inflow_records, dim3(blocks(m->inf.n)), dim3(kBlock), 0, s, m->inf, m->faces,
+                     m->W, m->G, m->PHI, (reconstructed && para) ? m->K : nullptr,
+                     (reconstructed && muscl) ? 1 : 0, reconstructed ? 1 : 0, st->gamma,
+                     st->gas_constant, m->status);
+  if (m->inflow_patches) {
+    size_t lds = sizeof(double) * ((size_t)m->inflow_max_n + 9 * (size_t)m->inflow_max_leaves + 65);
+    hipLaunchKernelGGL(k_mass_flow_inlet, dim3(m->inflow_patches), dim3(256), lds, s, m->inf,
+                       m->face_len, m->inflow_max_n, m->inflow_max_leaves, st->gamma,
+                       st->gas_constant, m->status);
+  }
+  HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
 int residual_front(McfdMesh* m, const McfdSettings* st, const double* u, hipStream_t s) {
   int n = m->n_cells;
+  if (check_inflow(m, st)) return -1;
   {
     TimedLaunch t(m, s, MCFD_KERNEL_PRIMITIVE);
     hipLaunchKernelGGL(k_primitive, dim3(blocks(n)), dim3(kBlock), 0, s, n, u, m->W, st->gamma,
@@ -2878,6 +3161,7 @@ int residual_front(McfdMesh* m, const McfdSettings* st, const double* u, hipStre
   }
   if (m->n_faces) {
     TimedLaunch t(m, s, MCFD_KERNEL_FLUX);
+    if (inflow_states(m, st, s, true, para, muscl)) return -1;
     hipLaunchKernelGGL(k_flux, dim3(blocks(m->n_faces)), dim3(kBlock), 0, s, m->n_faces,
                        m->faces, m->W, m->G, m->PHI, para ? m->K : nullptr, st->flux,
                        muscl ? 1 : 0, st->gamma, m->FLUX, m->status);
@@ -3179,6 +3463,8 @@ int mcfd_residual_vjp_impl(McfdMesh* m, const McfdSettings* st, const double* u,
   if (check_settings(st)) return -1;
   if (st->reconstruction == 2) return fail("mcfd_residual_vjp: MUSCL parabolic has no adjoint yet");
   if (st->viscosity) return fail("mcfd_residual_vjp: viscous terms have no adjoint yet");
+  if (m->needs_inflow)
+    return fail("mcfd_residual_vjp: pressure-inlet and mass-flow boundaries have no adjoint yet");
   int n = m->n_cells;
   bool muscl = st->reconstruction == 1 && m->n_grad > 0;
   hipLaunchKernelGGL(k_primitive, dim3(blocks(n)), dim3(kBlock), 0, s, n, u, m->W, st->gamma,
@@ -3223,8 +3509,17 @@ int mcfd_mesh_create(const McfdMeshDesc* d, McfdMesh** out) {
   m->n_acc = d->n_acc;
   m->n_dts = d->n_dts;
   m->n_bc = d->n_bc;
-  for (int b = 0; b < d->n_bc; ++b)
-    if (d->bc_kind && d->bc_kind[b] == MCFD_FACE_FAR_FIELD) m->has_far_field = true;
+  for (int b = 0; b < d->n_bc; ++b) {
+    int kind = d->bc_kind ? d->bc_kind[b] : -1;
+    if (kind < MCFD_FACE_WALL || kind > MCFD_FACE_MASS_FLOW_OUTLET) {
+      delete m;
+      return fail("mcfd_mesh_create: unknown boundary record kind");
+    }
+    if (kind == MCFD_FACE_FAR_FIELD) m->has_far_field = true;
+    if (kind >= MCFD_FACE_PRESSURE_INLET) m->needs_inflow = true;
+  }
+  if (d->n_bc > 0) m->host_bc_kind.assign(d->bc_kind, d->bc_kind + d->n_bc);
+  if (d->n_faces > 0 && d->face_bc) m->host_face_bc.assign(d->face_bc, d->face_bc + d->n_faces);
   (void)hipGetDevice(&m->device);
   size_t C = d->n_cells, F = d->n_faces;
   int rc = 0;
@@ -3380,12 +3675,13 @@ int mcfd_stage(McfdMesh* m, const McfdSettings* st, const McfdStage* sg, void* s
               sg->dt,   sg->out,         sg->r_out, sg->c[0], sg->c[1], sg->c[2],
               sg->c[3], sg->c[4],        sg->dt_next,
               sg->dt_min ? m->block_partial : nullptr, st->cfl};
-  if ((sg->dt_next || sg->dt_min) && (!m->fused || !sg->dt_min || sg->op == MCFD_OP_RESIDUAL))
+  if ((sg->dt_next || sg->dt_min) &&
+      (!m->fused || m->needs_inflow || !sg->dt_min || sg->op == MCFD_OP_RESIDUAL))
     return fail("mcfd_stage: dt_next needs the fused path, dt_min and a state update");
   if (check_viscous(m, st)) return -1;
   if (sg->dt_min && (st->viscosity || st->reconstruction == 2))
     return fail("mcfd_stage: the next step's dt needs the fused stage (inviscid, no MUSCL parabolic)");
-  if (m->fused && st->reconstruction != 2 && !st->viscosity) {
+  if (m->fused && st->reconstruction != 2 && !st->viscosity && !m->needs_inflow) {
     bool muscl = st->reconstruction == 1 && m->n_grad > 0;
     size_t lds = fused_lds_bytes(m, muscl);
     GeomArrays ga{m->area,   m->grad_ptr, m->grad_cx,  m->grad_cy, m->lim_ptr,
@@ -3417,12 +3713,17 @@ int mcfd_time_steps(McfdMesh* m, const McfdSettings* st, const double* u, double
                     double* dt_min, void* stream) {
   if (!m || !u || !dt_cell) return fail("mcfd_time_steps: NULL argument");
   if (check_settings(st)) return -1;
+  if (check_inflow(m, st)) return -1;
   hipStream_t s = static_cast<hipStream_t>(stream);
   int n = m->n_cells;
   {
     TimedLaunch t(m, s, MCFD_KERNEL_PRIMITIVE);
     hipLaunchKernelGGL(k_primitive, dim3(blocks(n)), dim3(kBlock), 0, s, n, u, m->W, st->gamma,
                        m->status);
+  }
+  if (m->inflow) {
+    TimedLaunch t(m, s, MCFD_KERNEL_TIME_STEP);
+    if (inflow_states(m, st, s, false, false, false)) return -1;
   }
   if (st->viscosity) {
     if (check_viscous(m, st)) return -1;
@@ -3752,6 +4053,106 @@ int mcfd_mesh_set_viscous(McfdMesh* m, const McfdViscousDesc* d) {
   m->vacc.VF = m->VF;
   m->vdt.eff = m->EFF;
   m->viscous = true;
+  return 0;
+}
+
+int mcfd_mesh_set_inflow(McfdMesh* m, const McfdInflowDesc* d) {
+  if (!m) return fail("mcfd_mesh_set_inflow: NULL mesh");
+  (void)hipDeviceSynchronize();
+  free_inflow(m);
+  if (!d) return 0;
+  const int n = d->n_records, first = d->first_record, np = d->n_patches;
+  if (n < 0 || np < 0 || d->n_leaves < 0 || d->n_program < 0 || first < 0 || first + n > m->n_bc)
+    return fail("mcfd_mesh_set_inflow: record range outside the boundary records");
+  if (n > 0 && (!d->face || !d->params || !d->projection))
+    return fail("mcfd_mesh_set_inflow: missing array");
+  if (np > 0 && (!d->patch_rec || !d->leaf_ptr || !d->leaf || !d->prog_ptr || !d->program))
+    return fail("mcfd_mesh_set_inflow: missing patch array");
+  // the records are exactly the boundary records of kinds 7-9, each on its own flux face
+  int count = 0;
+  for (int b = 0; b < m->n_bc; ++b) count += m->host_bc_kind[b] >= MCFD_FACE_PRESSURE_INLET;
+  if (count != n) return fail("mcfd_mesh_set_inflow: records do not match the inflow boundary records");
+  for (int r = 0; r < n; ++r) {
+    int k = d->face[r];
+    if (k < 0 || k >= m->n_faces || m->host_face_bc[k] != first + r ||
+        m->host_bc_kind[first + r] < MCFD_FACE_PRESSURE_INLET)
+      return fail("mcfd_mesh_set_inflow: record face does not carry the record");
+  }
+  // patches: contiguous mass-flow-inlet records covering all of them; leaves
+  // inside the patch; programs that leave one value on a stack of <= 64
+  int max_n = 0, max_leaves = 0, covered = 0;
+  for (int p = 0; p < np; ++p) {
+    int a = d->patch_rec[p], b = d->patch_rec[p + 1];
+    if (a < 0 || b <= a || b > n)
+      return fail("mcfd_mesh_set_inflow: bad patch record range");
+    if (b - a > 4096) return fail("Mass-flow inlet patches of more than 4096 faces are not supported on the GPU.");
+    for (int r = a; r < b; ++r)
+      if (m->host_bc_kind[first + r] != MCFD_FACE_MASS_FLOW_INLET)
+        return fail("mcfd_mesh_set_inflow: patch record is not a mass-flow inlet");
+    covered += b - a;
+    int l0 = d->leaf_ptr[p], l1 = d->leaf_ptr[p + 1];
+    if (l0 < 0 || l1 <= l0 || l1 > d->n_leaves || l1 - l0 > 128)
+      return fail("mcfd_mesh_set_inflow: bad leaf range");
+    for (int l = l0; l < l1; ++l) {
+      int st = d->leaf[2 * l], sz = d->leaf[2 * l + 1];
+      if (st < 0 || sz <= 0 || st + sz > b - a) return fail("mcfd_mesh_set_inflow: leaf outside its patch");
+    }
+    int q0 = d->prog_ptr[p], q1 = d->prog_ptr[p + 1];
+    if (q0 < 0 || q1 <= q0 || q1 > d->n_program) return fail("mcfd_mesh_set_inflow: bad program range");
+    int depth = 0;
+    for (int q = q0; q < q1; ++q) {
+      int tok = d->program[q];
+      if (tok >= 0) {
+        if (tok >= l1 - l0 || ++depth > 64) return fail("mcfd_mesh_set_inflow: bad summation program");
+      } else {
+        if (tok != -1 || depth < 2) return fail("mcfd_mesh_set_inflow: bad summation program");
+        --depth;
+      }
+    }
+    if (depth != 1) return fail("mcfd_mesh_set_inflow: bad summation program");
+    max_n = std::max(max_n, b - a);
+    max_leaves = std::max(max_leaves, l1 - l0);
+  }
+  int n_mass = 0;
+  for (int r = 0; r < n; ++r) n_mass += m->host_bc_kind[first + r] == MCFD_FACE_MASS_FLOW_INLET;
+  if (covered != n_mass) return fail("mcfd_mesh_set_inflow: patches do not cover the mass-flow inlets");
+  InflowArrays& ia = m->inf;
+  int rc = 0;
+  int* face = nullptr;
+  double *params = nullptr, *proj = nullptr;
+  int *patch_rec = nullptr, *leaf_ptr = nullptr, *leaf = nullptr, *prog_ptr = nullptr, *program = nullptr;
+  rc |= alloc_inflow(m, &face, d->face, (size_t)n);
+  rc |= alloc_inflow(m, &params, d->params, 4 * (size_t)n);
+  rc |= alloc_inflow(m, &proj, d->projection, (size_t)n);
+  rc |= alloc_inflow(m, &patch_rec, np ? d->patch_rec : nullptr, (size_t)np + 1);
+  rc |= alloc_inflow(m, &leaf_ptr, np ? d->leaf_ptr : nullptr, (size_t)np + 1);
+  rc |= alloc_inflow(m, &leaf, d->leaf, 2 * (size_t)d->n_leaves);
+  rc |= alloc_inflow(m, &prog_ptr, np ? d->prog_ptr : nullptr, (size_t)np + 1);
+  rc |= alloc_inflow(m, &program, d->program, (size_t)d->n_program);
+  rc |= alloc_inflow(m, &ia.ext, (const double*)nullptr, 4 * (size_t)n);
+  if (rc) {
+    std::string msg = g_last_error;
+    free_inflow(m);
+    return fail(msg);
+  }
+  ia.first = first;
+  ia.n = n;
+  ia.face = face;
+  ia.params = params;
+  ia.proj = proj;
+  ia.patch_rec = patch_rec;
+  ia.leaf_ptr = leaf_ptr;
+  ia.leaf = leaf;
+  ia.prog_ptr = prog_ptr;
+  ia.program = program;
+  m->inflow_patches = np;
+  m->inflow_max_n = max_n;
+  m->inflow_max_leaves = max_leaves;
+  m->faces.inflow = ia.ext;
+  m->faces.inflow_first = first;
+  m->dts.inflow = ia.ext;
+  m->dts.inflow_first = first;
+  m->inflow = true;
   return 0;
 }
 

@@ -19,7 +19,7 @@ import math
 import numpy as np
 import torch
 
-from . import native
+from . import inflow, native
 from .geometry import SolverError, build_geometry
 from .layout import build_layout
 from .model import FLUX_CODES, LIMITER_CODES, RECONSTRUCTION_CODES, Case, NumericalSettings
@@ -95,7 +95,15 @@ class DeviceMesh:
         self.handle = handle
         self.fuse_time_step = fuse_time_step
         self.blocks = None
-        if fused:
+        # pressure-inlet / mass-flow boundaries: exterior states from their own
+        # kernels, read by the unfused flux and time-step kernels
+        self.inflow = inflow.has_inflow(self.layout)
+        if self.inflow:
+            if self.settings.viscosity_model != "Inviscid":
+                raise SolverError("Viscous terms with pressure-inlet or mass-flow boundaries are not supported on "
+                                  "the GPU.")
+            inflow.upload(self.lib, self.handle, self.geometry, self.layout)
+        elif fused:
             self.set_blocks(block_cells)
         self.native_settings = native_settings(self.settings)
         if self.settings.reconstruction == "MUSCL parabolic":
@@ -188,6 +196,8 @@ class DeviceMesh:
         ``block_cells`` cells (None: back to the four-kernel path)."""
         from .blocks import build_blocks
 
+        if block_cells is not None and self.inflow:
+            raise SolverError("Pressure-inlet and mass-flow boundaries run the unfused kernels.")
         if block_cells is None:
             native.check(self.lib.mcfd_mesh_set_blocks(self.handle, None))
             self.blocks = None
@@ -202,6 +212,9 @@ class DeviceMesh:
         block, each block also limited to 256 faces) time ``steps`` explicit
         steps with HIP events, ``repeats`` times, from the same state; keep
         the fastest and restore the state. Returns the chosen size."""
+        if self.inflow:
+            self.timings = {}
+            return None
         saved = (self.u.clone(), self.clock.clone(), self.dt_fresh)
         stream = torch.cuda.current_stream(self.device)
         best, best_ms = None, math.inf
@@ -259,7 +272,10 @@ class DeviceMesh:
         native.check(self.lib.mcfd_status(self.handle, self.stream, C.byref(mask), C.byref(index)))
         if mask.value:
             code = min(k for k in native.ERROR_MESSAGES if mask.value & (1 << k))
-            raise SolverError(f"{context}{native.ERROR_MESSAGES[code]}")
+            message = native.ERROR_MESSAGES[code]
+            if "{name}" in message:  # inflow records report their record index
+                message = message.format(name=inflow.record_name(self.geometry, self.layout, index.value))
+            raise SolverError(f"{context}{message}")
 
     # Device cells are stored in the layout's (Hilbert) order; these convert
     # between mesh order (what callers see) and device order.

@@ -39,6 +39,9 @@ BOUNDARY_CODES = {
     "Pressure Outlet": 4,
     "Far Field": 5,
     "Transmissive": 6,
+    "Pressure Inlet": 7,
+    "Mass Flow Inlet": 8,
+    "Mass Flow Outlet": 9,
 }
 
 SUPPORTED_KINDS = (
@@ -49,6 +52,9 @@ SUPPORTED_KINDS = (
     "Pressure Outlet",
     "Far Field",
     "Transmissive",
+    "Pressure Inlet",
+    "Mass Flow Inlet",
+    "Mass Flow Outlet",
 )
 
 
@@ -83,6 +89,25 @@ class Geometry:
     far_field_faces: np.ndarray
     far_field_states: np.ndarray
     transmissive_faces: np.ndarray
+    # pressure inlets (solver.py:1930-1950): per face (total pressure, total
+    # temperature, unit direction x, y), the patch's ordinal and n.d;
+    # mass-flow inlets (:1951-1976): per face (mass flow, total temperature,
+    # direction x, y), the patch's ordinal and -(n.d); mass-flow outlets
+    # (:1977-1981): per face (mass flow, the patch's summed face length).
+    # Patches in condition order; n.d evaluated per patch as the reference
+    # does (normals @ direction, solver.py:4423, 4507).
+    pressure_inlet_faces: np.ndarray
+    pressure_inlet_params: np.ndarray
+    pressure_inlet_patch: np.ndarray
+    pressure_inlet_projection: np.ndarray
+    mass_flow_inlet_faces: np.ndarray
+    mass_flow_inlet_params: np.ndarray
+    mass_flow_inlet_patch: np.ndarray
+    mass_flow_inlet_projection: np.ndarray
+    mass_flow_outlet_faces: np.ndarray
+    mass_flow_outlet_params: np.ndarray
+    mass_flow_outlet_patch: np.ndarray
+    inflow_patch_names: tuple  # (pressure-inlet names, mass-flow-inlet names, mass-flow-outlet names)
     gradient_sources: np.ndarray
     gradient_neighbours: np.ndarray
     gradient_dx: np.ndarray
@@ -105,6 +130,16 @@ class Geometry:
     @property
     def cell_count(self) -> int:
         return int(self.areas.shape[0])
+
+
+def _flow_direction(x, y, description: str) -> np.ndarray:
+    """`_boundary_flow_direction` (solver.py:4599-4612)."""
+    d = np.array((_finite(x, f"{description} flow direction X"), _finite(y, f"{description} flow direction Y")),
+                 dtype=float)
+    magnitude = float(np.linalg.norm(d))
+    if not math.isfinite(magnitude) or magnitude <= 1e-12:
+        raise SolverError(f"{description} needs a non-zero flow direction.")
+    return d / magnitude
 
 
 def shoelace(mesh: FiniteVolumeMesh) -> tuple[np.ndarray, np.ndarray]:
@@ -301,6 +336,9 @@ def build_geometry(case: Case) -> Geometry:
     symmetry_faces = boundary[nonperiodic & (boundary_kinds == "Symmetry")]
 
     vi_faces, vi_states, po_faces, po_p, ff_faces, ff_states, tr_faces = [], [], [], [], [], [], []
+    pi_faces, pi_par, pi_patch, pi_proj, mi_faces, mi_par, mi_patch, mi_proj, mo_faces, mo_par, mo_patch = (
+        [], [], [], [], [], [], [], [], [], [], [])
+    names = ([], [], [])
     for pid, bc in conditions.items():
         faces = boundary[patch_ids[boundary] == pid]
         if bc.kind == "Velocity Inlet":
@@ -326,6 +364,45 @@ def build_geometry(case: Case) -> Geometry:
             ff_states.append(np.tile((rho, ux, uy, p), (len(faces), 1)))
         elif bc.kind == "Transmissive":
             tr_faces.append(faces)
+        elif bc.kind == "Pressure Inlet":
+            p0 = _positive(bc.values.get("pressure", 101325.0), f"Total pressure for inlet '{bc.patch_name}'")
+            t0 = _positive(bc.values.get("temperature", 288.15), f"Total temperature for inlet '{bc.patch_name}'")
+            d = _flow_direction(bc.values.get("velocity_x", 1.0), bc.values.get("velocity_y", 0.0),
+                                f"Pressure inlet '{bc.patch_name}'")
+            # checks of _pressure_inlet_boundary_states (solver.py:4419-4428) that depend on geometry only
+            if not math.isclose(float(np.linalg.norm(d)), 1.0):
+                raise SolverError(f"Pressure inlet '{bc.patch_name}' has an invalid flow direction.")
+            projection = normals[faces] @ d
+            if np.any(projection >= -1e-12):
+                raise SolverError(
+                    f"Pressure inlet '{bc.patch_name}' flow direction must point into the domain across every face."
+                )
+            pi_patch.append(np.full(len(faces), len(pi_faces), dtype=np.int64))
+            pi_faces.append(faces)
+            pi_par.append(np.tile((p0, t0, d[0], d[1]), (len(faces), 1)))
+            pi_proj.append(projection)
+            names[0].append(bc.patch_name)
+        elif bc.kind == "Mass Flow Inlet":
+            mdot = _positive(bc.values.get("mass_flow", 1.0), f"Mass flow for inlet '{bc.patch_name}'")
+            t0 = _positive(bc.values.get("temperature", 288.15), f"Total temperature for inlet '{bc.patch_name}'")
+            d = _flow_direction(bc.values.get("velocity_x", 1.0), bc.values.get("velocity_y", 0.0),
+                                f"Mass-flow inlet '{bc.patch_name}'")
+            inward = -(normals[faces] @ d)
+            if np.any(inward <= 1e-12):
+                raise SolverError(
+                    f"Mass-flow inlet '{bc.patch_name}' direction must point into the domain across every face."
+                )
+            mi_patch.append(np.full(len(faces), len(mi_faces), dtype=np.int64))
+            mi_faces.append(faces)
+            mi_par.append(np.tile((mdot, t0, d[0], d[1]), (len(faces), 1)))
+            mi_proj.append(inward)
+            names[1].append(bc.patch_name)
+        elif bc.kind == "Mass Flow Outlet":
+            mdot = _positive(bc.values.get("mass_flow", 1.0), f"Mass flow for outlet '{bc.patch_name}'")
+            mo_patch.append(np.full(len(faces), len(mo_faces), dtype=np.int64))
+            mo_faces.append(faces)
+            mo_par.append(np.tile((mdot, float(np.sum(lengths[faces]))), (len(faces), 1)))
+            names[2].append(bc.patch_name)
 
     def cat(parts, shape_tail=()):
         if parts:
@@ -450,6 +527,18 @@ def build_geometry(case: Case) -> Geometry:
         far_field_faces=cat(ff_faces),
         far_field_states=cat(ff_states, (4,)),
         transmissive_faces=cat(tr_faces),
+        pressure_inlet_faces=cat(pi_faces),
+        pressure_inlet_params=cat(pi_par, (4,)),
+        pressure_inlet_patch=cat(pi_patch),
+        pressure_inlet_projection=cat(pi_proj),
+        mass_flow_inlet_faces=cat(mi_faces),
+        mass_flow_inlet_params=cat(mi_par, (4,)),
+        mass_flow_inlet_patch=cat(mi_patch),
+        mass_flow_inlet_projection=cat(mi_proj),
+        mass_flow_outlet_faces=cat(mo_faces),
+        mass_flow_outlet_params=cat(mo_par, (2,)),
+        mass_flow_outlet_patch=cat(mo_patch),
+        inflow_patch_names=tuple(tuple(x) for x in names),
         gradient_sources=src,
         gradient_neighbours=nbr,
         gradient_dx=gdx,

@@ -6,8 +6,9 @@ gather reproduces the reference residual's sums term for term:
 * LSQ entries: the order of `gradient_sources` (`solver.py:2004-2019`);
 * accumulation: interior faces ascending (the CSR divergence operator's row
   order, `solver.py:2157-2174,2501`), then periodic a-sides, periodic
-  b-sides, wall/symmetry, velocity inlet, pressure outlet, far field and
-  transmissive faces, in that order (`solver.py:2502-2625`);
+  b-sides, wall/symmetry, velocity inlet, pressure outlet, far field,
+  pressure inlet, mass-flow inlet, mass-flow outlet and transmissive faces,
+  in that order (`solver.py:2502-2705`);
 * spectral radius: owners of all faces ascending, then neighbours of interior
   faces ascending (`solver.py:4226-4245`).
 
@@ -25,6 +26,8 @@ from .geometry import Geometry
 
 FACE_INTERIOR, FACE_PERIODIC, FACE_WALL, FACE_VELOCITY_INLET = 0, 1, 2, 3
 FACE_PRESSURE_OUTLET, FACE_FAR_FIELD, FACE_TRANSMISSIVE = 4, 5, 6
+FACE_PRESSURE_INLET, FACE_MASS_FLOW_INLET, FACE_MASS_FLOW_OUTLET = 7, 8, 9
+INFLOW_KINDS = (FACE_PRESSURE_INLET, FACE_MASS_FLOW_INLET, FACE_MASS_FLOW_OUTLET)
 ACC_OWNER, ACC_NEIGHBOUR, ACC_BOUNDARY, ACC_PERIODIC_B = 0, 1, 2, 3
 
 
@@ -167,8 +170,13 @@ def build_layout(g: Geometry, reorder: bool = True, order: np.ndarray | None = N
         bgroup(FACE_VELOCITY_INLET, g.velocity_inlet_faces, states=g.velocity_inlet_states),
         bgroup(FACE_PRESSURE_OUTLET, g.pressure_outlet_faces, pressures=g.pressure_outlet_pressures),
         bgroup(FACE_FAR_FIELD, g.far_field_faces, states=g.far_field_states),
+        bgroup(FACE_PRESSURE_INLET, g.pressure_inlet_faces),
+        bgroup(FACE_MASS_FLOW_INLET, g.mass_flow_inlet_faces),
+        bgroup(FACE_MASS_FLOW_OUTLET, g.mass_flow_outlet_faces),
         bgroup(FACE_TRANSMISSIVE, g.transmissive_faces),
     ]
+    bc_kinds_order = (FACE_WALL, FACE_VELOCITY_INLET, FACE_PRESSURE_OUTLET, FACE_FAR_FIELD) + INFLOW_KINDS + (
+        FACE_TRANSMISSIVE,)
     kinds = np.concatenate([np.full(len(gr[1]), gr[0], np.int32) for gr in groups])
     left = np.concatenate([gr[1] for gr in groups]).astype(np.int32)
     right = np.concatenate([gr[2] for gr in groups]).astype(np.int32)
@@ -244,10 +252,8 @@ def build_layout(g: Geometry, reorder: bool = True, order: np.ndarray | None = N
     # boundary records (numbered from the first boundary flux face) of faces
     # whose exterior state enters the time step
     rec = 0
-    for kind, faces in zip(
-        (FACE_WALL, FACE_VELOCITY_INLET, FACE_PRESSURE_OUTLET, FACE_FAR_FIELD, FACE_TRANSMISSIVE), bc_faces
-    ):
-        if kind in (FACE_VELOCITY_INLET, FACE_PRESSURE_OUTLET, FACE_FAR_FIELD):
+    for kind, faces in zip(bc_kinds_order, bc_faces):
+        if kind in (FACE_VELOCITY_INLET, FACE_PRESSURE_OUTLET, FACE_FAR_FIELD) + INFLOW_KINDS:
             mesh_bc[faces] = rec + np.arange(len(faces))
         rec += len(faces)
     d_cells = np.concatenate((g.owners, ine))

@@ -168,10 +168,32 @@ class Oracle:
                     interior, g.far_field_states, g.face_normals[faces], gamma
                 ),
             )
+        if len(g.pressure_inlet_faces):
+            from .inflow import pressure_inlet_states
+
+            owner_side(g.pressure_inlet_faces, lambda interior, faces: pressure_inlet_states(
+                interior, g.face_normals[faces], g.pressure_inlet_params, g.pressure_inlet_patch, gamma,
+                s.gas_constant, g.inflow_patch_names[0]))
+        if len(g.mass_flow_inlet_faces):
+            from .inflow import mass_flow_inlet_states
+
+            owner_side(g.mass_flow_inlet_faces, lambda interior, faces: mass_flow_inlet_states(
+                interior, g.face_normals[faces], g.face_lengths[faces], g.mass_flow_inlet_params,
+                g.mass_flow_inlet_patch, gamma, s.gas_constant, g.inflow_patch_names[1]))
+        if len(g.mass_flow_outlet_faces):
+            from .inflow import mass_flow_outlet_states
+
+            owner_side(g.mass_flow_outlet_faces, lambda interior, faces: mass_flow_outlet_states(
+                interior, g.face_normals[faces], g.mass_flow_outlet_params))
         if len(getattr(g, "transmissive_faces", ())):
             owner_side(g.transmissive_faces, lambda interior, faces: interior.copy())
         if s.viscosity_model != "Inviscid":
             from .viscous import viscous_residual
+
+            if len(g.pressure_inlet_faces) or len(g.mass_flow_inlet_faces) or len(g.mass_flow_outlet_faces):
+                # the reference's viscous terms take these inlets' states as
+                # boundary values (solver.py:2740-2785); not restated here
+                raise NotImplementedError("viscous terms with pressure-inlet or mass-flow boundaries")
 
             deriv += viscous_residual(primitive, g, s, self.D)
         if s.acceleration_x != 0.0 or s.acceleration_y != 0.0:
@@ -192,7 +214,9 @@ class Oracle:
         un = primitive[g.owners, 1] * n[:, 0] + primitive[g.owners, 2] * n[:, 1]
         c = sound_speed(primitive, gamma)
         radius = np.abs(un) + c[g.owners]
-        bfaces = np.concatenate((g.velocity_inlet_faces, g.pressure_outlet_faces, g.far_field_faces)).astype(np.int64)
+        bfaces = np.concatenate((g.velocity_inlet_faces, g.pressure_outlet_faces, g.far_field_faces,
+                                 g.pressure_inlet_faces, g.mass_flow_inlet_faces,
+                                 g.mass_flow_outlet_faces)).astype(np.int64)
         if len(bfaces):
             owners = g.owners[bfaces]
             ext = primitive[owners].copy()
@@ -202,8 +226,33 @@ class Oracle:
                 ext[:ni] = g.velocity_inlet_states
             if len(g.pressure_outlet_faces):
                 ext[ni:no, 3] = g.pressure_outlet_pressures
+            nf = no + len(g.far_field_faces)
             if len(g.far_field_faces):
-                ext[no:] = far_field_states(primitive[owners[no:]], g.far_field_states, n[bfaces[no:]], gamma)
+                ext[no:nf] = far_field_states(primitive[owners[no:nf]], g.far_field_states, n[bfaces[no:nf]], gamma)
+            k = nf
+            if len(g.pressure_inlet_faces):
+                from .inflow import pressure_inlet_states
+
+                m = len(g.pressure_inlet_faces)
+                ext[k:k + m] = pressure_inlet_states(primitive[owners[k:k + m]], n[bfaces[k:k + m]],
+                                                     g.pressure_inlet_params, g.pressure_inlet_patch, gamma,
+                                                     s.gas_constant, g.inflow_patch_names[0])
+                k += m
+            if len(g.mass_flow_inlet_faces):
+                from .inflow import mass_flow_inlet_states
+
+                m = len(g.mass_flow_inlet_faces)
+                ext[k:k + m] = mass_flow_inlet_states(primitive[owners[k:k + m]], n[bfaces[k:k + m]],
+                                                      g.face_lengths[bfaces[k:k + m]], g.mass_flow_inlet_params,
+                                                      g.mass_flow_inlet_patch, gamma, s.gas_constant,
+                                                      g.inflow_patch_names[1])
+                k += m
+            if len(g.mass_flow_outlet_faces):
+                from .inflow import mass_flow_outlet_states
+
+                m = len(g.mass_flow_outlet_faces)
+                ext[k:k + m] = mass_flow_outlet_states(primitive[owners[k:k + m]], n[bfaces[k:k + m]],
+                                                       g.mass_flow_outlet_params)
             eun = ext[:, 1] * n[bfaces, 0] + ext[:, 2] * n[bfaces, 1]
             er = np.abs(eun) + sound_speed(ext, gamma)
             radius[bfaces] = np.maximum(radius[bfaces], er)

@@ -27,6 +27,7 @@ def test_header_declares_entry_points():
 def test_library_exports_every_declared_symbol():
     import meshcfd_amd.adjoint  # noqa: F401  (registers its entry points)
     import meshcfd_amd.implicit  # noqa: F401
+    import meshcfd_amd.inflow  # noqa: F401
     import meshcfd_amd.viscous  # noqa: F401
     import meshcfd_amd.steady_adjoint  # noqa: F401
 
@@ -54,6 +55,9 @@ def test_ctypes_structs_match_header_layout():
     from meshcfd_amd.viscous import McfdViscousDesc
 
     assert C.sizeof(McfdViscousDesc) == 4 * 4 + 17 * 8
+    from meshcfd_amd.inflow import McfdInflowDesc
+
+    assert C.sizeof(McfdInflowDesc) == 6 * 4 + 8 * 8
     assert C.sizeof(native.McfdStage) == 8 + 7 * 8 + 5 * 8 + 2 * 8
 
 
