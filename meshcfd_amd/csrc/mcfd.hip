@@ -322,14 +322,19 @@ __device__ __forceinline__ void limiter_entry(double inc, double wk, double mxk,
 }
 
 // LSQ gradient (_kernels.py:252-262) + limiter factor (_kernels.py:9-67).
+// With ARGS (for the adjoint) also per variable k: ARG[4i+k] = limiting
+// limiter entry | max entry << 8 | min entry << 16, entries counted from the
+// cell's first, 0xFF = none (the limiter stays 1) / the cell itself.
+template <bool ARGS>
 __global__ void __launch_bounds__(kBlock)
-    k_gradient(int n, const double* __restrict__ W, const int* __restrict__ gptr,
-               const int* __restrict__ gnb, const double* __restrict__ gcx,
-               const double* __restrict__ gcy, const int* __restrict__ lptr,
-               const double* __restrict__ ldx, const double* __restrict__ ldy, int code,
-               double* __restrict__ G, double* __restrict__ PHI) {
+    k_gradient_t(int n, const double* __restrict__ W, const int* __restrict__ gptr,
+                 const int* __restrict__ gnb, const double* __restrict__ gcx,
+                 const double* __restrict__ gcy, const int* __restrict__ lptr,
+                 const double* __restrict__ ldx, const double* __restrict__ ldy, int code,
+                 double* __restrict__ G, double* __restrict__ PHI, unsigned* __restrict__ ARG) {
   int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  unsigned arg[4] = {0xFFFFFFu, 0xFFFFFFu, 0xFFFFFFu, 0xFFFFFFu};
   P4 w = load4(W, i);
   double wv[4] = {w.r, w.u, w.v, w.p};
   double gx[4] = {0.0, 0.0, 0.0, 0.0};
@@ -346,8 +351,14 @@ __global__ void __launch_bounds__(kBlock)
       double d = ov[k] - wv[k];
       gx[k] = gx[k] + cx * d;
       gy[k] = gy[k] + cy * d;
-      if (ov[k] < mn[k]) mn[k] = ov[k];
-      if (ov[k] > mx[k]) mx[k] = ov[k];
+      if (ov[k] < mn[k]) {
+        mn[k] = ov[k];
+        if (ARGS) arg[k] = (arg[k] & 0x00FFFFu) | ((unsigned)(e - e0) << 16);
+      }
+      if (ov[k] > mx[k]) {
+        mx[k] = ov[k];
+        if (ARGS) arg[k] = (arg[k] & 0xFF00FFu) | ((unsigned)(e - e0) << 8);
+      }
     }
   }
   double phi[4] = {1.0, 1.0, 1.0, 1.0};
@@ -356,8 +367,14 @@ __global__ void __launch_bounds__(kBlock)
     double dx = ldx[e], dy = ldy[e];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
+      double before = phi[k];
       limiter_entry(gx[k] * dx + gy[k] * dy, wv[k], mx[k], mn[k], code, phi[k]);
+      if (ARGS && phi[k] < before) arg[k] = (arg[k] & 0xFFFF00u) | (unsigned)(e - l0);
     }
+  }
+  if (ARGS) {
+    uint4* aq = reinterpret_cast<uint4*>(ARG + 4 * (long)i);
+    *aq = make_uint4(arg[0], arg[1], arg[2], arg[3]);
   }
   double2* gq = reinterpret_cast<double2*>(G + 8 * (long)i);
   gq[0] = make_double2(gx[0], gy[0]);
@@ -2284,6 +2301,287 @@ __device__ __forceinline__ P4D seed(const P4& w, int base) {
   return r;
 }
 
+// ---------------------------------------------------------------------------
+// Reverse mode of the Rusanov and HLLC fluxes: one primal pass that keeps the
+// intermediates in registers, one backward pass. (dF/dW_L)^T Fbar and
+// (dF/dW_R)^T Fbar cost about three primal fluxes instead of the nine of
+// eight-direction duals. Roe and far-field faces keep the dual path.
+
+struct SideR {
+  double r, u, v, p, un, E;
+};
+struct SideB {
+  double r = 0.0, u = 0.0, v = 0.0, p = 0.0, un = 0.0, E = 0.0;
+};
+
+__device__ __forceinline__ SideR side_r(const P4& w, double nx, double ny, double g) {
+  return SideR{w.r, w.u, w.v, w.p, w.u * nx + w.v * ny,
+               w.p / (g - 1.0) + 0.5 * w.r * (w.u * w.u + w.v * w.v)};
+}
+
+// F = (r un, r u un + p nx, r v un + p ny, (E + p) un): accumulate F^T fb.
+__device__ __forceinline__ void phys_flux_b(const SideR& s, const double* fb, double nx, double ny,
+                                            SideB& b) {
+  b.r += fb[0] * s.un + fb[1] * s.u * s.un + fb[2] * s.v * s.un;
+  b.u += fb[1] * s.r * s.un;
+  b.v += fb[2] * s.r * s.un;
+  b.p += fb[1] * nx + fb[2] * ny + fb[3] * s.un;
+  b.un += fb[0] * s.r + fb[1] * s.r * s.u + fb[2] * s.r * s.v + fb[3] * (s.E + s.p);
+  b.E += fb[3] * s.un;
+}
+
+// U = (r, r u, r v, E): accumulate U^T ub.
+__device__ __forceinline__ void cons_b(const SideR& s, const double* ub, SideB& b) {
+  b.r += ub[0] + ub[1] * s.u + ub[2] * s.v;
+  b.u += ub[1] * s.r;
+  b.v += ub[2] * s.r;
+  b.E += ub[3];
+}
+
+// c = sqrt(g p / r)
+__device__ __forceinline__ void sound_b(const SideR& s, double c, double cb, double g, SideB& b) {
+  double h = cb * 0.5 / c;
+  b.p += h * g / s.r;
+  b.r -= h * (g * s.p / s.r) / s.r;
+}
+
+// un = u nx + v ny and E = p/(g-1) + r (u^2 + v^2)/2 back to (r, u, v, p).
+__device__ __forceinline__ P4 side_finish_b(const SideR& s, const SideB& b, double nx, double ny,
+                                            double g) {
+  return P4{b.r + b.E * 0.5 * (s.u * s.u + s.v * s.v), b.u + b.E * s.r * s.u + b.un * nx,
+            b.v + b.E * s.r * s.v + b.un * ny, b.p + b.E / (g - 1.0)};
+}
+
+// Star-region flux of one side (hllc_star): F + S (U* - U).
+__device__ void hllc_star_b(const SideR& s, double S, double star, const double* fb, double nx,
+                            double ny, SideB& sb, double& Sb, double& starb) {
+  double q = S - s.un, d = S - star;
+  double factor = s.r * q / d;
+  double t = star - s.un;
+  double vx = s.u + t * nx, vy = s.v + t * ny;
+  double e1 = s.E / s.r;
+  double rq = s.r * q;
+  double e2 = star + s.p / rq;
+  double inner = e1 + t * e2;
+  double energy = factor * inner;
+  phys_flux_b(s, fb, nx, ny, sb);
+  Sb += fb[0] * (factor - s.r) + fb[1] * (factor * vx - s.r * s.u) +
+        fb[2] * (factor * vy - s.r * s.v) + fb[3] * (energy - s.E);
+  double factorb = fb[0] * S + fb[1] * S * vx + fb[2] * S * vy;
+  double vxb = fb[1] * S * factor, vyb = fb[2] * S * factor;
+  double ub[4] = {-fb[0] * S, -fb[1] * S, -fb[2] * S, -fb[3] * S};
+  double energyb = fb[3] * S;
+  factorb += energyb * inner;
+  double innerb = energyb * factor;
+  sb.E += innerb / s.r;
+  sb.r -= innerb * e1 / s.r;
+  double tb = innerb * e2;
+  double e2b = innerb * t;
+  starb += e2b;
+  sb.p += e2b / rq;
+  double rqb = -e2b * (s.p / rq) / rq;
+  sb.r += rqb * q;
+  double qb = rqb * s.r;
+  sb.u += vxb;
+  sb.v += vyb;
+  tb += vxb * nx + vyb * ny;
+  starb += tb;
+  sb.un -= tb;
+  sb.r += factorb * q / d;
+  qb += factorb * s.r / d;
+  double db = -factorb * factor / d;
+  Sb += qb + db;
+  sb.un -= qb;
+  starb -= db;
+  cons_b(s, ub, sb);
+}
+
+// (dF/dL)^T fb -> lb, (dF/dR)^T fb -> rb for hllc() (same branches).
+__device__ void hllc_b(const P4& L, const P4& R, double nx, double ny, double g, bool fused_order,
+                       const double* fb, P4& lb, P4& rb) {
+  SideR a = side_r(L, nx, ny, g), b = side_r(R, nx, ny, g);
+  SideB A, B;
+  double cl = sqrt(g * L.p / L.r), cr = sqrt(g * R.p / R.r);
+  double al = a.un - cl, ar = b.un - cr;
+  bool sl_left = al <= ar;
+  double sl = sl_left ? al : ar;
+  double bl = a.un + cl, br = b.un + cr;
+  bool sr_left = bl >= br;
+  double sr = sr_left ? bl : br;
+  double den = a.r * (sl - a.un) - b.r * (sr - b.un);
+  bool clamped = !(fabs(den) > 1e-14);
+  if (clamped) den = copysign(1e-14, den);
+  double num = b.p - a.p + a.r * a.un * (sl - a.un) - b.r * b.un * (sr - b.un);
+  double star = num / den;
+  int branch;  // 0 left flux, 1 right flux, 2 left star state, 3 right star state
+  if (sl >= 0.0)
+    branch = 0;
+  else if (fused_order)
+    branch = (sr <= 0.0) ? 1 : (star >= 0.0 ? 2 : 3);
+  else
+    branch = (star >= 0.0) ? 2 : (sr > 0.0 ? 3 : 1);
+  double clb = 0.0, crb = 0.0;
+  if (branch == 0) {
+    phys_flux_b(a, fb, nx, ny, A);
+  } else if (branch == 1) {
+    phys_flux_b(b, fb, nx, ny, B);
+  } else {
+    double slb = 0.0, srb = 0.0, starb = 0.0;
+    if (branch == 2)
+      hllc_star_b(a, sl, star, fb, nx, ny, A, slb, starb);
+    else
+      hllc_star_b(b, sr, star, fb, nx, ny, B, srb, starb);
+    double numb = starb / den;
+    double denb = clamped ? 0.0 : -starb * star / den;
+    B.p += numb;
+    A.p -= numb;
+    double t1 = numb * (sl - a.un), q1 = numb * a.r * a.un;
+    A.r += t1 * a.un;
+    A.un += t1 * a.r - q1;
+    slb += q1;
+    double t2 = -numb * (sr - b.un), q2 = -numb * b.r * b.un;
+    B.r += t2 * b.un;
+    B.un += t2 * b.r - q2;
+    srb += q2;
+    A.r += denb * (sl - a.un);
+    slb += denb * a.r;
+    A.un -= denb * a.r;
+    B.r -= denb * (sr - b.un);
+    srb -= denb * b.r;
+    B.un += denb * b.r;
+    if (sl_left) {
+      A.un += slb;
+      clb -= slb;
+    } else {
+      B.un += slb;
+      crb -= slb;
+    }
+    if (sr_left) {
+      A.un += srb;
+      clb += srb;
+    } else {
+      B.un += srb;
+      crb += srb;
+    }
+  }
+  if (clb != 0.0) sound_b(a, cl, clb, g, A);
+  if (crb != 0.0) sound_b(b, cr, crb, g, B);
+  lb = side_finish_b(a, A, nx, ny, g);
+  rb = side_finish_b(b, B, nx, ny, g);
+}
+
+// (dF/dL)^T fb, (dF/dR)^T fb for rusanov().
+__device__ void rusanov_b(const P4& L, const P4& R, double nx, double ny, double g, const double* fb,
+                          P4& lb, P4& rb) {
+  SideR a = side_r(L, nx, ny, g), b = side_r(R, nx, ny, g);
+  SideB A, B;
+  double cl = sqrt(g * L.p / L.r), cr = sqrt(g * R.p / R.r);
+  double xa = fabs(a.un) + cl, xb = fabs(b.un) + cr;
+  bool left = xa >= xb;
+  double sp = left ? xa : xb;
+  double h[4] = {0.5 * fb[0], 0.5 * fb[1], 0.5 * fb[2], 0.5 * fb[3]};
+  phys_flux_b(a, h, nx, ny, A);
+  phys_flux_b(b, h, nx, ny, B);
+  double spb = -(h[0] * (b.r - a.r) + h[1] * (b.r * b.u - a.r * a.u) + h[2] * (b.r * b.v - a.r * a.v) +
+                 h[3] * (b.E - a.E));
+  double ulb[4] = {h[0] * sp, h[1] * sp, h[2] * sp, h[3] * sp};
+  double urb[4] = {-h[0] * sp, -h[1] * sp, -h[2] * sp, -h[3] * sp};
+  cons_b(a, ulb, A);
+  cons_b(b, urb, B);
+  if (left) {
+    A.un += (a.un < 0.0) ? -spb : spb;
+    sound_b(a, cl, spb, g, A);
+  } else {
+    B.un += (b.un < 0.0) ? -spb : spb;
+    sound_b(b, cr, spb, g, B);
+  }
+  lb = side_finish_b(a, A, nx, ny, g);
+  rb = side_finish_b(b, B, nx, ny, g);
+}
+
+// Face adjoint in reverse mode (Rusanov, HLLC; far-field faces are left to
+// k_flux_adjoint). Same outputs as k_flux_adjoint: FADJ[k][0..3] the
+// adjoint of the reconstructed left state (through the ghost state on
+// boundary faces), [4..7] that of the right / exterior state.
+__global__ void __launch_bounds__(kBlock)
+    k_flux_adjoint_rev(int nf, FaceArrays fa, const double* __restrict__ W,
+                       const double* __restrict__ G, const double* __restrict__ PHI,
+                       const double* __restrict__ flen, const double* __restrict__ area,
+                       const double* __restrict__ lam, int scheme, int muscl, double g,
+                       double* __restrict__ FADJ, double* __restrict__ SBAR) {
+  int k = xcd_block() * blockDim.x + threadIdx.x;
+  if (k >= nf) return;
+  int kind = fa.kind[k];
+  if (kind == MCFD_FACE_FAR_FIELD) return;
+  double nx = fa.nx[k], ny = fa.ny[k];
+  int l = fa.left[k];
+  int r = fa.right[k];
+  double L = flen[k];
+  P4 ll = load4(lam, l);
+  double cl = -L / area[l];
+  double fb[4] = {cl * ll.r, cl * ll.u, cl * ll.v, cl * ll.p};
+  if (kind <= MCFD_FACE_PERIODIC) {
+    P4 lr = load4(lam, r);
+    double cr = L / area[r];
+    fb[0] += cr * lr.r;
+    fb[1] += cr * lr.u;
+    fb[2] += cr * lr.v;
+    fb[3] += cr * lr.p;
+  }
+  P4 wl = reconstruct(W, G, PHI, l, fa.lx[k], fa.ly[k], muscl != 0);
+  P4 wr;
+  int b = fa.bc[k];
+  if (kind <= MCFD_FACE_PERIODIC) {
+    wr = reconstruct(W, G, PHI, r, fa.rx[k], fa.ry[k], muscl != 0);
+  } else {
+    const double* st = fa.bc_state + 4 * (long)b;
+    wr = wl;
+    if (kind == MCFD_FACE_WALL) {
+      double un = (wl.u - fa.bc_wall[2 * (long)b]) * nx + (wl.v - fa.bc_wall[2 * (long)b + 1]) * ny;
+      wr.u = wl.u - 2.0 * un * nx;
+      wr.v = wl.v - 2.0 * un * ny;
+    } else if (kind == MCFD_FACE_VELOCITY_INLET) {
+      wr = P4{st[0], st[1], st[2], st[3]};
+    } else if (kind == MCFD_FACE_PRESSURE_OUTLET) {
+      wr.p = st[3];
+    }
+  }
+  P4 lb, rb;
+  if (scheme == 1)
+    hllc_b(wl, wr, nx, ny, g, kind == MCFD_FACE_INTERIOR, fb, lb, rb);
+  else
+    rusanov_b(wl, wr, nx, ny, g, fb, lb, rb);
+  P4 ext{0.0, 0.0, 0.0, 0.0};
+  if (kind > MCFD_FACE_PERIODIC) {
+    if (kind == MCFD_FACE_WALL) {
+      // mirrored velocity: d(u_R, v_R)/d(u_L, v_L) = I - 2 n n^T
+      double un_b = rb.u * nx + rb.v * ny;
+      lb.r += rb.r;
+      lb.u += rb.u - 2.0 * nx * un_b;
+      lb.v += rb.v - 2.0 * ny * un_b;
+      lb.p += rb.p;
+    } else if (kind == MCFD_FACE_VELOCITY_INLET) {
+      ext = rb;
+    } else if (kind == MCFD_FACE_PRESSURE_OUTLET) {
+      lb.r += rb.r;
+      lb.u += rb.u;
+      lb.v += rb.v;
+    } else {  // transmissive: exterior = interior
+      lb.r += rb.r;
+      lb.u += rb.u;
+      lb.v += rb.v;
+      lb.p += rb.p;
+    }
+    rb = ext;
+  }
+  double2* q = reinterpret_cast<double2*>(FADJ + 8 * (long)k);
+  q[0] = make_double2(lb.r, lb.u);
+  q[1] = make_double2(lb.v, lb.p);
+  q[2] = make_double2(rb.r, rb.u);
+  q[3] = make_double2(rb.v, rb.p);
+  if (SBAR && kind > MCFD_FACE_PERIODIC) store4(SBAR, b, ext.r, ext.u, ext.v, ext.p);
+}
+
 // Face adjoint: Fbar = sum of the face's residual coefficients times lambda,
 // then (dF/dW_L)^T Fbar and (dF/dW_R)^T Fbar -> FADJ[k][0..7].
 __global__ void __launch_bounds__(kBlock)
@@ -2291,10 +2589,11 @@ __global__ void __launch_bounds__(kBlock)
                    const double* __restrict__ G, const double* __restrict__ PHI,
                    const double* __restrict__ flen, const double* __restrict__ area,
                    const double* __restrict__ lam, int scheme, int muscl, double g,
-                   double* __restrict__ FADJ, double* __restrict__ SBAR) {
+                   double* __restrict__ FADJ, double* __restrict__ SBAR, int only_far_field) {
   int k = xcd_block() * blockDim.x + threadIdx.x;
   if (k >= nf) return;
   int kind = fa.kind[k];
+  if (only_far_field && kind != MCFD_FACE_FAR_FIELD) return;
   double nx = fa.nx[k], ny = fa.ny[k];
   int l = fa.left[k];
   int r = fa.right[k];
@@ -2364,9 +2663,10 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Cell pass 1: gather face adjoints (reconstruction backward), limiter
-// backward. Writes per cell: Wbar (local part), gbar, and for each variable
-// the extremum adjoints with the LSQ entry (block-global index) that holds
-// the extremum (-1: the cell itself).
+// backward, LSQ-gradient backward. Writes per cell Wbar without the
+// neighbours' contributions, and per LSQ entry e (i -> j) the 4-vector that
+// cell i's gradient and extremum adjoints contribute to W_j:
+// ebar[e] = c_e . gbar_i (+ the extremum adjoint if j holds i's extremum).
 struct AdjArrays {
   const int* acc_ptr;
   const int* acc_item;
@@ -2383,9 +2683,8 @@ struct AdjArrays {
   const double* lim_dx;
   const double* lim_dy;
   double* wbar;   // (C,4)
-  double* gbar;   // (C,8)
-  double* xbar;   // (C,8): max-adjoint (4), min-adjoint (4)
-  int* xarg;      // (C,8): entry of max (4), of min (4), -1 = own cell
+  double* ebar;   // (n_grad,4) per LSQ entry
+  unsigned* arg;  // (C,4) limiting / extremum entries of the forward limiter (k_gradient_t<true>)
 };
 
 __device__ __forceinline__ double limiter_value(double ratio, int code, double& dfdr) {
@@ -2433,111 +2732,106 @@ __global__ void __launch_bounds__(kBlock)
   double wb[4] = {0.0, 0.0, 0.0, 0.0};
   double pb[4] = {0.0, 0.0, 0.0, 0.0};
   double gb[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  P4 w = load4(W, i);
-  double wv[4] = {w.r, w.u, w.v, w.p};
   double gv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   double pv[4] = {1.0, 1.0, 1.0, 1.0};
   if (muscl) {
-    for (int k = 0; k < 8; ++k) gv[k] = G[8 * (long)i + k];
-    for (int k = 0; k < 4; ++k) pv[k] = PHI[4 * (long)i + k];
+    const double2* gq = reinterpret_cast<const double2*>(G + 8 * (long)i);
+    for (int k = 0; k < 4; ++k) {
+      double2 t = gq[k];
+      gv[2 * k] = t.x;
+      gv[2 * k + 1] = t.y;
+    }
+    P4 f = load4(PHI, i);
+    pv[0] = f.r;
+    pv[1] = f.u;
+    pv[2] = f.v;
+    pv[3] = f.p;
   }
   for (int e = aa.acc_ptr[i]; e < aa.acc_ptr[i + 1]; ++e) {
     int item = aa.acc_item[e];
     int f = item >> 2;
     int mode = item & 3;
     bool right = (mode == MCFD_ACC_NEIGHBOUR) || (mode == MCFD_ACC_PERIODIC_B);
-    const double* q = FADJ + 8 * (long)f + (right ? 4 : 0);
+    const double2* q = reinterpret_cast<const double2*>(FADJ + 8 * (long)f + (right ? 4 : 0));
+    double2 q0 = q[0], q1 = q[1];
+    double a[4] = {q0.x, q0.y, q1.x, q1.y};
     double dx = right ? aa.frx[f] : aa.flx[f];
     double dy = right ? aa.fry[f] : aa.fly[f];
     for (int k = 0; k < 4; ++k) {
-      double a = q[k];
-      wb[k] += a;
+      wb[k] += a[k];
       if (muscl) {
         double inc = gv[2 * k] * dx + gv[2 * k + 1] * dy;
-        pb[k] += a * inc;
-        gb[2 * k] += a * pv[k] * dx;
-        gb[2 * k + 1] += a * pv[k] * dy;
+        pb[k] += a[k] * inc;
+        gb[2 * k] += a[k] * pv[k] * dx;
+        gb[2 * k + 1] += a[k] * pv[k] * dy;
       }
     }
   }
-  double xb[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  int xa[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
   if (muscl) {
-    // extrema and their entries (first strict extremum in entry order)
-    double mn[4] = {wv[0], wv[1], wv[2], wv[3]};
-    double mx[4] = {wv[0], wv[1], wv[2], wv[3]};
-    for (int e = aa.grad_ptr[i]; e < aa.grad_ptr[i + 1]; ++e) {
-      P4 o = load4(W, aa.grad_nb[e]);
-      double ov[4] = {o.r, o.u, o.v, o.p};
+    P4 w = load4(W, i);
+    double wv[4] = {w.r, w.u, w.v, w.p};
+    const int g0 = aa.grad_ptr[i];
+    uint4 av = *reinterpret_cast<const uint4*>(aa.arg + 4 * (long)i);
+    unsigned arg[4] = {av.x, av.y, av.z, av.w};
+    // extremum adjoints: xb[k] goes to the entry holding the extremum
+    // (0xFF: the cell itself); at most one per variable is non-zero
+    double xb[4] = {0.0, 0.0, 0.0, 0.0};
+    int xe[4] = {-1, -1, -1, -1};
+    // limiter backward at the entry that set phi (the forward's first strict minimum)
+    for (int k = 0; k < 4; ++k) {
+      unsigned le = arg[k] & 0xFFu;
+      if (pb[k] == 0.0 || le == 0xFFu) continue;
+      int e = aa.lim_ptr[i] + (int)le;
+      double dx = aa.lim_dx[e], dy = aa.lim_dy[e];
+      double inc = gv[2 * k] * dx + gv[2 * k + 1] * dy;
+      double tol = 1e-14 * vmax(fabs(wv[k]), 1.0);
+      if (!(inc > tol) && !(inc < -tol)) continue;  // ratio 1: no dependence
+      bool up = inc > tol;
+      unsigned xa = up ? ((arg[k] >> 8) & 0xFFu) : ((arg[k] >> 16) & 0xFFu);
+      int xn = (xa == 0xFFu) ? -1 : g0 + (int)xa;
+      double ext = wv[k];
+      if (xn >= 0) {
+        P4 o = load4(W, aa.grad_nb[xn]);
+        double ov[4] = {o.r, o.u, o.v, o.p};
+        ext = ov[k];
+      }
+      double ratio = (ext - wv[k]) / inc;
+      if (0.0 > ratio) continue;  // clamped at 0
+      double dfdr;
+      (void)limiter_value(ratio, code, dfdr);
+      if (dfdr == 0.0) continue;
+      double rb = pb[k] * dfdr;      // d phi / d ratio
+      double ext_b = rb / inc;       // d ratio / d extremum
+      wb[k] -= ext_b;                // d ratio / d W_i
+      double incb = -rb * ratio / inc;
+      gb[2 * k] += incb * dx;
+      gb[2 * k + 1] += incb * dy;
+      if (xn < 0)
+        wb[k] += ext_b;
+      else {
+        xb[k] = ext_b;
+        xe[k] = xn;
+      }
+    }
+    // LSQ gradient backward: grad_i = sum_e c_e (W_j - W_i)
+    for (int e = g0; e < aa.grad_ptr[i + 1]; ++e) {
+      double cx = aa.grad_cx[e], cy = aa.grad_cy[e];
+      double c[4];
       for (int k = 0; k < 4; ++k) {
-        if (ov[k] < mn[k]) { mn[k] = ov[k]; xa[4 + k] = e; }
-        if (ov[k] > mx[k]) { mx[k] = ov[k]; xa[k] = e; }
+        double t = cx * gb[2 * k] + cy * gb[2 * k + 1];
+        wb[k] -= t;
+        if (xe[k] == e) t += xb[k];
+        c[k] = t;
       }
-    }
-    // limiter backward: phi = min over entries (first strict minimum below 1)
-    for (int k = 0; k < 4; ++k) {
-      if (pb[k] == 0.0) continue;
-      double best = 1.0;
-      int best_e = -1;
-      double best_d = 0.0, best_ratio = 0.0, best_inc = 0.0, best_dx = 0.0, best_dy = 0.0;
-      int best_side = 0;
-      for (int e = aa.lim_ptr[i]; e < aa.lim_ptr[i + 1]; ++e) {
-        double dx = aa.lim_dx[e], dy = aa.lim_dy[e];
-        double inc = gv[2 * k] * dx + gv[2 * k + 1] * dy;
-        double tol = 1e-14 * vmax(fabs(wv[k]), 1.0);
-        double ratio = 1.0;
-        int side = 0;
-        if (inc > tol) {
-          ratio = (mx[k] - wv[k]) / inc;
-          side = 1;
-        } else if (inc < -tol) {
-          ratio = (mn[k] - wv[k]) / inc;
-          side = -1;
-        }
-        bool clamped = 0.0 > ratio;
-        ratio = clamped ? 0.0 : ratio;
-        double dfdr;
-        double f = limiter_value(ratio, code, dfdr);
-        if (f < best) {
-          best = f;
-          best_e = e;
-          best_d = (side == 0 || clamped) ? 0.0 : dfdr;
-          best_ratio = ratio;
-          best_inc = inc;
-          best_dx = dx;
-          best_dy = dy;
-          best_side = side;
-        }
-      }
-      if (best_e < 0 || best_d == 0.0) continue;
-      double rb = pb[k] * best_d;           // d phi / d ratio
-      double ext_b = rb / best_inc;         // d ratio / d extremum
-      wb[k] -= ext_b;                       // d ratio / d W_i
-      double incb = -rb * best_ratio / best_inc;
-      gb[2 * k] += incb * best_dx;
-      gb[2 * k + 1] += incb * best_dy;
-      if (best_side > 0) xb[k] += ext_b; else xb[4 + k] += ext_b;
-    }
-    // extrema owned by the cell itself flow back to W_i
-    for (int k = 0; k < 4; ++k) {
-      if (xa[k] < 0) wb[k] += xb[k];
-      if (xa[4 + k] < 0) wb[k] += xb[4 + k];
+      store4(aa.ebar, e, c[0], c[1], c[2], c[3]);
     }
   }
   store4(aa.wbar, i, wb[0], wb[1], wb[2], wb[3]);
-  double* gq = aa.gbar + 8 * (long)i;
-  double* xq = aa.xbar + 8 * (long)i;
-  int* aq = aa.xarg + 8 * (long)i;
-  for (int k = 0; k < 8; ++k) {
-    gq[k] = gb[k];
-    xq[k] = xb[k];
-    aq[k] = xa[k];
-  }
 }
 
-// Cell pass 2: LSQ-gradient backward (own term and the neighbours' terms via
-// the reverse entries), extremum adjoints from neighbours, body force, and
-// the primitive -> conservative chain rule. Writes Ubar.
+// Cell pass 2: the neighbours' contributions (their entries pointing back at
+// the cell, in the cell's entry order), body force, and the primitive ->
+// conservative chain rule. Writes Ubar.
 __global__ void __launch_bounds__(kBlock)
     k_cell_adjoint2(int n, const double* __restrict__ U, const double* __restrict__ W,
                     const double* __restrict__ lam, AdjArrays aa, int muscl, double g, double ax,
@@ -2547,24 +2841,17 @@ __global__ void __launch_bounds__(kBlock)
   P4 wb4 = load4(aa.wbar, i);
   double wb[4] = {wb4.r, wb4.u, wb4.v, wb4.p};
   if (muscl) {
-    const double* gi = aa.gbar + 8 * (long)i;
     for (int e = aa.grad_ptr[i]; e < aa.grad_ptr[i + 1]; ++e) {
-      double cx = aa.grad_cx[e], cy = aa.grad_cy[e];
-      int j = aa.grad_nb[e];
-      int r = aa.grad_rev[e];  // entry j -> i
-      const double* gj = aa.gbar + 8 * (long)j;
-      double cxr = aa.grad_cx[r], cyr = aa.grad_cy[r];
-      const double* xj = aa.xbar + 8 * (long)j;
-      const int* aj = aa.xarg + 8 * (long)j;
-      for (int k = 0; k < 4; ++k) {
-        wb[k] -= cx * gi[2 * k] + cy * gi[2 * k + 1];
-        wb[k] += cxr * gj[2 * k] + cyr * gj[2 * k + 1];
-        if (aj[k] == r) wb[k] += xj[k];
-        if (aj[4 + k] == r) wb[k] += xj[4 + k];
-      }
+      P4 c = load4(aa.ebar, aa.grad_rev[e]);  // entry j -> i
+      wb[0] += c.r;
+      wb[1] += c.u;
+      wb[2] += c.v;
+      wb[3] += c.p;
     }
   }
-  P4 w = load4(W, i);
+  // primitive velocity of the cell, as k_primitive computes it
+  P4 q = load4(U, i);
+  P4 w{q.r, q.u / q.r, q.v / q.r, 0.0};
   if (ax != 0.0 || ay != 0.0) {
     P4 l = load4(lam, i);
     wb[0] += l.u * ax + l.v * ay + l.p * (ax * w.u + ay * w.v);
@@ -2578,9 +2865,271 @@ __global__ void __launch_bounds__(kBlock)
   double u2 = wb[2] / rho - wb[3] * (g - 1.0) * v;
   double u3 = wb[3] * (g - 1.0);
   store4(ubar, i, u0, u1, u2, u3);
-  (void)U;
+  (void)W;
 }
 
+
+// ---------------------------------------------------------------------------
+// Fused VJP, part 1: one workgroup per fused-stage block does the reverse of
+// the block's residual up to the per-cell results of k_cell_adjoint1, out of
+// LDS: W of own + ring cells, lambda/A and the gradient, limiter factors
+// and limiter arguments of own + ring-1 cells, the face adjoints of every
+// face touching an own cell. k_cell_adjoint2 then adds the neighbours'
+// contributions. Rusanov and HLLC (reverse mode), no far field.
+//   P0  W of local cells, lambda/A of own + ring-1 cells       -> LDS
+//   P1  gradient + limiter (+ limiting / extremum entries)     -> LDS (+ registers)
+//   P2  face adjoints, one face per thread                     -> LDS
+//   P3  per own cell: reconstruction, limiter and LSQ backward -> wbar, ebar
+template <int SCHEME, bool MUSCL, int LIM>
+__global__ void __launch_bounds__(kBlock)
+    k_vjp_fused(int max_local, int max_grad, int max_faces, BlockArrays ba, GeomArrays ga,
+                FaceArrays fa, const double* __restrict__ U, const double* __restrict__ lam, double g,
+                double* __restrict__ wbar, double* __restrict__ ebar, double* __restrict__ SBAR,
+                int* __restrict__ status) {
+  extern __shared__ double smem[];
+  const int b = xcd_block();
+  const int* inf = ba.info + kBlockInfo * b;
+  const int cs = inf[0], n_own = inf[1], n_r1 = inf[2], n_r2 = inf[3];
+  const int fs = inf[4], n_faces = inf[5], gs = inf[6], gps = inf[7];
+  const int first = inf[8], ls = inf[9];
+  const int n_loc = n_own + n_r1 + n_r2;
+  const int n_grad = n_own + n_r1;
+  double* sW = smem;                       // [4 max_local]
+  double* sG = sW + 4 * max_local;         // [8 max_grad]
+  double* sP = sG + 8 * max_grad;          // [4 max_grad]
+  double* sLam = sP + 4 * max_grad;        // [4 max_grad] lambda / A, until P2 ends
+  double* sA = sLam;                       // [8 max_faces] face adjoints, after P2
+  const int t = threadIdx.x;
+  int e0 = 0, e1 = 0, l0 = 0, l1 = 0;
+  if (MUSCL && t < n_grad) {
+    e0 = ba.gptr[gps + t];
+    e1 = ba.gptr[gps + t + 1];
+    l0 = ba.lptr[gps + t];
+    l1 = ba.lptr[gps + t + 1];
+  }
+  // P0
+  for (int j = t; j < n_loc; j += kBlock) {
+    int c = (j < n_own) ? first + j : ba.cells[cs + j];
+    P4 q = load4(U, c);
+    double rho = q.r;
+    double u = q.u / rho;
+    double v = q.v / rho;
+    double p = (g - 1.0) * (q.p - 0.5 * (q.u * q.u + q.v * q.v) / rho);
+    if (j < n_own && (!finite(rho) || !finite(u) || !finite(v) || !finite(p) || !(rho > 0.0) || !(p > 0.0)))
+      report(status, MCFD_ERR_STATE, c);
+    sW[4 * j] = rho;
+    sW[4 * j + 1] = u;
+    sW[4 * j + 2] = v;
+    sW[4 * j + 3] = p;
+    if (j < n_grad) {
+      P4 l = load4(lam, c);
+      double A = ga.area[c];
+      sLam[4 * j] = l.r / A;
+      sLam[4 * j + 1] = l.u / A;
+      sLam[4 * j + 2] = l.v / A;
+      sLam[4 * j + 3] = l.p / A;
+    }
+  }
+  __syncthreads();
+  // P1 (as k_gradient_t<true>)
+  unsigned arg[4] = {0xFFFFFFu, 0xFFFFFFu, 0xFFFFFFu, 0xFFFFFFu};
+  if constexpr (MUSCL) {
+    if (t < n_grad) {
+      const int j = t;
+      double wv[4] = {sW[4 * j], sW[4 * j + 1], sW[4 * j + 2], sW[4 * j + 3]};
+      double gx[4] = {0.0, 0.0, 0.0, 0.0};
+      double gy[4] = {0.0, 0.0, 0.0, 0.0};
+      double mn[4] = {wv[0], wv[1], wv[2], wv[3]};
+      double mx[4] = {wv[0], wv[1], wv[2], wv[3]};
+      for (int e = e0; e < e1; ++e) {
+        int o = ba.gnb[gs + e];
+        double2 c2 = ba.gent[gs + e];
+        double ov[4] = {sW[4 * o], sW[4 * o + 1], sW[4 * o + 2], sW[4 * o + 3]};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          double d = ov[k] - wv[k];
+          gx[k] = gx[k] + c2.x * d;
+          gy[k] = gy[k] + c2.y * d;
+          if (ov[k] < mn[k]) {
+            mn[k] = ov[k];
+            arg[k] = (arg[k] & 0x00FFFFu) | ((unsigned)(e - e0) << 16);
+          }
+          if (ov[k] > mx[k]) {
+            mx[k] = ov[k];
+            arg[k] = (arg[k] & 0xFF00FFu) | ((unsigned)(e - e0) << 8);
+          }
+        }
+      }
+      double phi[4] = {1.0, 1.0, 1.0, 1.0};
+      for (int e = l0; e < l1; ++e) {
+        double2 d2 = ba.lent[ls + e];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          double before = phi[k];
+          limiter_entry(gx[k] * d2.x + gy[k] * d2.y, wv[k], mx[k], mn[k], LIM, phi[k]);
+          if (phi[k] < before) arg[k] = (arg[k] & 0xFFFF00u) | (unsigned)(e - l0);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        sG[8 * j + 2 * k] = gx[k];
+        sG[8 * j + 2 * k + 1] = gy[k];
+        sP[4 * j + k] = phi[k];
+      }
+    }
+    __syncthreads();
+  }
+  // P2: face adjoints (as k_flux_adjoint_rev); results stay in registers
+  // until every thread has read lambda
+  P4 alb{0.0, 0.0, 0.0, 0.0}, arb{0.0, 0.0, 0.0, 0.0};
+  if (t < n_faces) {
+    const FaceRec fr = ba.frec[fs + t];
+    const int kind = fr.kind_bc & 15;
+    const int l = fr.lr & 0xFFFF, r = fr.lr >> 16;
+    const double nx = fr.nx, ny = fr.ny, L = fr.len;
+    double fb[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) fb[k] = -L * sLam[4 * l + k];
+    P4 wl = rec_lds(sW, sG, sP, l, fr.lx, fr.ly, MUSCL);
+    P4 wr = wl;
+    int bi = fr.kind_bc >> 4;
+    if (kind <= MCFD_FACE_PERIODIC) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) fb[k] += L * sLam[4 * r + k];
+      wr = rec_lds(sW, sG, sP, r, fr.rx, fr.ry, MUSCL);
+    } else if (kind == MCFD_FACE_WALL) {
+      double un = (wl.u - fa.bc_wall[2 * (long)bi]) * nx + (wl.v - fa.bc_wall[2 * (long)bi + 1]) * ny;
+      wr.u = wl.u - 2.0 * un * nx;
+      wr.v = wl.v - 2.0 * un * ny;
+    } else if (kind == MCFD_FACE_VELOCITY_INLET) {
+      const double* st = fa.bc_state + 4 * (long)bi;
+      wr = P4{st[0], st[1], st[2], st[3]};
+    } else if (kind == MCFD_FACE_PRESSURE_OUTLET) {
+      wr.p = fa.bc_state[4 * (long)bi + 3];
+    }
+    if constexpr (SCHEME == 1)
+      hllc_b(wl, wr, nx, ny, g, kind == MCFD_FACE_INTERIOR, fb, alb, arb);
+    else
+      rusanov_b(wl, wr, nx, ny, g, fb, alb, arb);
+    if (kind > MCFD_FACE_PERIODIC) {
+      P4 ext{0.0, 0.0, 0.0, 0.0};
+      if (kind == MCFD_FACE_WALL) {
+        double un_b = arb.u * nx + arb.v * ny;
+        alb.r += arb.r;
+        alb.u += arb.u - 2.0 * nx * un_b;
+        alb.v += arb.v - 2.0 * ny * un_b;
+        alb.p += arb.p;
+      } else if (kind == MCFD_FACE_VELOCITY_INLET) {
+        ext = arb;
+      } else if (kind == MCFD_FACE_PRESSURE_OUTLET) {
+        alb.r += arb.r;
+        alb.u += arb.u;
+        alb.v += arb.v;
+      } else {
+        alb.r += arb.r;
+        alb.u += arb.u;
+        alb.v += arb.v;
+        alb.p += arb.p;
+      }
+      arb = ext;
+      if (SBAR) store4(SBAR, bi, ext.r, ext.u, ext.v, ext.p);
+    }
+  }
+  __syncthreads();
+  if (t < n_faces) {
+    double* q = sA + 8 * t;
+    q[0] = alb.r;
+    q[1] = alb.u;
+    q[2] = alb.v;
+    q[3] = alb.p;
+    q[4] = arb.r;
+    q[5] = arb.u;
+    q[6] = arb.v;
+    q[7] = arb.p;
+  }
+  __syncthreads();
+  // P3: own cell t (as k_cell_adjoint1)
+  if (t >= n_own) return;
+  const int i = first + t;
+  double wb[4] = {0.0, 0.0, 0.0, 0.0};
+  double pb[4] = {0.0, 0.0, 0.0, 0.0};
+  double gb[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double gv[8], pv[4];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) gv[k] = MUSCL ? sG[8 * t + k] : 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) pv[k] = MUSCL ? sP[4 * t + k] : 1.0;
+  for (int e = ga.acc_ptr[i]; e < ga.acc_ptr[i + 1]; ++e) {
+    int item = ba.acc_local[e];
+    int slot = item >> 2;
+    int mode = item & 3;
+    bool right = (mode == MCFD_ACC_NEIGHBOUR) || (mode == MCFD_ACC_PERIODIC_B);
+    const double* q = sA + 8 * slot + (right ? 4 : 0);
+    double a[4] = {q[0], q[1], q[2], q[3]};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) wb[k] += a[k];
+    if (MUSCL) {
+      const double2* rec = reinterpret_cast<const double2*>(ba.frec + fs + slot);
+      double2 d = right ? rec[2] : rec[1];  // (rx, ry) or (lx, ly)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        double inc = gv[2 * k] * d.x + gv[2 * k + 1] * d.y;
+        pb[k] += a[k] * inc;
+        gb[2 * k] += a[k] * pv[k] * d.x;
+        gb[2 * k + 1] += a[k] * pv[k] * d.y;
+      }
+    }
+  }
+  if constexpr (MUSCL) {
+    double wv[4] = {sW[4 * t], sW[4 * t + 1], sW[4 * t + 2], sW[4 * t + 3]};
+    double xb[4] = {0.0, 0.0, 0.0, 0.0};
+    int xe[4] = {-1, -1, -1, -1};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      unsigned le = arg[k] & 0xFFu;
+      if (pb[k] == 0.0 || le == 0xFFu) continue;
+      double2 d2 = ba.lent[ls + l0 + (int)le];
+      double inc = gv[2 * k] * d2.x + gv[2 * k + 1] * d2.y;
+      double tol = 1e-14 * vmax(fabs(wv[k]), 1.0);
+      if (!(inc > tol) && !(inc < -tol)) continue;
+      bool up = inc > tol;
+      unsigned xa = up ? ((arg[k] >> 8) & 0xFFu) : ((arg[k] >> 16) & 0xFFu);
+      int xn = (xa == 0xFFu) ? -1 : e0 + (int)xa;
+      double ext = (xn < 0) ? wv[k] : sW[4 * (int)ba.gnb[gs + xn] + k];
+      double ratio = (ext - wv[k]) / inc;
+      if (0.0 > ratio) continue;
+      double dfdr;
+      (void)limiter_value(ratio, LIM, dfdr);
+      if (dfdr == 0.0) continue;
+      double rb = pb[k] * dfdr;
+      double ext_b = rb / inc;
+      wb[k] -= ext_b;
+      double incb = -rb * ratio / inc;
+      gb[2 * k] += incb * d2.x;
+      gb[2 * k + 1] += incb * d2.y;
+      if (xn < 0) {
+        wb[k] += ext_b;
+      } else {
+        xb[k] = ext_b;
+        xe[k] = xn;
+      }
+    }
+    const int gbase = ga.grad_ptr[i] - e0;
+    for (int e = e0; e < e1; ++e) {
+      double2 c2 = ba.gent[gs + e];
+      double c[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        double tv = c2.x * gb[2 * k] + c2.y * gb[2 * k + 1];
+        wb[k] -= tv;
+        if (xe[k] == e) tv += xb[k];
+        c[k] = tv;
+      }
+      store4(ebar, gbase + e, c[0], c[1], c[2], c[3]);
+    }
+  }
+  store4(wbar, i, wb[0], wb[1], wb[2], wb[3]);
+}
 
 // Batched cases: per-case CFL step = min over the case's cell range, clipped
 // to end_time - time_case, then broadcast to the case's cells. One block per
@@ -2840,6 +3389,7 @@ struct McfdMesh {
   std::vector<void*> inflow_allocations;
   std::vector<int> host_face_bc;  // [n_faces] for validating the inflow records
   std::vector<int> host_bc_kind;  // [n_bc]
+  int max_cell_entries = 0;       // longest LSQ or limiter list of a cell
   // timing
   bool timing = false;
   std::vector<hipEvent_t> event_pool;
@@ -3155,9 +3705,9 @@ int residual_front(McfdMesh* m, const McfdSettings* st, const double* u, hipStre
                        m->G, m->K, m->PHI);
   } else if (muscl) {
     TimedLaunch t(m, s, MCFD_KERNEL_GRADIENT);
-    hipLaunchKernelGGL(k_gradient, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->grad_ptr,
+    hipLaunchKernelGGL(k_gradient_t<false>, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->grad_ptr,
                        m->grad_nb, m->grad_cx, m->grad_cy, m->lim_ptr, m->lim_dx, m->lim_dy,
-                       st->limiter, m->G, m->PHI);
+                       st->limiter, m->G, m->PHI, nullptr);
   }
   if (m->n_faces) {
     TimedLaunch t(m, s, MCFD_KERNEL_FLUX);
@@ -3456,6 +4006,32 @@ int gmres_cycle(JacobianOp& op, const double* b, double* x, double rtol, int res
 
 namespace {
 
+template <int SCHEME, bool MUSCL, int LIM>
+void launch_vjp_fused_t(McfdMesh* m, size_t lds, hipStream_t s, const GeomArrays& ga, const double* u,
+                        const double* lam, double g, double* sbar) {
+  auto kern = k_vjp_fused<SCHEME, MUSCL, LIM>;
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024);
+  hipLaunchKernelGGL(kern, dim3(m->nb), dim3(kBlock), lds, s, m->max_local, m->max_grad, m->max_faces,
+                     m->blocks, ga, m->faces, u, lam, g, m->adj.wbar, m->adj.ebar, sbar, m->status);
+}
+
+template <int SCHEME>
+void launch_vjp_fused_s(McfdMesh* m, bool muscl, int lim, size_t lds, hipStream_t s, const GeomArrays& ga,
+                        const double* u, const double* lam, double g, double* sbar) {
+  if (!muscl) return launch_vjp_fused_t<SCHEME, false, 0>(m, lds, s, ga, u, lam, g, sbar);
+  if (lim <= 1) return launch_vjp_fused_t<SCHEME, true, 0>(m, lds, s, ga, u, lam, g, sbar);
+  if (lim == 2) return launch_vjp_fused_t<SCHEME, true, 2>(m, lds, s, ga, u, lam, g, sbar);
+  return launch_vjp_fused_t<SCHEME, true, 3>(m, lds, s, ga, u, lam, g, sbar);
+}
+
+// LDS of k_vjp_fused: W (4 max_local), G and phi (12 max_grad), then
+// lambda/A (4 max_grad) aliased by the face adjoints (8 max_faces).
+size_t vjp_fused_lds_bytes(const McfdMesh* m) {
+  size_t region = std::max(4 * (size_t)m->max_grad, 8 * (size_t)m->max_faces);
+  return (4 * (size_t)m->max_local + 12 * (size_t)m->max_grad + region) * sizeof(double);
+}
+
 int mcfd_residual_vjp_impl(McfdMesh* m, const McfdSettings* st, const double* u,
                            const double* lam, double* ubar, double* sbar, hipStream_t s) {
   if (!m || !u || !lam || !ubar) return fail("mcfd_residual_vjp: NULL argument");
@@ -3467,16 +4043,39 @@ int mcfd_residual_vjp_impl(McfdMesh* m, const McfdSettings* st, const double* u,
     return fail("mcfd_residual_vjp: pressure-inlet and mass-flow boundaries have no adjoint yet");
   int n = m->n_cells;
   bool muscl = st->reconstruction == 1 && m->n_grad > 0;
+  size_t vjp_lds = m->fused ? vjp_fused_lds_bytes(m) : 0;
+  if (m->fused && st->flux != 2 && !m->has_far_field && vjp_lds <= 64 * 1024) {
+    // one workgroup per block (reverse-mode fluxes), then the neighbour pass
+    GeomArrays ga{m->area,   m->grad_ptr, m->grad_cx,  m->grad_cy, m->lim_ptr,
+                  m->lim_dx, m->lim_dy,   m->acc_ptr,  m->face_len, m->acc_coef};
+    if (st->flux == 1)
+      launch_vjp_fused_s<1>(m, muscl, st->limiter, vjp_lds, s, ga, u, lam, st->gamma, sbar);
+    else
+      launch_vjp_fused_s<0>(m, muscl, st->limiter, vjp_lds, s, ga, u, lam, st->gamma, sbar);
+    hipLaunchKernelGGL(k_cell_adjoint2, dim3(blocks(n)), dim3(kBlock), 0, s, n, u, m->W, lam,
+                       m->adj, muscl ? 1 : 0, st->gamma, st->acceleration_x, st->acceleration_y,
+                       ubar);
+    HIP_CHECK(hipGetLastError());
+    return 0;
+  }
   hipLaunchKernelGGL(k_primitive, dim3(blocks(n)), dim3(kBlock), 0, s, n, u, m->W, st->gamma,
                      m->status);
   if (muscl)
-    hipLaunchKernelGGL(k_gradient, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->grad_ptr,
+    hipLaunchKernelGGL(k_gradient_t<true>, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->grad_ptr,
                        m->grad_nb, m->grad_cx, m->grad_cy, m->lim_ptr, m->lim_dx, m->lim_dy,
-                       st->limiter, m->G, m->PHI);
-  if (m->n_faces)
-    hipLaunchKernelGGL(k_flux_adjoint, dim3(blocks(m->n_faces)), dim3(kBlock), 0, s, m->n_faces,
-                       m->faces, m->W, m->G, m->PHI, m->face_len, m->area, lam, st->flux,
-                       muscl ? 1 : 0, st->gamma, m->FADJ, sbar);
+                       st->limiter, m->G, m->PHI, m->adj.arg);
+  if (m->n_faces) {
+    // Rusanov and HLLC in reverse mode; Roe and far-field faces with duals
+    bool reverse = st->flux != 2;
+    if (reverse)
+      hipLaunchKernelGGL(k_flux_adjoint_rev, dim3(blocks(m->n_faces)), dim3(kBlock), 0, s,
+                         m->n_faces, m->faces, m->W, m->G, m->PHI, m->face_len, m->area, lam,
+                         st->flux, muscl ? 1 : 0, st->gamma, m->FADJ, sbar);
+    if (!reverse || m->has_far_field)
+      hipLaunchKernelGGL(k_flux_adjoint, dim3(blocks(m->n_faces)), dim3(kBlock), 0, s, m->n_faces,
+                         m->faces, m->W, m->G, m->PHI, m->face_len, m->area, lam, st->flux,
+                         muscl ? 1 : 0, st->gamma, m->FADJ, sbar, reverse ? 1 : 0);
+  }
   hipLaunchKernelGGL(k_cell_adjoint1, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->G, m->PHI,
                      m->FADJ, m->adj, muscl ? 1 : 0, st->limiter);
   hipLaunchKernelGGL(k_cell_adjoint2, dim3(blocks(n)), dim3(kBlock), 0, s, n, u, m->W, lam,
@@ -3519,6 +4118,10 @@ int mcfd_mesh_create(const McfdMeshDesc* d, McfdMesh** out) {
     if (kind >= MCFD_FACE_PRESSURE_INLET) m->needs_inflow = true;
   }
   if (d->n_bc > 0) m->host_bc_kind.assign(d->bc_kind, d->bc_kind + d->n_bc);
+  for (int i = 0; i < d->n_cells && d->grad_ptr && d->lim_ptr; ++i) {
+    m->max_cell_entries = std::max(m->max_cell_entries, d->grad_ptr[i + 1] - d->grad_ptr[i]);
+    m->max_cell_entries = std::max(m->max_cell_entries, d->lim_ptr[i + 1] - d->lim_ptr[i]);
+  }
   if (d->n_faces > 0 && d->face_bc) m->host_face_bc.assign(d->face_bc, d->face_bc + d->n_faces);
   (void)hipGetDevice(&m->device);
   size_t C = d->n_cells, F = d->n_faces;
@@ -3958,15 +4561,16 @@ int mcfd_mesh_set_adjoint(McfdMesh* m, const int32_t* grad_rev) {
   (void)hipDeviceSynchronize();
   free_adjoint(m);
   if (!grad_rev) return 0;
+  if (m->max_cell_entries >= 255)
+    return fail("mcfd_mesh_set_adjoint: a cell has 255 or more LSQ or limiter entries");
   size_t C = m->n_cells;
   int rc = 0;
   AdjArrays& a = m->adj;
   rc |= alloc_adjoint(m, &m->grad_rev, grad_rev, (size_t)m->n_grad);
   rc |= alloc_adjoint(m, &m->FADJ, (const double*)nullptr, 8 * (size_t)m->n_faces);
   rc |= alloc_adjoint(m, &a.wbar, (const double*)nullptr, 4 * C);
-  rc |= alloc_adjoint(m, &a.gbar, (const double*)nullptr, 8 * C);
-  rc |= alloc_adjoint(m, &a.xbar, (const double*)nullptr, 8 * C);
-  rc |= alloc_adjoint(m, &a.xarg, (const int*)nullptr, 8 * C);
+  rc |= alloc_adjoint(m, &a.ebar, (const double*)nullptr, 4 * (size_t)m->n_grad);
+  rc |= alloc_adjoint(m, &a.arg, (const unsigned*)nullptr, 4 * C);
   if (rc) {
     std::string msg = g_last_error;
     free_adjoint(m);
