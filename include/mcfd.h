@@ -459,6 +459,20 @@ int mcfd_mesh_set_adjoint(McfdMesh* mesh, const int32_t* grad_rev);
 int mcfd_residual_vjp(McfdMesh* mesh, const McfdSettings* settings, const double* u,
                       const double* lambda, double* ubar, double* sbar, void* stream);
 
+/* Geometry adjoint lambda^T dR/d(geometry) at state u: per flux face
+ * face_bar[7k..7k+6] = (nx, ny, L, lx, ly, rx, ry)-bar (normal, length,
+ * reconstruction offsets of the left and right side), per LSQ entry
+ * grad_bar[2e..2e+1] = (cx, cy)-bar, per limiter entry lim_bar[2e..2e+1] =
+ * (dx, dy)-bar (device order). The cell-area adjoint is -lambda.(R - body
+ * force)/A and is left to the caller. Chained through the geometry's
+ * dependence on the mesh nodes this gives gradients with respect to shape
+ * parameters. Inviscid Rusanov or HLLC, first order or MUSCL linear, no
+ * far-field, pressure-inlet or mass-flow boundaries; needs
+ * mcfd_mesh_set_adjoint. No counterpart in the reference. */
+int mcfd_residual_geometry_vjp(McfdMesh* mesh, const McfdSettings* settings, const double* u,
+                               const double* lambda, double* face_bar, double* grad_bar,
+                               double* lim_bar, void* stream);
+
 /* Transposed LU-SGS: out = M^T v with M = U^-1 D L^-1 as assembled by
  * mcfd_lusgs_assemble (forward sweep on U^T, backward on L^T). The
  * preconditioner of the adjoint GMRES; no counterpart in the reference. */

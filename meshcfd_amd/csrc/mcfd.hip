@@ -1918,6 +1918,11 @@ __global__ void __launch_bounds__(kBlock)
   if (i < n) y[i] = a * x[i] + b * y[i];
 }
 
+__global__ void __launch_bounds__(kBlock) k_fill(long n, double v, double* __restrict__ y) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = v;
+}
+
 // out = x * s
 __global__ void __launch_bounds__(kBlock)
     k_scale(long n, const double* __restrict__ x, double s, double* __restrict__ out) {
@@ -2312,6 +2317,7 @@ struct SideR {
 };
 struct SideB {
   double r = 0.0, u = 0.0, v = 0.0, p = 0.0, un = 0.0, E = 0.0;
+  double nx = 0.0, ny = 0.0;  // adjoint of the face normal through this side's terms
 };
 
 __device__ __forceinline__ SideR side_r(const P4& w, double nx, double ny, double g) {
@@ -2328,6 +2334,8 @@ __device__ __forceinline__ void phys_flux_b(const SideR& s, const double* fb, do
   b.p += fb[1] * nx + fb[2] * ny + fb[3] * s.un;
   b.un += fb[0] * s.r + fb[1] * s.r * s.u + fb[2] * s.r * s.v + fb[3] * (s.E + s.p);
   b.E += fb[3] * s.un;
+  b.nx += fb[1] * s.p;
+  b.ny += fb[2] * s.p;
 }
 
 // U = (r, r u, r v, E): accumulate U^T ub.
@@ -2345,9 +2353,12 @@ __device__ __forceinline__ void sound_b(const SideR& s, double c, double cb, dou
   b.r -= h * (g * s.p / s.r) / s.r;
 }
 
-// un = u nx + v ny and E = p/(g-1) + r (u^2 + v^2)/2 back to (r, u, v, p).
+// un = u nx + v ny and E = p/(g-1) + r (u^2 + v^2)/2 back to (r, u, v, p);
+// the normal's adjoint from this side is added to (nbx, nby).
 __device__ __forceinline__ P4 side_finish_b(const SideR& s, const SideB& b, double nx, double ny,
-                                            double g) {
+                                            double g, double& nbx, double& nby) {
+  nbx += b.nx + b.un * s.u;
+  nby += b.ny + b.un * s.v;
   return P4{b.r + b.E * 0.5 * (s.u * s.u + s.v * s.v), b.u + b.E * s.r * s.u + b.un * nx,
             b.v + b.E * s.r * s.v + b.un * ny, b.p + b.E / (g - 1.0)};
 }
@@ -2384,6 +2395,8 @@ __device__ void hllc_star_b(const SideR& s, double S, double star, const double*
   double qb = rqb * s.r;
   sb.u += vxb;
   sb.v += vyb;
+  sb.nx += vxb * t;
+  sb.ny += vyb * t;
   tb += vxb * nx + vyb * ny;
   starb += tb;
   sb.un -= tb;
@@ -2398,7 +2411,7 @@ __device__ void hllc_star_b(const SideR& s, double S, double star, const double*
 
 // (dF/dL)^T fb -> lb, (dF/dR)^T fb -> rb for hllc() (same branches).
 __device__ void hllc_b(const P4& L, const P4& R, double nx, double ny, double g, bool fused_order,
-                       const double* fb, P4& lb, P4& rb) {
+                       const double* fb, P4& lb, P4& rb, double& nbx, double& nby) {
   SideR a = side_r(L, nx, ny, g), b = side_r(R, nx, ny, g);
   SideB A, B;
   double cl = sqrt(g * L.p / L.r), cr = sqrt(g * R.p / R.r);
@@ -2466,13 +2479,13 @@ __device__ void hllc_b(const P4& L, const P4& R, double nx, double ny, double g,
   }
   if (clb != 0.0) sound_b(a, cl, clb, g, A);
   if (crb != 0.0) sound_b(b, cr, crb, g, B);
-  lb = side_finish_b(a, A, nx, ny, g);
-  rb = side_finish_b(b, B, nx, ny, g);
+  lb = side_finish_b(a, A, nx, ny, g, nbx, nby);
+  rb = side_finish_b(b, B, nx, ny, g, nbx, nby);
 }
 
 // (dF/dL)^T fb, (dF/dR)^T fb for rusanov().
 __device__ void rusanov_b(const P4& L, const P4& R, double nx, double ny, double g, const double* fb,
-                          P4& lb, P4& rb) {
+                          P4& lb, P4& rb, double& nbx, double& nby) {
   SideR a = side_r(L, nx, ny, g), b = side_r(R, nx, ny, g);
   SideB A, B;
   double cl = sqrt(g * L.p / L.r), cr = sqrt(g * R.p / R.r);
@@ -2495,8 +2508,8 @@ __device__ void rusanov_b(const P4& L, const P4& R, double nx, double ny, double
     B.un += (b.un < 0.0) ? -spb : spb;
     sound_b(b, cr, spb, g, B);
   }
-  lb = side_finish_b(a, A, nx, ny, g);
-  rb = side_finish_b(b, B, nx, ny, g);
+  lb = side_finish_b(a, A, nx, ny, g, nbx, nby);
+  rb = side_finish_b(b, B, nx, ny, g, nbx, nby);
 }
 
 // Face adjoint in reverse mode (Rusanov, HLLC; far-field faces are left to
@@ -2547,10 +2560,11 @@ __global__ void __launch_bounds__(kBlock)
     }
   }
   P4 lb, rb;
+  double nbx = 0.0, nby = 0.0;  // unused here
   if (scheme == 1)
-    hllc_b(wl, wr, nx, ny, g, kind == MCFD_FACE_INTERIOR, fb, lb, rb);
+    hllc_b(wl, wr, nx, ny, g, kind == MCFD_FACE_INTERIOR, fb, lb, rb, nbx, nby);
   else
-    rusanov_b(wl, wr, nx, ny, g, fb, lb, rb);
+    rusanov_b(wl, wr, nx, ny, g, fb, lb, rb, nbx, nby);
   P4 ext{0.0, 0.0, 0.0, 0.0};
   if (kind > MCFD_FACE_PERIODIC) {
     if (kind == MCFD_FACE_WALL) {
@@ -2580,6 +2594,119 @@ __global__ void __launch_bounds__(kBlock)
   q[2] = make_double2(rb.r, rb.u);
   q[3] = make_double2(rb.v, rb.p);
   if (SBAR && kind > MCFD_FACE_PERIODIC) store4(SBAR, b, ext.r, ext.u, ext.v, ext.p);
+}
+
+// Geometry adjoint of the face terms, lambda^T dR/d(face geometry): per
+// flux face (nx, ny, L, lx, ly, rx, ry)-bar into out[7k..7k+6]. L enters
+// the residual as -L/A_l, +L/A_r; the normal through the flux and the wall's
+// mirror state; the offsets through the reconstruction W + phi (g . d).
+// Rusanov and HLLC; far-field faces are not covered (the entry point
+// rejects them).
+__global__ void __launch_bounds__(kBlock)
+    k_geom_face(int nf, FaceArrays fa, const double* __restrict__ W, const double* __restrict__ G,
+                const double* __restrict__ PHI, const double* __restrict__ flen,
+                const double* __restrict__ area, const double* __restrict__ lam, int scheme, int muscl,
+                double g, double* __restrict__ out) {
+  int k = xcd_block() * blockDim.x + threadIdx.x;
+  if (k >= nf) return;
+  int kind = fa.kind[k];
+  double nx = fa.nx[k], ny = fa.ny[k];
+  int l = fa.left[k];
+  int r = fa.right[k];
+  double L = flen[k];
+  P4 ll = load4(lam, l);
+  double Al = area[l];
+  double cl = -L / Al;
+  double fb[4] = {cl * ll.r, cl * ll.u, cl * ll.v, cl * ll.p};
+  double lc[4] = {-ll.r / Al, -ll.u / Al, -ll.v / Al, -ll.p / Al};  // dR/dL per flux component
+  const bool two = kind <= MCFD_FACE_PERIODIC;
+  if (two) {
+    P4 lr = load4(lam, r);
+    double Ar = area[r];
+    double cr = L / Ar;
+    fb[0] += cr * lr.r;
+    fb[1] += cr * lr.u;
+    fb[2] += cr * lr.v;
+    fb[3] += cr * lr.p;
+    lc[0] += lr.r / Ar;
+    lc[1] += lr.u / Ar;
+    lc[2] += lr.v / Ar;
+    lc[3] += lr.p / Ar;
+  }
+  P4 wl = reconstruct(W, G, PHI, l, fa.lx[k], fa.ly[k], muscl != 0);
+  P4 wr = wl;
+  int b = fa.bc[k];
+  double wrx = 0.0, wry = 0.0, wun = 0.0;
+  if (two) {
+    wr = reconstruct(W, G, PHI, r, fa.rx[k], fa.ry[k], muscl != 0);
+  } else if (kind == MCFD_FACE_WALL) {
+    wrx = wl.u - fa.bc_wall[2 * (long)b];
+    wry = wl.v - fa.bc_wall[2 * (long)b + 1];
+    wun = wrx * nx + wry * ny;
+    wr.u = wl.u - 2.0 * wun * nx;
+    wr.v = wl.v - 2.0 * wun * ny;
+  } else if (kind == MCFD_FACE_VELOCITY_INLET) {
+    const double* st = fa.bc_state + 4 * (long)b;
+    wr = P4{st[0], st[1], st[2], st[3]};
+  } else if (kind == MCFD_FACE_PRESSURE_OUTLET) {
+    wr.p = fa.bc_state[4 * (long)b + 3];
+  }
+  F4 F = (scheme == 1) ? hllc(wl, wr, nx, ny, g, kind == MCFD_FACE_INTERIOR) : rusanov(wl, wr, nx, ny, g);
+  double lbar = F.a * lc[0] + F.b * lc[1] + F.c * lc[2] + F.d * lc[3];
+  P4 lb, rb;
+  double nbx = 0.0, nby = 0.0;
+  if (scheme == 1)
+    hllc_b(wl, wr, nx, ny, g, kind == MCFD_FACE_INTERIOR, fb, lb, rb, nbx, nby);
+  else
+    rusanov_b(wl, wr, nx, ny, g, fb, lb, rb, nbx, nby);
+  if (!two) {
+    if (kind == MCFD_FACE_WALL) {
+      // u_R = u_L - 2 un nx, v_R = v_L - 2 un ny, un = (u_L - wu) nx + (v_L - wv) ny
+      nbx += rb.u * (-2.0 * (wrx * nx + wun)) + rb.v * (-2.0 * wrx * ny);
+      nby += rb.u * (-2.0 * wry * nx) + rb.v * (-2.0 * (wry * ny + wun));
+      double un_b = rb.u * nx + rb.v * ny;
+      lb.r += rb.r;
+      lb.u += rb.u - 2.0 * nx * un_b;
+      lb.v += rb.v - 2.0 * ny * un_b;
+      lb.p += rb.p;
+    } else if (kind == MCFD_FACE_PRESSURE_OUTLET) {
+      lb.r += rb.r;
+      lb.u += rb.u;
+      lb.v += rb.v;
+    } else if (kind != MCFD_FACE_VELOCITY_INLET) {
+      lb.r += rb.r;
+      lb.u += rb.u;
+      lb.v += rb.v;
+      lb.p += rb.p;
+    }
+  }
+  double dlx = 0.0, dly = 0.0, drx = 0.0, dry = 0.0;
+  if (muscl) {
+    const double* gl = G + 8 * (long)l;
+    P4 pl = load4(PHI, l);
+    double a[4] = {lb.r * pl.r, lb.u * pl.u, lb.v * pl.v, lb.p * pl.p};
+    for (int c = 0; c < 4; ++c) {
+      dlx += a[c] * gl[2 * c];
+      dly += a[c] * gl[2 * c + 1];
+    }
+    if (two) {
+      const double* gr = G + 8 * (long)r;
+      P4 pr = load4(PHI, r);
+      double q[4] = {rb.r * pr.r, rb.u * pr.u, rb.v * pr.v, rb.p * pr.p};
+      for (int c = 0; c < 4; ++c) {
+        drx += q[c] * gr[2 * c];
+        dry += q[c] * gr[2 * c + 1];
+      }
+    }
+  }
+  double* o = out + 7 * (long)k;
+  o[0] = nbx;
+  o[1] = nby;
+  o[2] = lbar;
+  o[3] = dlx;
+  o[4] = dly;
+  o[5] = drx;
+  o[6] = dry;
 }
 
 // Face adjoint: Fbar = sum of the face's residual coefficients times lambda,
@@ -2685,6 +2812,8 @@ struct AdjArrays {
   double* wbar;   // (C,4)
   double* ebar;   // (n_grad,4) per LSQ entry
   unsigned* arg;  // (C,4) limiting / extremum entries of the forward limiter (k_gradient_t<true>)
+  double* cbar;   // (n_grad,2) LSQ coefficient adjoints, or NULL
+  double* dlbar;  // (n_lim,2) limiter offset adjoints, or NULL
 };
 
 __device__ __forceinline__ double limiter_value(double ratio, int code, double& dfdr) {
@@ -2777,6 +2906,11 @@ __global__ void __launch_bounds__(kBlock)
     // (0xFF: the cell itself); at most one per variable is non-zero
     double xb[4] = {0.0, 0.0, 0.0, 0.0};
     int xe[4] = {-1, -1, -1, -1};
+    if (aa.dlbar)
+      for (int e = aa.lim_ptr[i]; e < aa.lim_ptr[i + 1]; ++e) {
+        aa.dlbar[2 * (long)e] = 0.0;
+        aa.dlbar[2 * (long)e + 1] = 0.0;
+      }
     // limiter backward at the entry that set phi (the forward's first strict minimum)
     for (int k = 0; k < 4; ++k) {
       unsigned le = arg[k] & 0xFFu;
@@ -2806,6 +2940,10 @@ __global__ void __launch_bounds__(kBlock)
       double incb = -rb * ratio / inc;
       gb[2 * k] += incb * dx;
       gb[2 * k + 1] += incb * dy;
+      if (aa.dlbar) {
+        aa.dlbar[2 * (long)e] += incb * gv[2 * k];
+        aa.dlbar[2 * (long)e + 1] += incb * gv[2 * k + 1];
+      }
       if (xn < 0)
         wb[k] += ext_b;
       else {
@@ -2824,6 +2962,17 @@ __global__ void __launch_bounds__(kBlock)
         c[k] = t;
       }
       store4(aa.ebar, e, c[0], c[1], c[2], c[3]);
+      if (aa.cbar) {  // grad_i = sum_e c_e (W_j - W_i)
+        P4 o = load4(W, aa.grad_nb[e]);
+        double d[4] = {o.r - wv[0], o.u - wv[1], o.v - wv[2], o.p - wv[3]};
+        double bx = 0.0, by = 0.0;
+        for (int k = 0; k < 4; ++k) {
+          bx += gb[2 * k] * d[k];
+          by += gb[2 * k + 1] * d[k];
+        }
+        aa.cbar[2 * (long)e] = bx;
+        aa.cbar[2 * (long)e + 1] = by;
+      }
     }
   }
   store4(aa.wbar, i, wb[0], wb[1], wb[2], wb[3]);
@@ -3007,10 +3156,11 @@ __global__ void __launch_bounds__(kBlock)
     } else if (kind == MCFD_FACE_PRESSURE_OUTLET) {
       wr.p = fa.bc_state[4 * (long)bi + 3];
     }
+    double nbx = 0.0, nby = 0.0;  // unused here
     if constexpr (SCHEME == 1)
-      hllc_b(wl, wr, nx, ny, g, kind == MCFD_FACE_INTERIOR, fb, alb, arb);
+      hllc_b(wl, wr, nx, ny, g, kind == MCFD_FACE_INTERIOR, fb, alb, arb, nbx, nby);
     else
-      rusanov_b(wl, wr, nx, ny, g, fb, alb, arb);
+      rusanov_b(wl, wr, nx, ny, g, fb, alb, arb, nbx, nby);
     if (kind > MCFD_FACE_PERIODIC) {
       P4 ext{0.0, 0.0, 0.0, 0.0};
       if (kind == MCFD_FACE_WALL) {
@@ -4757,6 +4907,46 @@ int mcfd_mesh_set_inflow(McfdMesh* m, const McfdInflowDesc* d) {
   m->dts.inflow = ia.ext;
   m->dts.inflow_first = first;
   m->inflow = true;
+  return 0;
+}
+
+int mcfd_residual_geometry_vjp(McfdMesh* m, const McfdSettings* st, const double* u, const double* lam,
+                               double* face_bar, double* grad_bar, double* lim_bar, void* stream) {
+  if (!m || !u || !lam || !face_bar || !grad_bar || !lim_bar)
+    return fail("mcfd_residual_geometry_vjp: NULL argument");
+  if (!m->adjoint) return fail("mcfd_residual_geometry_vjp: call mcfd_mesh_set_adjoint first");
+  if (check_settings(st)) return -1;
+  if (st->flux == 2 || st->reconstruction == 2 || st->viscosity || m->has_far_field || m->needs_inflow)
+    return fail("Geometry gradients cover the inviscid Rusanov and HLLC residual, first order or MUSCL "
+                "linear, without far-field, pressure-inlet or mass-flow boundaries.");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int n = m->n_cells;
+  bool muscl = st->reconstruction == 1 && m->n_grad > 0;
+  hipLaunchKernelGGL(k_primitive, dim3(blocks(n)), dim3(kBlock), 0, s, n, u, m->W, st->gamma, m->status);
+  if (muscl)
+    hipLaunchKernelGGL(k_gradient_t<true>, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->grad_ptr,
+                       m->grad_nb, m->grad_cx, m->grad_cy, m->lim_ptr, m->lim_dx, m->lim_dy,
+                       st->limiter, m->G, m->PHI, m->adj.arg);
+  if (m->n_faces) {
+    hipLaunchKernelGGL(k_flux_adjoint_rev, dim3(blocks(m->n_faces)), dim3(kBlock), 0, s, m->n_faces,
+                       m->faces, m->W, m->G, m->PHI, m->face_len, m->area, lam, st->flux,
+                       muscl ? 1 : 0, st->gamma, m->FADJ, nullptr);
+    hipLaunchKernelGGL(k_geom_face, dim3(blocks(m->n_faces)), dim3(kBlock), 0, s, m->n_faces, m->faces,
+                       m->W, m->G, m->PHI, m->face_len, m->area, lam, st->flux, muscl ? 1 : 0,
+                       st->gamma, face_bar);
+  }
+  if (muscl) {
+    AdjArrays aa = m->adj;
+    aa.cbar = grad_bar;
+    aa.dlbar = lim_bar;
+    hipLaunchKernelGGL(k_cell_adjoint1, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->G, m->PHI,
+                       m->FADJ, aa, 1, st->limiter);
+  } else {
+    long ng = 2L * m->n_grad, nl = 2L * m->n_lim;
+    if (ng) hipLaunchKernelGGL(k_fill, dim3(lblocks(ng)), dim3(kBlock), 0, s, ng, 0.0, grad_bar);
+    if (nl) hipLaunchKernelGGL(k_fill, dim3(lblocks(nl)), dim3(kBlock), 0, s, nl, 0.0, lim_bar);
+  }
+  HIP_CHECK(hipGetLastError());
   return 0;
 }
 

@@ -99,6 +99,10 @@ class DeviceLayout:
     mesh_face_flux: np.ndarray = None  # mesh face -> flux face (periodic b faces: their pair's)
     dts_flux: np.ndarray = None  # flux face of each spectral-radius item
     dts_own: np.ndarray = None  # item uses its flux face's normal and length (not a periodic b side, no exterior state)
+    flux_mesh_face: np.ndarray = None  # flux face -> mesh face of its normal, length and left offset
+    flux_right_face: np.ndarray = None  # flux face -> mesh face of its right offset (-1: boundary)
+    grad_src_order: np.ndarray = None  # device LSQ entry -> geometry LSQ entry
+    lim_src_order: np.ndarray = None  # device limiter entry -> geometry limiter entry
 
     @property
     def n_faces(self) -> int:
@@ -201,6 +205,7 @@ def build_layout(g: Geometry, reorder: bool = True, order: np.ndarray | None = N
 
     mesh_face_flux = np.full(len(g.owners), -1, dtype=np.int64)
     old_mesh = np.concatenate([inter, fa] + bc_faces).astype(np.int64)
+    old_right = np.concatenate([inter, fb, np.full(n_bc, -1, np.int64)]).astype(np.int64)
     mesh_face_flux[old_mesh] = fnew[np.arange(nf)]
     mesh_face_flux[fb] = fnew[n_int + np.arange(n_per)]
 
@@ -235,6 +240,7 @@ def build_layout(g: Geometry, reorder: bool = True, order: np.ndarray | None = N
 
     # --- LSQ and limiter entries -------------------------------------------
     grad_ptr, order = _csr(new_id[g.gradient_sources], n)
+    grad_src_order = order
     grad_nb = new_id[g.gradient_neighbours[order]].astype(np.int32)
     # LSQ entries come in (a -> b, b -> a) pairs: 2k and 2k+1 in the reference list
     pos = np.empty(len(order), dtype=np.int64)
@@ -243,6 +249,7 @@ def build_layout(g: Geometry, reorder: bool = True, order: np.ndarray | None = N
     grad_cx = g.gradient_coefficients_x[order]
     grad_cy = g.gradient_coefficients_y[order]
     lim_ptr, order = _csr(new_id[g.limiter_cells], n)
+    lim_src_order = order
     lim_dx = g.limiter_dx[order]
     lim_dy = g.limiter_dy[order]
 
@@ -308,4 +315,8 @@ def build_layout(g: Geometry, reorder: bool = True, order: np.ndarray | None = N
         mesh_face_flux=mesh_face_flux,
         dts_flux=mesh_face_flux[d_faces].astype(np.int32),
         dts_own=(~np.isin(d_faces, fb)) & (d_bc_sorted < 0),
+        flux_mesh_face=old_mesh[fperm],
+        flux_right_face=old_right[fperm],
+        grad_src_order=grad_src_order,
+        lim_src_order=lim_src_order,
     )

@@ -471,3 +471,65 @@ def cylinder_case(
     case = Case(mesh=mesh, boundary_conditions=[wall, far], numerics=numerics, name="cylinder")
     prim = np.tile(np.asarray(fs, dtype=float), (mesh.cell_count, 1))
     return case, prim
+
+
+def bump_surface(x, h: float, x0: float = 1.0, chord: float = 1.0, xp=np):
+    """Height of a circular-arc bump of height ``h`` over [x0, x0 + chord]
+    (zero elsewhere): the arc through (x0, 0), (x0 + chord/2, h),
+    (x0 + chord, 0). ``xp`` is numpy or torch (then ``x`` and ``h`` may be
+    tensors and the result is differentiable in ``h``)."""
+    radius = (0.25 * chord * chord + h * h) / (2.0 * h)
+    xc = x0 + 0.5 * chord
+    yc = h - radius
+    dx = x - xc
+    inside = xp.abs(dx) < 0.5 * chord
+    arg = radius * radius - xp.where(inside, dx, 0.0 * dx) ** 2
+    return xp.where(inside, yc + xp.sqrt(arg), 0.0 * dx)
+
+
+def bump_nodes(base, h, height: float = 1.0, x0: float = 1.0, chord: float = 1.0, xp=np):
+    """Channel nodes over a bump: base nodes (x, y) of the rectangle
+    [0, L] x [0, height] moved to y + y_b(x) (1 - y/height), so the bottom
+    follows the bump and the top stays straight."""
+    x, y = base[:, 0], base[:, 1]
+    moved = y + bump_surface(x, h, x0, chord, xp) * (1.0 - y / height)
+    return np.column_stack((x, moved)) if xp is np else xp.stack((x, moved), 1)
+
+
+def bump_channel(nx: int, ny: int, h: float, length: float = 3.0, height: float = 1.0, x0: float = 1.0,
+                 chord: float = 1.0) -> FiniteVolumeMesh:
+    """Quad grid of the channel [0, length] x [0, height] with a circular-arc
+    bump of height ``h`` on the bottom (patches: 0 bottom wall, 1 right,
+    2 top, 3 left). The connectivity does not depend on ``h``; the nodes are
+    `bump_nodes` of the rectangle's."""
+    if not (0.0 < h < 0.5 * height):
+        raise ValueError("bump height must be in (0, height/2)")
+    base = _grid_mesh(nx, ny, 0.0, length, 0.0, height, 0.0, None)
+    base.points = bump_nodes(base.points, h, height, x0, chord)
+    return base
+
+
+def bump_case(nx: int, ny: int, h: float, mach: float = 0.675, integrator: str = "Implicit Euler",
+              cfl: float = 50.0):
+    """Inviscid subsonic channel flow over the bump: velocity inlet at Mach
+    ``mach`` (rho = 1, T = 1, R = 1, so p = 1), pressure outlet p = 1, slip
+    walls; HLLC + MUSCL linear (Barth-Jespersen), local pseudo-time steps.
+    Returns (case, uniform initial primitive state)."""
+    gamma = 1.4
+    speed = mach * math.sqrt(gamma)
+    numerics = NumericalSettings(
+        flux="HLLC", reconstruction="MUSCL linear", time_integration=integrator,
+        time_step_mode="Local (steady)", cfl=cfl, gas_constant=1.0, end_time=1e30,
+        gmres_krylov_dimension=30, gmres_tolerance=1e-3, residual_tolerance=1e-9,
+    )
+    mesh = bump_channel(nx, ny, h)
+    bcs = [
+        BoundaryCondition(0, "bump", "Wall", {}, "Temperature", "Slip"),
+        BoundaryCondition(1, "outlet", "Pressure Outlet", {"pressure": 1.0}),
+        BoundaryCondition(2, "top", "Wall", {}, "Temperature", "Slip"),
+        BoundaryCondition(3, "inlet", "Velocity Inlet",
+                          {"density": 1.0, "temperature": 1.0, "velocity_x": speed, "velocity_y": 0.0}),
+    ]
+    case = Case(mesh=mesh, boundary_conditions=bcs, numerics=numerics, name=f"bump h={h}")
+    prim = np.tile((1.0, speed, 0.0, 1.0), (mesh.cell_count, 1))
+    return case, prim
