@@ -26,6 +26,7 @@ def test_header_declares_entry_points():
 
 def test_library_exports_every_declared_symbol():
     import meshcfd_amd.adjoint  # noqa: F401  (registers its entry points)
+    import meshcfd_amd.geometry_adjoint  # noqa: F401
     import meshcfd_amd.implicit  # noqa: F401
     import meshcfd_amd.inflow  # noqa: F401
     import meshcfd_amd.viscous  # noqa: F401
