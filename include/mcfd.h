@@ -467,8 +467,8 @@ int mcfd_residual_vjp(McfdMesh* mesh, const McfdSettings* settings, const double
  * force)/A and is left to the caller. Chained through the geometry's
  * dependence on the mesh nodes this gives gradients with respect to shape
  * parameters. Inviscid Rusanov or HLLC, first order or MUSCL linear, no
- * far-field, pressure-inlet or mass-flow boundaries; needs
- * mcfd_mesh_set_adjoint. No counterpart in the reference. */
+ * pressure-inlet or mass-flow boundaries; needs mcfd_mesh_set_adjoint. No
+ * counterpart in the reference. */
 int mcfd_residual_geometry_vjp(McfdMesh* mesh, const McfdSettings* settings, const double* u,
                                const double* lambda, double* face_bar, double* grad_bar,
                                double* lim_bar, void* stream);

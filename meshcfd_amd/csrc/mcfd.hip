@@ -2310,7 +2310,7 @@ __device__ __forceinline__ P4D seed(const P4& w, int base) {
 // Reverse mode of the Rusanov and HLLC fluxes: one primal pass that keeps the
 // intermediates in registers, one backward pass. (dF/dW_L)^T Fbar and
 // (dF/dW_R)^T Fbar cost about three primal fluxes instead of the nine of
-// eight-direction duals. Roe and far-field faces keep the dual path.
+// eight-direction duals. Roe keeps the dual path.
 
 struct SideR {
   double r, u, v, p, un, E;
@@ -2512,10 +2512,86 @@ __device__ void rusanov_b(const P4& L, const P4& R, double nx, double ny, double
   rb = side_finish_b(b, B, nx, ny, g, nbx, nby);
 }
 
-// Face adjoint in reverse mode (Rusanov, HLLC; far-field faces are left to
-// k_flux_adjoint). Same outputs as k_flux_adjoint: FADJ[k][0..3] the
-// adjoint of the reconstructed left state (through the ghost state on
-// boundary faces), [4..7] that of the right / exterior state.
+// Reverse of far_field(): e-bar -> interior-state, free-stream-state and
+// normal adjoints (accumulated). Same branches as the forward.
+__device__ void far_field_b(const P4& in, const P4& out, double nx, double ny, double g, const P4& eb,
+                            P4& inb, P4& outb, double& nbx, double& nby) {
+  double un_i = in.u * nx + in.v * ny;
+  double un_f = out.u * nx + out.v * ny;
+  double c_i = sound(in, g), c_f = sound(out, g);
+  if (un_i <= -c_i) {
+    outb.r += eb.r;
+    outb.u += eb.u;
+    outb.v += eb.v;
+    outb.p += eb.p;
+    return;
+  }
+  if (un_i >= c_i) {
+    inb.r += eb.r;
+    inb.u += eb.u;
+    inb.v += eb.v;
+    inb.p += eb.p;
+    return;
+  }
+  double r_out = un_i + 2.0 * c_i / (g - 1.0);
+  double r_in = un_f - 2.0 * c_f / (g - 1.0);
+  double vb = 0.5 * (r_out + r_in);
+  double cb = 0.25 * (g - 1.0) * (r_out - r_in);
+  bool inflow = vb < 0.0;
+  const P4& src = inflow ? out : in;
+  double sr_g = pow(src.r, g);
+  double entropy = src.p / sr_g;
+  double X = cb * cb / (g * entropy);
+  double rho = pow(X, 1.0 / (g - 1.0));
+  double rho_g = pow(rho, g);
+  double tx = -ny, ty = nx;
+  double vt = src.u * tx + src.v * ty;
+  // e = (rho, vb nx + vt tx, vb ny + vt ty, entropy rho^g)
+  double rhob = eb.r + eb.p * entropy * g * rho_g / rho;
+  double entb = eb.p * rho_g;
+  double vbb = eb.u * nx + eb.v * ny;
+  double vtb = eb.u * tx + eb.v * ty;
+  nbx += eb.u * vb + eb.v * vt;
+  nby += -eb.u * vt + eb.v * vb;
+  double Xb = rhob * rho / ((g - 1.0) * X);
+  double cbb = Xb * 2.0 * cb / (g * entropy);
+  entb -= Xb * X / entropy;
+  P4 sb{0.0, 0.0, 0.0, 0.0};
+  sb.u += vtb * tx;
+  sb.v += vtb * ty;
+  nbx += vtb * src.v;
+  nby -= vtb * src.u;
+  sb.p += entb / sr_g;
+  sb.r -= entb * g * entropy / src.r;
+  double routb = 0.5 * vbb + 0.25 * (g - 1.0) * cbb;
+  double rinb = 0.5 * vbb - 0.25 * (g - 1.0) * cbb;
+  double cib = 2.0 * routb / (g - 1.0);
+  double cfb = -2.0 * rinb / (g - 1.0);
+  inb.u += routb * nx;
+  inb.v += routb * ny;
+  nbx += routb * in.u;
+  nby += routb * in.v;
+  outb.u += rinb * nx;
+  outb.v += rinb * ny;
+  nbx += rinb * out.u;
+  nby += rinb * out.v;
+  double hi = cib * 0.5 / c_i;
+  inb.p += hi * g / in.r;
+  inb.r -= hi * (g * in.p / in.r) / in.r;
+  double hf = cfb * 0.5 / c_f;
+  outb.p += hf * g / out.r;
+  outb.r -= hf * (g * out.p / out.r) / out.r;
+  P4& tgt = inflow ? outb : inb;
+  tgt.r += sb.r;
+  tgt.u += sb.u;
+  tgt.v += sb.v;
+  tgt.p += sb.p;
+}
+
+// Face adjoint in reverse mode (Rusanov, HLLC). Same outputs as
+// k_flux_adjoint: FADJ[k][0..3] the adjoint of the reconstructed left state
+// (through the ghost state on boundary faces), [4..7] that of the right /
+// exterior state.
 __global__ void __launch_bounds__(kBlock)
     k_flux_adjoint_rev(int nf, FaceArrays fa, const double* __restrict__ W,
                        const double* __restrict__ G, const double* __restrict__ PHI,
@@ -2525,7 +2601,6 @@ __global__ void __launch_bounds__(kBlock)
   int k = xcd_block() * blockDim.x + threadIdx.x;
   if (k >= nf) return;
   int kind = fa.kind[k];
-  if (kind == MCFD_FACE_FAR_FIELD) return;
   double nx = fa.nx[k], ny = fa.ny[k];
   int l = fa.left[k];
   int r = fa.right[k];
@@ -2557,6 +2632,9 @@ __global__ void __launch_bounds__(kBlock)
       wr = P4{st[0], st[1], st[2], st[3]};
     } else if (kind == MCFD_FACE_PRESSURE_OUTLET) {
       wr.p = st[3];
+    } else if (kind == MCFD_FACE_FAR_FIELD) {
+      int err = 0;
+      wr = far_field(wl, P4{st[0], st[1], st[2], st[3]}, nx, ny, g, &err);
     }
   }
   P4 lb, rb;
@@ -2580,6 +2658,9 @@ __global__ void __launch_bounds__(kBlock)
       lb.r += rb.r;
       lb.u += rb.u;
       lb.v += rb.v;
+    } else if (kind == MCFD_FACE_FAR_FIELD) {
+      const double* st = fa.bc_state + 4 * (long)b;
+      far_field_b(wl, P4{st[0], st[1], st[2], st[3]}, nx, ny, g, rb, lb, ext, nbx, nby);
     } else {  // transmissive: exterior = interior
       lb.r += rb.r;
       lb.u += rb.u;
@@ -2600,8 +2681,7 @@ __global__ void __launch_bounds__(kBlock)
 // flux face (nx, ny, L, lx, ly, rx, ry)-bar into out[7k..7k+6]. L enters
 // the residual as -L/A_l, +L/A_r; the normal through the flux and the wall's
 // mirror state; the offsets through the reconstruction W + phi (g . d).
-// Rusanov and HLLC; far-field faces are not covered (the entry point
-// rejects them).
+// Rusanov and HLLC.
 __global__ void __launch_bounds__(kBlock)
     k_geom_face(int nf, FaceArrays fa, const double* __restrict__ W, const double* __restrict__ G,
                 const double* __restrict__ PHI, const double* __restrict__ flen,
@@ -2650,6 +2730,10 @@ __global__ void __launch_bounds__(kBlock)
     wr = P4{st[0], st[1], st[2], st[3]};
   } else if (kind == MCFD_FACE_PRESSURE_OUTLET) {
     wr.p = fa.bc_state[4 * (long)b + 3];
+  } else if (kind == MCFD_FACE_FAR_FIELD) {
+    const double* st = fa.bc_state + 4 * (long)b;
+    int err = 0;
+    wr = far_field(wl, P4{st[0], st[1], st[2], st[3]}, nx, ny, g, &err);
   }
   F4 F = (scheme == 1) ? hllc(wl, wr, nx, ny, g, kind == MCFD_FACE_INTERIOR) : rusanov(wl, wr, nx, ny, g);
   double lbar = F.a * lc[0] + F.b * lc[1] + F.c * lc[2] + F.d * lc[3];
@@ -2673,6 +2757,10 @@ __global__ void __launch_bounds__(kBlock)
       lb.r += rb.r;
       lb.u += rb.u;
       lb.v += rb.v;
+    } else if (kind == MCFD_FACE_FAR_FIELD) {
+      const double* st = fa.bc_state + 4 * (long)b;
+      P4 outb{0.0, 0.0, 0.0, 0.0};
+      far_field_b(wl, P4{st[0], st[1], st[2], st[3]}, nx, ny, g, rb, lb, outb, nbx, nby);
     } else if (kind != MCFD_FACE_VELOCITY_INLET) {
       lb.r += rb.r;
       lb.u += rb.u;
@@ -3024,7 +3112,7 @@ __global__ void __launch_bounds__(kBlock)
 // LDS: W of own + ring cells, lambda/A and the gradient, limiter factors
 // and limiter arguments of own + ring-1 cells, the face adjoints of every
 // face touching an own cell. k_cell_adjoint2 then adds the neighbours'
-// contributions. Rusanov and HLLC (reverse mode), no far field.
+// contributions. Rusanov and HLLC (reverse mode).
 //   P0  W of local cells, lambda/A of own + ring-1 cells       -> LDS
 //   P1  gradient + limiter (+ limiting / extremum entries)     -> LDS (+ registers)
 //   P2  face adjoints, one face per thread                     -> LDS
@@ -3155,6 +3243,10 @@ __global__ void __launch_bounds__(kBlock)
       wr = P4{st[0], st[1], st[2], st[3]};
     } else if (kind == MCFD_FACE_PRESSURE_OUTLET) {
       wr.p = fa.bc_state[4 * (long)bi + 3];
+    } else if (kind == MCFD_FACE_FAR_FIELD) {
+      const double* st = fa.bc_state + 4 * (long)bi;
+      int err = 0;
+      wr = far_field(wl, P4{st[0], st[1], st[2], st[3]}, nx, ny, g, &err);
     }
     double nbx = 0.0, nby = 0.0;  // unused here
     if constexpr (SCHEME == 1)
@@ -3175,6 +3267,9 @@ __global__ void __launch_bounds__(kBlock)
         alb.r += arb.r;
         alb.u += arb.u;
         alb.v += arb.v;
+      } else if (kind == MCFD_FACE_FAR_FIELD) {
+        const double* st = fa.bc_state + 4 * (long)bi;
+        far_field_b(wl, P4{st[0], st[1], st[2], st[3]}, nx, ny, g, arb, alb, ext, nbx, nby);
       } else {
         alb.r += arb.r;
         alb.u += arb.u;
@@ -4194,7 +4289,7 @@ int mcfd_residual_vjp_impl(McfdMesh* m, const McfdSettings* st, const double* u,
   int n = m->n_cells;
   bool muscl = st->reconstruction == 1 && m->n_grad > 0;
   size_t vjp_lds = m->fused ? vjp_fused_lds_bytes(m) : 0;
-  if (m->fused && st->flux != 2 && !m->has_far_field && vjp_lds <= 64 * 1024) {
+  if (m->fused && st->flux != 2 && vjp_lds <= 64 * 1024) {
     // one workgroup per block (reverse-mode fluxes), then the neighbour pass
     GeomArrays ga{m->area,   m->grad_ptr, m->grad_cx,  m->grad_cy, m->lim_ptr,
                   m->lim_dx, m->lim_dy,   m->acc_ptr,  m->face_len, m->acc_coef};
@@ -4215,16 +4310,16 @@ int mcfd_residual_vjp_impl(McfdMesh* m, const McfdSettings* st, const double* u,
                        m->grad_nb, m->grad_cx, m->grad_cy, m->lim_ptr, m->lim_dx, m->lim_dy,
                        st->limiter, m->G, m->PHI, m->adj.arg);
   if (m->n_faces) {
-    // Rusanov and HLLC in reverse mode; Roe and far-field faces with duals
+    // Rusanov and HLLC in reverse mode; Roe with duals
     bool reverse = st->flux != 2;
     if (reverse)
       hipLaunchKernelGGL(k_flux_adjoint_rev, dim3(blocks(m->n_faces)), dim3(kBlock), 0, s,
                          m->n_faces, m->faces, m->W, m->G, m->PHI, m->face_len, m->area, lam,
                          st->flux, muscl ? 1 : 0, st->gamma, m->FADJ, sbar);
-    if (!reverse || m->has_far_field)
+    else
       hipLaunchKernelGGL(k_flux_adjoint, dim3(blocks(m->n_faces)), dim3(kBlock), 0, s, m->n_faces,
                          m->faces, m->W, m->G, m->PHI, m->face_len, m->area, lam, st->flux,
-                         muscl ? 1 : 0, st->gamma, m->FADJ, sbar, reverse ? 1 : 0);
+                         muscl ? 1 : 0, st->gamma, m->FADJ, sbar, 0);
   }
   hipLaunchKernelGGL(k_cell_adjoint1, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->G, m->PHI,
                      m->FADJ, m->adj, muscl ? 1 : 0, st->limiter);
@@ -4916,9 +5011,9 @@ int mcfd_residual_geometry_vjp(McfdMesh* m, const McfdSettings* st, const double
     return fail("mcfd_residual_geometry_vjp: NULL argument");
   if (!m->adjoint) return fail("mcfd_residual_geometry_vjp: call mcfd_mesh_set_adjoint first");
   if (check_settings(st)) return -1;
-  if (st->flux == 2 || st->reconstruction == 2 || st->viscosity || m->has_far_field || m->needs_inflow)
+  if (st->flux == 2 || st->reconstruction == 2 || st->viscosity || m->needs_inflow)
     return fail("Geometry gradients cover the inviscid Rusanov and HLLC residual, first order or MUSCL "
-                "linear, without far-field, pressure-inlet or mass-flow boundaries.");
+                "linear, without pressure-inlet or mass-flow boundaries.");
   hipStream_t s = static_cast<hipStream_t>(stream);
   int n = m->n_cells;
   bool muscl = st->reconstruction == 1 && m->n_grad > 0;
