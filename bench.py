@@ -217,7 +217,7 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=1000, help="quads per side")
-    ap.add_argument("--cpu-steps", type=int, default=5, help="SSP-RK3 steps timed on the CPU oracle (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=20, help="SSP-RK3 steps timed on the CPU oracle (0 = skip)")
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--unfused", action="store_true", help="four-kernel path instead of the fused stage kernel")
     ap.add_argument("--block-cells", type=int, default=128, help="cells per workgroup of the fused stage kernel")

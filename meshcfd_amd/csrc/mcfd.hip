@@ -2879,9 +2879,10 @@ __global__ void __launch_bounds__(kBlock)
 
 // Cell pass 1: gather face adjoints (reconstruction backward), limiter
 // backward, LSQ-gradient backward. Writes per cell Wbar without the
-// neighbours' contributions, and per LSQ entry e (i -> j) the 4-vector that
-// cell i's gradient and extremum adjoints contribute to W_j:
-// ebar[e] = c_e . gbar_i (+ the extremum adjoint if j holds i's extremum).
+// neighbours' contributions, and for each LSQ entry e (i -> j) the 4-vector
+// that cell i's gradient and extremum adjoints contribute to W_j,
+// c_e . gbar_i (+ the extremum adjoint if j holds i's extremum), into
+// ebar[rev(e)], the slot of j's entry for i.
 struct AdjArrays {
   const int* acc_ptr;
   const int* acc_item;
@@ -3049,7 +3050,7 @@ __global__ void __launch_bounds__(kBlock)
         if (xe[k] == e) t += xb[k];
         c[k] = t;
       }
-      store4(aa.ebar, e, c[0], c[1], c[2], c[3]);
+      store4(aa.ebar, aa.grad_rev[e], c[0], c[1], c[2], c[3]);  // read back by cell j in entry order
       if (aa.cbar) {  // grad_i = sum_e c_e (W_j - W_i)
         P4 o = load4(W, aa.grad_nb[e]);
         double d[4] = {o.r - wv[0], o.u - wv[1], o.v - wv[2], o.p - wv[3]};
@@ -3066,9 +3067,10 @@ __global__ void __launch_bounds__(kBlock)
   store4(aa.wbar, i, wb[0], wb[1], wb[2], wb[3]);
 }
 
-// Cell pass 2: the neighbours' contributions (their entries pointing back at
-// the cell, in the cell's entry order), body force, and the primitive ->
-// conservative chain rule. Writes Ubar.
+// Cell pass 2: the neighbours' contributions (stored by each neighbour at
+// the cell's own entry for it, so they are read contiguously in entry
+// order), body force, and the primitive -> conservative chain rule. Writes
+// Ubar.
 __global__ void __launch_bounds__(kBlock)
     k_cell_adjoint2(int n, const double* __restrict__ U, const double* __restrict__ W,
                     const double* __restrict__ lam, AdjArrays aa, int muscl, double g, double ax,
@@ -3079,7 +3081,7 @@ __global__ void __launch_bounds__(kBlock)
   double wb[4] = {wb4.r, wb4.u, wb4.v, wb4.p};
   if (muscl) {
     for (int e = aa.grad_ptr[i]; e < aa.grad_ptr[i + 1]; ++e) {
-      P4 c = load4(aa.ebar, aa.grad_rev[e]);  // entry j -> i
+      P4 c = load4(aa.ebar, e);  // written by the neighbour for its entry j -> i
       wb[0] += c.r;
       wb[1] += c.u;
       wb[2] += c.v;
@@ -3121,8 +3123,8 @@ template <int SCHEME, bool MUSCL, int LIM>
 __global__ void __launch_bounds__(kBlock)
     k_vjp_fused(int max_local, int max_grad, int max_faces, BlockArrays ba, GeomArrays ga,
                 FaceArrays fa, const double* __restrict__ U, const double* __restrict__ lam, double g,
-                double* __restrict__ wbar, double* __restrict__ ebar, double* __restrict__ SBAR,
-                int* __restrict__ status) {
+                const int* __restrict__ grad_rev, double* __restrict__ wbar, double* __restrict__ ebar,
+                double* __restrict__ SBAR, int* __restrict__ status) {
   extern __shared__ double smem[];
   const int b = xcd_block();
   const int* inf = ba.info + kBlockInfo * b;
@@ -3370,7 +3372,7 @@ __global__ void __launch_bounds__(kBlock)
         if (xe[k] == e) tv += xb[k];
         c[k] = tv;
       }
-      store4(ebar, gbase + e, c[0], c[1], c[2], c[3]);
+      store4(ebar, grad_rev[gbase + e], c[0], c[1], c[2], c[3]);
     }
   }
   store4(wbar, i, wb[0], wb[1], wb[2], wb[3]);
@@ -4258,7 +4260,8 @@ void launch_vjp_fused_t(McfdMesh* m, size_t lds, hipStream_t s, const GeomArrays
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                             hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024);
   hipLaunchKernelGGL(kern, dim3(m->nb), dim3(kBlock), lds, s, m->max_local, m->max_grad, m->max_faces,
-                     m->blocks, ga, m->faces, u, lam, g, m->adj.wbar, m->adj.ebar, sbar, m->status);
+                     m->blocks, ga, m->faces, u, lam, g, m->grad_rev, m->adj.wbar, m->adj.ebar, sbar,
+                     m->status);
 }
 
 template <int SCHEME>
