@@ -1,0 +1,124 @@
+"""`solve_case` on the GPU: the reference's driver contract (settings
+validation messages on the CPU; trajectories, callbacks, end time and steady
+convergence on the GPU against the oracle)."""
+
+import math
+
+import numpy as np
+import pytest
+
+from meshcfd_amd import meshgen
+from meshcfd_amd.geometry import SolverError
+from meshcfd_amd.model import InitialState
+from meshcfd_amd.solver import SolverProgress, conservative_to_primitive, primitive_to_conservative, solve_case
+
+
+@pytest.mark.parametrize(
+    "field, value, message",
+    [
+        ("cfl", 1.5, r"Explicit Euler requires a CFL number in \(0, 1\]\."),
+        ("flux", "Upwind", "Unsupported convective flux 'Upwind'."),
+        ("limiter", "Superbee", "Unsupported limiter 'Superbee'."),
+        ("time_integration", "Explicit RK9", "Unsupported time integration 'Explicit RK9'."),
+        ("time_step_mode", "Adaptive", "Unsupported time-stepping mode 'Adaptive'."),
+        ("max_iterations", 0, "Maximum iterations must be a positive integer."),
+        ("gmres_tolerance", 1.0, r"GMRES tolerance must be finite and in \(0, 1\)\."),
+        ("end_time", -1.0, "End time must be positive and finite."),
+        ("gamma", 1.0, "Specific heat ratio must be finite and greater than 1."),
+        ("prandtl_number", 0.0, "Prandtl number must be positive and finite."),
+        ("acceleration_x", math.inf, "Body-force acceleration must be finite."),
+    ],
+)
+def test_settings_are_validated_with_the_reference_messages(field, value, message):
+    """Reference `solver.py:194-374` (messages matched by the reference's
+    tests, e.g. `tests/test_solver.py`)."""
+    case, prim = meshgen.vortex_workload(8, 10)
+    setattr(case.numerics, field, value)
+    with pytest.raises(SolverError, match=message):
+        solve_case(case, prim)
+
+
+def test_initial_state_checks():
+    case, prim = meshgen.vortex_workload(8, 10)
+    with pytest.raises(SolverError, match="Initial state count must match the mesh cell count."):
+        solve_case(case, prim[:-1])
+    bad = prim.copy()
+    bad[3, 3] = -1.0
+    with pytest.raises(SolverError, match="Invalid initial state: Euler state has non-finite"):
+        solve_case(case, bad)
+
+
+def test_state_conversions_round_trip():
+    rng = np.random.default_rng(0)
+    w = np.column_stack((1 + rng.random(50), rng.normal(size=50), rng.normal(size=50), 1 + rng.random(50)))
+    np.testing.assert_allclose(conservative_to_primitive(primitive_to_conservative(w, 1.4), 1.4), w, rtol=1e-14)
+    with pytest.raises(SolverError, match="non-positive density or pressure"):
+        conservative_to_primitive(np.array([[1.0, 0.0, 0.0, -1.0]]), 1.4)
+
+
+@pytest.mark.gpu
+def test_gpu_solve_case_vortex_matches_oracle(cuda_device):
+    """Config 1 (64x64 periodic vortex, Rusanov, first order, explicit Euler,
+    CFL 0.5, 100 steps) through `solve_case`: states, time and step count equal
+    the oracle's; progress reports every convergence_history_interval steps
+    and at the last step."""
+    from oracle.solver import Oracle
+
+    case, prim = meshgen.vortex_workload(64, 100)
+    case.numerics.convergence_history_interval = 25
+    seen = []
+    states = tuple(InitialState(*row) for row in prim)
+    print("case: solve_case vortex", flush=True)
+    res = solve_case(case, states, progress_callback=seen.append, device=cuda_device)
+    from meshcfd_amd.geometry import build_geometry
+
+    ref, t_ref = Oracle(build_geometry(case), case.numerics).run(prim, 100)
+    wref = conservative_to_primitive(ref, 1.4)
+    err = np.max(np.abs(res.primitive - wref), axis=0) / np.max(np.abs(wref), axis=0)
+    print("rel err", err, "time", res.time, t_ref)
+    assert np.all(err <= 1e-12)
+    assert res.steps == 100 and not res.completed
+    assert math.isclose(res.time, t_ref, rel_tol=1e-13)
+    assert [p.step for p in seen] == [25, 50, 75, 100]
+    assert all(isinstance(p, SolverProgress) and len(p.residuals) == 4 for p in seen)
+    assert len(res.states) == case.mesh.cell_count and res.states[0].density == res.primitive[0, 0]
+
+
+@pytest.mark.gpu
+def test_gpu_solve_case_stops_at_end_time_and_on_cancel(cuda_device):
+    case, prim = meshgen.vortex_workload(16, 1000)
+    case.numerics.time_integration = "Explicit RK3"
+    case.numerics.end_time = 0.05
+    print("case: solve_case end time", flush=True)
+    res = solve_case(case, prim, device=cuda_device)
+    assert res.completed and res.time == 0.05 and 1 < res.steps < 1000
+    case.numerics.end_time = 10.0
+    calls = []
+    res = solve_case(case, prim, cancel_callback=lambda: len(calls) >= 3,
+                     solution_callback=lambda s, t, st: calls.append(s), device=cuda_device)
+    completed, steps = res.completed, res.steps
+    assert not completed and steps == 3 and calls == [1, 2, 3]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("integrator", ["Explicit RK3", "Implicit Euler"])
+def test_gpu_solve_case_converges_the_steady_bump(cuda_device, integrator):
+    """Local (steady) pseudo-time on the bump channel (first-order HLLC):
+    completes when the RMS residual reaches the tolerance; the reported
+    residual is the post-step residual. Implicit steps stop their Newton
+    iteration at max|U - U^n - dt R| <= 1e-9 (`solver.py:1418-1425`), which
+    floors the steady residual near 1e-9/dt: tolerance 1e-8 at CFL 100."""
+    implicit = integrator == "Implicit Euler"
+    tol = 1e-8 if implicit else 1e-10
+    case, prim = meshgen.bump_case(30, 10, 0.06, integrator=integrator, cfl=100.0 if implicit else 0.8)
+    case.numerics.reconstruction = "First order"
+    case.numerics.residual_tolerance = tol
+    case.numerics.max_iterations = 8000
+    case.numerics.convergence_history_interval = 500
+    seen = []
+    print(f"case: solve_case bump {integrator}", flush=True)
+    res = solve_case(case, prim, progress_callback=seen.append, device=cuda_device)
+    steps, residual, completed = res.steps, res.residual, res.completed
+    print("steps", steps, "residual", residual)
+    assert completed and residual <= tol
+    assert math.sqrt(sum(r * r for r in seen[-1].residuals) / 4) <= tol
