@@ -449,6 +449,12 @@ int mcfd_mesh_set_inflow(McfdMesh* mesh, const McfdInflowDesc* desc);
  * neighbour that points back) and allocate the adjoint scratch; NULL frees. */
 int mcfd_mesh_set_adjoint(McfdMesh* mesh, const int32_t* grad_rev);
 
+/* MUSCL parabolic adjoint: per two-ring stencil entry e (i -> j at offset
+ * d) the entry of j's list that points back at i (offset -d); the stencils
+ * are symmetric. Needs mcfd_mesh_set_parabolic and mcfd_mesh_set_adjoint;
+ * afterwards mcfd_residual_vjp covers MUSCL parabolic settings. */
+int mcfd_mesh_set_parabolic_adjoint(McfdMesh* mesh, const int32_t* par_rev);
+
 /* ubar = lambda^T dR/dU at state u: the hand-written reverse of `_residual`
  * (solver.py:2437-2718) through fluxes, ghost states, reconstruction,
  * limiter, LSQ gradient and primitive conversion. If sbar is not NULL it

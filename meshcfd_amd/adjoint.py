@@ -1,9 +1,11 @@
 """Adjoint (reverse-mode) residual and differentiable time stepping.
 
 No counterpart in the reference, which has no gradients. `mcfd_residual_vjp`
-is a hand-written backward kernel chain for the residual (fluxes by 8-way
-forward-mode duals per face, then reconstruction, limiter, LSQ gradient and
-primitive conversion reversed as deterministic per-cell gathers).
+is a hand-written backward kernel chain for the residual (Rusanov and HLLC
+face fluxes and their ghost states in reverse mode, Roe by 8-way
+forward-mode duals per face, then reconstruction, limiter, LSQ gradient or
+quadratic fit and primitive conversion reversed as deterministic per-cell
+gathers).
 `ResidualFunction` exposes it to `torch.autograd`; `rk3_rollout` advances
 SSP-RK3 steps with a fixed step size, checkpointing the state at every step
 and recomputing the stages during the backward pass, so long transients need
@@ -23,6 +25,7 @@ from . import native
 native.EXPORTS.update(
     {
         "mcfd_mesh_set_adjoint": (C.c_int, [C.c_void_p, C.c_void_p]),
+        "mcfd_mesh_set_parabolic_adjoint": (C.c_int, [C.c_void_p, C.c_void_p]),
         "mcfd_residual_vjp": (
             C.c_int,
             [C.c_void_p, C.POINTER(native.McfdSettings), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -41,12 +44,16 @@ def enable(dm) -> None:
     if getattr(dm, "_adjoint_ready", False):
         return
     lib = native.load()
-    for name in ("mcfd_mesh_set_adjoint", "mcfd_residual_vjp"):
+    for name in ("mcfd_mesh_set_adjoint", "mcfd_residual_vjp", "mcfd_mesh_set_parabolic_adjoint"):
         restype, argtypes = native.EXPORTS[name]
         fn = getattr(lib, name)
         fn.restype, fn.argtypes = restype, argtypes
     rev = np.ascontiguousarray(dm.layout.grad_rev, dtype=np.int32)
     native.check(lib.mcfd_mesh_set_adjoint(dm.handle, C.c_void_p(rev.ctypes.data)))
+    par = getattr(dm, "_parabolic", None)
+    if par is not None:
+        prev = np.ascontiguousarray(par["rev"], dtype=np.int32)
+        native.check(lib.mcfd_mesh_set_parabolic_adjoint(dm.handle, C.c_void_p(prev.ctypes.data)))
     dm._adjoint_ready = True
 
 

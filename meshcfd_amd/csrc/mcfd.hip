@@ -389,16 +389,22 @@ __global__ void __launch_bounds__(kBlock)
 // k_gradient; for full-rank cells the quadratic fit replaces them:
 // gradient/curvature = sum_e P_e (W_e - W_i) with P_e precomputed per stencil
 // entry, min/max over the two-ring stencil, quadratic face increments.
+// With ARGS (for the adjoint) the limiting and extremum entries as in
+// k_gradient_t; extremum entries count in the two-ring list for full-rank
+// cells and in the LSQ list otherwise.
+template <bool ARGS>
 __global__ void __launch_bounds__(kBlock)
-    k_gradient_parabolic(int n, const double* __restrict__ W, const int* __restrict__ gptr,
-                         const int* __restrict__ gnb, const double* __restrict__ gcx,
-                         const double* __restrict__ gcy, const int* __restrict__ lptr,
-                         const double* __restrict__ ldx, const double* __restrict__ ldy, int code,
-                         const int* __restrict__ pptr, const int* __restrict__ pnb,
-                         const double* __restrict__ pcoef, const int* __restrict__ pfull,
-                         double* __restrict__ G, double* __restrict__ K, double* __restrict__ PHI) {
+    k_gradient_parabolic_t(int n, const double* __restrict__ W, const int* __restrict__ gptr,
+                           const int* __restrict__ gnb, const double* __restrict__ gcx,
+                           const double* __restrict__ gcy, const int* __restrict__ lptr,
+                           const double* __restrict__ ldx, const double* __restrict__ ldy, int code,
+                           const int* __restrict__ pptr, const int* __restrict__ pnb,
+                           const double* __restrict__ pcoef, const int* __restrict__ pfull,
+                           double* __restrict__ G, double* __restrict__ K, double* __restrict__ PHI,
+                           unsigned* __restrict__ ARG) {
   int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  unsigned arg[4] = {0xFFFFFFu, 0xFFFFFFu, 0xFFFFFFu, 0xFFFFFFu};
   P4 w = load4(W, i);
   double wv[4] = {w.r, w.u, w.v, w.p};
   double gx[4] = {0.0, 0.0, 0.0, 0.0};
@@ -409,15 +415,22 @@ __global__ void __launch_bounds__(kBlock)
   double phi[4] = {1.0, 1.0, 1.0, 1.0};
   if (pfull[i]) {
     double c[5][4] = {{0}};
-    for (int e = pptr[i]; e < pptr[i + 1]; ++e) {
+    const int p0 = pptr[i];
+    for (int e = p0; e < pptr[i + 1]; ++e) {
       P4 o = load4(W, pnb[e]);
       double ov[4] = {o.r, o.u, o.v, o.p};
       const double* pe = pcoef + 5 * (long)e;
       for (int k = 0; k < 4; ++k) {
         double d = ov[k] - wv[k];
         for (int a = 0; a < 5; ++a) c[a][k] = c[a][k] + pe[a] * d;
-        if (ov[k] < mn[k]) mn[k] = ov[k];
-        if (ov[k] > mx[k]) mx[k] = ov[k];
+        if (ov[k] < mn[k]) {
+          mn[k] = ov[k];
+          if (ARGS) arg[k] = (arg[k] & 0x00FFFFu) | ((unsigned)(e - p0) << 16);
+        }
+        if (ov[k] > mx[k]) {
+          mx[k] = ov[k];
+          if (ARGS) arg[k] = (arg[k] & 0xFF00FFu) | ((unsigned)(e - p0) << 8);
+        }
       }
     }
     for (int k = 0; k < 4; ++k) {
@@ -432,11 +445,14 @@ __global__ void __launch_bounds__(kBlock)
       for (int k = 0; k < 4; ++k) {
         double inc = gx[k] * dx + gy[k] * dy + 0.5 * kk[3 * k] * (dx * dx) +
                      kk[3 * k + 1] * (dx * dy) + 0.5 * kk[3 * k + 2] * (dy * dy);
+        double before = phi[k];
         limiter_entry(inc, wv[k], mx[k], mn[k], code, phi[k]);
+        if (ARGS && phi[k] < before) arg[k] = (arg[k] & 0xFFFF00u) | (unsigned)(e - lptr[i]);
       }
     }
   } else {
-    for (int e = gptr[i]; e < gptr[i + 1]; ++e) {
+    const int g0 = gptr[i];
+    for (int e = g0; e < gptr[i + 1]; ++e) {
       P4 o = load4(W, gnb[e]);
       double ov[4] = {o.r, o.u, o.v, o.p};
       double cx = gcx[e], cy = gcy[e];
@@ -444,15 +460,28 @@ __global__ void __launch_bounds__(kBlock)
         double d = ov[k] - wv[k];
         gx[k] = gx[k] + cx * d;
         gy[k] = gy[k] + cy * d;
-        if (ov[k] < mn[k]) mn[k] = ov[k];
-        if (ov[k] > mx[k]) mx[k] = ov[k];
+        if (ov[k] < mn[k]) {
+          mn[k] = ov[k];
+          if (ARGS) arg[k] = (arg[k] & 0x00FFFFu) | ((unsigned)(e - g0) << 16);
+        }
+        if (ov[k] > mx[k]) {
+          mx[k] = ov[k];
+          if (ARGS) arg[k] = (arg[k] & 0xFF00FFu) | ((unsigned)(e - g0) << 8);
+        }
       }
     }
     for (int e = lptr[i]; e < lptr[i + 1]; ++e) {
       double dx = ldx[e], dy = ldy[e];
-      for (int k = 0; k < 4; ++k)
+      for (int k = 0; k < 4; ++k) {
+        double before = phi[k];
         limiter_entry(gx[k] * dx + gy[k] * dy, wv[k], mx[k], mn[k], code, phi[k]);
+        if (ARGS && phi[k] < before) arg[k] = (arg[k] & 0xFFFF00u) | (unsigned)(e - lptr[i]);
+      }
     }
+  }
+  if (ARGS) {
+    uint4* aq = reinterpret_cast<uint4*>(ARG + 4 * (long)i);
+    *aq = make_uint4(arg[0], arg[1], arg[2], arg[3]);
   }
   double2* gq = reinterpret_cast<double2*>(G + 8 * (long)i);
   for (int k = 0; k < 4; ++k) gq[k] = make_double2(gx[k], gy[k]);
@@ -2595,7 +2624,8 @@ __device__ void far_field_b(const P4& in, const P4& out, double nx, double ny, d
 __global__ void __launch_bounds__(kBlock)
     k_flux_adjoint_rev(int nf, FaceArrays fa, const double* __restrict__ W,
                        const double* __restrict__ G, const double* __restrict__ PHI,
-                       const double* __restrict__ flen, const double* __restrict__ area,
+                       const double* __restrict__ K, const double* __restrict__ flen,
+                       const double* __restrict__ area,
                        const double* __restrict__ lam, int scheme, int muscl, double g,
                        double* __restrict__ FADJ, double* __restrict__ SBAR) {
   int k = xcd_block() * blockDim.x + threadIdx.x;
@@ -2616,11 +2646,13 @@ __global__ void __launch_bounds__(kBlock)
     fb[2] += cr * lr.v;
     fb[3] += cr * lr.p;
   }
-  P4 wl = reconstruct(W, G, PHI, l, fa.lx[k], fa.ly[k], muscl != 0);
+  P4 wl = K ? reconstruct_q(W, G, K, PHI, l, fa.lx[k], fa.ly[k])
+            : reconstruct(W, G, PHI, l, fa.lx[k], fa.ly[k], muscl != 0);
   P4 wr;
   int b = fa.bc[k];
   if (kind <= MCFD_FACE_PERIODIC) {
-    wr = reconstruct(W, G, PHI, r, fa.rx[k], fa.ry[k], muscl != 0);
+    wr = K ? reconstruct_q(W, G, K, PHI, r, fa.rx[k], fa.ry[k])
+           : reconstruct(W, G, PHI, r, fa.rx[k], fa.ry[k], muscl != 0);
   } else {
     const double* st = fa.bc_state + 4 * (long)b;
     wr = wl;
@@ -2802,7 +2834,8 @@ __global__ void __launch_bounds__(kBlock)
 __global__ void __launch_bounds__(kBlock)
     k_flux_adjoint(int nf, FaceArrays fa, const double* __restrict__ W,
                    const double* __restrict__ G, const double* __restrict__ PHI,
-                   const double* __restrict__ flen, const double* __restrict__ area,
+                   const double* __restrict__ K, const double* __restrict__ flen,
+                   const double* __restrict__ area,
                    const double* __restrict__ lam, int scheme, int muscl, double g,
                    double* __restrict__ FADJ, double* __restrict__ SBAR, int only_far_field) {
   int k = xcd_block() * blockDim.x + threadIdx.x;
@@ -2825,11 +2858,13 @@ __global__ void __launch_bounds__(kBlock)
     fb[2] += cr * lr.v;
     fb[3] += cr * lr.p;
   }
-  P4 wl = reconstruct(W, G, PHI, l, fa.lx[k], fa.ly[k], muscl != 0);
+  P4 wl = K ? reconstruct_q(W, G, K, PHI, l, fa.lx[k], fa.ly[k])
+            : reconstruct(W, G, PHI, l, fa.lx[k], fa.ly[k], muscl != 0);
   P4D DL = seed(wl, 0);
   P4D DR;
   if (kind <= MCFD_FACE_PERIODIC) {
-    P4 wr = reconstruct(W, G, PHI, r, fa.rx[k], fa.ry[k], muscl != 0);
+    P4 wr = K ? reconstruct_q(W, G, K, PHI, r, fa.rx[k], fa.ry[k])
+              : reconstruct(W, G, PHI, r, fa.rx[k], fa.ry[k], muscl != 0);
     DR = seed(wr, 4);
   } else {
     int b = fa.bc[k];
@@ -2903,6 +2938,15 @@ struct AdjArrays {
   unsigned* arg;  // (C,4) limiting / extremum entries of the forward limiter (k_gradient_t<true>)
   double* cbar;   // (n_grad,2) LSQ coefficient adjoints, or NULL
   double* dlbar;  // (n_lim,2) limiter offset adjoints, or NULL
+  // MUSCL parabolic (all NULL otherwise): curvatures, two-ring fit entries,
+  // their reverse entries and the per-entry contributions
+  const double* K;
+  const int* pfull;
+  const int* par_ptr;
+  const int* par_nb;
+  const double* par_coef;
+  const int* par_rev;
+  double* ebar_par;
 };
 
 __device__ __forceinline__ double limiter_value(double ratio, int code, double& dfdr) {
@@ -2952,6 +2996,11 @@ __global__ void __launch_bounds__(kBlock)
   double gb[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double gv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   double pv[4] = {1.0, 1.0, 1.0, 1.0};
+  const bool quad = aa.K != nullptr;
+  double kv[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // curvature [k][xx, xy, yy]
+  double kb[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (quad)
+    for (int a = 0; a < 12; ++a) kv[a] = aa.K[12 * (long)i + a];
   if (muscl) {
     const double2* gq = reinterpret_cast<const double2*>(G + 8 * (long)i);
     for (int k = 0; k < 4; ++k) {
@@ -2979,9 +3028,17 @@ __global__ void __launch_bounds__(kBlock)
       wb[k] += a[k];
       if (muscl) {
         double inc = gv[2 * k] * dx + gv[2 * k + 1] * dy;
+        if (quad)
+          inc += 0.5 * kv[3 * k] * (dx * dx) + kv[3 * k + 1] * (dx * dy) + 0.5 * kv[3 * k + 2] * (dy * dy);
         pb[k] += a[k] * inc;
-        gb[2 * k] += a[k] * pv[k] * dx;
-        gb[2 * k + 1] += a[k] * pv[k] * dy;
+        double ap = a[k] * pv[k];
+        gb[2 * k] += ap * dx;
+        gb[2 * k + 1] += ap * dy;
+        if (quad) {
+          kb[3 * k] += ap * 0.5 * (dx * dx);
+          kb[3 * k + 1] += ap * (dx * dy);
+          kb[3 * k + 2] += ap * 0.5 * (dy * dy);
+        }
       }
     }
   }
@@ -2989,6 +3046,8 @@ __global__ void __launch_bounds__(kBlock)
     P4 w = load4(W, i);
     double wv[4] = {w.r, w.u, w.v, w.p};
     const int g0 = aa.grad_ptr[i];
+    const bool full = quad && aa.pfull[i];
+    const int p0 = quad ? aa.par_ptr[i] : 0;
     uint4 av = *reinterpret_cast<const uint4*>(aa.arg + 4 * (long)i);
     unsigned arg[4] = {av.x, av.y, av.z, av.w};
     // extremum adjoints: xb[k] goes to the entry holding the extremum
@@ -3007,14 +3066,17 @@ __global__ void __launch_bounds__(kBlock)
       int e = aa.lim_ptr[i] + (int)le;
       double dx = aa.lim_dx[e], dy = aa.lim_dy[e];
       double inc = gv[2 * k] * dx + gv[2 * k + 1] * dy;
+      if (quad)
+        inc += 0.5 * kv[3 * k] * (dx * dx) + kv[3 * k + 1] * (dx * dy) + 0.5 * kv[3 * k + 2] * (dy * dy);
       double tol = 1e-14 * vmax(fabs(wv[k]), 1.0);
       if (!(inc > tol) && !(inc < -tol)) continue;  // ratio 1: no dependence
       bool up = inc > tol;
       unsigned xa = up ? ((arg[k] >> 8) & 0xFFu) : ((arg[k] >> 16) & 0xFFu);
-      int xn = (xa == 0xFFu) ? -1 : g0 + (int)xa;
+      // extremum entries count in the two-ring list of full-rank cells
+      int xn = (xa == 0xFFu) ? -1 : (full ? p0 : g0) + (int)xa;
       double ext = wv[k];
       if (xn >= 0) {
-        P4 o = load4(W, aa.grad_nb[xn]);
+        P4 o = load4(W, full ? aa.par_nb[xn] : aa.grad_nb[xn]);
         double ov[4] = {o.r, o.u, o.v, o.p};
         ext = ov[k];
       }
@@ -3029,6 +3091,11 @@ __global__ void __launch_bounds__(kBlock)
       double incb = -rb * ratio / inc;
       gb[2 * k] += incb * dx;
       gb[2 * k + 1] += incb * dy;
+      if (quad) {
+        kb[3 * k] += incb * 0.5 * (dx * dx);
+        kb[3 * k + 1] += incb * (dx * dy);
+        kb[3 * k + 2] += incb * 0.5 * (dy * dy);
+      }
       if (aa.dlbar) {
         aa.dlbar[2 * (long)e] += incb * gv[2 * k];
         aa.dlbar[2 * (long)e + 1] += incb * gv[2 * k + 1];
@@ -3040,14 +3107,31 @@ __global__ void __launch_bounds__(kBlock)
         xe[k] = xn;
       }
     }
+    if (quad) {
+      // quadratic fit backward (full rank): (grad, curvature)_i = sum_e P_e (W_j - W_i)
+      for (int e = p0; e < aa.par_ptr[i + 1]; ++e) {
+        double c[4] = {0.0, 0.0, 0.0, 0.0};
+        if (full) {
+          const double* pe = aa.par_coef + 5 * (long)e;
+          for (int k = 0; k < 4; ++k) {
+            double t = pe[0] * gb[2 * k] + pe[1] * gb[2 * k + 1] + pe[2] * kb[3 * k] +
+                       pe[3] * kb[3 * k + 1] + pe[4] * kb[3 * k + 2];
+            wb[k] -= t;
+            if (xe[k] == e) t += xb[k];
+            c[k] = t;
+          }
+        }
+        store4(aa.ebar_par, aa.par_rev[e], c[0], c[1], c[2], c[3]);
+      }
+    }
     // LSQ gradient backward: grad_i = sum_e c_e (W_j - W_i)
     for (int e = g0; e < aa.grad_ptr[i + 1]; ++e) {
       double cx = aa.grad_cx[e], cy = aa.grad_cy[e];
       double c[4];
       for (int k = 0; k < 4; ++k) {
-        double t = cx * gb[2 * k] + cy * gb[2 * k + 1];
+        double t = full ? 0.0 : cx * gb[2 * k] + cy * gb[2 * k + 1];
         wb[k] -= t;
-        if (xe[k] == e) t += xb[k];
+        if (!full && xe[k] == e) t += xb[k];
         c[k] = t;
       }
       store4(aa.ebar, aa.grad_rev[e], c[0], c[1], c[2], c[3]);  // read back by cell j in entry order
@@ -3087,6 +3171,14 @@ __global__ void __launch_bounds__(kBlock)
       wb[2] += c.v;
       wb[3] += c.p;
     }
+    if (aa.ebar_par)
+      for (int e = aa.par_ptr[i]; e < aa.par_ptr[i + 1]; ++e) {
+        P4 c = load4(aa.ebar_par, e);
+        wb[0] += c.r;
+        wb[1] += c.u;
+        wb[2] += c.v;
+        wb[3] += c.p;
+      }
   }
   // primitive velocity of the cell, as k_primitive computes it
   P4 q = load4(U, i);
@@ -3721,6 +3813,7 @@ void free_adjoint(McfdMesh* m) {
   for (void* p : m->adjoint_allocations) (void)hipFree(p);
   m->adjoint_allocations.clear();
   m->adjoint = false;
+  m->adj = AdjArrays{};
 }
 
 template <typename T>
@@ -3946,10 +4039,10 @@ int residual_front(McfdMesh* m, const McfdSettings* st, const double* u, hipStre
   bool muscl = (st->reconstruction == 1 || para) && m->n_grad > 0;
   if (para) {
     TimedLaunch t(m, s, MCFD_KERNEL_GRADIENT);
-    hipLaunchKernelGGL(k_gradient_parabolic, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W,
+    hipLaunchKernelGGL(k_gradient_parabolic_t<false>, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W,
                        m->grad_ptr, m->grad_nb, m->grad_cx, m->grad_cy, m->lim_ptr, m->lim_dx,
                        m->lim_dy, st->limiter, m->par_ptr, m->par_nb, m->par_coef, m->par_full,
-                       m->G, m->K, m->PHI);
+                       m->G, m->K, m->PHI, nullptr);
   } else if (muscl) {
     TimedLaunch t(m, s, MCFD_KERNEL_GRADIENT);
     hipLaunchKernelGGL(k_gradient_t<false>, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->grad_ptr,
@@ -4285,14 +4378,16 @@ int mcfd_residual_vjp_impl(McfdMesh* m, const McfdSettings* st, const double* u,
   if (!m || !u || !lam || !ubar) return fail("mcfd_residual_vjp: NULL argument");
   if (!m->adjoint) return fail("mcfd_residual_vjp: call mcfd_mesh_set_adjoint first");
   if (check_settings(st)) return -1;
-  if (st->reconstruction == 2) return fail("mcfd_residual_vjp: MUSCL parabolic has no adjoint yet");
+  const bool para = st->reconstruction == 2 && m->n_grad > 0;
+  if (para && !(m->parabolic && m->adj.par_rev))
+    return fail("mcfd_residual_vjp: MUSCL parabolic needs mcfd_mesh_set_parabolic_adjoint first");
   if (st->viscosity) return fail("mcfd_residual_vjp: viscous terms have no adjoint yet");
   if (m->needs_inflow)
     return fail("mcfd_residual_vjp: pressure-inlet and mass-flow boundaries have no adjoint yet");
   int n = m->n_cells;
-  bool muscl = st->reconstruction == 1 && m->n_grad > 0;
+  bool muscl = (st->reconstruction == 1 || para) && m->n_grad > 0;
   size_t vjp_lds = m->fused ? vjp_fused_lds_bytes(m) : 0;
-  if (m->fused && st->flux != 2 && vjp_lds <= 64 * 1024) {
+  if (m->fused && !para && st->flux != 2 && vjp_lds <= 64 * 1024) {
     // one workgroup per block (reverse-mode fluxes), then the neighbour pass
     GeomArrays ga{m->area,   m->grad_ptr, m->grad_cx,  m->grad_cy, m->lim_ptr,
                   m->lim_dx, m->lim_dy,   m->acc_ptr,  m->face_len, m->acc_coef};
@@ -4308,26 +4403,42 @@ int mcfd_residual_vjp_impl(McfdMesh* m, const McfdSettings* st, const double* u,
   }
   hipLaunchKernelGGL(k_primitive, dim3(blocks(n)), dim3(kBlock), 0, s, n, u, m->W, st->gamma,
                      m->status);
-  if (muscl)
-    hipLaunchKernelGGL(k_gradient_t<true>, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->grad_ptr,
-                       m->grad_nb, m->grad_cx, m->grad_cy, m->lim_ptr, m->lim_dx, m->lim_dy,
-                       st->limiter, m->G, m->PHI, m->adj.arg);
+  AdjArrays aa = m->adj;
+  if (para) {
+    hipLaunchKernelGGL(k_gradient_parabolic_t<true>, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W,
+                       m->grad_ptr, m->grad_nb, m->grad_cx, m->grad_cy, m->lim_ptr, m->lim_dx,
+                       m->lim_dy, st->limiter, m->par_ptr, m->par_nb, m->par_coef, m->par_full,
+                       m->G, m->K, m->PHI, m->adj.arg);
+    aa.K = m->K;
+    aa.pfull = m->par_full;
+    aa.par_ptr = m->par_ptr;
+    aa.par_nb = m->par_nb;
+    aa.par_coef = m->par_coef;
+  } else {
+    aa.par_rev = nullptr;
+    aa.ebar_par = nullptr;
+    if (muscl)
+      hipLaunchKernelGGL(k_gradient_t<true>, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->grad_ptr,
+                         m->grad_nb, m->grad_cx, m->grad_cy, m->lim_ptr, m->lim_dx, m->lim_dy,
+                         st->limiter, m->G, m->PHI, m->adj.arg);
+  }
   if (m->n_faces) {
     // Rusanov and HLLC in reverse mode; Roe with duals
     bool reverse = st->flux != 2;
+    const double* Kq = para ? m->K : nullptr;
     if (reverse)
       hipLaunchKernelGGL(k_flux_adjoint_rev, dim3(blocks(m->n_faces)), dim3(kBlock), 0, s,
-                         m->n_faces, m->faces, m->W, m->G, m->PHI, m->face_len, m->area, lam,
+                         m->n_faces, m->faces, m->W, m->G, m->PHI, Kq, m->face_len, m->area, lam,
                          st->flux, muscl ? 1 : 0, st->gamma, m->FADJ, sbar);
     else
       hipLaunchKernelGGL(k_flux_adjoint, dim3(blocks(m->n_faces)), dim3(kBlock), 0, s, m->n_faces,
-                         m->faces, m->W, m->G, m->PHI, m->face_len, m->area, lam, st->flux,
+                         m->faces, m->W, m->G, m->PHI, Kq, m->face_len, m->area, lam, st->flux,
                          muscl ? 1 : 0, st->gamma, m->FADJ, sbar, 0);
   }
   hipLaunchKernelGGL(k_cell_adjoint1, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->G, m->PHI,
-                     m->FADJ, m->adj, muscl ? 1 : 0, st->limiter);
+                     m->FADJ, aa, muscl ? 1 : 0, st->limiter);
   hipLaunchKernelGGL(k_cell_adjoint2, dim3(blocks(n)), dim3(kBlock), 0, s, n, u, m->W, lam,
-                     m->adj, muscl ? 1 : 0, st->gamma, st->acceleration_x, st->acceleration_y,
+                     aa, muscl ? 1 : 0, st->gamma, st->acceleration_x, st->acceleration_y,
                      ubar);
   HIP_CHECK(hipGetLastError());
   return 0;
@@ -5027,7 +5138,7 @@ int mcfd_residual_geometry_vjp(McfdMesh* m, const McfdSettings* st, const double
                        st->limiter, m->G, m->PHI, m->adj.arg);
   if (m->n_faces) {
     hipLaunchKernelGGL(k_flux_adjoint_rev, dim3(blocks(m->n_faces)), dim3(kBlock), 0, s, m->n_faces,
-                       m->faces, m->W, m->G, m->PHI, m->face_len, m->area, lam, st->flux,
+                       m->faces, m->W, m->G, m->PHI, nullptr, m->face_len, m->area, lam, st->flux,
                        muscl ? 1 : 0, st->gamma, m->FADJ, nullptr);
     hipLaunchKernelGGL(k_geom_face, dim3(blocks(m->n_faces)), dim3(kBlock), 0, s, m->n_faces, m->faces,
                        m->W, m->G, m->PHI, m->face_len, m->area, lam, st->flux, muscl ? 1 : 0,
@@ -5037,6 +5148,8 @@ int mcfd_residual_geometry_vjp(McfdMesh* m, const McfdSettings* st, const double
     AdjArrays aa = m->adj;
     aa.cbar = grad_bar;
     aa.dlbar = lim_bar;
+    aa.par_rev = nullptr;
+    aa.ebar_par = nullptr;
     hipLaunchKernelGGL(k_cell_adjoint1, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->G, m->PHI,
                        m->FADJ, aa, 1, st->limiter);
   } else {
@@ -5045,6 +5158,26 @@ int mcfd_residual_geometry_vjp(McfdMesh* m, const McfdSettings* st, const double
     if (nl) hipLaunchKernelGGL(k_fill, dim3(lblocks(nl)), dim3(kBlock), 0, s, nl, 0.0, lim_bar);
   }
   HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int mcfd_mesh_set_parabolic_adjoint(McfdMesh* m, const int32_t* par_rev) {
+  if (!m) return fail("mcfd_mesh_set_parabolic_adjoint: NULL mesh");
+  if (!m->adjoint || !m->parabolic)
+    return fail("mcfd_mesh_set_parabolic_adjoint: call mcfd_mesh_set_parabolic and mcfd_mesh_set_adjoint first");
+  if (!par_rev) return fail("mcfd_mesh_set_parabolic_adjoint: NULL reverse map");
+  for (int e = 0; e < m->n_par; ++e)
+    if (par_rev[e] < 0 || par_rev[e] >= m->n_par)
+      return fail("mcfd_mesh_set_parabolic_adjoint: reverse entry out of range");
+  (void)hipDeviceSynchronize();
+  int* rev = nullptr;
+  double* eb = nullptr;
+  int rc = 0;
+  rc |= alloc_adjoint(m, &rev, par_rev, (size_t)m->n_par);
+  rc |= alloc_adjoint(m, &eb, (const double*)nullptr, 4 * (size_t)m->n_par);
+  if (rc) return -1;
+  m->adj.par_rev = rev;
+  m->adj.ebar_par = eb;
   return 0;
 }
 

@@ -114,4 +114,22 @@ def build_parabolic(g: Geometry, new_id: np.ndarray):
         "nb": new_id[nb[order]].astype(np.int32),
         "coef": np.ascontiguousarray(P[order]),
         "full": full_dev,
+        "rev": reverse_entries(src[order], nb[order], dx[order], dy[order], g.domain_scale),
     }
+
+
+def reverse_entries(src, nb, dx, dy, scale) -> np.ndarray:
+    """For every stencil entry (i -> j at offset d) the entry (j -> i at
+    offset -d); the two-ring stencils are symmetric (paths reversed), which
+    this checks. Entries are matched on the dedup key of the stencil."""
+    kx = np.round(dx / scale, 10)
+    ky = np.round(dy / scale, 10)
+    keys = np.rec.fromarrays((src, nb, kx + 0.0, ky + 0.0))
+    back = np.rec.fromarrays((nb, src, -kx + 0.0, -ky + 0.0))
+    order = np.argsort(keys, kind="stable")
+    pos = np.searchsorted(keys[order], back)
+    pos = np.minimum(pos, len(order) - 1)
+    rev = order[pos]
+    if len(rev) and not np.array_equal(keys[rev], back):
+        raise ValueError("The two-ring stencil is not symmetric; the parabolic adjoint needs it to be.")
+    return rev.astype(np.int32)

@@ -20,7 +20,7 @@ def _dm(case, cuda_device, fused=True):
 
 
 @pytest.mark.parametrize("flux", ["Rusanov", "HLLC", "Roe"])
-@pytest.mark.parametrize("reconstruction", ["First order", "MUSCL linear"])
+@pytest.mark.parametrize("reconstruction", ["First order", "MUSCL linear", "MUSCL parabolic"])
 @pytest.mark.parametrize("kind", ["periodic", "mixed", "farfield", "transmissive"])
 def test_vjp_matches_central_difference_of_oracle(cuda_device, flux, reconstruction, kind):
     from test_gpu_parity import _case, _perturbed_primitive
