@@ -2947,6 +2947,9 @@ struct AdjArrays {
   const double* par_coef;
   const int* par_rev;
   double* ebar_par;
+  // viscous terms (NULL otherwise): per LSQ entry the (u, v, T) adjoint a
+  // cell's WLSQ gradient gives its neighbour, stored at the reverse entry
+  double* vbar;
 };
 
 __device__ __forceinline__ double limiter_value(double ratio, int code, double& dfdr) {
@@ -3151,6 +3154,413 @@ __global__ void __launch_bounds__(kBlock)
   store4(aa.wbar, i, wb[0], wb[1], wb[2], wb[3]);
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of the laminar viscous terms (reverse of k_visc_gradient,
+// k_visc_flux and visc_gather). Per flux face and side a 12-double record
+// {gbar[6] (cell gradient of u, v, T), mubar, kbar (cell viscosity and
+// conductivity), Wbar[4] (direct primitive adjoint)}; k_visc_cell_adjoint
+// gathers them per cell and reverses the WLSQ gradient with its ghost
+// entries, adding to wbar and to the per-entry neighbour slots vbar.
+
+__device__ __forceinline__ double visc_dmu_dT(double T, double mu, const ViscParams& v) {
+  if (v.model == MCFD_VISCOSITY_CONSTANT) return 0.0;
+  return mu * (1.5 / T - 1.0 / (T + v.S));
+}
+
+// Reverse of visc_flux: F-bar (components 1..3) -> g-bar, u, v, mu, k adjoints.
+__device__ __forceinline__ void visc_flux_b(const double g[6], double u, double v, double mu, double k,
+                                            double nx, double ny, const double Fb[4], double gb[6],
+                                            double& ub, double& vb, double& mub, double& kb) {
+  double div = g[0] + g[3];
+  double txx = mu * (2.0 * g[0] - (2.0 / 3.0) * div);
+  double tyy = mu * (2.0 * g[3] - (2.0 / 3.0) * div);
+  double txy = mu * (g[1] + g[2]);
+  double tx = txx * nx + txy * ny;
+  double ty = txy * nx + tyy * ny;
+  double txb = Fb[1] + Fb[3] * u;
+  double tyb = Fb[2] + Fb[3] * v;
+  ub += Fb[3] * tx;
+  vb += Fb[3] * ty;
+  double qb = Fb[3];
+  double txxb = txb * nx, txyb = txb * ny + tyb * nx, tyyb = tyb * ny;
+  kb += qb * (g[4] * nx + g[5] * ny);
+  gb[4] += qb * k * nx;
+  gb[5] += qb * k * ny;
+  gb[0] += txxb * (4.0 / 3.0) * mu + tyyb * (-2.0 / 3.0) * mu;
+  gb[3] += txxb * (-2.0 / 3.0) * mu + tyyb * (4.0 / 3.0) * mu;
+  gb[1] += txyb * mu;
+  gb[2] += txyb * mu;
+  mub += txxb * (2.0 * g[0] - (2.0 / 3.0) * div) + tyyb * (2.0 * g[3] - (2.0 / 3.0) * div) +
+         txyb * (g[1] + g[2]);
+}
+
+// (u, v, T)-bar of a cell state -> primitive (rho, u, v, p)-bar, T = p/(rho R)
+__device__ __forceinline__ void uvT_to_prim_b(const P4& w, double R, double ub, double vb, double Tb,
+                                              double out[4]) {
+  out[0] += -Tb * w.p / (w.r * w.r * R);
+  out[1] += ub;
+  out[2] += vb;
+  out[3] += Tb / (w.r * R);
+}
+
+// Exterior (u, v, T) adjoint of visc_inlet -> interior primitive adjoint
+// (far field through far_field_b; a velocity inlet's state is fixed).
+__device__ __forceinline__ void visc_inlet_b(int code, const P4& w, const double* st, double nx, double ny,
+                                             const ViscParams& v, const double xb[3], double wb[4]) {
+  if (code != kVisFarField) return;
+  int err = 0;
+  P4 out{st[0], st[1], st[2], st[3]};
+  P4 e = far_field(w, out, nx, ny, v.gamma, &err);
+  P4 eb{-xb[2] * e.p / (e.r * e.r * v.R), xb[0], xb[1], xb[2] / (e.r * v.R)};
+  P4 inb{0.0, 0.0, 0.0, 0.0}, outb{0.0, 0.0, 0.0, 0.0};
+  double nbx = 0.0, nby = 0.0;
+  far_field_b(w, out, nx, ny, v.gamma, eb, inb, outb, nbx, nby);
+  wb[0] += inb.r;
+  wb[1] += inb.u;
+  wb[2] += inb.v;
+  wb[3] += inb.p;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_visc_face_adjoint(int nf, FaceArrays fa, const double* __restrict__ W,
+                        const double* __restrict__ VG, ViscArrays va, ViscParams vp,
+                        const double* __restrict__ flen, const double* __restrict__ area,
+                        const double* __restrict__ lam, double* __restrict__ VFA) {
+  int f = xcd_block() * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  int kind = fa.kind[f];
+  double nx = fa.nx[f], ny = fa.ny[f];
+  int o = fa.left[f];
+  const double* go = VG + 8 * (long)o;
+  double L = flen[f];
+  // the viscous terms enter the owner as +(F L)/A and the other side as -(F L)/A
+  P4 lo = load4(lam, o);
+  double co = L / area[o];
+  double Fb[4] = {co * lo.r, co * lo.u, co * lo.v, co * lo.p};
+  double rec[2][12];
+#pragma unroll
+  for (int a = 0; a < 12; ++a) rec[0][a] = rec[1][a] = 0.0;
+  if (kind == MCFD_FACE_INTERIOR || kind == MCFD_FACE_PERIODIC) {
+    int r = fa.right[f];
+    const double* gr = VG + 8 * (long)r;
+    P4 lr = load4(lam, r);
+    double cr = L / area[r];
+    Fb[0] -= cr * lr.r;
+    Fb[1] -= cr * lr.u;
+    Fb[2] -= cr * lr.v;
+    Fb[3] -= cr * lr.p;
+    double g[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) g[k] = 0.5 * (go[k] + gr[k]);
+    P4 a = load4(W, o), c = load4(W, r);
+    double gb[6] = {0, 0, 0, 0, 0, 0}, ub = 0.0, vb = 0.0, mub = 0.0, kb = 0.0;
+    visc_flux_b(g, 0.5 * (a.u + c.u), 0.5 * (a.v + c.v), 0.5 * (go[6] + gr[6]), 0.5 * (go[7] + gr[7]), nx,
+                ny, Fb, gb, ub, vb, mub, kb);
+    for (int sd = 0; sd < 2; ++sd) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) rec[sd][k] = 0.5 * gb[k];
+      rec[sd][6] = 0.5 * mub;
+      rec[sd][7] = 0.5 * kb;
+      rec[sd][9] = 0.5 * ub;
+      rec[sd][10] = 0.5 * vb;
+    }
+  } else {
+    int b = fa.bc[f];
+    int code = va.bv_code[b];
+    const double* bg = va.bv_geom + 4 * (long)b;
+    double dn = bg[0], tx = bg[1], ty = bg[2];
+    double offx = fa.lx[f], offy = fa.ly[f];
+    P4 w = load4(W, o);
+    double T = w.p / (w.r * vp.R);
+    int flags = va.bv_flags[b];
+    int thermal = (flags >> 1) & 3;
+    // forward (as k_visc_flux), keeping what the reverse needs
+    double g[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) g[k] = go[k];
+    double bv[3] = {w.u, w.v, T};
+    double x[3] = {0.0, 0.0, 0.0};
+    if (code == kVisWall) {
+      if (flags & 1) {
+        const double* wv = fa.bc_wall + 2 * (long)b;
+        for (int c = 0; c < 2; ++c) {
+          double nd = (wv[c] - bv[c] - (g[2 * c] * tx + g[2 * c + 1] * ty)) / dn;
+          double sc = nd - (g[2 * c] * nx + g[2 * c + 1] * ny);
+          g[2 * c] = g[2 * c] + sc * nx;
+          g[2 * c + 1] = g[2 * c + 1] + sc * ny;
+          bv[c] = wv[c];
+        }
+      } else {
+        double un = bv[0] * nx + bv[1] * ny;
+        bv[0] = bv[0] + (g[0] * offx + g[1] * offy);
+        bv[1] = bv[1] + (g[2] * offx + g[3] * offy);
+        double un2 = bv[0] * nx + bv[1] * ny;
+        bv[0] = bv[0] - un2 * nx;
+        bv[1] = bv[1] - un2 * ny;
+        double gnx = nx * g[0] + ny * g[2];
+        double gny = nx * g[1] + ny * g[3];
+        double nd = -(un + (gnx * tx + gny * ty)) / dn;
+        double vn0 = g[0] * nx + g[1] * ny;
+        double vn1 = g[2] * nx + g[3] * ny;
+        g[0] = g[0] + (nd * nx * nx - vn0 * nx);
+        g[1] = g[1] + (nd * nx * ny - vn0 * ny);
+        g[2] = g[2] + (nd * ny * nx - vn1 * nx);
+        g[3] = g[3] + (nd * ny * ny - vn1 * ny);
+      }
+      if (thermal == 1) {
+        double nd = (bg[3] - bv[2] - (g[4] * tx + g[5] * ty)) / dn;
+        bv[2] = bg[3];
+        double sc = nd - (g[4] * nx + g[5] * ny);
+        g[4] = g[4] + sc * nx;
+        g[5] = g[5] + sc * ny;
+      } else {
+        double nd = bg[3] / go[7];
+        bv[2] = bv[2] + (g[4] * offx + g[5] * offy);
+        double sc = nd - (g[4] * nx + g[5] * ny);
+        g[4] = g[4] + sc * nx;
+        g[5] = g[5] + sc * ny;
+      }
+    } else if (code == kVisSymmetry) {
+      double un = bv[0] * nx + bv[1] * ny;
+      double dv[2] = {-un / dn * nx, -un / dn * ny};
+      for (int c = 0; c < 2; ++c) {
+        double sc = dv[c] - (g[2 * c] * nx + g[2 * c + 1] * ny);
+        g[2 * c] = g[2 * c] + sc * nx;
+        g[2 * c + 1] = g[2 * c + 1] + sc * ny;
+      }
+      double gt = g[4] * nx + g[5] * ny;
+      g[4] = g[4] - gt * nx;
+      g[5] = g[5] - gt * ny;
+      bv[0] = bv[0] - un * nx;
+      bv[1] = bv[1] - un * ny;
+    } else if (code == kVisInlet || code == kVisFarField) {
+      int dummy = 0;
+      visc_inlet(code, w, fa.bc_state + 4 * (long)b, nx, ny, vp, x, &dummy, o);
+      double cv[3] = {w.u, w.v, T};
+      for (int c = 0; c < 3; ++c) {
+        bv[c] = x[c];
+        double nd = (bv[c] - cv[c] - (g[2 * c] * tx + g[2 * c + 1] * ty)) / dn;
+        double sc = nd - (g[2 * c] * nx + g[2 * c + 1] * ny);
+        g[2 * c] = g[2 * c] + sc * nx;
+        g[2 * c + 1] = g[2 * c + 1] + sc * ny;
+      }
+    } else {
+      for (int c = 0; c < 3; ++c) {
+        double gn = g[2 * c] * nx + g[2 * c + 1] * ny;
+        g[2 * c] = g[2 * c] - gn * nx;
+        g[2 * c + 1] = g[2 * c + 1] - gn * ny;
+        bv[c] = bv[c] + (g[2 * c] * offx + g[2 * c + 1] * offy);
+      }
+    }
+    int dummy = 0;
+    double fmu = visc_mu(bv[2], vp, &dummy, o);
+    double fk = visc_k(fmu, vp);
+    // reverse
+    if (code == kVisWall && thermal == 2) Fb[3] = 0.0;  // F[3] = prescribed heat flux
+    double gb[6] = {0, 0, 0, 0, 0, 0}, bvb[3] = {0.0, 0.0, 0.0}, fmub = 0.0, fkb = 0.0;
+    visc_flux_b(g, bv[0], bv[1], fmu, fk, nx, ny, Fb, gb, bvb[0], bvb[1], fmub, fkb);
+    fmub += fkb * vp.gamma * vp.R / ((vp.gamma - 1.0) * vp.Pr);
+    bvb[2] += fmub * visc_dmu_dT(bv[2], fmu, vp);
+    double wb[4] = {0.0, 0.0, 0.0, 0.0};   // direct primitive adjoint of the owner
+    double uvTb[3] = {0.0, 0.0, 0.0};      // adjoint of the owner's (u, v, T)
+    double kob = 0.0;                      // adjoint of the owner's conductivity
+    if (code == kVisWall) {
+      // thermal part (applied last in the forward)
+      if (thermal == 1) {
+        double sb = gb[4] * nx + gb[5] * ny;
+        gb[4] += sb * (-tx / dn - nx);
+        gb[5] += sb * (-ty / dn - ny);
+        uvTb[2] += -sb / dn;  // bv[2] before the reset was T
+      } else {
+        double sb = gb[4] * nx + gb[5] * ny;
+        gb[4] -= sb * nx;
+        gb[5] -= sb * ny;
+        kob += -sb * bg[3] / (go[7] * go[7]);
+        uvTb[2] += bvb[2];
+        gb[4] += bvb[2] * offx;
+        gb[5] += bvb[2] * offy;
+      }
+      if (flags & 1) {
+        for (int c = 0; c < 2; ++c) {
+          double sb = gb[2 * c] * nx + gb[2 * c + 1] * ny;
+          gb[2 * c] += sb * (-tx / dn - nx);
+          gb[2 * c + 1] += sb * (-ty / dn - ny);
+          uvTb[c] += -sb / dn;
+        }
+      } else {
+        // bv'' = (I - n n^T) bv'
+        double pb = bvb[0] * nx + bvb[1] * ny;
+        double b0 = bvb[0] - nx * pb, b1 = bvb[1] - ny * pb;
+        double ndb = gb[0] * nx * nx + gb[1] * nx * ny + gb[2] * ny * nx + gb[3] * ny * ny;
+        double vn0b = -(gb[0] * nx + gb[1] * ny);
+        double vn1b = -(gb[2] * nx + gb[3] * ny);
+        gb[0] += vn0b * nx;
+        gb[1] += vn0b * ny;
+        gb[2] += vn1b * nx;
+        gb[3] += vn1b * ny;
+        double unb = -ndb / dn;
+        double gnxb = -ndb * tx / dn, gnyb = -ndb * ty / dn;
+        gb[0] += gnxb * nx;
+        gb[2] += gnxb * ny;
+        gb[1] += gnyb * nx;
+        gb[3] += gnyb * ny;
+        uvTb[0] += b0;
+        gb[0] += b0 * offx;
+        gb[1] += b0 * offy;
+        uvTb[1] += b1;
+        gb[2] += b1 * offx;
+        gb[3] += b1 * offy;
+        uvTb[0] += unb * nx;
+        uvTb[1] += unb * ny;
+      }
+    } else if (code == kVisSymmetry) {
+      double unb = -(bvb[0] * nx + bvb[1] * ny);
+      uvTb[0] += bvb[0];
+      uvTb[1] += bvb[1];
+      uvTb[2] += bvb[2];
+      double gtb = gb[4] * nx + gb[5] * ny;
+      gb[4] -= gtb * nx;
+      gb[5] -= gtb * ny;
+      for (int c = 0; c < 2; ++c) {
+        double sb = gb[2 * c] * nx + gb[2 * c + 1] * ny;
+        gb[2 * c] -= sb * nx;
+        gb[2 * c + 1] -= sb * ny;
+        unb += -sb * (c == 0 ? nx : ny) / dn;
+      }
+      uvTb[0] += unb * nx;
+      uvTb[1] += unb * ny;
+    } else if (code == kVisInlet || code == kVisFarField) {
+      double xb[3];
+      for (int c = 0; c < 3; ++c) {
+        double sb = gb[2 * c] * nx + gb[2 * c + 1] * ny;
+        gb[2 * c] += sb * (-tx / dn - nx);
+        gb[2 * c + 1] += sb * (-ty / dn - ny);
+        xb[c] = sb / dn + bvb[c];
+        uvTb[c] += -sb / dn;
+      }
+      visc_inlet_b(code, w, fa.bc_state + 4 * (long)b, nx, ny, vp, xb, wb);
+    } else {
+      for (int c = 0; c < 3; ++c) {
+        gb[2 * c] += bvb[c] * offx;
+        gb[2 * c + 1] += bvb[c] * offy;
+        uvTb[c] += bvb[c];
+        double gnb = gb[2 * c] * nx + gb[2 * c + 1] * ny;
+        gb[2 * c] -= gnb * nx;
+        gb[2 * c + 1] -= gnb * ny;
+      }
+    }
+    uvT_to_prim_b(w, vp.R, uvTb[0], uvTb[1], uvTb[2], wb);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) rec[0][k] = gb[k];
+    rec[0][7] = kob;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) rec[0][8 + k] = wb[k];
+  }
+  double2* q = reinterpret_cast<double2*>(VFA + 24 * (long)f);
+#pragma unroll
+  for (int a = 0; a < 6; ++a) q[a] = make_double2(rec[0][2 * a], rec[0][2 * a + 1]);
+#pragma unroll
+  for (int a = 0; a < 6; ++a) q[6 + a] = make_double2(rec[1][2 * a], rec[1][2 * a + 1]);
+}
+
+// Per cell: gather the face records, reverse the viscosity law and the WLSQ
+// gradient with its ghost entries; adds to wbar and writes the neighbours'
+// (u, v, T) adjoints into vbar at the reverse entries.
+__global__ void __launch_bounds__(kBlock)
+    k_visc_cell_adjoint(int n, const double* __restrict__ W, const double* __restrict__ VG,
+                        const int* __restrict__ acc_ptr, const int* __restrict__ acc_item,
+                        const int* __restrict__ grad_ptr, const int* __restrict__ grad_rev,
+                        ViscArrays va, FaceArrays fa, ViscParams vp, const double* __restrict__ VFA,
+                        double* __restrict__ wbar, double* __restrict__ vbar) {
+  int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double G[6] = {0, 0, 0, 0, 0, 0}, mub = 0.0, kb = 0.0, wb[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int e = acc_ptr[i]; e < acc_ptr[i + 1]; ++e) {
+    int item = acc_item[e];
+    int f = item >> 2, mode = item & 3;
+    bool right = (mode == MCFD_ACC_NEIGHBOUR) || (mode == MCFD_ACC_PERIODIC_B);
+    const double* r = VFA + 24 * (long)f + (right ? 12 : 0);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) G[k] += r[k];
+    mub += r[6];
+    kb += r[7];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) wb[k] += r[8 + k];
+  }
+  P4 w = load4(W, i);
+  double T = w.p / (w.r * vp.R);
+  double val[3] = {w.u, w.v, T};
+  const double* vg = VG + 8 * (long)i;
+  double mu = vg[6], kc = vg[7];
+  const double* m = va.wl_cell + 4 * (long)i;
+  double xx = m[0], xy = m[1], yy = m[2], inv = m[3];
+  double rxb[3] = {0.0, 0.0, 0.0}, ryb[3] = {0.0, 0.0, 0.0};
+  if (inv != 0.0)
+    for (int k = 0; k < 3; ++k) {
+      rxb[k] = (G[2 * k] * yy - G[2 * k + 1] * xy) * inv;
+      ryb[k] = (G[2 * k + 1] * xx - G[2 * k] * xy) * inv;
+    }
+  double valb[3] = {0.0, 0.0, 0.0};
+  for (int e = grad_ptr[i]; e < grad_ptr[i + 1]; ++e) {
+    double wx = va.wl_wx[e], wy = va.wl_wy[e];
+    double d[3];
+    for (int k = 0; k < 3; ++k) {
+      d[k] = wx * rxb[k] + wy * ryb[k];
+      valb[k] -= d[k];
+    }
+    double* vq = vbar + 3 * (long)grad_rev[e];
+    vq[0] = d[0];
+    vq[1] = d[1];
+    vq[2] = d[2];
+  }
+  for (int e = va.gh_ptr[i]; e < va.gh_ptr[i + 1]; ++e) {
+    int f = va.gh_face[e];
+    int b = fa.bc[f];
+    int code = va.bv_code[b];
+    double nx = fa.nx[f], ny = fa.ny[f];
+    double wx = va.gh_wx[e], wy = va.gh_wy[e];
+    double ghb[3];
+    for (int k = 0; k < 3; ++k) {
+      ghb[k] = wx * rxb[k] + wy * ryb[k];
+      valb[k] -= ghb[k];
+    }
+    if (code == kVisWall) {
+      int flags = va.bv_flags[b];
+      const double* bg = va.bv_geom + 4 * (long)b;
+      if (flags & 1) {
+        valb[0] -= ghb[0];
+        valb[1] -= ghb[1];
+      } else {
+        double t = 2.0 * (ghb[0] * nx + ghb[1] * ny);
+        valb[0] += ghb[0] - t * nx;
+        valb[1] += ghb[1] - t * ny;
+      }
+      if (((flags >> 1) & 3) == 1) {
+        valb[2] -= ghb[2];
+      } else {
+        valb[2] += ghb[2];
+        kb += -ghb[2] * 2.0 * bg[0] * bg[3] / (kc * kc);
+      }
+    } else if (code == kVisSymmetry) {
+      double t = 2.0 * (ghb[0] * nx + ghb[1] * ny);
+      valb[0] += ghb[0] - t * nx;
+      valb[1] += ghb[1] - t * ny;
+      valb[2] += ghb[2];
+    } else if (code == kVisInlet || code == kVisFarField) {
+      double xb[3] = {2.0 * ghb[0], 2.0 * ghb[1], 2.0 * ghb[2]};
+      for (int k = 0; k < 3; ++k) valb[k] -= ghb[k];
+      visc_inlet_b(code, w, fa.bc_state + 4 * (long)b, nx, ny, vp, xb, wb);
+    } else {
+      for (int k = 0; k < 3; ++k) valb[k] += ghb[k];
+    }
+  }
+  mub += kb * vp.gamma * vp.R / ((vp.gamma - 1.0) * vp.Pr);
+  valb[2] += mub * visc_dmu_dT(T, mu, vp);
+  uvT_to_prim_b(w, vp.R, valb[0], valb[1], valb[2], wb);
+  P4 cur = load4(wbar, i);
+  store4(wbar, i, cur.r + wb[0], cur.u + wb[1], cur.v + wb[2], cur.p + wb[3]);
+}
+
 // Cell pass 2: the neighbours' contributions (stored by each neighbour at
 // the cell's own entry for it, so they are read contiguously in entry
 // order), body force, and the primitive -> conservative chain rule. Writes
@@ -3158,7 +3568,7 @@ __global__ void __launch_bounds__(kBlock)
 __global__ void __launch_bounds__(kBlock)
     k_cell_adjoint2(int n, const double* __restrict__ U, const double* __restrict__ W,
                     const double* __restrict__ lam, AdjArrays aa, int muscl, double g, double ax,
-                    double ay, double* __restrict__ ubar) {
+                    double ay, double vR, double* __restrict__ ubar) {
   int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   P4 wb4 = load4(aa.wbar, i);
@@ -3179,6 +3589,23 @@ __global__ void __launch_bounds__(kBlock)
         wb[2] += c.v;
         wb[3] += c.p;
       }
+  }
+  if (aa.vbar) {
+    // the neighbours' WLSQ adjoints of this cell's (u, v, T), T = p/(rho R)
+    double ub = 0.0, vb = 0.0, Tb = 0.0;
+    for (int e = aa.grad_ptr[i]; e < aa.grad_ptr[i + 1]; ++e) {
+      const double* vq = aa.vbar + 3 * (long)e;
+      ub += vq[0];
+      vb += vq[1];
+      Tb += vq[2];
+    }
+    P4 q = load4(U, i);
+    double rho = q.r;
+    double pr = (g - 1.0) * (q.p - 0.5 * (q.u * q.u + q.v * q.v) / rho);
+    wb[0] += -Tb * pr / (rho * rho * vR);
+    wb[1] += ub;
+    wb[2] += vb;
+    wb[3] += Tb / (rho * vR);
   }
   // primitive velocity of the cell, as k_primitive computes it
   P4 q = load4(U, i);
@@ -3709,6 +4136,8 @@ struct McfdMesh {
   bool adjoint = false;
   int* grad_rev = nullptr;
   double* FADJ = nullptr;
+  double* VFA = nullptr;   // viscous face adjoint records [n_faces*24]
+  double* VBAR = nullptr;  // viscous neighbour adjoints [n_grad*3]
   AdjArrays adj{};
   std::vector<void*> adjoint_allocations;
   // laminar viscous terms
@@ -4381,13 +4810,14 @@ int mcfd_residual_vjp_impl(McfdMesh* m, const McfdSettings* st, const double* u,
   const bool para = st->reconstruction == 2 && m->n_grad > 0;
   if (para && !(m->parabolic && m->adj.par_rev))
     return fail("mcfd_residual_vjp: MUSCL parabolic needs mcfd_mesh_set_parabolic_adjoint first");
-  if (st->viscosity) return fail("mcfd_residual_vjp: viscous terms have no adjoint yet");
+  if (st->viscosity && !(m->viscous && m->VFA))
+    return fail("mcfd_residual_vjp: viscous settings need mcfd_mesh_set_viscous before mcfd_mesh_set_adjoint");
   if (m->needs_inflow)
     return fail("mcfd_residual_vjp: pressure-inlet and mass-flow boundaries have no adjoint yet");
   int n = m->n_cells;
   bool muscl = (st->reconstruction == 1 || para) && m->n_grad > 0;
   size_t vjp_lds = m->fused ? vjp_fused_lds_bytes(m) : 0;
-  if (m->fused && !para && st->flux != 2 && vjp_lds <= 64 * 1024) {
+  if (m->fused && !para && !st->viscosity && st->flux != 2 && vjp_lds <= 64 * 1024) {
     // one workgroup per block (reverse-mode fluxes), then the neighbour pass
     GeomArrays ga{m->area,   m->grad_ptr, m->grad_cx,  m->grad_cy, m->lim_ptr,
                   m->lim_dx, m->lim_dy,   m->acc_ptr,  m->face_len, m->acc_coef};
@@ -4396,7 +4826,7 @@ int mcfd_residual_vjp_impl(McfdMesh* m, const McfdSettings* st, const double* u,
     else
       launch_vjp_fused_s<0>(m, muscl, st->limiter, vjp_lds, s, ga, u, lam, st->gamma, sbar);
     hipLaunchKernelGGL(k_cell_adjoint2, dim3(blocks(n)), dim3(kBlock), 0, s, n, u, m->W, lam,
-                       m->adj, muscl ? 1 : 0, st->gamma, st->acceleration_x, st->acceleration_y,
+                       m->adj, muscl ? 1 : 0, st->gamma, st->acceleration_x, st->acceleration_y, st->gas_constant,
                        ubar);
     HIP_CHECK(hipGetLastError());
     return 0;
@@ -4437,8 +4867,21 @@ int mcfd_residual_vjp_impl(McfdMesh* m, const McfdSettings* st, const double* u,
   }
   hipLaunchKernelGGL(k_cell_adjoint1, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->G, m->PHI,
                      m->FADJ, aa, muscl ? 1 : 0, st->limiter);
+  aa.vbar = nullptr;
+  if (st->viscosity) {
+    ViscParams vp = visc_params(st);
+    hipLaunchKernelGGL(k_visc_gradient, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->grad_ptr,
+                       m->grad_nb, m->visc, m->faces, vp, m->VG, m->status);
+    if (m->n_faces)
+      hipLaunchKernelGGL(k_visc_face_adjoint, dim3(blocks(m->n_faces)), dim3(kBlock), 0, s, m->n_faces,
+                         m->faces, m->W, m->VG, m->visc, vp, m->face_len, m->area, lam, m->VFA);
+    hipLaunchKernelGGL(k_visc_cell_adjoint, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->VG,
+                       m->acc_ptr, m->acc_item, m->grad_ptr, m->grad_rev, m->visc, m->faces, vp, m->VFA,
+                       m->adj.wbar, m->VBAR);
+    aa.vbar = m->VBAR;
+  }
   hipLaunchKernelGGL(k_cell_adjoint2, dim3(blocks(n)), dim3(kBlock), 0, s, n, u, m->W, lam,
-                     aa, muscl ? 1 : 0, st->gamma, st->acceleration_x, st->acceleration_y,
+                     aa, muscl ? 1 : 0, st->gamma, st->acceleration_x, st->acceleration_y, st->gas_constant,
                      ubar);
   HIP_CHECK(hipGetLastError());
   return 0;
@@ -4929,6 +5372,10 @@ int mcfd_mesh_set_adjoint(McfdMesh* m, const int32_t* grad_rev) {
   rc |= alloc_adjoint(m, &m->FADJ, (const double*)nullptr, 8 * (size_t)m->n_faces);
   rc |= alloc_adjoint(m, &a.wbar, (const double*)nullptr, 4 * C);
   rc |= alloc_adjoint(m, &a.ebar, (const double*)nullptr, 4 * (size_t)m->n_grad);
+  if (m->viscous) {
+    rc |= alloc_adjoint(m, &m->VFA, (const double*)nullptr, 24 * (size_t)m->n_faces);
+    rc |= alloc_adjoint(m, &m->VBAR, (const double*)nullptr, 3 * (size_t)m->n_grad);
+  }
   rc |= alloc_adjoint(m, &a.arg, (const unsigned*)nullptr, 4 * C);
   if (rc) {
     std::string msg = g_last_error;

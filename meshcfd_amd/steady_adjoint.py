@@ -160,8 +160,6 @@ class SteadyAdjoint:
     """Adjoint solves at the converged state of a DeviceMesh."""
 
     def __init__(self, dm, cfl: float = 1e3, krylov: int = 30, rtol: float = 1e-3, max_gmres: int = 300):
-        if dm.settings.viscosity_model != "Inviscid":
-            raise SolverError("The adjoint covers the inviscid residual (first order, MUSCL linear and parabolic).")
         self.dm = dm
         # the LU-SGS structure on the mesh (shared with the forward implicit steps)
         if getattr(dm, "_implicit", None) is None:
