@@ -1252,13 +1252,13 @@ __device__ __forceinline__ void visc_gather(int i, const ViscAcc& v, const doubl
 // cell's faces come from the block's LDS slots; items flagged 0x8000 (periodic
 // b sides and faces with an exterior state) use the global item as above.
 __device__ double block_time_step(int i, const P4& w, int e0, int e1,
-                                  const unsigned short* __restrict__ dloc,
+                                  const unsigned short* dloc, int dbase,
                                   const double* sN, const double* sL, const DtArrays& d, double g,
                                   double cfl, int* status) {
   double c = sound(w, g);
   double sum = 0.0;
   for (int e = e0; e < e1; ++e) {
-    int item = dloc[e];
+    int item = dloc[e - dbase];
     double nx, ny, len;
     int b = -1;
     if (item & 0x8000) {
@@ -1455,7 +1455,7 @@ __device__ __forceinline__ P4 rec_lds(const double* sW, const double* sG, const 
 // kernel is ~48 KB, most of a CU pair's instruction cache).
 template <int SCHEME, bool MUSCL, int LIM, bool FARFIELD, bool DTNEXT>
 __global__ void __launch_bounds__(kBlock)
-    k_stage_fused(int block_cells, int max_local, int max_grad, int max_faces, BlockArrays ba,
+    k_stage_fused(int max_acc, int max_local, int max_grad, int max_faces, BlockArrays ba,
                   GeomArrays ga, FaceArrays fa, DtArrays da, double g, double ax, double ay,
                   StageArgs s, int* __restrict__ status) {
   constexpr int limiter = LIM;
@@ -1478,8 +1478,11 @@ __global__ void __launch_bounds__(kBlock)
   double* sF = sG;
   double* sL = sF + 4 * max_faces;
   double* sN = sL + max_faces;  // face normals (final stage with the next step's dt)
+  // own cells' accumulation coefficients and items, and spectral-radius items
+  double* sAC = sG + std::max(std::max(7 * max_faces, MUSCL ? 12 * max_grad : 0), kBlock);
+  unsigned short* sAI = reinterpret_cast<unsigned short*>(sAC + max_acc);
+  unsigned short* sDI = sAI + max_acc;
   const double* __restrict__ U = s.u;
-  (void)block_cells;
   const int t = threadIdx.x;
 
   // Index loads of the later phases are issued now, so that their latency
@@ -1662,6 +1665,23 @@ __global__ void __launch_bounds__(kBlock)
       if (s.op != MCFD_OP_EULER) z2 = reinterpret_cast<const double2*>(s.u0 + 4 * (long)i)[hf];
     }
   }
+  // the block's accumulation (and spectral-radius) items are contiguous in
+  // the global lists: staged with coalesced loads, so that P3's loop reads
+  // LDS instead of a chain of dependent global loads
+  const int abase = ga.acc_ptr[first];
+  {
+    const int n_acc = ga.acc_ptr[first + n_own] - abase;
+    for (int k = t; k < n_acc; k += kBlock) {
+      sAC[k] = ga.acc_coef[abase + k];
+      sAI[k] = ba.acc_local[abase + k];
+    }
+  }
+  int dbase = 0;
+  if (DTNEXT) {
+    dbase = da.ptr[first];
+    const int n_dts = da.ptr[first + n_own] - dbase;
+    for (int k = t; k < n_dts; k += kBlock) sDI[k] = ba.dts_local[dbase + k];
+  }
   if constexpr (MUSCL) __syncthreads();
   if (f < n_faces) {
     sF[4 * f] = fl.a;
@@ -1683,12 +1703,12 @@ __global__ void __launch_bounds__(kBlock)
     const int i = first + j3;
     double ra = 0.0, rb = 0.0;
     for (int e = a0; e < a1; ++e) {
-      int item = ba.acc_local[e];
+      int item = sAI[e - abase];
       int slot = item >> 2;
       int mode = item & 3;
       double2 F = reinterpret_cast<const double2*>(sF + 4 * slot)[hf];
       if (mode <= MCFD_ACC_NEIGHBOUR) {
-        double c = ga.acc_coef[e];  // (-L)/A or L/A, precomputed like the reference's CSR data
+        double c = sAC[e - abase];  // (-L)/A or L/A, precomputed like the reference's CSR data
         ra = ra + c * F.x;
         rb = rb + c * F.y;
         continue;
@@ -1746,7 +1766,7 @@ __global__ void __launch_bounds__(kBlock)
         double p = (g - 1.0) * (o3 - 0.5 * (o1 * o1 + o2 * o2) / rho);
         if (!finite(rho) || !finite(uu) || !finite(vv) || !finite(p) || !(rho > 0.0) || !(p > 0.0))
           report(status, MCFD_ERR_STATE, i);
-        double dt = block_time_step(i, P4{rho, uu, vv, p}, d0, d1, ba.dts_local, sN, sL, da, g,
+        double dt = block_time_step(i, P4{rho, uu, vv, p}, d0, d1, sDI, dbase, sN, sL, da, g,
                                     s.cfl, status);
         if (s.dt_next) s.dt_next[i] = dt;
         dt_local = dt;
@@ -4107,6 +4127,7 @@ struct McfdMesh {
   // fused-stage blocks
   bool fused = false;
   int nb = 0, block_cells = 0, max_local = 0, max_grad = 0, max_faces = 0;
+  int max_acc = 0, max_dts = 0;  // most accumulation / spectral-radius items of a block's own cells
   double* block_partial = nullptr;  // [nb] per-block minima of the fused final stage
   BlockArrays blocks{};
   std::vector<void*> block_allocations;
@@ -4212,12 +4233,16 @@ int upload_block(McfdMesh* m, T** dst, const T* src, size_t count) {
 }
 
 // W of the local cells + one region shared by (gradients, limiter factors)
-// and then (face fluxes, lengths, normals) and the final block reduction.
+// and then (face fluxes, lengths, normals) and the final block reduction,
+// then the own cells' accumulation coefficients and items and their
+// spectral-radius items, staged for the gather.
 size_t fused_lds_bytes(const McfdMesh* m, bool muscl) {
   size_t region = 7 * (size_t)m->max_faces;
   if (muscl) region = std::max(region, 12 * (size_t)m->max_grad);
   region = std::max(region, (size_t)kBlock);
-  return (4 * (size_t)m->max_local + region) * sizeof(double);
+  size_t doubles = 4 * (size_t)m->max_local + region + (size_t)m->max_acc;
+  size_t shorts = (size_t)m->max_acc + (size_t)m->max_dts;
+  return doubles * sizeof(double) + ((shorts * sizeof(unsigned short) + 7) & ~(size_t)7);
 }
 
 void free_implicit(McfdMesh* m) {
@@ -4391,7 +4416,7 @@ void launch_fused_t(McfdMesh* m, size_t lds, hipStream_t s, const GeomArrays& ga
   auto kern = k_stage_fused<SCHEME, MUSCL, LIM, FARFIELD, DTNEXT>;
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                             hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024);
-  hipLaunchKernelGGL(kern, dim3(m->nb), dim3(kBlock), lds, s, m->block_cells, m->max_local,
+  hipLaunchKernelGGL(kern, dim3(m->nb), dim3(kBlock), lds, s, m->max_acc, m->max_local,
                      m->max_grad, m->max_faces, m->blocks, ga, m->faces, m->dts, st->gamma,
                      st->acceleration_x, st->acceleration_y, a, m->status);
 }
@@ -5009,6 +5034,19 @@ int mcfd_mesh_set_blocks(McfdMesh* m, const McfdBlockDesc* d) {
   m->max_local = d->max_local;
   m->max_grad = d->max_grad;
   m->max_faces = d->max_faces;
+  {
+    // the gather stages each block's accumulation and spectral-radius items
+    std::vector<int> aptr((size_t)m->n_cells + 1), dptr((size_t)m->n_cells + 1);
+    HIP_CHECK(hipMemcpy(aptr.data(), m->acc_ptr, sizeof(int) * aptr.size(), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(dptr.data(), m->dts_ptr, sizeof(int) * dptr.size(), hipMemcpyDeviceToHost));
+    m->max_acc = 0;
+    m->max_dts = 0;
+    for (int b = 0; b < d->n_blocks; ++b) {
+      const int32_t* inf = d->info + (size_t)kBlockInfo * b;
+      m->max_acc = std::max(m->max_acc, aptr[inf[8] + inf[1]] - aptr[inf[8]]);
+      m->max_dts = std::max(m->max_dts, dptr[inf[8] + inf[1]] - dptr[inf[8]]);
+    }
+  }
   if (fused_lds_bytes(m, true) > 64 * 1024) {
     free_blocks(m);
     return fail("mcfd_mesh_set_blocks: block needs more than 64 KiB of LDS; use smaller blocks");
