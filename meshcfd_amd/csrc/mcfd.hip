@@ -4178,6 +4178,7 @@ struct McfdMesh {
   std::vector<void*> inflow_allocations;
   std::vector<int> host_face_bc;  // [n_faces] for validating the inflow records
   std::vector<int> host_bc_kind;  // [n_bc]
+  std::vector<int> host_acc_ptr, host_dts_ptr;  // [n_cells + 1], for the fused blocks' LDS sizes
   int max_cell_entries = 0;       // longest LSQ or limiter list of a cell
   // timing
   bool timing = false;
@@ -4950,6 +4951,8 @@ int mcfd_mesh_create(const McfdMeshDesc* d, McfdMesh** out) {
     m->max_cell_entries = std::max(m->max_cell_entries, d->lim_ptr[i + 1] - d->lim_ptr[i]);
   }
   if (d->n_faces > 0 && d->face_bc) m->host_face_bc.assign(d->face_bc, d->face_bc + d->n_faces);
+  if (d->acc_ptr) m->host_acc_ptr.assign(d->acc_ptr, d->acc_ptr + (size_t)d->n_cells + 1);
+  if (d->dts_ptr) m->host_dts_ptr.assign(d->dts_ptr, d->dts_ptr + (size_t)d->n_cells + 1);
   (void)hipGetDevice(&m->device);
   size_t C = d->n_cells, F = d->n_faces;
   int rc = 0;
@@ -5036,9 +5039,10 @@ int mcfd_mesh_set_blocks(McfdMesh* m, const McfdBlockDesc* d) {
   m->max_faces = d->max_faces;
   {
     // the gather stages each block's accumulation and spectral-radius items
-    std::vector<int> aptr((size_t)m->n_cells + 1), dptr((size_t)m->n_cells + 1);
-    HIP_CHECK(hipMemcpy(aptr.data(), m->acc_ptr, sizeof(int) * aptr.size(), hipMemcpyDeviceToHost));
-    HIP_CHECK(hipMemcpy(dptr.data(), m->dts_ptr, sizeof(int) * dptr.size(), hipMemcpyDeviceToHost));
+    const std::vector<int>& aptr = m->host_acc_ptr;
+    const std::vector<int>& dptr = m->host_dts_ptr;
+    if (aptr.size() != (size_t)m->n_cells + 1 || dptr.size() != (size_t)m->n_cells + 1)
+      return fail("mcfd_mesh_set_blocks: the mesh has no accumulation lists");
     m->max_acc = 0;
     m->max_dts = 0;
     for (int b = 0; b < d->n_blocks; ++b) {
