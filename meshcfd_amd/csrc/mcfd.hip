@@ -3225,8 +3225,21 @@ __device__ __forceinline__ void uvT_to_prim_b(const P4& w, double R, double ub, 
 
 // Exterior (u, v, T) adjoint of visc_inlet -> interior primitive adjoint
 // (far field through far_field_b; a velocity inlet's state is fixed).
+// Reverse of visc_inlet: the boundary values' adjoint xb (u, v, T) into the
+// owner state (far field) and, when sb is given, into the boundary record's
+// state adjoint (rho, u, v, p), added to what the convective faces stored.
 __device__ __forceinline__ void visc_inlet_b(int code, const P4& w, const double* st, double nx, double ny,
-                                             const ViscParams& v, const double xb[3], double wb[4]) {
+                                             const ViscParams& v, const double xb[3], double wb[4],
+                                             double* sb) {
+  if (code == kVisInlet) {
+    if (sb) {
+      sb[0] += -xb[2] * st[3] / (st[0] * st[0] * v.R);
+      sb[1] += xb[0];
+      sb[2] += xb[1];
+      sb[3] += xb[2] / (st[0] * v.R);
+    }
+    return;
+  }
   if (code != kVisFarField) return;
   int err = 0;
   P4 out{st[0], st[1], st[2], st[3]};
@@ -3239,13 +3252,20 @@ __device__ __forceinline__ void visc_inlet_b(int code, const P4& w, const double
   wb[1] += inb.u;
   wb[2] += inb.v;
   wb[3] += inb.p;
+  if (sb) {
+    sb[0] += outb.r;
+    sb[1] += outb.u;
+    sb[2] += outb.v;
+    sb[3] += outb.p;
+  }
 }
 
 __global__ void __launch_bounds__(kBlock)
     k_visc_face_adjoint(int nf, FaceArrays fa, const double* __restrict__ W,
                         const double* __restrict__ VG, ViscArrays va, ViscParams vp,
                         const double* __restrict__ flen, const double* __restrict__ area,
-                        const double* __restrict__ lam, double* __restrict__ VFA) {
+                        const double* __restrict__ lam, double* __restrict__ VFA,
+                        double* __restrict__ SBAR) {
   int f = xcd_block() * blockDim.x + threadIdx.x;
   if (f >= nf) return;
   int kind = fa.kind[f];
@@ -3458,7 +3478,7 @@ __global__ void __launch_bounds__(kBlock)
         xb[c] = sb / dn + bvb[c];
         uvTb[c] += -sb / dn;
       }
-      visc_inlet_b(code, w, fa.bc_state + 4 * (long)b, nx, ny, vp, xb, wb);
+      visc_inlet_b(code, w, fa.bc_state + 4 * (long)b, nx, ny, vp, xb, wb, SBAR ? SBAR + 4 * (long)b : nullptr);
     } else {
       for (int c = 0; c < 3; ++c) {
         gb[2 * c] += bvb[c] * offx;
@@ -3491,7 +3511,7 @@ __global__ void __launch_bounds__(kBlock)
                         const int* __restrict__ acc_ptr, const int* __restrict__ acc_item,
                         const int* __restrict__ grad_ptr, const int* __restrict__ grad_rev,
                         ViscArrays va, FaceArrays fa, ViscParams vp, const double* __restrict__ VFA,
-                        double* __restrict__ wbar, double* __restrict__ vbar) {
+                        double* __restrict__ wbar, double* __restrict__ vbar, double* __restrict__ SBAR) {
   int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double G[6] = {0, 0, 0, 0, 0, 0}, mub = 0.0, kb = 0.0, wb[4] = {0.0, 0.0, 0.0, 0.0};
@@ -3569,7 +3589,7 @@ __global__ void __launch_bounds__(kBlock)
     } else if (code == kVisInlet || code == kVisFarField) {
       double xb[3] = {2.0 * ghb[0], 2.0 * ghb[1], 2.0 * ghb[2]};
       for (int k = 0; k < 3; ++k) valb[k] -= ghb[k];
-      visc_inlet_b(code, w, fa.bc_state + 4 * (long)b, nx, ny, vp, xb, wb);
+      visc_inlet_b(code, w, fa.bc_state + 4 * (long)b, nx, ny, vp, xb, wb, SBAR ? SBAR + 4 * (long)b : nullptr);
     } else {
       for (int k = 0; k < 3; ++k) valb[k] += ghb[k];
     }
@@ -4900,10 +4920,10 @@ int mcfd_residual_vjp_impl(McfdMesh* m, const McfdSettings* st, const double* u,
                        m->grad_nb, m->visc, m->faces, vp, m->VG, m->status);
     if (m->n_faces)
       hipLaunchKernelGGL(k_visc_face_adjoint, dim3(blocks(m->n_faces)), dim3(kBlock), 0, s, m->n_faces,
-                         m->faces, m->W, m->VG, m->visc, vp, m->face_len, m->area, lam, m->VFA);
+                         m->faces, m->W, m->VG, m->visc, vp, m->face_len, m->area, lam, m->VFA, sbar);
     hipLaunchKernelGGL(k_visc_cell_adjoint, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->VG,
                        m->acc_ptr, m->acc_item, m->grad_ptr, m->grad_rev, m->visc, m->faces, vp, m->VFA,
-                       m->adj.wbar, m->VBAR);
+                       m->adj.wbar, m->VBAR, sbar);
     aa.vbar = m->VBAR;
   }
   hipLaunchKernelGGL(k_cell_adjoint2, dim3(blocks(n)), dim3(kBlock), 0, s, n, u, m->W, lam,

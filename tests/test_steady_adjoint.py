@@ -191,11 +191,35 @@ def test_gpu_adjoint_gmres_true_residual(cuda_device):
     dm.close()
 
 
-def _steady_force(mach, alpha, warm=None, lift=False):
+def _laminar_cylinder(n_theta=32, n_r=16, mach=0.3, alpha=5.0):
+    """Unit free stream (rho = 1, c = 1, gas constant 1) past the unit
+    cylinder with constant viscosity mu = 0.3 / 40 (Re 40 at M 0.3), an
+    isothermal no-slip wall at the free-stream temperature; HLLC first
+    order, implicit Euler, local steps, CFL 200."""
+    from meshcfd_amd.model import BoundaryCondition
+
+    numerics = NumericalSettings(
+        flux="HLLC", reconstruction="First order", time_integration="Implicit Euler",
+        time_step_mode="Local (steady)", cfl=200.0, gmres_tolerance=1e-3, gmres_krylov_dimension=30,
+        residual_tolerance=1e-12, max_iterations=400, end_time=1e30,
+        viscosity_model="Constant", dynamic_viscosity=0.3 / 40.0, gas_constant=1.0,
+    )
+    case, prim = meshgen.cylinder_case(n_theta, n_r, mach=mach, alpha_deg=alpha, reynolds=None, r_outer=10.0,
+                                       first_spacing=0.02, seed=5, numerics=numerics)
+    t_inf = float(prim[0, 3] / prim[0, 0])
+    case.boundary_conditions[0] = BoundaryCondition(0, "wall", "Wall", {"temperature": t_inf}, "Temperature",
+                                                    "No-slip")
+    return case, prim
+
+
+def _steady_force(mach, alpha, warm=None, lift=False, laminar=False):
     from meshcfd_amd.device import DeviceMesh
     from meshcfd_amd.steady_adjoint import WallForce, solve_steady
 
-    case, prim = _inviscid_cylinder(32, 16, mach=mach, alpha=alpha)
+    if laminar:
+        case, prim = _laminar_cylinder(32, 16, mach=mach, alpha=alpha)
+    else:
+        case, prim = _inviscid_cylinder(32, 16, mach=mach, alpha=alpha)
     dm = DeviceMesh(case, "cuda:0", fused=True, block_cells=64)
     dm.set_state(primitive_to_conservative(prim, 1.4) if warm is None else warm)
 
@@ -254,3 +278,44 @@ def test_gpu_wall_force_inflow_gradients_match_fd(cuda_device, lift):
     frac = 5e-4 if lift else 2e-5
     assert abs(g_mach - fd_mach) <= frac * scale
     assert abs(g_alpha - fd_alpha) <= frac * scale
+
+
+@pytest.mark.gpu
+def test_gpu_laminar_drag_gradients_match_fd(cuda_device):
+    """Laminar cylinder (Re 40 at M 0.3, alpha 5 deg, 32x16 O-grid, HLLC first
+    order): the drag includes the viscous wall traction; d(drag)/dM and
+    d(drag)/d(alpha) from one adjoint solve (viscous reverse kernels in the
+    adjoint operator) equal central differences of re-converged steady
+    states (step 1e-3) to 2e-5 of the larger derivative."""
+    from meshcfd_amd.steady_adjoint import SteadyAdjoint, free_stream_derivatives, objective_gradient
+
+    print("case: laminar cylinder drag gradients", flush=True)
+    mach, alpha = 0.3, 5.0
+    dm, force, j0, hist = _steady_force(mach, alpha, laminar=True)
+    fp, fv = force.components(dm.u)
+    print("steady", len(hist), "steps, rms", hist[-1], "drag", j0, "pressure", fp.tolist(), "viscous", fv.tolist())
+    assert abs(float(fv[0])) > 0.1 * abs(j0)
+    u_star = dm.state()
+    sa = SteadyAdjoint(dm, cfl=1e3, rtol=1e-4)
+    _, dj_du = objective_gradient(force, dm.u)
+    lam, ahist = sa.solve(dj_du, tol=1e-11, max_iterations=80,
+                          progress=lambda k, rel: print(f"  adjoint iteration {k}: {rel:.3e}", flush=True))
+    assert ahist[-1] <= 1e-11, ahist[-5:]
+    sbar = sa.inflow_gradient(lam)
+    d_mach, d_alpha = free_stream_derivatives(mach, alpha, 1.4, 1.0, 1.0 / 1.4)
+    g_mach = float(sbar @ d_mach)
+    g_alpha = float(sbar @ d_alpha) + force.d_alpha(dm.u)
+    print("adjoint", g_mach, g_alpha, "iterations", len(ahist), "gmres", sa.gmres_iterations)
+    dm.close()
+    h = 1e-3
+    fd = []
+    for dmach, dalpha in ((h, 0.0), (-h, 0.0), (0.0, math.degrees(h)), (0.0, -math.degrees(h))):
+        d2, _, j, _ = _steady_force(mach + dmach, alpha + dalpha, warm=u_star, laminar=True)
+        d2.close()
+        fd.append(j)
+    fd_mach = (fd[0] - fd[1]) / (2 * h)
+    fd_alpha = (fd[2] - fd[3]) / (2 * h)
+    print("fd", fd_mach, fd_alpha)
+    scale = max(abs(fd_mach), abs(fd_alpha))
+    assert abs(g_mach - fd_mach) <= 2e-5 * scale
+    assert abs(g_alpha - fd_alpha) <= 2e-5 * scale
