@@ -507,6 +507,14 @@ int mcfd_batch_time_steps(McfdMesh* mesh, const McfdSettings* settings, const do
                           int32_t n_cases, const int32_t* case_ptr, double end_time,
                           double* clock, double* dt_cell, void* stream);
 
+/* The per-case part of mcfd_batch_time_steps alone, when dt_cell already
+ * holds the per-cell stable steps (written by the final fused stage of the
+ * previous step through McfdStage.dt_next): per case dt = min(case minimum,
+ * end_time - time_c), time_c += dt, dt written to every cell of the case.
+ * No counterpart in the reference (one case per run). */
+int mcfd_batch_case_steps(McfdMesh* mesh, int32_t n_cases, const int32_t* case_ptr, double end_time,
+                          double* clock, double* dt_cell, void* stream);
+
 /* Per-component sum of R^2 over cells into sums[4] (device), deterministic. */
 int mcfd_sum_squares(McfdMesh* mesh, const double* r, double* sums, void* stream);
 

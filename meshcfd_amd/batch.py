@@ -34,17 +34,20 @@ class BatchSpec:
 
 def batch_specs(count: int = 256, seed: int = 2, cells=(2e3, 2e5)) -> list[BatchSpec]:
     """Cases alternate vortex / shock tube; cell counts log-uniform in
-    [cells[0], cells[1]] and inflow Mach uniform in [0.1, 2.0] (seed 2)."""
+    [cells[0], cells[1]] and inflow Mach uniform in [0.1, 2.0] (seed 2).
+    A mesh of q quads has 1.3 q cells after its 30% triangle split, so the
+    quad counts aim at target / 1.3."""
     rng = np.random.default_rng(seed)
     target = np.exp(rng.uniform(np.log(cells[0]), np.log(cells[1]), count))
     mach = rng.uniform(0.1, 2.0, count)
+    quads = target / 1.3
     specs = []
     for k in range(count):
         if k % 2 == 0:
-            n = max(4, int(round(math.sqrt(target[k]))))
+            n = max(4, int(round(math.sqrt(quads[k]))))
             specs.append(BatchSpec("vortex", n, n, float(mach[k]), 5.0))
         else:
-            m = max(2, int(round(math.sqrt(target[k] / 10.0))))
+            m = max(2, int(round(math.sqrt(quads[k] / 10.0))))
             specs.append(BatchSpec("shock tube", 10 * m, m, float(mach[k]), 0.0))
     return specs
 

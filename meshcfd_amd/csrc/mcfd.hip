@@ -5766,6 +5766,20 @@ int mcfd_batch_time_steps(McfdMesh* m, const McfdSettings* st, const double* u, 
   return 0;
 }
 
+int mcfd_batch_case_steps(McfdMesh* m, int32_t n_cases, const int32_t* case_ptr, double end_time,
+                          double* clock, double* dt_cell, void* stream) {
+  if (!m || !case_ptr || !clock || !dt_cell || n_cases <= 0)
+    return fail("mcfd_batch_case_steps: NULL argument or no cases");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  {
+    TimedLaunch t(m, s, MCFD_KERNEL_REDUCE);
+    hipLaunchKernelGGL(k_batch_dt, dim3(n_cases), dim3(1024), 0, s, n_cases, case_ptr, end_time,
+                       clock, dt_cell, m->status);
+  }
+  HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
 int mcfd_status(McfdMesh* m, void* stream, int32_t* mask, int32_t* index) {
   if (!m) return fail("mcfd_status: NULL mesh");
   hipStream_t s = static_cast<hipStream_t>(stream);

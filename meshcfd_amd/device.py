@@ -380,12 +380,20 @@ class DeviceMesh:
         if time is not None:
             self.clock[0] = time
         if self.case_ptr is not None:
-            native.check(
-                self.lib.mcfd_batch_time_steps(
-                    self.handle, C.byref(self.native_settings), _ptr(self.u), self.n_cases, _ptr(self.case_ptr),
-                    s.end_time, _ptr(self.case_clock), _ptr(self.dt_cell), self.stream,
+            if self.dt_fresh:  # the previous final stage wrote the per-cell steps
+                native.check(
+                    self.lib.mcfd_batch_case_steps(
+                        self.handle, self.n_cases, _ptr(self.case_ptr), s.end_time, _ptr(self.case_clock),
+                        _ptr(self.dt_cell), self.stream,
+                    )
                 )
-            )
+            else:
+                native.check(
+                    self.lib.mcfd_batch_time_steps(
+                        self.handle, C.byref(self.native_settings), _ptr(self.u), self.n_cases, _ptr(self.case_ptr),
+                        s.end_time, _ptr(self.case_clock), _ptr(self.dt_cell), self.stream,
+                    )
+                )
             dt, per_cell = self.dt_cell, True
         elif local:
             if not self.dt_fresh:
@@ -409,12 +417,13 @@ class DeviceMesh:
         u = self.u
         # the fused final stage also computes the next step's CFL steps
         fuse_dt = (
-            self.fuse_time_step and self.blocks is not None and self.case_ptr is None and mode != "Global fixed"
+            self.fuse_time_step and self.blocks is not None and mode != "Global fixed"
             and self.settings.viscosity_model == "Inviscid" and self.settings.reconstruction != "MUSCL parabolic"
         )
         nxt = {}
-        if fuse_dt:  # the global modes need only the minimum
-            nxt = dict(dt_next=self.dt_cell, dt_min=self.dt_min) if local else dict(dt_min=self.dt_min)
+        if fuse_dt:  # the global modes need only the minimum; batches and local steps the per-cell steps
+            per_cell_next = local or self.case_ptr is not None
+            nxt = dict(dt_next=self.dt_cell, dt_min=self.dt_min) if per_cell_next else dict(dt_min=self.dt_min)
         if ti == "Explicit Euler":
             self._stage(native.OP_EULER, u, out=self.s1, dt=dt, per_cell=per_cell, coeffs=(1.0,), r_out=residual_out,
                         **nxt)
