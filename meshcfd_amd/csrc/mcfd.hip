@@ -1455,7 +1455,7 @@ __device__ __forceinline__ P4 rec_lds(const double* sW, const double* sG, const 
 // kernel is ~48 KB, most of a CU pair's instruction cache).
 template <int SCHEME, bool MUSCL, int LIM, bool FARFIELD, bool DTNEXT>
 __global__ void __launch_bounds__(kBlock)
-    k_stage_fused(int max_acc, int max_local, int max_grad, int max_faces, BlockArrays ba,
+    k_stage_fused(int max_acc, int stage_off, int max_local, int max_grad, int max_faces, BlockArrays ba,
                   GeomArrays ga, FaceArrays fa, DtArrays da, double g, double ax, double ay,
                   StageArgs s, int* __restrict__ status) {
   constexpr int limiter = LIM;
@@ -1478,8 +1478,10 @@ __global__ void __launch_bounds__(kBlock)
   double* sF = sG;
   double* sL = sF + 4 * max_faces;
   double* sN = sL + max_faces;  // face normals (final stage with the next step's dt)
-  // own cells' accumulation coefficients and items, and spectral-radius items
-  double* sAC = sG + std::max(std::max(7 * max_faces, MUSCL ? 12 * max_grad : 0), kBlock);
+  // own cells' accumulation coefficients and items, and spectral-radius
+  // items, for P3: written after P2's last read of W and the gradients, over
+  // W when they fit there (stage_off 0), else after the shared region
+  double* sAC = smem + stage_off;
   unsigned short* sAI = reinterpret_cast<unsigned short*>(sAC + max_acc);
   unsigned short* sDI = sAI + max_acc;
   const double* __restrict__ U = s.u;
@@ -1665,24 +1667,55 @@ __global__ void __launch_bounds__(kBlock)
       if (s.op != MCFD_OP_EULER) z2 = reinterpret_cast<const double2*>(s.u0 + 4 * (long)i)[hf];
     }
   }
-  // the block's accumulation (and spectral-radius) items are contiguous in
-  // the global lists: staged with coalesced loads, so that P3's loop reads
-  // LDS instead of a chain of dependent global loads
+  // The block's accumulation (and spectral-radius) items are contiguous in
+  // the global lists: loaded here with coalesced loads (up to kStageItems
+  // per thread in registers), stored to LDS after the barrier, so that P3's
+  // loop reads LDS instead of a chain of dependent global loads.
+  constexpr int kStageItems = 3;
   const int abase = ga.acc_ptr[first];
-  {
-    const int n_acc = ga.acc_ptr[first + n_own] - abase;
-    for (int k = t; k < n_acc; k += kBlock) {
-      sAC[k] = ga.acc_coef[abase + k];
-      sAI[k] = ba.acc_local[abase + k];
-    }
+  const int n_acc = ga.acc_ptr[first + n_own] - abase;
+  double ac_r[kStageItems];
+  unsigned short ai_r[kStageItems];
+#pragma unroll
+  for (int q = 0; q < kStageItems; ++q) {
+    const int k = t + q * kBlock;
+    ac_r[q] = (k < n_acc) ? ga.acc_coef[abase + k] : 0.0;
+    ai_r[q] = (k < n_acc) ? ba.acc_local[abase + k] : 0;
   }
-  int dbase = 0;
+  int dbase = 0, n_dts = 0;
+  unsigned short di_r[kStageItems];
   if (DTNEXT) {
     dbase = da.ptr[first];
-    const int n_dts = da.ptr[first + n_own] - dbase;
-    for (int k = t; k < n_dts; k += kBlock) sDI[k] = ba.dts_local[dbase + k];
+    n_dts = da.ptr[first + n_own] - dbase;
+#pragma unroll
+    for (int q = 0; q < kStageItems; ++q) {
+      const int k = t + q * kBlock;
+      di_r[q] = (k < n_dts) ? ba.dts_local[dbase + k] : 0;
+    }
   }
-  if constexpr (MUSCL) __syncthreads();
+  // own cell state for the body force (W is overwritten by the staging)
+  double bw_r = 0.0, bw_u = 0.0, bw_v = 0.0;
+  if ((ax != 0.0 || ay != 0.0) && j3 < n_own) {
+    bw_r = sW[4 * j3];
+    bw_u = sW[4 * j3 + 1];
+    bw_v = sW[4 * j3 + 2];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kStageItems; ++q) {
+    const int k = t + q * kBlock;
+    if (k < n_acc) {
+      sAC[k] = ac_r[q];
+      sAI[k] = ai_r[q];
+    }
+    if (DTNEXT && k < n_dts) sDI[k] = di_r[q];
+  }
+  for (int k = t + kStageItems * kBlock; k < n_acc; k += kBlock) {
+    sAC[k] = ga.acc_coef[abase + k];
+    sAI[k] = ba.acc_local[abase + k];
+  }
+  if (DTNEXT)
+    for (int k = t + kStageItems * kBlock; k < n_dts; k += kBlock) sDI[k] = ba.dts_local[dbase + k];
   if (f < n_faces) {
     sF[4 * f] = fl.a;
     sF[4 * f + 1] = fl.b;
@@ -1723,12 +1756,11 @@ __global__ void __launch_bounds__(kBlock)
       }
     }
     if (ax != 0.0 || ay != 0.0) {
-      P4 w = lds4(sW, j3);
       if (hf == 0) {
-        rb = rb + w.r * ax;
+        rb = rb + bw_r * ax;
       } else {
-        ra = ra + w.r * ay;
-        rb = rb + w.r * (ax * w.u + ay * w.v);
+        ra = ra + bw_r * ay;
+        rb = rb + bw_r * (ax * bw_u + ay * bw_v);
       }
     }
     if (!(finite(ra) && finite(rb))) report(status, MCFD_ERR_RESIDUAL, i);
@@ -4257,13 +4289,27 @@ int upload_block(McfdMesh* m, T** dst, const T* src, size_t count) {
 // and then (face fluxes, lengths, normals) and the final block reduction,
 // then the own cells' accumulation coefficients and items and their
 // spectral-radius items, staged for the gather.
-size_t fused_lds_bytes(const McfdMesh* m, bool muscl) {
+// The staging is written after P2, over W when it fits there.
+size_t fused_region(const McfdMesh* m, bool muscl) {
   size_t region = 7 * (size_t)m->max_faces;
   if (muscl) region = std::max(region, 12 * (size_t)m->max_grad);
-  region = std::max(region, (size_t)kBlock);
-  size_t doubles = 4 * (size_t)m->max_local + region + (size_t)m->max_acc;
+  return std::max(region, (size_t)kBlock);
+}
+
+size_t fused_stage_doubles(const McfdMesh* m) {
   size_t shorts = (size_t)m->max_acc + (size_t)m->max_dts;
-  return doubles * sizeof(double) + ((shorts * sizeof(unsigned short) + 7) & ~(size_t)7);
+  return (size_t)m->max_acc + (shorts * sizeof(unsigned short) + 7) / 8;
+}
+
+int fused_stage_offset(const McfdMesh* m, bool muscl) {
+  if (fused_stage_doubles(m) <= 4 * (size_t)m->max_local) return 0;
+  return (int)(4 * (size_t)m->max_local + fused_region(m, muscl));
+}
+
+size_t fused_lds_bytes(const McfdMesh* m, bool muscl) {
+  size_t doubles = 4 * (size_t)m->max_local + fused_region(m, muscl);
+  if (fused_stage_offset(m, muscl) != 0) doubles += fused_stage_doubles(m);
+  return doubles * sizeof(double);
 }
 
 void free_implicit(McfdMesh* m) {
@@ -4437,7 +4483,7 @@ void launch_fused_t(McfdMesh* m, size_t lds, hipStream_t s, const GeomArrays& ga
   auto kern = k_stage_fused<SCHEME, MUSCL, LIM, FARFIELD, DTNEXT>;
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                             hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024);
-  hipLaunchKernelGGL(kern, dim3(m->nb), dim3(kBlock), lds, s, m->max_acc, m->max_local,
+  hipLaunchKernelGGL(kern, dim3(m->nb), dim3(kBlock), lds, s, m->max_acc, fused_stage_offset(m, MUSCL), m->max_local,
                      m->max_grad, m->max_faces, m->blocks, ga, m->faces, m->dts, st->gamma,
                      st->acceleration_x, st->acceleration_y, a, m->status);
 }
