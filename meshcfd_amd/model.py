@@ -55,6 +55,29 @@ class MeshPatch:
     face_ids: np.ndarray
 
 
+@dataclass(frozen=True)
+class BoundaryCurve:
+    """A boundary face's piece of a MeshCAD curve (reference `model.py`
+    BoundaryCurve): the curve descriptor (line / arc / spline /
+    spline_segment, evaluated by `curves.evaluate_boundary_curve`) and the
+    parameters of the face's two nodes."""
+
+    geometry: dict
+    start_parameter: float
+    end_parameter: float
+    patch_id: int
+
+
+@dataclass(frozen=True)
+class RefinementRecord:
+    """One refined parent cell (reference `model.py` RefinementRecord)."""
+
+    parent_id: str
+    child_ids: tuple
+    vertices: tuple
+    level: int
+
+
 @dataclass
 class FiniteVolumeMesh:
     """Polygon mesh (reference `model.py:60-74`) with array storage.
@@ -72,6 +95,16 @@ class FiniteVolumeMesh:
     face_neighbours: np.ndarray  # (F,) int64
     face_patch_ids: np.ndarray  # (F,) int64, -1 on interior faces
     patches: tuple[MeshPatch, ...]
+    # the file's cell -> face lists (padded with -1) and signs (+1: the face's
+    # node order runs along the cell, -1: against it), kept so that a mesh
+    # read from a MeshCAD file is written back as it was; None for meshes
+    # built in memory (the writer derives them)
+    cell_faces: np.ndarray | None = None
+    cell_face_signs: np.ndarray | None = None
+    cell_ids: tuple = ()
+    cell_levels: tuple = ()
+    refinement_records: tuple = ()
+    face_curves: tuple = ()  # per face: BoundaryCurve or None; () for none
 
     @property
     def cell_count(self) -> int:
@@ -148,3 +181,7 @@ class InitialState:
     velocity_x: float = 0.0
     velocity_y: float = 0.0
     pressure: float = 101325.0
+    # carried (and written to VTU files) with the reference's defaults; the
+    # turbulence model itself is outside this package
+    turbulence_k: float = 1e-6
+    turbulence_omega: float = 1.0
