@@ -162,17 +162,60 @@ class NumericalSettings:
     prandtl_number: float = 0.72
     acceleration_x: float = 0.0
     acceleration_y: float = 0.0
+    # carried by case files with the reference's defaults; adaptation and the
+    # SST model are outside this package (solve_case rejects a turbulence
+    # model other than "None" and ignores the adaptation settings)
+    adaptive_mesh: bool = False
+    adaptation_interval: int = 20
+    adaptation_max_level: int = 3
+    adaptation_refine_threshold: float = 0.25
+    adaptation_variable: str = "Density"
+    turbulence_model: str = "None"
+    turbulence_wall_treatment: str = "Wall-resolved"
+    turbulence_omega_formulation: str = "Standard SST"
+
+
+@dataclass
+class InitialConditionBlock:
+    """A region (boolean expression of the centroid x, y) and the field
+    expressions set there (reference `model.py` InitialConditionBlock)."""
+
+    name: str = "Block 1"
+    domain: str = "True"
+    density: str = "1.225"
+    velocity_x: str = "0"
+    velocity_y: str = "0"
+    pressure: str = "101325"
+
+
+@dataclass
+class InitialConditionSettings:
+    base_state: "InitialState" = None
+    blocks: list = field(default_factory=list)
+
+    def __post_init__(self):
+        if self.base_state is None:
+            self.base_state = InitialState()
 
 
 @dataclass
 class Case:
-    """A mesh with its boundary conditions, periodic pairs and numerics."""
+    """A mesh with its boundary conditions, periodic pairs and numerics; the
+    remaining fields are what a case file carries besides (the reference
+    `model.py` Case)."""
 
     mesh: FiniteVolumeMesh
     boundary_conditions: list[BoundaryCondition]
     periodic_pairs: list[PeriodicBoundaryPair] = field(default_factory=list)
     numerics: NumericalSettings = field(default_factory=NumericalSettings)
     name: str = "case"
+    mesh_path: object = None
+    initial_conditions: InitialConditionSettings = field(default_factory=InitialConditionSettings)
+    final_solution: object = None  # tuple of InitialState, or None
+    solution_time: float = 0.0
+    solution_steps: int = 0
+    solution_cpu_time: float = 0.0
+    convergence_history: list = field(default_factory=list)
 
 
 @dataclass(frozen=True)

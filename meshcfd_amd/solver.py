@@ -92,6 +92,10 @@ def validate(case: Case, start_time: float, step_offset: int, solution_interval:
     if case.mesh is None:
         raise SolverError("Import a mesh before running the solver.")
     s = case.numerics
+    if getattr(s, "turbulence_model", "None") != "None":
+        raise SolverError("The Menter k-omega SST model is not implemented by the GPU path.")
+    if getattr(s, "adaptive_mesh", False):
+        raise SolverError("Adaptive mesh refinement is not implemented by the GPU path.")
     if s.reconstruction not in ("First order", "MUSCL linear", "MUSCL parabolic"):
         raise SolverError(f"Unsupported reconstruction '{s.reconstruction}'.")
     if s.limiter not in ("Barth-Jespersen", "Venkatakrishnan", "Minmod", "Van Leer"):
