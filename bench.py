@@ -175,6 +175,10 @@ def cylinder_bench(args) -> int:
     rho, u, v, p = prim.T
     dm.set_state(np.column_stack((rho, rho * u, rho * v, p / (gamma - 1.0) + 0.5 * rho * (u**2 + v**2))))
     # first step: warm-up (kernel loading, GMRES buffers)
+    from meshcfd_amd.implicit import ImplicitStepper
+
+    dm._implicit = ImplicitStepper(dm)
+    dm._implicit.on_newton = lambda k, g, r: note(f"  Newton iteration {k}: {g} GMRES iterations, |G| {r:.3e}")
     t = dm.step_implicit(0.0)
     r0 = residual_rms(dm)
     note(f"warm-up step done, rms {r0:.3e}")

@@ -1853,6 +1853,23 @@ struct LusgsArrays {
   double* coef;          // per record
   double* diag;          // per cell
   double* off_val;       // per off item
+  // The sweeps' data in level-position order (position k of a sweep is
+  // lvl_cells[k]): per position the number of items, and the first kf / kb
+  // items' column cells and coefficients (normal and transposed), padded;
+  // the diagonal in position order. Filled by k_lusgs_pack after assembly.
+  int kf, kb;
+  const int* pcnt_f;
+  const int* pcnt_b;
+  const int* pnb_f;      // [n * kf] column cell, -1 padding
+  const int* pnb_b;      // [n * kb]
+  const int* pit_f;      // [n * kf] the off item, -1 padding
+  const int* pit_b;
+  double* pval_f;        // [n * kf] coefficients
+  double* pval_ft;       // transposed coefficients
+  double* pval_b;
+  double* pval_bt;
+  double* pdiag_f;       // [n]
+  double* pdiag_b;
 };
 
 // Per record: 0.5*max(|u_o.n_o|+c_o, |u_n.n_n|+c_n)*L for face pairs, the
@@ -1895,6 +1912,24 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// Coefficients and diagonal in the sweeps' position order (after assembly).
+__global__ void __launch_bounds__(kBlock) k_lusgs_pack(int n, LusgsArrays la) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  la.pdiag_f[k] = la.diag[la.lvl_cells_f[k]];
+  la.pdiag_b[k] = la.diag[la.lvl_cells_b[k]];
+  for (int q = 0; q < la.kf; ++q) {
+    int e = la.pit_f[(long)k * la.kf + q];
+    la.pval_f[(long)k * la.kf + q] = e >= 0 ? la.off_val[e] : 0.0;
+    la.pval_ft[(long)k * la.kf + q] = e >= 0 ? la.off_val[la.off_t[e]] : 0.0;
+  }
+  for (int q = 0; q < la.kb; ++q) {
+    int e = la.pit_b[(long)k * la.kb + q];
+    la.pval_b[(long)k * la.kb + q] = e >= 0 ? la.off_val[e] : 0.0;
+    la.pval_bt[(long)k * la.kb + q] = e >= 0 ? la.off_val[la.off_t[e]] : 0.0;
+  }
+}
+
 // Level-scheduled triangular sweeps in one workgroup: every cell of a level
 // depends only on cells of earlier levels, so the sweep equals the
 // sequential natural-order substitution. Forward: y = L^-1 v. Backward (in
@@ -1903,51 +1938,120 @@ __global__ void __launch_bounds__(kBlock)
 // solves U^T x = v and the backward sweep L^T z = D x, over the same lower /
 // upper items with the coefficient of the column cell's item for the same
 // record, S[j][i] = off_val[te[e]]. Same levels: S is structurally symmetric.
-__global__ void __launch_bounds__(1024)
-    k_lusgs_forward(LusgsArrays la, const double* __restrict__ v, double* __restrict__ y,
-                    const int* __restrict__ te) {
-  for (int lev = 0; lev < la.n_levels_f; ++lev) {
-    int a = la.lvl_ptr_f[lev], b = la.lvl_ptr_f[lev + 1];
-    for (int idx = a + threadIdx.x; idx < b; idx += blockDim.x) {
-      int i = la.lvl_cells_f[idx];
-      P4 s = load4(v, i);
-      for (int e = la.off_ptr[i]; e < la.off_split[i]; ++e) {
-        double c = te ? la.off_val[te[e]] : la.off_val[e];
-        P4 o = load4(y, la.off_nb[e]);
-        s.r = s.r - c * o.r;
-        s.u = s.u - c * o.u;
-        s.v = s.v - c * o.v;
-        s.p = s.p - c * o.p;
-      }
-      double d = la.diag[i];
-      store4(y, i, s.r / d, s.u / d, s.v / d, s.p / d);
-    }
-    __threadfence();
-    __syncthreads();
+//
+// The per-position data (cell, item count, diagonal, own value, the first
+// kLusgsK items' column cells and coefficients) of the next level do not
+// depend on the sweep, so they are loaded while the current level computes
+// and waits at the barrier; a level then costs one round trip for the
+// column values written by earlier levels. Thread t owns positions a + t and
+// a + t + 1024 of a level in registers; further positions (levels of more
+// than 2048 cells) and items beyond kLusgsK take the direct path.
+constexpr int kLusgsK = 4;
+constexpr int kLusgsSlots = 2;
+
+struct LusgsSlot {
+  int cell, cnt;
+  double d;
+  P4 v;
+  int nb[kLusgsK];
+  double c[kLusgsK];
+};
+
+template <bool FORWARD>
+__device__ __forceinline__ void lusgs_fetch(const LusgsArrays& la, int k, int K, const int* cells,
+                                            const int* cnt, const int* pnb, const double* pval,
+                                            const double* pdiag, const double* own, LusgsSlot& sl) {
+  sl.cell = cells[k];
+  sl.cnt = cnt[k];
+  sl.d = pdiag[k];
+  sl.v = load4(own, sl.cell);
+#pragma unroll
+  for (int q = 0; q < kLusgsK; ++q) {
+    bool ok = q < K;
+    sl.nb[q] = ok ? pnb[(long)k * K + q] : -1;
+    sl.c[q] = ok ? pval[(long)k * K + q] : 0.0;
   }
 }
 
-__global__ void __launch_bounds__(1024)
-    k_lusgs_backward(LusgsArrays la, double* __restrict__ x, const int* __restrict__ te) {
-  for (int lev = 0; lev < la.n_levels_b; ++lev) {
-    int a = la.lvl_ptr_b[lev], b = la.lvl_ptr_b[lev + 1];
-    for (int idx = a + threadIdx.x; idx < b; idx += blockDim.x) {
-      int i = la.lvl_cells_b[idx];
-      double d = la.diag[i];
-      double dm = vmax(d, 1e-14);
-      P4 y = load4(x, i);
-      P4 s{dm * y.r, dm * y.u, dm * y.v, dm * y.p};
-      for (int e = la.off_split[i]; e < la.off_ptr[i + 1]; ++e) {
-        double c = te ? la.off_val[te[e]] : la.off_val[e];
-        P4 o = load4(x, la.off_nb[e]);
-        s.r = s.r - c * o.r;
-        s.u = s.u - c * o.u;
-        s.v = s.v - c * o.v;
-        s.p = s.p - c * o.p;
-      }
-      store4(x, i, s.r / d, s.u / d, s.v / d, s.p / d);
+template <bool FORWARD>
+__device__ __forceinline__ void lusgs_finish(const LusgsArrays& la, const LusgsSlot& sl, int K,
+                                             const double* src, double* dst, const int* te) {
+  P4 s;
+  if (FORWARD) {
+    s = sl.v;
+  } else {
+    double dm = vmax(sl.d, 1e-14);
+    s = P4{dm * sl.v.r, dm * sl.v.u, dm * sl.v.v, dm * sl.v.p};
+  }
+  const int lim = K < kLusgsK ? K : kLusgsK;
+#pragma unroll
+  for (int q = 0; q < kLusgsK; ++q) {
+    if (q >= lim || q >= sl.cnt) break;
+    double c = sl.c[q];
+    P4 o = load4(src, sl.nb[q]);
+    s.r = s.r - c * o.r;
+    s.u = s.u - c * o.u;
+    s.v = s.v - c * o.v;
+    s.p = s.p - c * o.p;
+  }
+  if (sl.cnt > lim) {  // items beyond the packed ones, in item order
+    const int i = sl.cell;
+    const int e0 = (FORWARD ? la.off_ptr[i] : la.off_split[i]) + lim;
+    const int e1 = FORWARD ? la.off_split[i] : la.off_ptr[i + 1];
+    for (int e = e0; e < e1; ++e) {
+      double c = te ? la.off_val[te[e]] : la.off_val[e];
+      P4 o = load4(src, la.off_nb[e]);
+      s.r = s.r - c * o.r;
+      s.u = s.u - c * o.u;
+      s.v = s.v - c * o.v;
+      s.p = s.p - c * o.p;
     }
-    __threadfence();
+  }
+  const double d = sl.d;
+  store4(dst, sl.cell, s.r / d, s.u / d, s.v / d, s.p / d);
+}
+
+// Forward: own = v, src = dst = y. Backward: own = src = dst = x (in place).
+template <bool FORWARD>
+__global__ void __launch_bounds__(1024)
+    k_lusgs_sweep(LusgsArrays la, const double* __restrict__ own, double* y, const int* __restrict__ te) {
+  const int nl = FORWARD ? la.n_levels_f : la.n_levels_b;
+  const int* ptr = FORWARD ? la.lvl_ptr_f : la.lvl_ptr_b;
+  const int* cells = FORWARD ? la.lvl_cells_f : la.lvl_cells_b;
+  const int* cnt = FORWARD ? la.pcnt_f : la.pcnt_b;
+  const int* pnb = FORWARD ? la.pnb_f : la.pnb_b;
+  const int K = FORWARD ? la.kf : la.kb;
+  const double* pval = FORWARD ? (te ? la.pval_ft : la.pval_f) : (te ? la.pval_bt : la.pval_b);
+  const double* pdiag = FORWARD ? la.pdiag_f : la.pdiag_b;
+  const int t = threadIdx.x, T = blockDim.x;
+  LusgsSlot nxt[kLusgsSlots];
+  int a = ptr[0], b = ptr[1];
+#pragma unroll
+  for (int q = 0; q < kLusgsSlots; ++q)
+    if (a + t + q * T < b) lusgs_fetch<FORWARD>(la, a + t + q * T, K, cells, cnt, pnb, pval, pdiag, own, nxt[q]);
+  for (int lev = 0; lev < nl; ++lev) {
+    LusgsSlot cur[kLusgsSlots];
+#pragma unroll
+    for (int q = 0; q < kLusgsSlots; ++q) cur[q] = nxt[q];
+    const int ca = a, cb = b;
+    if (lev + 1 < nl) {
+      a = cb;
+      b = ptr[lev + 2];
+#pragma unroll
+      for (int q = 0; q < kLusgsSlots; ++q)
+        if (a + t + q * T < b) lusgs_fetch<FORWARD>(la, a + t + q * T, K, cells, cnt, pnb, pval, pdiag, own, nxt[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < kLusgsSlots; ++q)
+      if (ca + t + q * T < cb) lusgs_finish<FORWARD>(la, cur[q], K, y, y, te);
+    for (int k = ca + t + kLusgsSlots * T; k < cb; k += T) {
+      LusgsSlot sl;
+      lusgs_fetch<FORWARD>(la, k, K, cells, cnt, pnb, pval, pdiag, own, sl);
+      lusgs_finish<FORWARD>(la, sl, K, y, y, te);
+    }
+    // workgroup-scope release / acquire: the one workgroup's waves share
+    // the CU's write-through L1, so the next level reads these values (an
+    // agent-scope fence would write L2 back at every level)
     __syncthreads();
   }
 }
@@ -3561,7 +3665,6 @@ __global__ void __launch_bounds__(kBlock)
   }
   P4 w = load4(W, i);
   double T = w.p / (w.r * vp.R);
-  double val[3] = {w.u, w.v, T};
   const double* vg = VG + 8 * (long)i;
   double mu = vg[6], kc = vg[7];
   const double* m = va.wl_cell + 4 * (long)i;
@@ -4616,8 +4719,8 @@ int lblocks(long n) { return (int)((n + kBlock - 1) / kBlock); }
 
 int lusgs_apply(McfdMesh* m, const double* v, double* out, hipStream_t s, bool transpose = false) {
   const int* te = transpose ? m->lusgs.off_t : nullptr;
-  hipLaunchKernelGGL(k_lusgs_forward, dim3(1), dim3(1024), 0, s, m->lusgs, v, out, te);
-  hipLaunchKernelGGL(k_lusgs_backward, dim3(1), dim3(1024), 0, s, m->lusgs, out, te);
+  hipLaunchKernelGGL(k_lusgs_sweep<true>, dim3(1), dim3(1024), 0, s, m->lusgs, v, out, te);
+  hipLaunchKernelGGL(k_lusgs_sweep<false>, dim3(1), dim3(1024), 0, s, m->lusgs, out, out, te);
   HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -5329,7 +5432,33 @@ int mcfd_mesh_set_implicit(McfdMesh* m, const McfdImplicitDesc* d) {
   std::vector<int> pf, cf, pb, cb;
   bucket(lev_f, nlf, pf, cf);
   bucket(lev_b, nlb, pb, cb);
+  // per position: item count and the first kf / kb items (column cell, item)
+  int kf = 1, kb = 1;
+  for (int i = 0; i < n; ++i) {
+    kf = std::max(kf, d->off_split[i] - d->off_ptr[i]);
+    kb = std::max(kb, d->off_ptr[i + 1] - d->off_split[i]);
+  }
+  kf = std::min(kf, kLusgsK);
+  kb = std::min(kb, kLusgsK);
+  std::vector<int> cnt_f(n), cnt_b(n), nb_f((size_t)n * kf, -1), it_f((size_t)n * kf, -1),
+      nb_b((size_t)n * kb, -1), it_b((size_t)n * kb, -1);
+  for (int k = 0; k < n; ++k) {
+    int i = cf[k];
+    cnt_f[k] = d->off_split[i] - d->off_ptr[i];
+    for (int q = 0; q < kf && q < cnt_f[k]; ++q) {
+      nb_f[(size_t)k * kf + q] = d->off_nb[d->off_ptr[i] + q];
+      it_f[(size_t)k * kf + q] = d->off_ptr[i] + q;
+    }
+    int j = cb[k];
+    cnt_b[k] = d->off_ptr[j + 1] - d->off_split[j];
+    for (int q = 0; q < kb && q < cnt_b[k]; ++q) {
+      nb_b[(size_t)k * kb + q] = d->off_nb[d->off_split[j] + q];
+      it_b[(size_t)k * kb + q] = d->off_split[j] + q;
+    }
+  }
   LusgsArrays& la = m->lusgs;
+  la.kf = kf;
+  la.kb = kb;
   la.n_rec = d->n_records;
   la.n_levels_f = nlf;
   la.n_levels_b = nlb;
@@ -5356,6 +5485,18 @@ int mcfd_mesh_set_implicit(McfdMesh* m, const McfdImplicitDesc* d) {
   rc |= upload_implicit(m, &la.coef, (const double*)nullptr, R);
   rc |= upload_implicit(m, &la.diag, (const double*)nullptr, (size_t)n);
   rc |= upload_implicit(m, &la.off_val, (const double*)nullptr, NO);
+  rc |= upload_implicit(m, const_cast<int**>(&la.pcnt_f), cnt_f.data(), (size_t)n);
+  rc |= upload_implicit(m, const_cast<int**>(&la.pcnt_b), cnt_b.data(), (size_t)n);
+  rc |= upload_implicit(m, const_cast<int**>(&la.pnb_f), nb_f.data(), nb_f.size());
+  rc |= upload_implicit(m, const_cast<int**>(&la.pnb_b), nb_b.data(), nb_b.size());
+  rc |= upload_implicit(m, const_cast<int**>(&la.pit_f), it_f.data(), it_f.size());
+  rc |= upload_implicit(m, const_cast<int**>(&la.pit_b), it_b.data(), it_b.size());
+  rc |= upload_implicit(m, &la.pval_f, (const double*)nullptr, nb_f.size());
+  rc |= upload_implicit(m, &la.pval_ft, (const double*)nullptr, nb_f.size());
+  rc |= upload_implicit(m, &la.pval_b, (const double*)nullptr, nb_b.size());
+  rc |= upload_implicit(m, &la.pval_bt, (const double*)nullptr, nb_b.size());
+  rc |= upload_implicit(m, &la.pdiag_f, (const double*)nullptr, (size_t)n);
+  rc |= upload_implicit(m, &la.pdiag_b, (const double*)nullptr, (size_t)n);
   if (rc) {
     std::string msg = g_last_error;
     free_implicit(m);
@@ -5378,6 +5519,7 @@ int mcfd_lusgs_assemble(McfdMesh* m, const McfdSettings* st, const double* u, co
                      m->W, st->gamma);
   hipLaunchKernelGGL(k_lusgs_assemble, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->lusgs, m->area,
                      dt, dt_per_cell);
+  hipLaunchKernelGGL(k_lusgs_pack, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->lusgs);
   HIP_CHECK(hipGetLastError());
   return 0;
 }

@@ -223,6 +223,9 @@ class ImplicitStepper:
         self.norms = torch.zeros((2,), **opts)
         self.gmres_iterations = 0
         self.newton_iterations = 0
+        # called after each Newton iteration's GMRES solve with (Newton
+        # iterations so far, GMRES iterations of this solve, residual norm)
+        self.on_newton = None
         del s
 
     # -- pieces -----------------------------------------------------------
@@ -307,6 +310,8 @@ class ImplicitStepper:
             args = self.gmres_args(state, self.rate1, dt, per_cell, sigma, rhs=self.rhs, x=self.x)
             native.check(self.lib.mcfd_gmres(self.dm.handle, C.byref(self.first_settings), C.byref(args), self.dm.stream))
             self.gmres_iterations += args.iterations
+            if self.on_newton is not None:
+                self.on_newton(self.newton_iterations, args.iterations, norm)
             if args.info != 0:
                 raise SolverError(
                     f"Implicit Euler GMRES did not converge within {s.gmres_max_iterations} iterations "

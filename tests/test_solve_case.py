@@ -124,3 +124,31 @@ def test_gpu_solve_case_converges_the_steady_bump(cuda_device, integrator):
     print("steps", steps, "residual", residual)
     assert completed and residual <= tol
     assert math.sqrt(sum(r * r for r in seen[-1].residuals) / 4) <= tol
+
+
+@pytest.mark.gpu
+def test_gpu_solve_case_from_a_case_file(cuda_device, tmp_path):
+    """The vortex case saved as a case file with its initial state as the
+    saved solution, reloaded, solved with solve_case and written to VTU:
+    the states equal the in-memory run bit for bit and the VTU holds them."""
+    import xml.etree.ElementTree as ET
+
+    from meshcfd_amd.io.project_io import initial_primitive, load_case, save_case
+    from meshcfd_amd.io.vtk_io import export_vtu
+
+    case, prim = meshgen.vortex_workload(16, 20)
+    case.name = "vortex"
+    case.final_solution = tuple(InitialState(*map(float, row)) for row in prim)
+    save_case(tmp_path / "vortex.json", case)
+    loaded = load_case(tmp_path / "vortex.json")
+    start = initial_primitive(loaded)
+    assert np.array_equal(start, prim)
+    print("case: solve_case from a case file", flush=True)
+    a = solve_case(loaded, start, device=cuda_device)
+    case.final_solution = None
+    b = solve_case(case, prim, device=cuda_device)
+    assert np.array_equal(a.primitive, b.primitive) and a.time == b.time
+    path = export_vtu(tmp_path / "vortex.vtu", loaded, a.primitive, a.time, a.steps)
+    piece = ET.parse(path).getroot().find("./UnstructuredGrid/Piece")
+    rho = np.array(piece.find("./CellData/DataArray[@Name='Density']").text.split(), dtype=float)
+    assert np.array_equal(rho, a.primitive[:, 0])
