@@ -450,12 +450,18 @@ def cell_face_lists(mesh: FiniteVolumeMesh) -> tuple[np.ndarray, np.ndarray]:
     key_b = np.concatenate((f[:, 1], f[inner, 0]))
     key_face = np.concatenate((fid, fid[inner]))
     key_sign = np.concatenate((np.ones(nf, np.int64), -np.ones(int(inner.sum()), np.int64)))
-    keys = (key_cell * n_nodes + key_a) * n_nodes + key_b
-    order = np.argsort(keys)
+    # a directed edge (n_a -> n_b) lies on the left of exactly one cell, so
+    # n_a * N + n_b identifies it (a key with the cell in it would overflow
+    # 64 bits at a few million cells); the cell is checked after the lookup
+    keys = key_a * n_nodes + key_b
+    order = np.argsort(keys, kind="stable")
     ks = keys[order]
-    q = (cells[active] * n_nodes + a[active]) * n_nodes + b[active]
+    if len(ks) > 1 and np.any(ks[1:] == ks[:-1]):
+        raise ValueError("Two faces run along the same directed edge.")
+    q = a[active] * n_nodes + b[active]
     pos = np.searchsorted(ks, q)
     hit = (pos < len(ks)) & (ks[np.minimum(pos, len(ks) - 1)] == q)
+    hit &= key_cell[order[np.minimum(pos, len(ks) - 1)]] == cells[active]
     if not hit.all():
         bad = int(cells[active][np.flatnonzero(~hit)[0]])
         raise ValueError(f"Cell {bad} has an edge that is not one of its faces.")

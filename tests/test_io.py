@@ -226,3 +226,19 @@ def test_y_plus_next_to_a_no_slip_wall():
     d = 0.5 / ny
     np.testing.assert_allclose(f["y_plus"][bottom], math.sqrt(rho * speed * d / 0.01), rtol=1e-13)
     assert np.all(f["y_plus"][~bottom] == 0.0)
+
+
+def test_cell_face_lists_with_many_nodes():
+    """Edge keys stay within 64 bits: a two-cell mesh whose node array is
+    padded to 10^7 points (a (cell, n_a, n_b) key would overflow at cell 1)."""
+    from dataclasses import replace
+
+    from meshcfd_amd.io.mesh_io import cell_face_lists
+
+    m = meshgen.rectangle_mesh(2, 1, (0.0, 2.0, 0.0, 1.0))
+    pad = np.zeros((10_000_000, 2))
+    big = replace(m, points=np.concatenate((m.points, pad)))
+    faces, signs = cell_face_lists(big)
+    ref_faces, ref_signs = cell_face_lists(m)
+    assert np.array_equal(faces, ref_faces) and np.array_equal(signs, ref_signs)
+    assert sorted(faces[0][:4].tolist() + faces[1][:4].tolist()).count(int(np.flatnonzero(m.face_neighbours >= 0)[0])) == 2
