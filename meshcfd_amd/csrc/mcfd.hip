@@ -3815,7 +3815,7 @@ __global__ void __launch_bounds__(kBlock)
 //   P3  per own cell: reconstruction, limiter and LSQ backward -> wbar, ebar
 template <int SCHEME, bool MUSCL, int LIM>
 __global__ void __launch_bounds__(kBlock)
-    k_vjp_fused(int max_local, int max_grad, int max_faces, BlockArrays ba, GeomArrays ga,
+    k_vjp_fused(int max_local, int max_grad, int max_faces, int max_acc, BlockArrays ba, GeomArrays ga,
                 FaceArrays fa, const double* __restrict__ U, const double* __restrict__ lam, double g,
                 const int* __restrict__ grad_rev, double* __restrict__ wbar, double* __restrict__ ebar,
                 double* __restrict__ SBAR, int* __restrict__ status) {
@@ -3831,7 +3831,10 @@ __global__ void __launch_bounds__(kBlock)
   double* sG = sW + 4 * max_local;         // [8 max_grad]
   double* sP = sG + 8 * max_grad;          // [4 max_grad]
   double* sLam = sP + 4 * max_grad;        // [4 max_grad] lambda / A, until P2 ends
-  double* sA = sLam;                       // [8 max_faces] face adjoints, after P2
+  double* sA = sLam;                       // [12 max_faces] face adjoints and offsets, after P2
+  unsigned short* sAI = reinterpret_cast<unsigned short*>(
+      sLam + (4 * max_grad > 12 * max_faces ? 4 * max_grad : 12 * max_faces));  // [max_acc] gather items
+  (void)max_acc;
   const int t = threadIdx.x;
   int e0 = 0, e1 = 0, l0 = 0, l1 = 0;
   if (MUSCL && t < n_grad) {
@@ -3874,22 +3877,37 @@ __global__ void __launch_bounds__(kBlock)
       double gy[4] = {0.0, 0.0, 0.0, 0.0};
       double mn[4] = {wv[0], wv[1], wv[2], wv[3]};
       double mx[4] = {wv[0], wv[1], wv[2], wv[3]};
-      for (int e = e0; e < e1; ++e) {
-        int o = ba.gnb[gs + e];
-        double2 c2 = ba.gent[gs + e];
-        double ov[4] = {sW[4 * o], sW[4 * o + 1], sW[4 * o + 2], sW[4 * o + 3]};
+      // entries in batches of 4: the batch's loads are issued together, the
+      // sums and min / max still run in entry order
+      for (int base = e0; base < e1; base += 4) {
+        int ob[4];
+        double2 cb[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          double d = ov[k] - wv[k];
-          gx[k] = gx[k] + c2.x * d;
-          gy[k] = gy[k] + c2.y * d;
-          if (ov[k] < mn[k]) {
-            mn[k] = ov[k];
-            arg[k] = (arg[k] & 0x00FFFFu) | ((unsigned)(e - e0) << 16);
-          }
-          if (ov[k] > mx[k]) {
-            mx[k] = ov[k];
-            arg[k] = (arg[k] & 0xFF00FFu) | ((unsigned)(e - e0) << 8);
+        for (int q = 0; q < 4; ++q) {
+          bool ok = base + q < e1;
+          ob[q] = ok ? (int)ba.gnb[gs + base + q] : 0;
+          cb[q] = ok ? ba.gent[gs + base + q] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int e = base + q;
+          if (e >= e1) break;
+          const int o = ob[q];
+          const double2 c2 = cb[q];
+          double ov[4] = {sW[4 * o], sW[4 * o + 1], sW[4 * o + 2], sW[4 * o + 3]};
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            double d = ov[k] - wv[k];
+            gx[k] = gx[k] + c2.x * d;
+            gy[k] = gy[k] + c2.y * d;
+            if (ov[k] < mn[k]) {
+              mn[k] = ov[k];
+              arg[k] = (arg[k] & 0x00FFFFu) | ((unsigned)(e - e0) << 16);
+            }
+            if (ov[k] > mx[k]) {
+              mx[k] = ov[k];
+              arg[k] = (arg[k] & 0xFF00FFu) | ((unsigned)(e - e0) << 8);
+            }
           }
         }
       }
@@ -3912,11 +3930,33 @@ __global__ void __launch_bounds__(kBlock)
     }
     __syncthreads();
   }
+  // P3's gather items (contiguous for the block's own cells), loaded now and
+  // stored to LDS after P2, and the own cell's item range
+  constexpr int kStageItems = 3;
+  const int abase = ga.acc_ptr[first];
+  const int n_acc = ga.acc_ptr[first + n_own] - abase;
+  unsigned short ai_r[kStageItems];
+#pragma unroll
+  for (int q = 0; q < kStageItems; ++q) {
+    const int k = t + q * kBlock;
+    ai_r[q] = (k < n_acc) ? ba.acc_local[abase + k] : 0;
+  }
+  int a0 = 0, a1 = 0;
+  if (t < n_own) {
+    a0 = ga.acc_ptr[first + t];
+    a1 = ga.acc_ptr[first + t + 1];
+  }
   // P2: face adjoints (as k_flux_adjoint_rev); results stay in registers
-  // until every thread has read lambda
+  // until every thread has read lambda; the face's reconstruction offsets
+  // go to LDS beside them for P3
   P4 alb{0.0, 0.0, 0.0, 0.0}, arb{0.0, 0.0, 0.0, 0.0};
+  double olx = 0.0, oly = 0.0, orx = 0.0, ory = 0.0;
   if (t < n_faces) {
     const FaceRec fr = ba.frec[fs + t];
+    olx = fr.lx;
+    oly = fr.ly;
+    orx = fr.rx;
+    ory = fr.ry;
     const int kind = fr.kind_bc & 15;
     const int l = fr.lr & 0xFFFF, r = fr.lr >> 16;
     const double nx = fr.nx, ny = fr.ny, L = fr.len;
@@ -3978,7 +4018,7 @@ __global__ void __launch_bounds__(kBlock)
   }
   __syncthreads();
   if (t < n_faces) {
-    double* q = sA + 8 * t;
+    double* q = sA + 12 * t;
     q[0] = alb.r;
     q[1] = alb.u;
     q[2] = alb.v;
@@ -3987,7 +4027,17 @@ __global__ void __launch_bounds__(kBlock)
     q[5] = arb.u;
     q[6] = arb.v;
     q[7] = arb.p;
+    q[8] = olx;
+    q[9] = oly;
+    q[10] = orx;
+    q[11] = ory;
   }
+#pragma unroll
+  for (int q = 0; q < kStageItems; ++q) {
+    const int k = t + q * kBlock;
+    if (k < n_acc) sAI[k] = ai_r[q];
+  }
+  for (int k = t + kStageItems * kBlock; k < n_acc; k += kBlock) sAI[k] = ba.acc_local[abase + k];
   __syncthreads();
   // P3: own cell t (as k_cell_adjoint1)
   if (t >= n_own) return;
@@ -4000,18 +4050,18 @@ __global__ void __launch_bounds__(kBlock)
   for (int k = 0; k < 8; ++k) gv[k] = MUSCL ? sG[8 * t + k] : 0.0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) pv[k] = MUSCL ? sP[4 * t + k] : 1.0;
-  for (int e = ga.acc_ptr[i]; e < ga.acc_ptr[i + 1]; ++e) {
-    int item = ba.acc_local[e];
+  for (int e = a0; e < a1; ++e) {
+    int item = sAI[e - abase];
     int slot = item >> 2;
     int mode = item & 3;
     bool right = (mode == MCFD_ACC_NEIGHBOUR) || (mode == MCFD_ACC_PERIODIC_B);
-    const double* q = sA + 8 * slot + (right ? 4 : 0);
+    const double* q = sA + 12 * slot + (right ? 4 : 0);
     double a[4] = {q[0], q[1], q[2], q[3]};
 #pragma unroll
     for (int k = 0; k < 4; ++k) wb[k] += a[k];
     if (MUSCL) {
-      const double2* rec = reinterpret_cast<const double2*>(ba.frec + fs + slot);
-      double2 d = right ? rec[2] : rec[1];  // (rx, ry) or (lx, ly)
+      const double* o = sA + 12 * slot + (right ? 10 : 8);
+      double2 d = make_double2(o[0], o[1]);  // (rx, ry) or (lx, ly)
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         double inc = gv[2 * k] * d.x + gv[2 * k + 1] * d.y;
@@ -4976,7 +5026,7 @@ void launch_vjp_fused_t(McfdMesh* m, size_t lds, hipStream_t s, const GeomArrays
   auto kern = k_vjp_fused<SCHEME, MUSCL, LIM>;
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                             hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024);
-  hipLaunchKernelGGL(kern, dim3(m->nb), dim3(kBlock), lds, s, m->max_local, m->max_grad, m->max_faces,
+  hipLaunchKernelGGL(kern, dim3(m->nb), dim3(kBlock), lds, s, m->max_local, m->max_grad, m->max_faces, m->max_acc,
                      m->blocks, ga, m->faces, u, lam, g, m->grad_rev, m->adj.wbar, m->adj.ebar, sbar,
                      m->status);
 }
@@ -4991,10 +5041,12 @@ void launch_vjp_fused_s(McfdMesh* m, bool muscl, int lim, size_t lds, hipStream_
 }
 
 // LDS of k_vjp_fused: W (4 max_local), G and phi (12 max_grad), then
-// lambda/A (4 max_grad) aliased by the face adjoints (8 max_faces).
+// lambda/A (4 max_grad) aliased by the face adjoints and offsets
+// (12 max_faces), then the gather items (max_acc shorts).
 size_t vjp_fused_lds_bytes(const McfdMesh* m) {
-  size_t region = std::max(4 * (size_t)m->max_grad, 8 * (size_t)m->max_faces);
-  return (4 * (size_t)m->max_local + 12 * (size_t)m->max_grad + region) * sizeof(double);
+  size_t region = std::max(4 * (size_t)m->max_grad, 12 * (size_t)m->max_faces);
+  size_t items = ((size_t)m->max_acc * sizeof(unsigned short) + 7) & ~(size_t)7;
+  return (4 * (size_t)m->max_local + 12 * (size_t)m->max_grad + region) * sizeof(double) + items;
 }
 
 int mcfd_residual_vjp_impl(McfdMesh* m, const McfdSettings* st, const double* u,
