@@ -17,7 +17,6 @@ from __future__ import annotations
 
 import ctypes as C
 import math
-from dataclasses import replace
 
 import numpy as np
 import torch

@@ -97,7 +97,6 @@ def test_per_instance_strength_gradients(cuda_device):
 
     from meshcfd_amd.adjoint import kinetic_energy, rk3_rollout, vortex_state
     from meshcfd_amd.device import DeviceMesh
-    from meshcfd_amd.geometry import shoelace
 
     specs = [s for s in mb.batch_specs(8, seed=2, cells=(200, 800)) if s.kind == "vortex"][:3]
     b = mb.build_batch(specs)

@@ -7,7 +7,6 @@ residuals and time steps (the mass-flow bisection compares a patch sum whose
 rounding differs between NumPy's pairwise sum and the device's tree sum; the
 pressure-inlet state goes through pow), 1e-10 after 10 steps."""
 
-import math
 
 import numpy as np
 import pytest

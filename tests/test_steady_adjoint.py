@@ -19,7 +19,7 @@ from meshcfd_amd import meshgen
 from meshcfd_amd.geometry import build_geometry
 from meshcfd_amd.model import NumericalSettings
 from oracle import implicit as oimp
-from oracle.euler import conservative_to_primitive, primitive_to_conservative
+from oracle.euler import primitive_to_conservative
 
 
 def test_geometric_radii_ratio_and_ends():
