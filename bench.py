@@ -169,6 +169,7 @@ def cylinder_bench(args) -> int:
     case, prim = meshgen.cylinder_case(args.n_theta, args.n_r)
     note(f"mesh {case.mesh.cell_count} cells in {time.perf_counter() - t0:.1f}s; building device data ...")
     dm = DeviceMesh(case, "cuda:0", fused=False)  # parabolic + viscous stages use the four-kernel path
+    dm.lusgs_ordering = args.lusgs_ordering
     setup_s = time.perf_counter() - t0
     note(f"setup {setup_s:.1f}s")
     gamma = case.numerics.gamma
@@ -205,7 +206,7 @@ def cylinder_bench(args) -> int:
         "data": "synthetic (seeded O-grid, free-stream start)",
         "config": {"workload": f"laminar cylinder O-grid {args.n_theta}x{args.n_r}, r 0.5..30, first spacing 1e-3, "
                                "Mach 0.2, Re 40, implicit Euler local CFL 100, GMRES(30) rtol 1e-3, LU-SGS",
-                   "cells": cells},
+                   "cells": cells, "lusgs_ordering": args.lusgs_ordering},
         "newton_per_step": round((st.newton_iterations - n0) / args.steps, 2),
         "gmres_per_step": round((st.gmres_iterations - g0) / args.steps, 1),
         "residual_rms": [r0, r1],
@@ -310,6 +311,8 @@ def main(argv=None) -> int:
     ap.add_argument("--graph-steps", type=int, default=10, help="batch: steps per captured HIP graph")
     ap.add_argument("--n-theta", type=int, default=2048, help="cylinder: cells around")
     ap.add_argument("--n-r", type=int, default=2048, help="cylinder: cells outward")
+    ap.add_argument("--lusgs-ordering", choices=("natural", "multicolour"), default="natural",
+                    help="cylinder: LU-SGS ordering (natural: the reference's; multicolour: few parallel levels)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use --block-cells as given instead of timing candidate block sizes first")
     ap.add_argument("--separate-time-step", action="store_true",

@@ -1948,6 +1948,8 @@ __global__ void __launch_bounds__(kBlock) k_lusgs_pack(int n, LusgsArrays la) {
 // than 2048 cells) and items beyond kLusgsK take the direct path.
 constexpr int kLusgsK = 4;
 constexpr int kLusgsSlots = 2;
+// average cells per level from which the sweeps launch a grid per level
+constexpr int kLusgsWideLevel = 4096;
 
 struct LusgsSlot {
   int cell, cnt;
@@ -2054,6 +2056,26 @@ __global__ void __launch_bounds__(1024)
     // agent-scope fence would write L2 back at every level)
     __syncthreads();
   }
+}
+
+// One level of a sweep across the whole GPU (orderings with few, large
+// levels, such as the multicolour one): a thread per position; the levels
+// are launched in order on the stream.
+template <bool FORWARD>
+__global__ void __launch_bounds__(kBlock)
+    k_lusgs_level(LusgsArrays la, const double* __restrict__ own, double* y, const int* __restrict__ te, int a,
+                  int b) {
+  const int k = a + blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= b) return;
+  const int* cells = FORWARD ? la.lvl_cells_f : la.lvl_cells_b;
+  const int* cnt = FORWARD ? la.pcnt_f : la.pcnt_b;
+  const int* pnb = FORWARD ? la.pnb_f : la.pnb_b;
+  const int K = FORWARD ? la.kf : la.kb;
+  const double* pval = FORWARD ? (te ? la.pval_ft : la.pval_f) : (te ? la.pval_bt : la.pval_b);
+  const double* pdiag = FORWARD ? la.pdiag_f : la.pdiag_b;
+  LusgsSlot sl;
+  lusgs_fetch<FORWARD>(la, k, K, cells, cnt, pnb, pval, pdiag, own, sl);
+  lusgs_finish<FORWARD>(la, sl, K, y, y, te);
 }
 
 // Deterministic dot product: fixed-shape block partials, then one block.
@@ -4340,6 +4362,7 @@ struct McfdMesh {
   bool implicit = false;
   LusgsArrays lusgs{};
   std::vector<void*> implicit_allocations;
+  std::vector<int> host_lvl_f, host_lvl_b;  // level pointers of the LU-SGS sweeps
   int gmres_m = 0;
   double* gm_V = nullptr;       // (m+1) * 4C Krylov basis
   double* gm_w = nullptr;       // 4C
@@ -4769,8 +4792,24 @@ int lblocks(long n) { return (int)((n + kBlock - 1) / kBlock); }
 
 int lusgs_apply(McfdMesh* m, const double* v, double* out, hipStream_t s, bool transpose = false) {
   const int* te = transpose ? m->lusgs.off_t : nullptr;
-  hipLaunchKernelGGL(k_lusgs_sweep<true>, dim3(1), dim3(1024), 0, s, m->lusgs, v, out, te);
-  hipLaunchKernelGGL(k_lusgs_sweep<false>, dim3(1), dim3(1024), 0, s, m->lusgs, out, out, te);
+  const int n = m->n_cells;
+  if (m->lusgs.n_levels_f > 0 && n / m->lusgs.n_levels_f >= kLusgsWideLevel &&
+      n / m->lusgs.n_levels_b >= kLusgsWideLevel) {
+    // few, large levels: one launch per level over the whole GPU
+    const std::vector<int>& pf = m->host_lvl_f;
+    for (int l = 0; l < m->lusgs.n_levels_f; ++l)
+      if (pf[l + 1] > pf[l])
+        hipLaunchKernelGGL(k_lusgs_level<true>, dim3(blocks(pf[l + 1] - pf[l])), dim3(kBlock), 0, s, m->lusgs, v,
+                         out, te, pf[l], pf[l + 1]);
+    const std::vector<int>& pb = m->host_lvl_b;
+    for (int l = 0; l < m->lusgs.n_levels_b; ++l)
+      if (pb[l + 1] > pb[l])
+        hipLaunchKernelGGL(k_lusgs_level<false>, dim3(blocks(pb[l + 1] - pb[l])), dim3(kBlock), 0, s, m->lusgs,
+                         out, out, te, pb[l], pb[l + 1]);
+  } else {
+    hipLaunchKernelGGL(k_lusgs_sweep<true>, dim3(1), dim3(1024), 0, s, m->lusgs, v, out, te);
+    hipLaunchKernelGGL(k_lusgs_sweep<false>, dim3(1), dim3(1024), 0, s, m->lusgs, out, out, te);
+  }
   HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -5511,6 +5550,8 @@ int mcfd_mesh_set_implicit(McfdMesh* m, const McfdImplicitDesc* d) {
   LusgsArrays& la = m->lusgs;
   la.kf = kf;
   la.kb = kb;
+  m->host_lvl_f = pf;
+  m->host_lvl_b = pb;
   la.n_rec = d->n_records;
   la.n_levels_f = nlf;
   la.n_levels_b = nlb;

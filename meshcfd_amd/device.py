@@ -129,6 +129,10 @@ class DeviceMesh:
         self.n_cases = 0
         self.case_clock = None
         self.graph = None
+        # LU-SGS ordering of the implicit path: "natural" (the reference's) or
+        # "multicolour" (few fully parallel levels); read when the implicit
+        # structure is first built
+        self.lusgs_ordering = "natural"
 
     @classmethod
     def from_batch(cls, batch, device="cuda:0", fused: bool = True, block_cells: int = 128):

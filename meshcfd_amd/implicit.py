@@ -26,7 +26,48 @@ from .geometry import Geometry, SolverError
 from .layout import DeviceLayout
 
 
-def implicit_desc(g: Geometry, L: DeviceLayout):
+LUSGS_ORDERINGS = ("natural", "multicolour")
+
+
+def colour_cells(ptr: np.ndarray, nbr: np.ndarray, seed: int = 0) -> np.ndarray:
+    """Greedy colouring of the cell graph (CSR ``ptr``, ``nbr``) in
+    Jones-Plassmann rounds: each round the uncoloured cells whose seeded
+    random priority exceeds that of every uncoloured neighbour take the
+    smallest colour no coloured neighbour has (two such cells are never
+    neighbours). At most max degree + 1 colours."""
+    n = len(ptr) - 1
+    prio = np.random.default_rng(seed).permutation(n).astype(np.int64)
+    colour = np.full(n, -1, dtype=np.int64)
+    deg = np.diff(ptr)
+    has = deg > 0
+    starts = ptr[:-1][has]
+    if len(nbr) and deg.max() > 60:
+        raise ValueError("a cell has more than 60 neighbours")
+    while True:
+        unc = colour < 0
+        if not unc.any():
+            return colour
+        m = np.full(n, -1, dtype=np.int64)
+        if len(nbr):
+            m[has] = np.maximum.reduceat(np.where(unc[nbr], prio[nbr], -1), starts)
+        sel = unc & (prio > m)
+        used = np.zeros(n, dtype=np.int64)
+        if len(nbr):
+            bits = np.where(colour[nbr] >= 0, np.left_shift(np.int64(1), np.maximum(colour[nbr], 0)), 0)
+            used[has] = np.bitwise_or.reduceat(bits, starts)
+        free = ~used & (used + 1)  # lowest colour not used
+        colour[sel] = np.log2(free[sel].astype(np.float64)).astype(np.int64)
+
+
+def implicit_desc(g: Geometry, L: DeviceLayout, ordering: str = "natural"):
+    """The LU-SGS structure on the device cells. ``ordering`` "natural" is
+    the reference's (`solver.py:1309-1323`: triangular solves in mesh
+    order); "multicolour" (not in the reference; a different, equally valid
+    SGS preconditioner, so GMRES takes other iterations to the same
+    tolerance) orders the cells colour by colour, so each sweep has as many
+    levels as colours, each of them fully parallel."""
+    if ordering not in LUSGS_ORDERINGS:
+        raise ValueError(f"Unsupported LU-SGS ordering '{ordering}'.")
     n = g.cell_count
     new_id = np.empty(n, dtype=np.int64)
     new_id[L.cell_order] = np.arange(n)
@@ -55,11 +96,22 @@ def implicit_desc(g: Geometry, L: DeviceLayout):
     o_cols = np.concatenate((g.neighbours[inner], g.owners[inner], g.owners[fb], g.owners[fa]))
     o_items = np.concatenate((2 * r_int, 2 * r_int + 1, 2 * r_per, 2 * r_per + 1))
     seq = np.arange(len(o_cells))
-    o = np.lexsort((seq, o_cols, new_id[o_cells]))
+    if ordering == "natural":
+        rank = np.arange(n)  # position of each mesh cell in the sweep order
+    else:
+        # colour the mesh cells (neighbours through the off-diagonal items)
+        co = np.argsort(o_cells, kind="stable")
+        cptr = np.zeros(n + 1, dtype=np.int64)
+        np.cumsum(np.bincount(o_cells, minlength=n), out=cptr[1:])
+        colour = colour_cells(cptr, o_cols[co])
+        rank = np.empty(n, dtype=np.int64)
+        rank[np.lexsort((np.arange(n), colour))] = np.arange(n)
+    # a cell's items: lower (earlier in the sweep order) first
+    o = np.lexsort((seq, rank[o_cols], new_id[o_cells]))
     off_cells, off_cols, off_items = o_cells[o], o_cols[o], o_items[o]
     off_ptr = np.zeros(n + 1, dtype=np.int64)
     np.cumsum(np.bincount(new_id[off_cells], minlength=n), out=off_ptr[1:])
-    lower = off_cols < off_cells  # natural (mesh) order
+    lower = rank[off_cols] < rank[off_cells]
     n_lower = np.bincount(new_id[off_cells[lower]], minlength=n)
     off_split = off_ptr[:-1] + n_lower
     # item of (record, side) -> position; the transposed entry is the other side's item
@@ -81,7 +133,7 @@ def implicit_desc(g: Geometry, L: DeviceLayout):
         off_split=off_split,
         off_item=off_items,
         off_nb=new_id[off_cols],
-        natural_order=new_id,
+        natural_order=new_id[np.argsort(rank)],
         off_t=off_t,
     )
 
@@ -179,7 +231,7 @@ class ImplicitStepper:
             restype, argtypes = native.EXPORTS[name]
             fn = getattr(self.lib, name)
             fn.restype, fn.argtypes = restype, argtypes
-        d = implicit_desc(dm.geometry, dm.layout)
+        d = implicit_desc(dm.geometry, dm.layout, getattr(dm, "lusgs_ordering", "natural"))
         keep = []
 
         def arr(a, dt):
