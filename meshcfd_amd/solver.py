@@ -172,11 +172,16 @@ def solve_case(
     cpu_time_offset: float = 0.0,
     device: str = "cuda:0",
     mesh_device=None,
+    *,
+    lusgs_ordering: str = "natural",
 ) -> SolverResult:
     """Advance a transient case or iterate a steady pseudo-time solution on
     the GPU (`solver.py:179-1008`). ``initial_states`` is a tuple of
     `InitialState` or a (C, 4) primitive array in mesh order. A prepared
-    `DeviceMesh` of the same case may be passed as ``mesh_device``."""
+    `DeviceMesh` of the same case may be passed as ``mesh_device``.
+    ``lusgs_ordering`` "multicolour" sweeps the implicit preconditioner colour
+    by colour (a different preconditioner than the reference's natural
+    order, with the same converged solutions; much faster on large meshes)."""
     from .device import DeviceMesh
 
     cpu_start = thread_time()
@@ -197,7 +202,13 @@ def solve_case(
         conservative_to_primitive(conservative, s.gamma)
     except SolverError as error:
         raise SolverError(f"Invalid initial state: {error}") from error
+    if lusgs_ordering not in ("natural", "multicolour"):
+        raise SolverError(f"Unsupported LU-SGS ordering '{lusgs_ordering}'.")
     dm = mesh_device if mesh_device is not None else DeviceMesh(case, device)
+    if getattr(dm, "_implicit", None) is None:
+        dm.lusgs_ordering = lusgs_ordering
+    elif dm.lusgs_ordering != lusgs_ordering:
+        raise SolverError("The device mesh's implicit structure was built with another LU-SGS ordering.")
     dm.set_state(conservative)
     n = dm.n
     implicit = s.time_integration == "Implicit Euler"

@@ -103,14 +103,16 @@ def test_gpu_solve_case_stops_at_end_time_and_on_cancel(cuda_device):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("integrator", ["Explicit RK3", "Implicit Euler"])
+@pytest.mark.parametrize("integrator", ["Explicit RK3", "Implicit Euler", "Implicit Euler multicolour"])
 def test_gpu_solve_case_converges_the_steady_bump(cuda_device, integrator):
     """Local (steady) pseudo-time on the bump channel (first-order HLLC):
     completes when the RMS residual reaches the tolerance; the reported
     residual is the post-step residual. Implicit steps stop their Newton
     iteration at max|U - U^n - dt R| <= 1e-9 (`solver.py:1418-1425`), which
     floors the steady residual near 1e-9/dt: tolerance 1e-8 at CFL 100."""
-    implicit = integrator == "Implicit Euler"
+    implicit = integrator.startswith("Implicit Euler")
+    ordering = "multicolour" if integrator.endswith("multicolour") else "natural"
+    integrator = integrator.split(" multicolour")[0]
     tol = 1e-8 if implicit else 1e-10
     case, prim = meshgen.bump_case(30, 10, 0.06, integrator=integrator, cfl=100.0 if implicit else 0.8)
     case.numerics.reconstruction = "First order"
@@ -119,7 +121,7 @@ def test_gpu_solve_case_converges_the_steady_bump(cuda_device, integrator):
     case.numerics.convergence_history_interval = 500
     seen = []
     print(f"case: solve_case bump {integrator}", flush=True)
-    res = solve_case(case, prim, progress_callback=seen.append, device=cuda_device)
+    res = solve_case(case, prim, progress_callback=seen.append, device=cuda_device, lusgs_ordering=ordering)
     steps, residual, completed = res.steps, res.residual, res.completed
     print("steps", steps, "residual", residual)
     assert completed and residual <= tol
