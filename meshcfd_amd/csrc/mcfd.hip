@@ -2096,26 +2096,77 @@ __global__ void __launch_bounds__(kBlock)
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// Block b reduces partial[b*nb .. b*nb+nb) into out[b]; bit b of sqrt_mask
+// takes the square root.
 __global__ void __launch_bounds__(kBlock)
-    k_dot_final(int nb, const double* __restrict__ partial, double* __restrict__ out, int sqrt_out) {
+    k_dot_final(int nb, const double* __restrict__ partial, double* __restrict__ out, int sqrt_mask) {
   __shared__ double red[kBlock];
+  const double* part = partial + (long)blockIdx.x * nb;
   double acc = 0.0;
-  for (int j = threadIdx.x; j < nb; j += blockDim.x) acc += partial[j];
+  for (int j = threadIdx.x; j < nb; j += blockDim.x) acc += part[j];
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int s = kBlock / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[0] = sqrt_out ? sqrt(red[0]) : red[0];
+  if (threadIdx.x == 0) out[blockIdx.x] = ((sqrt_mask >> blockIdx.x) & 1) ? sqrt(red[0]) : red[0];
 }
 
-// y -= (*coef) * x  (MGS projection with a device-resident coefficient)
+// (w, w) and (a, w) in one pass: partials of the first at partial[0..grid),
+// of the second at partial[grid..2*grid); per-thread order and tree as
+// k_dot_partial, so each result is bit-identical to a separate dot.
 __global__ void __launch_bounds__(kBlock)
-    k_axpy_dev(long n, const double* __restrict__ x, double* __restrict__ y,
-               const double* __restrict__ coef) {
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] = y[i] - coef[0] * x[i];
+    k_dot2_partial(long n, const double* __restrict__ a, const double* __restrict__ w,
+                   double* __restrict__ partial) {
+  __shared__ double red[2][kBlock];
+  double ww = 0.0, aw = 0.0;
+  long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double wi = w[i];
+    ww += wi * wi;
+    aw += a[i] * wi;
+  }
+  red[0][threadIdx.x] = ww;
+  red[1][threadIdx.x] = aw;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + s];
+      red[1][threadIdx.x] += red[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = red[0][0];
+    partial[gridDim.x + blockIdx.x] = red[1][0];
+  }
+}
+
+// One modified Gram-Schmidt step fused with the next one's dot: w -= (*coef) * p
+// with a device-resident coefficient, then the block partial of (q, w) over the updated w, or of
+// (w, w) with SELF; per-thread order and tree as k_dot_partial.
+template <bool SELF>
+__global__ void __launch_bounds__(kBlock)
+    k_axpy_dot_partial(long n, const double* __restrict__ p, double* __restrict__ w,
+                       const double* __restrict__ coef, const double* __restrict__ q,
+                       double* __restrict__ partial) {
+  __shared__ double red[kBlock];
+  const double c = coef[0];
+  double acc = 0.0;
+  long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double wi = w[i] - c * p[i];
+    w[i] = wi;
+    acc += SELF ? wi * wi : q[i] * wi;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
 // y = a*x + b*y (host scalars)
@@ -4823,7 +4874,7 @@ int ensure_gmres(McfdMesh* m, int krylov) {
   for (int k = 0; k < 7; ++k) {
     if (upload_implicit(m, &bufs[k], (const double*)nullptr, sizes[k])) return -1;
   }
-  if (upload_implicit(m, &m->gm_partial, (const double*)nullptr, (size_t)kDotBlocks)) return -1;
+  if (upload_implicit(m, &m->gm_partial, (const double*)nullptr, 2 * (size_t)kDotBlocks)) return -1;
   m->gm_V = bufs[0];
   m->gm_w = bufs[1];
   m->gm_r = bufs[2];
@@ -4990,14 +5041,21 @@ int gmres_cycle(JacobianOp& op, const double* b, double* x, double rtol, int res
     double* vn = V + (long)(col + 1) * N;
     if (op_apply(op, vc, m->gm_av, s)) return -1;
     if (op_precond(op, m->gm_av, m->gm_w, s)) return -1;
-    // h0 = ||w||, then modified Gram-Schmidt with device coefficients
-    if (dot_dev(m, N, m->gm_w, m->gm_w, sc + 4, true, s)) return -1;
-    for (int k = 0; k <= col; ++k) {
-      if (dot_dev(m, N, V + (long)k * N, m->gm_w, sc + 5 + k, false, s)) return -1;
-      hipLaunchKernelGGL(k_axpy_dev, dim3(nbl), dim3(kBlock), 0, s, N, V + (long)k * N, m->gm_w,
-                         sc + 5 + k);
+    // h0 = ||w||, then modified Gram-Schmidt with device coefficients: each
+    // projection w -= h_k v_k runs in the pass that forms h_{k+1} = (v_{k+1}, w)
+    // (and the last in the pass forming ||w||), same arithmetic as one dot and
+    // one axpy per step.
+    hipLaunchKernelGGL(k_dot2_partial, dim3(kDotBlocks), dim3(kBlock), 0, s, N, V, m->gm_w, m->gm_partial);
+    hipLaunchKernelGGL(k_dot_final, dim3(2), dim3(kBlock), 0, s, kDotBlocks, m->gm_partial, sc + 4, 1);
+    for (int k = 1; k <= col; ++k) {
+      hipLaunchKernelGGL(k_axpy_dot_partial<false>, dim3(kDotBlocks), dim3(kBlock), 0, s, N,
+                         V + (long)(k - 1) * N, m->gm_w, sc + 5 + k - 1, V + (long)k * N, m->gm_partial);
+      hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kBlock), 0, s, kDotBlocks, m->gm_partial, sc + 5 + k, 0);
     }
-    if (dot_dev(m, N, m->gm_w, m->gm_w, sc + 5 + col + 1, true, s)) return -1;
+    hipLaunchKernelGGL(k_axpy_dot_partial<true>, dim3(kDotBlocks), dim3(kBlock), 0, s, N, V + (long)col * N,
+                       m->gm_w, sc + 5 + col, (const double*)nullptr, m->gm_partial);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kBlock), 0, s, kDotBlocks, m->gm_partial, sc + 5 + col + 1, 1);
+    HIP_CHECK(hipGetLastError());
     if (read_scalars(host, sc + 4, col + 3, s)) return -1;
     double h0 = host[0];
     double* hc = h.data() + (size_t)col * (restart + 1);
