@@ -257,7 +257,7 @@ typedef struct McfdGmres {
   int32_t info;           /* out: 0 converged, 1 not converged */
   int32_t reserved;
   double sigma[4];        /* variable scales */
-  double state_norm;      /* ||state / sigma||_2 */
+  double state_norm;      /* ||state / sigma||_2; negative: computed on the device and stored here */
   double tolerance;       /* relative tolerance rtol */
   double residual;        /* out: final true residual ||b - A x|| */
 } McfdGmres;

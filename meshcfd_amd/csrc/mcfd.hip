@@ -2143,6 +2143,28 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// Block partials of sum_i (u_i / sigma_(i mod 4))^2 over n = 4 * cells values.
+__global__ void __launch_bounds__(kBlock)
+    k_scaled_sq_partial(long n, const double* __restrict__ u, double s0, double s1, double s2, double s3,
+                        double* __restrict__ partial) {
+  __shared__ double red[kBlock];
+  double acc = 0.0;
+  long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int k = (int)(i & 3);
+    const double sig = k == 0 ? s0 : (k == 1 ? s1 : (k == 2 ? s2 : s3));
+    double q = u[i] / sig;
+    acc += q * q;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
 // One modified Gram-Schmidt step fused with the next one's dot: w -= (*coef) * p
 // with a device-resident coefficient, then the block partial of (q, w) over the updated w, or of
 // (w, w) with SELF; per-thread order and tree as k_dot_partial.
@@ -4841,6 +4863,20 @@ int read_scalars(double* host, const double* dev, int count, hipStream_t s) {
 
 int lblocks(long n) { return (int)((n + kBlock - 1) / kBlock); }
 
+// g->state_norm, computed on the device (||state / sigma||_2) when the caller
+// passes a negative value; the result is stored back into g.
+int resolve_state_norm(McfdMesh* m, McfdGmres* g, hipStream_t s) {
+  if (g->state_norm >= 0.0) return 0;
+  for (int k = 0; k < 4; ++k)
+    if (!(g->sigma[k] > 0.0)) return fail("mcfd_gmres: sigma must be positive");
+  double* out = m->gm_scalars + 63;
+  hipLaunchKernelGGL(k_scaled_sq_partial, dim3(kDotBlocks), dim3(kBlock), 0, s, 4L * m->n_cells, g->state,
+                     g->sigma[0], g->sigma[1], g->sigma[2], g->sigma[3], m->gm_partial);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kBlock), 0, s, kDotBlocks, m->gm_partial, out, 1);
+  HIP_CHECK(hipGetLastError());
+  return read_scalars(&g->state_norm, out, 1, s);
+}
+
 int lusgs_apply(McfdMesh* m, const double* v, double* out, hipStream_t s, bool transpose = false) {
   const int* te = transpose ? m->lusgs.off_t : nullptr;
   const int n = m->n_cells;
@@ -5687,6 +5723,7 @@ int mcfd_jacobian_apply(McfdMesh* m, const McfdSettings* jst, McfdGmres* g, cons
   if (check_settings(jst)) return -1;
   if (ensure_gmres(m, 1)) return -1;
   hipStream_t s = static_cast<hipStream_t>(stream);
+  if (resolve_state_norm(m, g, s)) return -1;
   JacobianOp op{m, jst, g->state, g->rate, g->dt, g->dt_per_cell,
                 {g->sigma[0], g->sigma[1], g->sigma[2], g->sigma[3]}, g->state_norm};
   return jacobian_apply(op, v, out, s);
@@ -5701,6 +5738,7 @@ int mcfd_gmres(McfdMesh* m, const McfdSettings* jst, McfdGmres* g, void* stream)
   if (ensure_gmres(m, g->krylov)) return -1;
   hipStream_t s = static_cast<hipStream_t>(stream);
   long N = 4L * m->n_cells;
+  if (resolve_state_norm(m, g, s)) return -1;
   HIP_CHECK(hipMemsetAsync(g->x, 0, sizeof(double) * N, s));
   JacobianOp op{m, jst, g->state, g->rate, g->dt, g->dt_per_cell,
                 {g->sigma[0], g->sigma[1], g->sigma[2], g->sigma[3]}, g->state_norm};

@@ -301,8 +301,7 @@ class ImplicitStepper:
         g.max_iterations = s.gmres_max_iterations
         for k in range(4):
             g.sigma[k] = float(sigma[k])
-        host = state.cpu().numpy()
-        g.state_norm = float(np.linalg.norm(host / np.asarray(sigma)[None, :]))
+        g.state_norm = -1.0  # ||state / sigma||_2, reduced on the device by the call
         g.tolerance = s.gmres_tolerance
         return g
 

@@ -159,9 +159,16 @@ def test_gpu_jacobian_operator_matches_oracle(cuda_device):
     dt = torch.full((1,), 0.02, dtype=torch.float64, device=cuda_device)
     args = st.gmres_args(ud, rate_d, dt, False, sc)
     out = torch.empty_like(ud)
+    assert args.state_norm < 0.0  # asks the call to reduce ||u / sigma|| on the device
     st.jacobian_apply(args, dm.to_device_order(v), out)
     dm.check_status()
+    # the device reduction stored back equals the two-norm up to summation order
+    assert args.state_norm == pytest.approx(float(np.linalg.norm(u / sc[None, :])), rel=1e-13)
     np.testing.assert_allclose(dm.to_mesh_order(out), want, rtol=1e-6, atol=1e-7)
+    # a caller-supplied norm is used as given
+    args.state_norm = 2.5
+    st.jacobian_apply(args, dm.to_device_order(v), out)
+    assert args.state_norm == 2.5
     dm.close()
 
 
