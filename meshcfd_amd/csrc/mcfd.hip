@@ -4868,7 +4868,7 @@ int lblocks(long n) { return (int)((n + kBlock - 1) / kBlock); }
 int resolve_state_norm(McfdMesh* m, McfdGmres* g, hipStream_t s) {
   if (g->state_norm >= 0.0) return 0;
   for (int k = 0; k < 4; ++k)
-    if (!(g->sigma[k] > 0.0)) return fail("mcfd_gmres: sigma must be positive");
+    if (!(g->sigma[k] > 0.0)) return fail("McfdGmres.sigma must be positive to compute state_norm");
   double* out = m->gm_scalars + 63;
   hipLaunchKernelGGL(k_scaled_sq_partial, dim3(kDotBlocks), dim3(kBlock), 0, s, 4L * m->n_cells, g->state,
                      g->sigma[0], g->sigma[1], g->sigma[2], g->sigma[3], m->gm_partial);
