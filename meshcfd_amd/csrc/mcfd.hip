@@ -4436,6 +4436,7 @@ struct McfdMesh {
   LusgsArrays lusgs{};
   std::vector<void*> implicit_allocations;
   std::vector<int> host_lvl_f, host_lvl_b;  // level pointers of the LU-SGS sweeps
+  int lusgs_path = -1;  // -1 undecided, 0 one-workgroup sweeps, 1 one launch per level
   int gmres_m = 0;
   double* gm_V = nullptr;       // (m+1) * 4C Krylov basis
   double* gm_w = nullptr;       // 4C
@@ -4877,11 +4878,8 @@ int resolve_state_norm(McfdMesh* m, McfdGmres* g, hipStream_t s) {
   return read_scalars(&g->state_norm, out, 1, s);
 }
 
-int lusgs_apply(McfdMesh* m, const double* v, double* out, hipStream_t s, bool transpose = false) {
-  const int* te = transpose ? m->lusgs.off_t : nullptr;
-  const int n = m->n_cells;
-  if (m->lusgs.n_levels_f > 0 && n / m->lusgs.n_levels_f >= kLusgsWideLevel &&
-      n / m->lusgs.n_levels_b >= kLusgsWideLevel) {
+void lusgs_launch(McfdMesh* m, const double* v, double* out, hipStream_t s, const int* te, bool wide) {
+  if (wide) {
     // few, large levels: one launch per level over the whole GPU
     const std::vector<int>& pf = m->host_lvl_f;
     for (int l = 0; l < m->lusgs.n_levels_f; ++l)
@@ -4897,6 +4895,56 @@ int lusgs_apply(McfdMesh* m, const double* v, double* out, hipStream_t s, bool t
     hipLaunchKernelGGL(k_lusgs_sweep<true>, dim3(1), dim3(1024), 0, s, m->lusgs, v, out, te);
     hipLaunchKernelGGL(k_lusgs_sweep<false>, dim3(1), dim3(1024), 0, s, m->lusgs, out, out, te);
   }
+}
+
+// Both launch paths compute every cell with the same fetch and finish, so
+// they give identical bits and the choice only decides speed. Many small
+// levels favour the one-workgroup sweeps (a level costs a barrier, not a
+// launch), few large ones a launch per level; between the two bounds the
+// first application on a mesh times both (one warm-up and two timed
+// applications each) and keeps the faster.
+constexpr int kLusgsTimedLevel = 384;  // average cells per level from which both are timed
+
+int lusgs_choose_path(McfdMesh* m, const double* v, double* out, hipStream_t s, const int* te) {
+  const int n = m->n_cells;
+  const int lf = m->lusgs.n_levels_f, lb = m->lusgs.n_levels_b;
+  if (lf <= 0 || lb <= 0) {
+    m->lusgs_path = 0;
+    return 0;
+  }
+  const int per = std::min(n / lf, n / lb);
+  if (per >= kLusgsWideLevel || per < kLusgsTimedLevel) {
+    m->lusgs_path = per >= kLusgsWideLevel ? 1 : 0;
+    return 0;
+  }
+  if (v == out) return 0;  // an in-place call is not repeatable: decide on a later call
+  hipEvent_t ev[2];
+  for (int k = 0; k < 2; ++k) HIP_CHECK(hipEventCreate(&ev[k]));
+  float ms[2] = {0.0f, 0.0f};
+  for (int wide = 0; wide < 2; ++wide) {
+    lusgs_launch(m, v, out, s, te, wide == 1);
+    HIP_CHECK(hipEventRecord(ev[0], s));
+    lusgs_launch(m, v, out, s, te, wide == 1);
+    lusgs_launch(m, v, out, s, te, wide == 1);
+    HIP_CHECK(hipEventRecord(ev[1], s));
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventSynchronize(ev[1]));
+    HIP_CHECK(hipEventElapsedTime(&ms[wide], ev[0], ev[1]));
+  }
+  for (int k = 0; k < 2; ++k) HIP_CHECK(hipEventDestroy(ev[k]));
+  m->lusgs_path = ms[1] < ms[0] ? 1 : 0;
+  return 0;
+}
+
+int lusgs_apply(McfdMesh* m, const double* v, double* out, hipStream_t s, bool transpose = false) {
+  const int* te = transpose ? m->lusgs.off_t : nullptr;
+  if (m->lusgs_path < 0 && lusgs_choose_path(m, v, out, s, te)) return -1;
+  const int n = m->n_cells;
+  bool wide = m->lusgs_path == 1;
+  if (m->lusgs_path < 0)  // undecided (in-place call): the size rule
+    wide = m->lusgs.n_levels_f > 0 && n / m->lusgs.n_levels_f >= kLusgsWideLevel &&
+           n / m->lusgs.n_levels_b >= kLusgsWideLevel;
+  lusgs_launch(m, v, out, s, te, wide);
   HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -5646,6 +5694,7 @@ int mcfd_mesh_set_implicit(McfdMesh* m, const McfdImplicitDesc* d) {
   la.kb = kb;
   m->host_lvl_f = pf;
   m->host_lvl_b = pb;
+  m->lusgs_path = -1;
   la.n_rec = d->n_records;
   la.n_levels_f = nlf;
   la.n_levels_b = nlb;
