@@ -916,6 +916,8 @@ __device__ __forceinline__ double visc_k(double mu, const ViscParams& v) {
 }
 
 struct ViscArrays {
+  const double* inf_ext;  // [4 * inflow records] exterior states from the owner cells (kVisInflow)
+  int inf_first;          // boundary record of inflow record 0
   const double* wl_wx;
   const double* wl_wy;
   const double* wl_cell;
@@ -928,7 +930,14 @@ struct ViscArrays {
   const double* bv_geom;
 };
 
-enum { kVisWall = 1, kVisSymmetry = 2, kVisInlet = 3, kVisFarField = 5 };
+enum { kVisWall = 1, kVisSymmetry = 2, kVisInlet = 3, kVisInflow = 4, kVisFarField = 5 };
+
+// Exterior state read by visc_inlet: the boundary record's own state, or for
+// pressure / mass-flow inlets the inflow record's state from the owner cell.
+__device__ __forceinline__ const double* visc_inlet_state(int code, int b, const ViscArrays& va,
+                                                          const FaceArrays& fa) {
+  return code == kVisInflow ? va.inf_ext + 4 * (long)(b - va.inf_first) : fa.bc_state + 4 * (long)b;
+}
 
 // Exterior (u, v, T) of a velocity inlet or far-field face from the owner's
 // cell state (_viscous_inlet_states, solver.py:2740-2785).
@@ -1000,9 +1009,9 @@ __global__ void __launch_bounds__(kBlock)
       double t = 2.0 * (val[0] * nx + val[1] * ny);
       gh[0] = gh[0] - t * nx;
       gh[1] = gh[1] - t * ny;
-    } else if (code == kVisInlet || code == kVisFarField) {
+    } else if (code == kVisInlet || code == kVisInflow || code == kVisFarField) {
       double x[3];
-      visc_inlet(code, w, fa.bc_state + 4 * (long)b, nx, ny, vp, x, status, i);
+      visc_inlet(code, w, visc_inlet_state(code, b, va, fa), nx, ny, vp, x, status, i);
 #pragma unroll
       for (int k = 0; k < 3; ++k) gh[k] = 2.0 * x[k] - gh[k];
     }
@@ -1141,9 +1150,9 @@ __global__ void __launch_bounds__(kBlock)
     g[5] = g[5] - gt * ny;
     bv[0] = bv[0] - un * nx;
     bv[1] = bv[1] - un * ny;
-  } else if (code == kVisInlet || code == kVisFarField) {
+  } else if (code == kVisInlet || code == kVisInflow || code == kVisFarField) {
     double x[3];
-    visc_inlet(code, w, fa.bc_state + 4 * (long)b, nx, ny, vp, x, status, o);
+    visc_inlet(code, w, visc_inlet_state(code, b, va, fa), nx, ny, vp, x, status, o);
     double cv[3] = {w.u, w.v, T};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -4476,6 +4485,7 @@ struct McfdMesh {
   bool needs_inflow = false;  // the mesh has boundary records of kinds 7-9
   bool inflow = false;        // mcfd_mesh_set_inflow done
   InflowArrays inf{};
+  double* inf_ext_v = nullptr;  // [4 * inflow records] exterior states for the viscous terms
   int inflow_patches = 0, inflow_max_n = 0, inflow_max_leaves = 0;
   std::vector<void*> inflow_allocations;
   std::vector<int> host_face_bc;  // [n_faces] for validating the inflow records
@@ -4659,6 +4669,7 @@ void free_inflow(McfdMesh* m) {
   m->inflow_allocations.clear();
   m->inflow = false;
   m->inf = InflowArrays{};
+  m->inf_ext_v = nullptr;
   m->faces.inflow = nullptr;
   m->dts.inflow = nullptr;
   m->inflow_patches = m->inflow_max_n = m->inflow_max_leaves = 0;
@@ -4772,8 +4783,6 @@ void launch_fused(McfdMesh* m, int scheme, bool muscl, bool far, bool dtn, size_
 int check_inflow(const McfdMesh* m, const McfdSettings* st) {
   if (!m->needs_inflow) return 0;
   if (!m->inflow) return fail("Pressure-inlet and mass-flow boundaries need mcfd_mesh_set_inflow first.");
-  if (st->viscosity)
-    return fail("Viscous terms with pressure-inlet or mass-flow boundaries are not supported on the GPU.");
   if (!(st->gas_constant > 0.0)) return fail("Inflow boundaries need a positive gas constant.");
   return 0;
 }
@@ -4781,15 +4790,17 @@ int check_inflow(const McfdMesh* m, const McfdSettings* st) {
 // Exterior states of the inflow records from the reconstructed face states
 // (residual) or the owner cells' primitive states in m->W (time steps).
 int inflow_states(McfdMesh* m, const McfdSettings* st, hipStream_t s, bool reconstructed, bool para,
-                  bool muscl) {
+                  bool muscl, double* ext = nullptr) {
   if (!m->inflow || m->inf.n == 0) return 0;
-  hipLaunchKernelGGL(k_inflow_records, dim3(blocks(m->inf.n)), dim3(kBlock), 0, s, m->inf, m->faces,
+  InflowArrays ia = m->inf;
+  if (ext) ia.ext = ext;
+  hipLaunchKernelGGL(k_inflow_records, dim3(blocks(ia.n)), dim3(kBlock), 0, s, ia, m->faces,
                      m->W, m->G, m->PHI, (reconstructed && para) ? m->K : nullptr,
                      (reconstructed && muscl) ? 1 : 0, reconstructed ? 1 : 0, st->gamma,
                      st->gas_constant, m->status);
   if (m->inflow_patches) {
     size_t lds = sizeof(double) * ((size_t)m->inflow_max_n + 9 * (size_t)m->inflow_max_leaves + 65);
-    hipLaunchKernelGGL(k_mass_flow_inlet, dim3(m->inflow_patches), dim3(256), lds, s, m->inf,
+    hipLaunchKernelGGL(k_mass_flow_inlet, dim3(m->inflow_patches), dim3(256), lds, s, ia,
                        m->face_len, m->inflow_max_n, m->inflow_max_leaves, st->gamma,
                        st->gas_constant, m->status);
   }
@@ -4830,11 +4841,19 @@ int residual_front(McfdMesh* m, const McfdSettings* st, const double* u, hipStre
   if (st->viscosity) {
     TimedLaunch t(m, s, MCFD_KERNEL_VISCOUS);
     ViscParams vp = visc_params(st);
+    ViscArrays va = m->visc;
+    if (m->inflow && m->inf.n) {
+      // pressure / mass-flow inlets: the viscous boundary values are their
+      // exterior states from the owner cells' states (solver.py:2749-2772)
+      if (inflow_states(m, st, s, false, false, false, m->inf_ext_v)) return -1;
+      va.inf_ext = m->inf_ext_v;
+      va.inf_first = m->inf.first;
+    }
     hipLaunchKernelGGL(k_visc_gradient, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W,
-                       m->grad_ptr, m->grad_nb, m->visc, m->faces, vp, m->VG, m->status);
+                       m->grad_ptr, m->grad_nb, va, m->faces, vp, m->VG, m->status);
     if (m->n_faces)
       hipLaunchKernelGGL(k_visc_flux, dim3(blocks(m->n_faces)), dim3(kBlock), 0, s, m->n_faces,
-                         m->faces, m->W, m->VG, m->visc, vp, m->VF, m->status);
+                         m->faces, m->W, m->VG, va, vp, m->VF, m->status);
   }
   HIP_CHECK(hipGetLastError());
   return 0;
@@ -6028,6 +6047,7 @@ int mcfd_mesh_set_inflow(McfdMesh* m, const McfdInflowDesc* d) {
   rc |= alloc_inflow(m, &prog_ptr, np ? d->prog_ptr : nullptr, (size_t)np + 1);
   rc |= alloc_inflow(m, &program, d->program, (size_t)d->n_program);
   rc |= alloc_inflow(m, &ia.ext, (const double*)nullptr, 4 * (size_t)n);
+  rc |= alloc_inflow(m, &m->inf_ext_v, (const double*)nullptr, 4 * (size_t)n);
   if (rc) {
     std::string msg = g_last_error;
     free_inflow(m);

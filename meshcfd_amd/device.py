@@ -99,9 +99,6 @@ class DeviceMesh:
         # kernels, read by the unfused flux and time-step kernels
         self.inflow = inflow.has_inflow(self.layout)
         if self.inflow:
-            if self.settings.viscosity_model != "Inviscid":
-                raise SolverError("Viscous terms with pressure-inlet or mass-flow boundaries are not supported on "
-                                  "the GPU.")
             inflow.upload(self.lib, self.handle, self.geometry, self.layout)
         elif fused:
             self.set_blocks(block_cells)

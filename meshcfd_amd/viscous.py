@@ -29,7 +29,7 @@ from . import native
 from .geometry import BOUNDARY_CODES, Geometry, SolverError
 from .layout import DeviceLayout
 
-VIS_WALL, VIS_SYMMETRY, VIS_INLET, VIS_FAR_FIELD = 1, 2, 3, 5
+VIS_WALL, VIS_SYMMETRY, VIS_INLET, VIS_INFLOW, VIS_FAR_FIELD = 1, 2, 3, 4, 5
 VIS_MODELS = {"Inviscid": 0, "Constant": 1, "Sutherland": 2}
 
 
@@ -103,7 +103,9 @@ def build_viscous(g: Geometry, L: DeviceLayout) -> dict:
     n_bc = len(L.bc_kind)
     face_bc = L.face_bc[flux[dbf]] if nb_ else np.zeros(0, dtype=np.int64)
     code_map = {BOUNDARY_CODES["Wall"]: VIS_WALL, BOUNDARY_CODES["Symmetry"]: VIS_SYMMETRY,
-                BOUNDARY_CODES["Velocity Inlet"]: VIS_INLET, BOUNDARY_CODES["Far Field"]: VIS_FAR_FIELD}
+                BOUNDARY_CODES["Velocity Inlet"]: VIS_INLET, BOUNDARY_CODES["Far Field"]: VIS_FAR_FIELD,
+                # exterior states of the inflow records, from the owner cells' states
+                BOUNDARY_CODES["Pressure Inlet"]: VIS_INFLOW, BOUNDARY_CODES["Mass Flow Inlet"]: VIS_INFLOW}
     bv_code = np.zeros(n_bc, dtype=np.int32)
     bv_flags = np.zeros(n_bc, dtype=np.int32)
     bv_geom = np.zeros((n_bc, 4))

@@ -190,11 +190,6 @@ class Oracle:
         if s.viscosity_model != "Inviscid":
             from .viscous import viscous_residual
 
-            if len(g.pressure_inlet_faces) or len(g.mass_flow_inlet_faces) or len(g.mass_flow_outlet_faces):
-                # the reference's viscous terms take these inlets' states as
-                # boundary values (solver.py:2740-2785); not restated here
-                raise NotImplementedError("viscous terms with pressure-inlet or mass-flow boundaries")
-
             deriv += viscous_residual(primitive, g, s, self.D)
         if s.acceleration_x != 0.0 or s.acceleration_y != 0.0:
             deriv[:, 1] += primitive[:, 0] * s.acceleration_x

@@ -75,11 +75,28 @@ def weighted_least_squares(n, sources, dx, dy, diff):
 
 
 def inlet_states(primitive, g, s):
-    """Boundary-face -> exterior primitive state for velocity inlets and far
-    field (`_viscous_inlet_states`, `solver.py:2740-2785`)."""
+    """Boundary-face -> exterior primitive state for velocity inlets, pressure
+    and mass-flow inlets (their exterior states from the owner cells'
+    unreconstructed states) and far field (`_viscous_inlet_states`,
+    `solver.py:2740-2785`)."""
     from .solver import far_field_states
 
     states = {int(f): st for f, st in zip(g.velocity_inlet_faces, g.velocity_inlet_states)}
+    if len(getattr(g, "pressure_inlet_faces", ())):
+        from .inflow import pressure_inlet_states
+
+        faces = g.pressure_inlet_faces
+        ext = pressure_inlet_states(primitive[g.owners[faces]], g.face_normals[faces], g.pressure_inlet_params,
+                                    g.pressure_inlet_patch, s.gamma, s.gas_constant, g.inflow_patch_names[0])
+        states.update({int(f): st for f, st in zip(faces, ext)})
+    if len(getattr(g, "mass_flow_inlet_faces", ())):
+        from .inflow import mass_flow_inlet_states
+
+        faces = g.mass_flow_inlet_faces
+        ext = mass_flow_inlet_states(primitive[g.owners[faces]], g.face_normals[faces], g.face_lengths[faces],
+                                     g.mass_flow_inlet_params, g.mass_flow_inlet_patch, s.gamma, s.gas_constant,
+                                     g.inflow_patch_names[1])
+        states.update({int(f): st for f, st in zip(faces, ext)})
     if len(g.far_field_faces):
         faces = g.far_field_faces
         ext = far_field_states(primitive[g.owners[faces]], g.far_field_states, g.face_normals[faces], s.gamma)

@@ -22,9 +22,10 @@ T0 = 1.0 + 0.2**2 / 7.0  # total temperature of (rho, u, p, R) = (1, 0.2, 1, 1)
 
 
 def channel(inlet_kind, outlet_kind="Mass Flow Outlet", n=8, flux="HLLC", reconstruction="MUSCL linear",
-            integrator="Explicit RK3", inlet_values=None, outlet_values=None):
+            integrator="Explicit RK3", inlet_values=None, outlet_values=None, viscous=False):
+    extra = {"viscosity_model": "Constant", "dynamic_viscosity": 0.02} if viscous else {}
     numerics = NumericalSettings(flux=flux, reconstruction=reconstruction, time_integration=integrator,
-                                 gas_constant=1.0, cfl=0.5, end_time=1e30)
+                                 gas_constant=1.0, cfl=0.5, end_time=1e30, **extra)
     mesh = meshgen.rectangle_mesh(2 * n, n, (0.0, 2.0, 0.0, 1.0), jitter=0.2, seed=4, split_fraction=0.3)
     if inlet_values is None:
         inlet_values = ({"pressure": T0**3.5, "temperature": T0, "velocity_x": 1.0, "velocity_y": 0.1}
@@ -279,6 +280,50 @@ def test_gpu_inflow_implicit_step_matches_oracle(cuda_device, kinds):
     err = _rel(gpu, ref)
     print("implicit rel err", err)
     assert np.all(err <= 1e-8), err
+    dm.close()
+
+
+@pytest.mark.parametrize("kinds", CASES)
+def test_viscous_oracle_takes_inflow_states_as_boundary_values(kinds):
+    """Viscous channels with pressure / mass-flow inlets: the inlets' exterior
+    states from the owner cells are the viscous boundary values
+    (`solver.py:2749-2772`), so the viscous residual changes with the inlet's
+    total conditions; mass-flow outlets are not in that map."""
+    from oracle.viscous import inlet_states
+
+    case = channel(kinds[0], kinds[1], viscous=True)
+    o = Oracle(build_geometry(case), case.numerics)
+    prim = flow_state(case)
+    r = o.residual(euler.primitive_to_conservative(prim, 1.4))
+    assert np.all(np.isfinite(r))
+    g = o.g
+    states = inlet_states(prim, g, case.numerics)
+    inlet_faces = g.pressure_inlet_faces if kinds[0] == "Pressure Inlet" else g.mass_flow_inlet_faces
+    assert set(int(f) for f in inlet_faces) <= set(states)
+    assert not set(int(f) for f in g.mass_flow_outlet_faces) & set(states)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kinds", CASES)
+def test_gpu_viscous_inflow_residual_and_dt_match_oracle(cuda_device, kinds):
+    from meshcfd_amd.device import DeviceMesh
+
+    case = channel(kinds[0], kinds[1], viscous=True)
+    print(f"case: viscous {case.name}", flush=True)
+    dm = DeviceMesh(case, cuda_device, block_cells=64)
+    prim = flow_state(case)
+    u = euler.primitive_to_conservative(prim, 1.4)
+    o = Oracle(dm.geometry, case.numerics)
+    gpu = dm.to_mesh_order(dm.residual(dm.to_device_order(u)))
+    dm.check_status()
+    err = _rel(gpu, o.residual(u))
+    print("residual rel err", err)
+    assert np.all(err <= 1e-12), err
+    dm.time_steps(dm.to_device_order(u))
+    dm.check_status()
+    dt = dm.to_mesh_order(dm.dt_cell)
+    ref = o.stable_time_steps(prim)
+    assert np.max(np.abs(dt - ref) / ref) <= 1e-12
     dm.close()
 
 
