@@ -309,7 +309,12 @@ int mcfd_mesh_set_implicit(McfdMesh* mesh, const McfdImplicitDesc* desc);
 int mcfd_lusgs_assemble(McfdMesh* mesh, const McfdSettings* settings, const double* u,
                         const double* dt, int32_t dt_per_cell, void* stream);
 
-/* out = U^-1 (D (L^-1 v)) (replaces `apply_lu_sgs`, solver.py:1315-1319). */
+/* out = U^-1 (D (L^-1 v)) (replaces `apply_lu_sgs`, solver.py:1315-1319).
+ * On a mesh whose levels average 384-4096 cells, the first call with
+ * v != out (here or through mcfd_lusgs_apply_transpose / mcfd_gmres) times
+ * the one-workgroup sweeps against one launch per level, waits for the
+ * stream, and keeps the faster; both give identical results. Make that
+ * first call outside HIP graph capture. */
 int mcfd_lusgs_apply(McfdMesh* mesh, const double* v, double* out, void* stream);
 
 /* out = v - dt * J v / sigma, one-sided difference of the first-order residual
