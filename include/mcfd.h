@@ -224,7 +224,10 @@ typedef struct McfdStage {
  */
 typedef struct McfdImplicitDesc {
   int32_t n_records;
-  int32_t reserved;
+  int32_t launch_path;         /* LU-SGS sweeps: 0 chosen per mesh (by level sizes, or by timing
+                                  both paths on the first application, which synchronises the
+                                  stream), 1 one workgroup, 2 one launch per level; every path
+                                  gives the same bits */
   const int32_t* rec_o;
   const int32_t* rec_n;        /* -1 for boundary records */
   const double* rec_nox;       /* normal paired with the owner-side state */
@@ -233,7 +236,9 @@ typedef struct McfdImplicitDesc {
   const double* rec_nny;
   const double* rec_len;
   const int32_t* diag_ptr;     /* [n_cells+1] */
-  const int32_t* diag_item;
+  const int32_t* diag_item;    /* record*2 + side; -(record*2 + side) - 1 for the entries of a
+                                  periodic pair whose two faces share one owner, which the
+                                  reference's sparse assembly sums onto the diagonal */
   const int32_t* off_ptr;      /* [n_cells+1] */
   const int32_t* off_split;    /* [n_cells] */
   const int32_t* off_item;
@@ -251,7 +256,7 @@ typedef struct McfdGmres {
   double* x;              /* (C,4) solution */
   const double* dt;       /* [1] or [C] */
   int32_t dt_per_cell;
-  int32_t krylov;         /* restart length m (<= 31) */
+  int32_t krylov;         /* restart length m */
   int32_t max_iterations;
   int32_t iterations;     /* out: inner iterations used */
   int32_t info;           /* out: 0 converged, 1 not converged */

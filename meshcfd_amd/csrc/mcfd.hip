@@ -1848,7 +1848,7 @@ struct LusgsArrays {
   const double* nny;
   const double* len;
   const int* diag_ptr;   // per cell: diagonal items in reference order
-  const int* diag_item;  // record*2 + side
+  const int* diag_item;  // record*2 + side; -(record*2 + side) - 1: a self-paired periodic entry
   const int* off_ptr;    // per cell: off-diagonal items, lower (natural order) first
   const int* off_split;  // per cell: first upper item
   const int* off_item;   // record*2 + side
@@ -1908,7 +1908,12 @@ __global__ void __launch_bounds__(kBlock)
   double A = area[i];
   double d = 1.0;
   for (int e = la.diag_ptr[i]; e < la.diag_ptr[i + 1]; ++e) {
-    int r = la.diag_item[e] >> 1;
+    const int it = la.diag_item[e];
+    if (it < 0) {  // a periodic pair whose faces share this owner: its off-diagonal entry
+      d = d + (-(h * la.coef[(-it - 1) >> 1]) / A);
+      continue;
+    }
+    int r = it >> 1;
     if (la.rec_n[r] < 0)
       d = d + h * la.coef[r] * la.len[r] / A;
     else
@@ -2219,16 +2224,13 @@ __global__ void __launch_bounds__(kBlock)
   if (i < n) out[i] = x[i] * s;
 }
 
-// x += sum_k y_k V_k, k = 0..m-1 in order (y on the host, passed by value)
-struct Coeffs32 {
-  double y[32];
-};
+// x += sum_k y_k V_k, k = 0..m-1 in order (y in device memory)
 __global__ void __launch_bounds__(kBlock)
-    k_combine(long n, int m, const double* __restrict__ V, Coeffs32 c, double* __restrict__ x) {
+    k_combine(long n, int m, const double* __restrict__ V, const double* __restrict__ y, double* __restrict__ x) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double acc = 0.0;
-  for (int k = 0; k < m; ++k) acc = acc + c.y[k] * V[(long)k * n + i];
+  for (int k = 0; k < m; ++k) acc = acc + y[k] * V[(long)k * n + i];
   x[i] = x[i] + acc;
 }
 
@@ -4453,7 +4455,8 @@ struct McfdMesh {
   double* gm_av = nullptr;      // 4C
   double* gm_pert = nullptr;    // 4C
   double* gm_rp = nullptr;      // 4C
-  double* gm_scalars = nullptr; // 64
+  double* gm_scalars = nullptr; // 64 fixed slots, then 3 + m per-column dots, then m coefficients
+  double* gm_y_host = nullptr;  // pinned, m coefficients of the basis combination
   double* gm_partial = nullptr; // dot partials
   // MUSCL parabolic
   bool parabolic = false;
@@ -4575,6 +4578,8 @@ size_t fused_lds_bytes(const McfdMesh* m, bool muscl) {
 void free_implicit(McfdMesh* m) {
   for (void* p : m->implicit_allocations) (void)hipFree(p);
   m->implicit_allocations.clear();
+  if (m->gm_y_host) (void)hipHostFree(m->gm_y_host);
+  m->gm_y_host = nullptr;
   m->implicit = false;
   m->lusgs = LusgsArrays{};
   m->gmres_m = 0;
@@ -4968,12 +4973,17 @@ int lusgs_apply(McfdMesh* m, const double* v, double* out, hipStream_t s, bool t
   return 0;
 }
 
+// gm_scalars: slots 0-3 (||b||, ||M b||, ||r||, ||M r||), 40 (||v|| of the
+// Jacobian product), 63 (||state / sigma||); from kGmScalarCols the column's
+// ||w|| and projections (col + 3), after them the m combination coefficients.
+constexpr int kGmScalarCols = 64;
+
 int ensure_gmres(McfdMesh* m, int krylov) {
   if (krylov <= m->gmres_m) return 0;
-  if (krylov > 31) return fail("mcfd_gmres: Krylov dimension above 31 is not supported");
   long N = 4L * m->n_cells;
   double* bufs[7] = {nullptr};
-  size_t sizes[7] = {(size_t)(krylov + 1) * N, (size_t)N, (size_t)N, (size_t)N, (size_t)N, (size_t)N, 64};
+  size_t sizes[7] = {(size_t)(krylov + 1) * N, (size_t)N, (size_t)N, (size_t)N, (size_t)N, (size_t)N,
+                     (size_t)kGmScalarCols + 2 * (size_t)krylov + 3};
   for (int k = 0; k < 7; ++k) {
     if (upload_implicit(m, &bufs[k], (const double*)nullptr, sizes[k])) return -1;
   }
@@ -4985,6 +4995,9 @@ int ensure_gmres(McfdMesh* m, int krylov) {
   m->gm_pert = bufs[4];
   m->gm_rp = bufs[5];
   m->gm_scalars = bufs[6];
+  if (m->gm_y_host) (void)hipHostFree(m->gm_y_host);
+  m->gm_y_host = nullptr;
+  HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&m->gm_y_host), sizeof(double) * krylov));
   m->gmres_m = krylov;
   return 0;
 }
@@ -5091,7 +5104,8 @@ int gmres_cycle(JacobianOp& op, const double* b, double* x, double rtol, int res
   int nbl = lblocks(N);
   double* V = m->gm_V;
   double* sc = m->gm_scalars;
-  double host[40];
+  double* cs = sc + kGmScalarCols;  // per-column scalars
+  std::vector<double> host((size_t)restart + 3);
   inner = 0;
   if (dot_dev(m, N, b, b, sc + 0, true, s)) return -1;
   double bnrm2 = 0.0;
@@ -5149,17 +5163,17 @@ int gmres_cycle(JacobianOp& op, const double* b, double* x, double rtol, int res
     // (and the last in the pass forming ||w||), same arithmetic as one dot and
     // one axpy per step.
     hipLaunchKernelGGL(k_dot2_partial, dim3(kDotBlocks), dim3(kBlock), 0, s, N, V, m->gm_w, m->gm_partial);
-    hipLaunchKernelGGL(k_dot_final, dim3(2), dim3(kBlock), 0, s, kDotBlocks, m->gm_partial, sc + 4, 1);
+    hipLaunchKernelGGL(k_dot_final, dim3(2), dim3(kBlock), 0, s, kDotBlocks, m->gm_partial, cs, 1);
     for (int k = 1; k <= col; ++k) {
       hipLaunchKernelGGL(k_axpy_dot_partial<false>, dim3(kDotBlocks), dim3(kBlock), 0, s, N,
-                         V + (long)(k - 1) * N, m->gm_w, sc + 5 + k - 1, V + (long)k * N, m->gm_partial);
-      hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kBlock), 0, s, kDotBlocks, m->gm_partial, sc + 5 + k, 0);
+                         V + (long)(k - 1) * N, m->gm_w, cs + 1 + k - 1, V + (long)k * N, m->gm_partial);
+      hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kBlock), 0, s, kDotBlocks, m->gm_partial, cs + 1 + k, 0);
     }
     hipLaunchKernelGGL(k_axpy_dot_partial<true>, dim3(kDotBlocks), dim3(kBlock), 0, s, N, V + (long)col * N,
-                       m->gm_w, sc + 5 + col, (const double*)nullptr, m->gm_partial);
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kBlock), 0, s, kDotBlocks, m->gm_partial, sc + 5 + col + 1, 1);
+                       m->gm_w, cs + 1 + col, (const double*)nullptr, m->gm_partial);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kBlock), 0, s, kDotBlocks, m->gm_partial, cs + 1 + col + 1, 1);
     HIP_CHECK(hipGetLastError());
-    if (read_scalars(host, sc + 4, col + 3, s)) return -1;
+    if (read_scalars(host.data(), cs, col + 3, s)) return -1;
     double h0 = host[0];
     double* hc = h.data() + (size_t)col * (restart + 1);
     for (int k = 0; k <= col; ++k) hc[k] = host[1 + k];
@@ -5194,17 +5208,21 @@ int gmres_cycle(JacobianOp& op, const double* b, double* x, double rtol, int res
   if (col == restart) col = restart - 1;
   // solve the upper-triangular system
   if (h[(size_t)col * (restart + 1) + col] == 0.0) S[col] = 0.0;
-  Coeffs32 y{};
-  for (int k = 0; k <= col; ++k) y.y[k] = S[k];
+  // y in pinned host memory: the stream synchronises on the residual norm
+  // below, before the next cycle writes y again
+  double* y = m->gm_y_host;
+  for (int k = 0; k <= col; ++k) y[k] = S[k];
   for (int k = col; k > 0; --k) {
-    if (y.y[k] != 0.0) {
-      y.y[k] /= h[(size_t)k * (restart + 1) + k];
-      double t = y.y[k];
-      for (int i = 0; i < k; ++i) y.y[i] -= t * h[(size_t)k * (restart + 1) + i];
+    if (y[k] != 0.0) {
+      y[k] /= h[(size_t)k * (restart + 1) + k];
+      double t = y[k];
+      for (int i = 0; i < k; ++i) y[i] -= t * h[(size_t)k * (restart + 1) + i];
     }
   }
-  if (y.y[0] != 0.0) y.y[0] /= h[0];
-  hipLaunchKernelGGL(k_combine, dim3(nbl), dim3(kBlock), 0, s, N, col + 1, V, y, x);
+  if (y[0] != 0.0) y[0] /= h[0];
+  double* y_dev = cs + restart + 3;
+  HIP_CHECK(hipMemcpyAsync(y_dev, y, sizeof(double) * (col + 1), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_combine, dim3(nbl), dim3(kBlock), 0, s, N, col + 1, V, y_dev, x);
   // true residual
   if (op_apply(op, x, m->gm_av, s)) return -1;
   HIP_CHECK(hipMemcpyAsync(m->gm_r, b, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
@@ -5651,6 +5669,7 @@ int mcfd_mesh_set_implicit(McfdMesh* m, const McfdImplicitDesc* d) {
   free_implicit(m);
   if (!d) return 0;
   if (!d->off_t) return fail("mcfd_mesh_set_implicit: off_t is required");
+  if (d->launch_path < 0 || d->launch_path > 2) return fail("mcfd_mesh_set_implicit: launch_path must be 0, 1 or 2");
   int n = m->n_cells;
   // Level schedules of the natural-order triangular solves: a cell's level
   // is one more than the largest level among the cells it depends on.
@@ -5713,7 +5732,7 @@ int mcfd_mesh_set_implicit(McfdMesh* m, const McfdImplicitDesc* d) {
   la.kb = kb;
   m->host_lvl_f = pf;
   m->host_lvl_b = pb;
-  m->lusgs_path = -1;
+  m->lusgs_path = d->launch_path - 1;  // -1 undecided
   la.n_rec = d->n_records;
   la.n_levels_f = nlf;
   la.n_levels_b = nlb;

@@ -216,7 +216,7 @@ class DeviceMesh:
         if self.inflow:
             self.timings = {}
             return None
-        saved = (self.u.clone(), self.clock.clone(), self.dt_fresh)
+        saved = (self.u.clone(), self.clock.clone())
         stream = torch.cuda.current_stream(self.device)
         best, best_ms = None, math.inf
         self.timings = {}
@@ -247,7 +247,8 @@ class DeviceMesh:
         self.set_blocks(best)
         self.u.copy_(saved[0])
         self.clock.copy_(saved[1])
-        self.dt_fresh = saved[2]
+        # the timed steps overwrote the CFL steps: the next step recomputes them
+        self.dt_fresh = False
         self.check_status("autotune: ")
         return best
 

@@ -43,6 +43,9 @@ def colour_cells(ptr: np.ndarray, nbr: np.ndarray, seed: int = 0) -> np.ndarray:
     starts = ptr[:-1][has]
     if len(nbr) and deg.max() > 60:
         raise ValueError("a cell has more than 60 neighbours")
+    owner = np.repeat(np.arange(n), deg)
+    if np.any(nbr == owner):
+        raise ValueError("the cell graph lists a cell as its own neighbour")
     while True:
         unc = colour < 0
         if not unc.any():
@@ -51,6 +54,8 @@ def colour_cells(ptr: np.ndarray, nbr: np.ndarray, seed: int = 0) -> np.ndarray:
         if len(nbr):
             m[has] = np.maximum.reduceat(np.where(unc[nbr], prio[nbr], -1), starts)
         sel = unc & (prio > m)
+        if not sel.any():  # cannot happen without self-edges: the highest uncoloured priority wins
+            raise ValueError("colouring made no progress")
         used = np.zeros(n, dtype=np.int64)
         if len(nbr):
             bits = np.where(colour[nbr] >= 0, np.left_shift(np.int64(1), np.maximum(colour[nbr], 0)), 0)
@@ -84,9 +89,17 @@ def implicit_desc(g: Geometry, L: DeviceLayout, ordering: str = "natural"):
     r_int = np.arange(ni)
     r_per = ni + np.arange(npair)
     r_bnd = ni + npair + np.arange(len(bnd))
-    # diagonal: interior owners, interior neighbours, periodic a, periodic b, boundary
-    d_cells = np.concatenate((g.owners[inner], g.neighbours[inner], g.owners[fa], g.owners[fb], g.owners[bnd]))
-    d_items = np.concatenate((2 * r_int, 2 * r_int + 1, 2 * r_per, 2 * r_per + 1, 2 * r_bnd))
+    # A cell that owns both faces of a periodic pair (a mesh one cell wide)
+    # has the pair's two off-diagonal entries on its own row and column: the
+    # reference's COO sum adds them to the diagonal after the diagonal value,
+    # a->b then b->a. They are listed as diagonal items -(item) - 1.
+    self_pair = g.owners[fa] == g.owners[fb]
+    sp_cells = np.repeat(g.owners[fa][self_pair], 2)
+    sp_items = -np.column_stack((2 * r_per[self_pair], 2 * r_per[self_pair] + 1)).ravel() - 1
+    # diagonal: interior owners, interior neighbours, periodic a, periodic b,
+    # boundary, then the self-pair entries
+    d_cells = np.concatenate((g.owners[inner], g.neighbours[inner], g.owners[fa], g.owners[fb], g.owners[bnd], sp_cells))
+    d_items = np.concatenate((2 * r_int, 2 * r_int + 1, 2 * r_per, 2 * r_per + 1, 2 * r_bnd, sp_items))
     o = np.argsort(new_id[d_cells], kind="stable")
     diag_ptr = np.zeros(n + 1, dtype=np.int64)
     np.cumsum(np.bincount(new_id[d_cells], minlength=n), out=diag_ptr[1:])
@@ -95,6 +108,8 @@ def implicit_desc(g: Geometry, L: DeviceLayout, ordering: str = "natural"):
     o_cells = np.concatenate((g.owners[inner], g.neighbours[inner], g.owners[fa], g.owners[fb]))
     o_cols = np.concatenate((g.neighbours[inner], g.owners[inner], g.owners[fb], g.owners[fa]))
     o_items = np.concatenate((2 * r_int, 2 * r_int + 1, 2 * r_per, 2 * r_per + 1))
+    off_diag = o_cells != o_cols
+    o_cells, o_cols, o_items = o_cells[off_diag], o_cols[off_diag], o_items[off_diag]
     seq = np.arange(len(o_cells))
     if ordering == "natural":
         rank = np.arange(n)  # position of each mesh cell in the sweep order
@@ -115,7 +130,7 @@ def implicit_desc(g: Geometry, L: DeviceLayout, ordering: str = "natural"):
     n_lower = np.bincount(new_id[off_cells[lower]], minlength=n)
     off_split = off_ptr[:-1] + n_lower
     # item of (record, side) -> position; the transposed entry is the other side's item
-    pos = np.empty(2 * (ni + npair), dtype=np.int64)
+    pos = np.full(2 * (ni + npair), -1, dtype=np.int64)
     pos[off_items] = np.arange(len(off_items))
     off_t = pos[off_items ^ 1]
     return dict(
@@ -138,10 +153,13 @@ def implicit_desc(g: Geometry, L: DeviceLayout, ordering: str = "natural"):
     )
 
 
+LAUNCH_PATHS = {"auto": 0, "workgroup": 1, "levels": 2}
+
+
 class McfdImplicitDesc(C.Structure):
     _fields_ = [
         ("n_records", C.c_int32),
-        ("reserved", C.c_int32),
+        ("launch_path", C.c_int32),
         ("rec_o", C.c_void_p),
         ("rec_n", C.c_void_p),
         ("rec_nox", C.c_void_p),
@@ -220,9 +238,14 @@ def _p(t):
 
 
 class ImplicitStepper:
-    """Implicit-Euler steps of one DeviceMesh."""
+    """Implicit-Euler steps of one DeviceMesh. ``launch_path`` fixes how the
+    LU-SGS sweeps launch ("workgroup": one workgroup per sweep, "levels": a
+    launch per level); "auto" lets the library choose per mesh. All paths
+    give the same bits."""
 
-    def __init__(self, dm):
+    def __init__(self, dm, launch_path: str = "auto"):
+        if launch_path not in LAUNCH_PATHS:
+            raise ValueError(f"Unsupported LU-SGS launch path '{launch_path}'.")
         self.dm = dm
         self.lib = native.load()
         for name in ("mcfd_mesh_set_implicit", "mcfd_lusgs_assemble", "mcfd_lusgs_apply", "mcfd_jacobian_apply",
@@ -241,7 +264,7 @@ class ImplicitStepper:
 
         desc = McfdImplicitDesc(
             n_records=d["n_records"],
-            reserved=0,
+            launch_path=LAUNCH_PATHS[launch_path],
             rec_o=arr(d["rec_o"], np.int32),
             rec_n=arr(d["rec_n"], np.int32),
             rec_nox=arr(d["rec_nox"], np.float64),

@@ -326,3 +326,126 @@ def test_gpu_multicolour_steady_state_equals_natural(cuda_device):
     err = np.max(np.abs(states["multicolour"] - ref), axis=0) / np.max(np.abs(ref), axis=0)
     print("max rel diff", err)
     assert np.all(err <= 1e-7)
+
+
+def _self_periodic(nx, ny):
+    mesh = meshgen.rectangle_mesh(nx, ny, (0.0, 1.0, 0.0, 0.7), jitter=0.2, seed=5)
+    numerics = NumericalSettings(flux="Rusanov", reconstruction="First order", time_integration="Implicit Euler",
+                                 cfl=100.0)
+    case = meshgen.periodic_case(mesh, numerics)
+    g = build_geometry(case)
+    x, y = g.centers[:, 0], g.centers[:, 1]
+    prim = np.column_stack((1.0 + 0.05 * np.sin(2 * np.pi * x + y), np.full_like(x, 0.3), np.full_like(x, -0.2),
+                            1.0 + 0.04 * np.cos(2 * np.pi * y)))
+    return case, g, prim
+
+
+def test_colouring_rejects_self_edges_and_one_cell_wide_meshes_colour():
+    """A cell that owns both faces of a periodic pair (a mesh one cell wide)
+    is not its own neighbour in the sweep structure: its pair entries go onto
+    the diagonal, so colouring terminates."""
+    from meshcfd_amd.implicit import colour_cells, implicit_desc
+    from meshcfd_amd.layout import build_layout
+
+    with pytest.raises(ValueError, match="own neighbour"):
+        colour_cells(np.array([0, 1, 2]), np.array([1, 1]))
+    for nx, ny in ((1, 4), (4, 1), (1, 1)):
+        _, g, _ = _self_periodic(nx, ny)
+        L = build_layout(g)
+        for ordering in ("natural", "multicolour"):
+            d = implicit_desc(g, L, ordering)
+            cells = np.repeat(np.arange(g.cell_count), np.diff(d["off_ptr"]))
+            assert np.all(d["off_nb"] != cells)
+            assert np.sum(d["diag_item"] < 0) == 2 * np.sum(g.owners[g.periodic_faces_a] == g.owners[g.periodic_faces_b])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("size", [(1, 6), (5, 1)])
+@pytest.mark.parametrize("ordering", ["natural", "multicolour"])
+def test_gpu_lusgs_one_cell_wide_periodic_matches_oracle(cuda_device, size, ordering):
+    """The pair entries of self-paired periodic cells sit on the diagonal, as
+    the reference's sparse assembly sums them (solver.py:1262-1311)."""
+    import torch
+    from meshcfd_amd.implicit import ImplicitStepper
+
+    case, g, prim = _self_periodic(*size)
+    print(f"case: one-cell-wide periodic LU-SGS {size} {ordering}", flush=True)
+    dm = _dm(case, cuda_device, True)
+    dm.lusgs_ordering = ordering
+    st = ImplicitStepper(dm)
+    _, rank, _ = _sweep_rank(g, ordering)
+    perm = np.argsort(rank)
+    u = primitive_to_conservative(prim, 1.4)
+    h = np.linspace(0.01, 0.05, g.cell_count)
+    S, _ = oimp.lusgs_matrix(prim, g, 1.4, h)
+    Sp = S[perm][:, perm].tocsr()
+    v = np.random.default_rng(8).normal(size=u.shape)
+    want = np.empty_like(v)
+    want[perm] = oimp.lusgs_apply(Sp, v[perm].ravel()).reshape(v.shape)
+    st.assemble(dm.to_device_order(u), dm.to_device_order(h), True)
+    out = torch.empty((g.cell_count, 4), dtype=torch.float64, device=cuda_device)
+    st.lusgs_apply(dm.to_device_order(v), out)
+    dm.check_status()
+    np.testing.assert_allclose(dm.to_mesh_order(out), want, rtol=1e-12, atol=1e-12)
+    dm.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ordering", ["natural", "multicolour"])
+def test_gpu_lusgs_launch_paths_give_identical_bits(cuda_device, ordering):
+    """The one-workgroup sweeps and the launch-per-level path compute every
+    cell with the same code: forced onto each path, the forward and the
+    transposed applications agree bit for bit."""
+    import ctypes as C
+
+    import torch
+    from meshcfd_amd.implicit import ImplicitStepper
+
+    case, g, prim = periodic(100, 60, flux="Rusanov", time_integration="Implicit Euler", cfl=100.0)
+    print(f"case: LU-SGS launch paths {g.cell_count} cells {ordering}", flush=True)
+    u = primitive_to_conservative(prim, 1.4)
+    h = np.linspace(0.01, 0.05, g.cell_count)
+    v = np.random.default_rng(9).normal(size=u.shape)
+    results = {}
+    for path in ("workgroup", "levels"):
+        dm = _dm(case, cuda_device, True)
+        dm.lusgs_ordering = ordering
+        st = ImplicitStepper(dm, launch_path=path)
+        st.assemble(dm.to_device_order(u), dm.to_device_order(h), True)
+        vd = dm.to_device_order(v)
+        fwd = torch.empty((g.cell_count, 4), dtype=torch.float64, device=cuda_device)
+        tr = torch.empty_like(fwd)
+        st.lusgs_apply(vd, fwd)
+        dm.lib.mcfd_lusgs_apply_transpose(dm.handle, C.c_void_p(vd.data_ptr()), C.c_void_p(tr.data_ptr()), dm.stream)
+        dm.check_status()
+        results[path] = (fwd.cpu(), tr.cpu())
+        dm.close()
+    assert torch.equal(results["workgroup"][0], results["levels"][0])
+    assert torch.equal(results["workgroup"][1], results["levels"][1])
+
+
+@pytest.mark.gpu
+def test_gpu_implicit_step_with_krylov_above_31_matches_oracle(cuda_device):
+    """Any positive Krylov dimension is accepted (the reference's limit):
+    GMRES(40) implicit steps match the oracle's."""
+    case, g, prim = periodic(8, 6, flux="HLLC", reconstruction="MUSCL linear", time_integration="Implicit Euler",
+                             cfl=50.0, end_time=1.0, gmres_tolerance=1e-8, gmres_krylov_dimension=40)
+    print("case: implicit GMRES(40)", flush=True)
+    dm = _dm(case, cuda_device, True)
+    u0 = primitive_to_conservative(prim, 1.4)
+    dm.set_state(u0)
+    t = 0.0
+    for _ in range(2):
+        t = dm.step_implicit(t)
+    gpu = dm.state()
+    o = Oracle(g, case.numerics)
+    u, tr = u0.copy(), 0.0
+    for _ in range(2):
+        steps = o.stable_time_steps(conservative_to_primitive(u, 1.4))
+        u, h = oimp.implicit_euler_step(g, case.numerics, u, min(float(np.min(steps)), 1.0 - tr))
+        tr += h
+    err = np.max(np.abs(gpu - u), axis=0) / np.max(np.abs(u), axis=0)
+    print("GMRES(40) implicit rel err", err)
+    assert np.all(err <= 1e-7)
+    assert math.isclose(t, tr, rel_tol=1e-12)
+    dm.close()
