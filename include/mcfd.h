@@ -293,6 +293,16 @@ int mcfd_residual(McfdMesh* mesh, const McfdSettings* settings, const double* u,
 int mcfd_time_steps(McfdMesh* mesh, const McfdSettings* settings, const double* u,
                     double* dt_cell, double* dt_min, void* stream);
 
+/* mcfd_time_steps from primitive states w (C,4) = (rho, u, v, p) on the device,
+ * as `_stable_time_steps` takes them (solver.py:4132). */
+int mcfd_time_steps_primitive(McfdMesh* mesh, const McfdSettings* settings, const double* w,
+                              double* dt_cell, double* dt_min, void* stream);
+
+/* The end-of-step state check of `solve_case` (solver.py:806-808:
+ * `_primitive_from_conservative` of the new state): sets MCFD_ERR_STATE with
+ * the first bad cell in the device status. */
+int mcfd_check_state(McfdMesh* mesh, const McfdSettings* settings, const double* u, void* stream);
+
 /* clock = [time, dt]: dt = min(dt_source[0], end_time - time) (dt_source NULL:
  * fixed_dt), checked positive/finite, then time += dt (solver.py:452-465, 812).
  * reset != 0 sets dt_source[0] = +inf after reading it (for dt_min of the

@@ -259,6 +259,20 @@ __device__ P4 far_field(const P4& in, const P4& out, double nx, double ny, doubl
 // Kernels
 // ---------------------------------------------------------------------------
 
+// The validity test of euler.py:20-40 alone (no primitive states written).
+__global__ void __launch_bounds__(kBlock)
+    k_check_state(int n, const double* __restrict__ U, double g, int* __restrict__ status) {
+  int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  P4 q = load4(U, i);
+  double rho = q.r;
+  double u = q.u / rho;
+  double v = q.v / rho;
+  double p = (g - 1.0) * (q.p - 0.5 * (q.u * q.u + q.v * q.v) / rho);
+  if (!finite(rho) || !finite(u) || !finite(v) || !finite(p) || !(rho > 0.0) || !(p > 0.0))
+    report(status, MCFD_ERR_STATE, i);
+}
+
 // euler.py:20-40
 __global__ void __launch_bounds__(kBlock)
     k_primitive(int n, const double* __restrict__ U, double* __restrict__ W, double g,
@@ -5595,18 +5609,10 @@ int mcfd_stage(McfdMesh* m, const McfdSettings* st, const McfdStage* sg, void* s
   return 0;
 }
 
-int mcfd_time_steps(McfdMesh* m, const McfdSettings* st, const double* u, double* dt_cell,
-                    double* dt_min, void* stream) {
-  if (!m || !u || !dt_cell) return fail("mcfd_time_steps: NULL argument");
-  if (check_settings(st)) return -1;
-  if (check_inflow(m, st)) return -1;
-  hipStream_t s = static_cast<hipStream_t>(stream);
+namespace {
+// Stable steps from the primitive states already in m->W.
+int time_steps_from_w(McfdMesh* m, const McfdSettings* st, double* dt_cell, double* dt_min, hipStream_t s) {
   int n = m->n_cells;
-  {
-    TimedLaunch t(m, s, MCFD_KERNEL_PRIMITIVE);
-    hipLaunchKernelGGL(k_primitive, dim3(blocks(n)), dim3(kBlock), 0, s, n, u, m->W, st->gamma,
-                       m->status);
-  }
   if (m->inflow) {
     TimedLaunch t(m, s, MCFD_KERNEL_TIME_STEP);
     if (inflow_states(m, st, s, false, false, false)) return -1;
@@ -5627,6 +5633,42 @@ int mcfd_time_steps(McfdMesh* m, const McfdSettings* st, const double* u, double
     TimedLaunch t(m, s, MCFD_KERNEL_REDUCE);
     hipLaunchKernelGGL(k_min_reduce, dim3(1), dim3(1024), 0, s, blocks(n), m->partial, dt_min);
   }
+  HIP_CHECK(hipGetLastError());
+  return 0;
+}
+}  // namespace
+
+int mcfd_time_steps(McfdMesh* m, const McfdSettings* st, const double* u, double* dt_cell,
+                    double* dt_min, void* stream) {
+  if (!m || !u || !dt_cell) return fail("mcfd_time_steps: NULL argument");
+  if (check_settings(st)) return -1;
+  if (check_inflow(m, st)) return -1;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int n = m->n_cells;
+  {
+    TimedLaunch t(m, s, MCFD_KERNEL_PRIMITIVE);
+    hipLaunchKernelGGL(k_primitive, dim3(blocks(n)), dim3(kBlock), 0, s, n, u, m->W, st->gamma,
+                       m->status);
+  }
+  return time_steps_from_w(m, st, dt_cell, dt_min, s);
+}
+
+int mcfd_time_steps_primitive(McfdMesh* m, const McfdSettings* st, const double* w, double* dt_cell,
+                              double* dt_min, void* stream) {
+  if (!m || !w || !dt_cell) return fail("mcfd_time_steps_primitive: NULL argument");
+  if (check_settings(st)) return -1;
+  if (check_inflow(m, st)) return -1;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  HIP_CHECK(hipMemcpyAsync(m->W, w, sizeof(double) * 4 * (size_t)m->n_cells, hipMemcpyDeviceToDevice, s));
+  return time_steps_from_w(m, st, dt_cell, dt_min, s);
+}
+
+int mcfd_check_state(McfdMesh* m, const McfdSettings* st, const double* u, void* stream) {
+  if (!m || !u) return fail("mcfd_check_state: NULL argument");
+  if (check_settings(st)) return -1;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int n = m->n_cells;
+  hipLaunchKernelGGL(k_check_state, dim3(blocks(n)), dim3(kBlock), 0, s, n, u, st->gamma, m->status);
   HIP_CHECK(hipGetLastError());
   return 0;
 }

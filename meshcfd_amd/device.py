@@ -277,6 +277,9 @@ class DeviceMesh:
             message = native.ERROR_MESSAGES[code]
             if "{name}" in message:  # inflow records report their record index
                 message = message.format(name=inflow.record_name(self.geometry, self.layout, index.value))
+            elif not mask.value & ~native.CELL_ERROR_MASK and 0 <= index.value < self.n:
+                # cell-indexed errors only: the lowest failing cell, in mesh order
+                message = f"{message} First invalid cell: {int(self.layout.cell_order[index.value])}."
             raise SolverError(f"{context}{message}")
 
     # Device cells are stored in the layout's (Hilbert) order; these convert
@@ -330,10 +333,7 @@ class DeviceMesh:
     def step_implicit(self, time: float) -> float:
         """One implicit-Euler step of `solve_case` (`solver.py:447-513`);
         returns the new time (host-synchronising: Newton needs the norms)."""
-        from .implicit import ImplicitStepper
-
-        if getattr(self, "_implicit", None) is None:
-            self._implicit = ImplicitStepper(self)
+        self.implicit_stepper()
         s = self.settings
         mode = s.time_step_mode
         if mode == "Local (steady)":
@@ -354,6 +354,35 @@ class DeviceMesh:
         state, accepted = self._implicit.step(self.u, dt, per_cell, allow_reduction=(mode != "Global fixed"))
         self.u.copy_(state)
         return time + float(accepted.mean().item())
+
+    def bind_settings(self, settings: NumericalSettings) -> None:
+        """Evaluate with ``settings`` from now on (the reconstruction and
+        viscosity data uploaded at creation must cover them)."""
+        if settings is self.settings:
+            return
+        self.settings = settings
+        self.case.numerics = settings
+        self.native_settings = native_settings(settings)
+        if getattr(self, "_implicit", None) is not None:
+            self._implicit.bind_settings()
+
+    def implicit_stepper(self):
+        """The implicit-Euler machinery of this mesh (built on first use)."""
+        from .implicit import ImplicitStepper
+
+        if getattr(self, "_implicit", None) is None:
+            self._implicit = ImplicitStepper(self)
+        return self._implicit
+
+    def time_steps_primitive(self, w: torch.Tensor) -> torch.Tensor:
+        """`_stable_time_steps` of device primitive states (C,4): per-cell
+        steps into ``dt_cell`` and their minimum into ``dt_min``."""
+        native.check(
+            self.lib.mcfd_time_steps_primitive(
+                self.handle, C.byref(self.native_settings), _ptr(w), _ptr(self.dt_cell), _ptr(self.dt_min), self.stream
+            )
+        )
+        return self.dt_cell
 
     def time_steps(self, u: torch.Tensor) -> torch.Tensor:
         """`_stable_time_steps` (`solver.py:4132-4343`): per-cell steps into
@@ -415,8 +444,6 @@ class DeviceMesh:
                                             self.stream)
             )
             dt, per_cell = self.clock[1:], False
-        ti = s.time_integration
-        u = self.u
         # the fused final stage also computes the next step's CFL steps
         fuse_dt = (
             self.fuse_time_step and self.blocks is not None and mode != "Global fixed"
@@ -426,6 +453,21 @@ class DeviceMesh:
         if fuse_dt:  # the global modes need only the minimum; batches and local steps the per-cell steps
             per_cell_next = local or self.case_ptr is not None
             nxt = dict(dt_next=self.dt_cell, dt_min=self.dt_min) if per_cell_next else dict(dt_min=self.dt_min)
+        self.advance(s.time_integration, dt, per_cell, residual_out, nxt)
+        if not fuse_dt:
+            # the end-of-step state check (solver.py:806-808); the fused final
+            # stage makes it while computing the next step's CFL steps
+            native.check(self.lib.mcfd_check_state(self.handle, C.byref(self.native_settings), _ptr(self.u),
+                                                   self.stream))
+        self.dt_fresh = fuse_dt
+
+    def advance(self, ti: str, dt: torch.Tensor, per_cell: bool, residual_out: torch.Tensor | None = None,
+                nxt: dict | None = None) -> None:
+        """The stages of one explicit step of integrator ``ti`` on ``self.u``
+        with the step sizes ``dt`` (device, [1] or [C]); ``nxt`` asks the
+        final fused stage for the next step's CFL steps."""
+        nxt = nxt or {}
+        u = self.u
         if ti == "Explicit Euler":
             self._stage(native.OP_EULER, u, out=self.s1, dt=dt, per_cell=per_cell, coeffs=(1.0,), r_out=residual_out,
                         **nxt)
@@ -453,7 +495,6 @@ class DeviceMesh:
                         coeffs=c[4], **nxt)
         else:
             raise SolverError(f"Unsupported time integration '{ti}' for the explicit GPU path.")
-        self.dt_fresh = fuse_dt
 
     def run(self, steps: int, time: float = 0.0, check_every: int = 0) -> float:
         """Advance ``steps`` steps; returns the final time (one host sync)."""

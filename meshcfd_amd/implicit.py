@@ -29,6 +29,26 @@ from .layout import DeviceLayout
 LUSGS_ORDERINGS = ("natural", "multicolour")
 
 
+def halve_until_solved(solve_once, time_step, allow_reduction: bool = True):
+    """The retry loop of `_implicit_euler_step` (`solver.py:1337-1368`):
+    ``solve_once(step)``, halving the step after each `SolverError`, at most
+    12 attempts (1 without reduction). Returns (solution, accepted step);
+    ``time_step`` is a float, a NumPy array or a device tensor."""
+    h = time_step
+    last = None
+    for _ in range(12 if allow_reduction else 1):
+        try:
+            return solve_once(h), h
+        except SolverError as error:
+            last = error
+            h = h * 0.5
+    if not allow_reduction:
+        raise SolverError(f"Implicit Euler could not converge at the configured fixed time step: {last}") from last
+    raise SolverError(
+        f"Implicit Euler could not converge after reducing the time step 12 times; last failure: {last}"
+    ) from last
+
+
 def colour_cells(ptr: np.ndarray, nbr: np.ndarray, seed: int = 0) -> np.ndarray:
     """Greedy colouring of the cell graph (CSR ``ptr``, ``nbr``) in
     Jones-Plassmann rounds: each round the uncoloured cells whose seeded
@@ -284,8 +304,7 @@ class ImplicitStepper:
         native.check(self.lib.mcfd_mesh_set_implicit(dm.handle, C.byref(desc)))
         del keep
         s = dm.settings
-        self.first_settings = native.McfdSettings.from_buffer_copy(dm.native_settings)
-        self.first_settings.reconstruction = 0
+        self.bind_settings()
         opts = dict(dtype=torch.float64, device=dm.device)
         n = dm.n
         self.rate1 = torch.empty((n, 4), **opts)
@@ -301,6 +320,12 @@ class ImplicitStepper:
         # iterations so far, GMRES iterations of this solve, residual norm)
         self.on_newton = None
         del s
+
+    def bind_settings(self) -> None:
+        """The Jacobian's settings: the mesh's, with first-order
+        reconstruction (`solver.py:1378`)."""
+        self.first_settings = native.McfdSettings.from_buffer_copy(self.dm.native_settings)
+        self.first_settings.reconstruction = 0
 
     # -- pieces -----------------------------------------------------------
     def assemble(self, u, dt, per_cell):
@@ -412,16 +437,4 @@ class ImplicitStepper:
 
     def step(self, initial: torch.Tensor, dt, per_cell: bool, allow_reduction: bool = True):
         """`_implicit_euler_step`: returns (state, accepted dt tensor)."""
-        h = dt.clone()
-        last = None
-        for _ in range(12 if allow_reduction else 1):
-            try:
-                return self.solve_once(initial, h, per_cell), h
-            except SolverError as error:
-                last = error
-                h.mul_(0.5)
-        if not allow_reduction:
-            raise SolverError(f"Implicit Euler could not converge at the configured fixed time step: {last}")
-        raise SolverError(
-            f"Implicit Euler could not converge after reducing the time step 12 times; last failure: {last}"
-        )
+        return halve_until_solved(lambda h: self.solve_once(initial, h, per_cell), dt.clone(), allow_reduction)

@@ -28,6 +28,9 @@ ERROR_MESSAGES = {
     11: "Sutherland law produced an invalid viscosity.",
 }
 
+# codes whose status index is a device cell (state, residual, local step, temperature)
+CELL_ERROR_MASK = (1 << 1) | (1 << 7) | (1 << 8) | (1 << 10)
+
 OP_EULER, OP_SSP, OP_RK4, OP_RK4_FINAL, OP_RESIDUAL = 0, 1, 2, 3, 4
 KERNEL_NAMES = ("primitive", "gradient", "flux", "update", "time_step", "reduce", "stage", "viscous")
 
@@ -190,6 +193,11 @@ EXPORTS = {
         C.c_int,
         [C.c_void_p, C.POINTER(McfdSettings), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     ),
+    "mcfd_time_steps_primitive": (
+        C.c_int,
+        [C.c_void_p, C.POINTER(McfdSettings), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+    ),
+    "mcfd_check_state": (C.c_int, [C.c_void_p, C.POINTER(McfdSettings), C.c_void_p, C.c_void_p]),
     "mcfd_advance_clock": (
         C.c_int,
         [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_int32, C.c_void_p],

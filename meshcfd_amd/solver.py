@@ -17,6 +17,7 @@ the initial states).
 from __future__ import annotations
 
 import math
+import weakref
 from dataclasses import dataclass
 from time import thread_time
 from typing import Callable
@@ -25,7 +26,8 @@ import numpy as np
 import torch
 
 from .geometry import SolverError
-from .model import Case, FiniteVolumeMesh, InitialState
+from .implicit import halve_until_solved
+from .model import Case, FiniteVolumeMesh, InitialState, NumericalSettings
 
 
 @dataclass(frozen=True)
@@ -158,6 +160,309 @@ def validate(case: Case, start_time: float, step_offset: int, solution_interval:
         raise SolverError("Body-force acceleration must be finite.")
 
 
+# ---------------------------------------------------------------------------
+# Seams: the reference's module functions (`solver.py`), NumPy in and out, in
+# mesh order. Each runs its arithmetic on the GPU (a device copy of the
+# geometry is kept per geometry object). `solve_case` calls them by module
+# name whenever one of them has been replaced (as the reference's tests
+# replace them with monkeypatch); otherwise it keeps the state on the device
+# and runs the same step sequence with the fused stage kernels.
+# ---------------------------------------------------------------------------
+
+SEAM_DEVICE = "cuda:0"
+_seam_meshes: dict = {}
+
+
+def _drop_seam_mesh(key) -> None:
+    dm = _seam_meshes.pop(key, None)
+    if dm is not None:
+        dm.close()
+
+
+def _seam_mesh(geometry, settings: NumericalSettings):
+    """The device mesh of ``geometry`` bound to ``settings`` (kept while the
+    geometry object lives)."""
+    from .device import DeviceMesh
+
+    key = (id(geometry), settings.reconstruction == "MUSCL parabolic", settings.viscosity_model != "Inviscid")
+    dm = _seam_meshes.get(key)
+    if dm is None:
+        dm = DeviceMesh(Case(mesh=None, boundary_conditions=[], numerics=settings), SEAM_DEVICE, geometry=geometry)
+        _seam_meshes[key] = dm
+        weakref.finalize(geometry, _drop_seam_mesh, key)
+    dm.bind_settings(settings)
+    return dm
+
+
+def _no_turbulence(turbulence) -> None:
+    if turbulence is not None:
+        raise SolverError("The Menter k-omega SST model is not implemented by the GPU path.")
+
+
+def _device_steps(dm, time_step):
+    """A scalar step or per-cell steps (mesh order) as a device tensor."""
+    if np.ndim(time_step) == 0:
+        return torch.full((1,), float(time_step), dtype=torch.float64, device=dm.device), False
+    return dm.to_device_order(np.asarray(time_step, dtype=np.float64)), True
+
+
+def _residual(conservative, geometry, settings, turbulence_conservative=None) -> np.ndarray:
+    """`_residual` (`solver.py:2437-2718`): the spatial rate dU/dt of a (C,4)
+    conservative state."""
+    _no_turbulence(turbulence_conservative)
+    dm = _seam_mesh(geometry, settings)
+    r = dm.residual(dm.to_device_order(conservative))
+    dm.check_status()
+    return dm.to_mesh_order(r)
+
+
+def _stable_time_steps(primitive, geometry, settings, turbulence_conservative=None) -> np.ndarray:
+    """`_stable_time_steps` (`solver.py:4132-4343`): per-cell CFL steps of a
+    (C,4) primitive state."""
+    _no_turbulence(turbulence_conservative)
+    dm = _seam_mesh(geometry, settings)
+    w = dm.to_device_order(primitive)
+    dm.time_steps_primitive(w)
+    dm.check_status()
+    return dm.to_mesh_order(dm.dt_cell)
+
+
+def _stable_time_step(primitive, geometry, settings, turbulence_conservative=None) -> float:
+    """`_stable_time_step` (`solver.py:4117-4129`): the smallest local step."""
+    return float(np.min(_stable_time_steps(primitive, geometry, settings, turbulence_conservative)))
+
+
+def _scale_cell_rate(rate: np.ndarray, time_step) -> np.ndarray:
+    """`_scale_cell_rate` (`solver.py:1119-1125`)."""
+    if np.ndim(time_step) == 0:
+        return time_step * rate
+    step_sizes = np.asarray(time_step, dtype=float)
+    return step_sizes * rate if rate.ndim == 1 else step_sizes[:, None] * rate
+
+
+def _time_state_rhs(state, geometry, settings) -> np.ndarray:
+    """`_time_state_rhs` (`solver.py:1051-1105`) for the four flow variables."""
+    if np.shape(state)[1] != 4:
+        _no_turbulence(state)
+    return _residual(state, geometry, settings)
+
+
+def _explicit_rk4_step(state, time_step, geometry, settings) -> np.ndarray:
+    """`_explicit_rk4_step` (`solver.py:1128-1182`), SSPRK(5,4): the five
+    fused stages on the GPU; through `_time_state_rhs` on the host arrays if
+    that seam or `_residual` has been replaced."""
+    if _time_state_rhs is not _DEVICE_SEAMS["_time_state_rhs"] or _residual is not _DEVICE_SEAMS["_residual"]:
+        return _rk4_through_rhs(np.asarray(state, dtype=np.float64), time_step, geometry, settings)
+    dm = _seam_mesh(geometry, settings)
+    dm.u.copy_(dm.to_device_order(state))
+    dt, per_cell = _device_steps(dm, time_step)
+    dm.advance("Explicit RK4", dt, per_cell)
+    dm.check_status()
+    return dm.to_mesh_order(dm.u)
+
+
+def _rk4_through_rhs(initial, h, geometry, settings):
+    """The SSPRK(5,4) stage sequence of `solver.py:1142-1181` over
+    `_time_state_rhs` (its coefficients are the device kernels' RK4_COEFFS)."""
+    from .device import RK4_COEFFS as c
+
+    def scaled(rate, k):
+        return _scale_cell_rate(rate, k * h) if np.ndim(h) == 0 else _scale_cell_rate(rate, k * np.asarray(h))
+
+    d = _time_state_rhs(initial, geometry, settings)
+    s1 = initial + scaled(d, c[0][0])
+    d = _time_state_rhs(s1, geometry, settings)
+    s2 = c[1][0] * initial + c[1][1] * s1 + scaled(d, c[1][2])
+    d = _time_state_rhs(s2, geometry, settings)
+    s3 = c[2][0] * initial + c[2][1] * s2 + scaled(d, c[2][2])
+    d3 = _time_state_rhs(s3, geometry, settings)
+    s4 = c[3][0] * initial + c[3][1] * s3 + scaled(d3, c[3][2])
+    d4 = _time_state_rhs(s4, geometry, settings)
+    return c[4][0] * s2 + c[4][1] * s3 + scaled(d3, c[4][2]) + c[4][3] * s4 + scaled(d4, c[4][4])
+
+
+def _implicit_euler_solve_once(initial, time_step, geometry, settings) -> np.ndarray:
+    """`_implicit_euler_solve_once` (`solver.py:1371-1486`): one Newton solve
+    of U - U^n - dt R(U) = 0 at the given step (matrix-free GMRES with the
+    LU-SGS preconditioner on the GPU)."""
+    dm = _seam_mesh(geometry, settings)
+    stepper = dm.implicit_stepper()
+    dt, per_cell = _device_steps(dm, time_step)
+    state = stepper.solve_once(dm.to_device_order(initial), dt, per_cell)
+    return dm.to_mesh_order(state)
+
+
+def _implicit_euler_step(initial, time_step, geometry, settings, allow_time_step_reduction: bool = True):
+    """`_implicit_euler_step` (`solver.py:1337-1368`): `_implicit_euler_solve_once`
+    with the step halved up to 12 times; returns (state, accepted step)."""
+    h = float(time_step) if np.ndim(time_step) == 0 else np.asarray(time_step, dtype=float).copy()
+    return halve_until_solved(lambda step: _implicit_euler_solve_once(initial, step, geometry, settings), h,
+                              allow_time_step_reduction)
+
+
+_SEAM_NAMES = ("_residual", "_stable_time_steps", "_stable_time_step", "_time_state_rhs", "_explicit_rk4_step",
+               "_implicit_euler_solve_once", "_implicit_euler_step")
+_DEVICE_SEAMS = {name: globals()[name] for name in _SEAM_NAMES}
+
+
+def seams_replaced() -> bool:
+    """True when a caller has replaced one of this module's seams."""
+    g = globals()
+    return any(g[name] is not fn for name, fn in _DEVICE_SEAMS.items())
+
+
+# ---------------------------------------------------------------------------
+# solve_case
+# ---------------------------------------------------------------------------
+
+
+class _DeviceRun:
+    """The step sequence with the state resident on the GPU (`DeviceMesh`)."""
+
+    def __init__(self, case, conservative, device, mesh_device, lusgs_ordering):
+        from .device import DeviceMesh
+
+        self.own = mesh_device is None
+        dm = mesh_device if mesh_device is not None else DeviceMesh(case, device)
+        if getattr(dm, "_implicit", None) is None:
+            dm.lusgs_ordering = lusgs_ordering
+        elif dm.lusgs_ordering != lusgs_ordering:
+            raise SolverError("The device mesh's implicit structure was built with another LU-SGS ordering.")
+        dm.set_state(conservative)
+        self.dm = dm
+        self.s = case.numerics
+        self.r = torch.empty((dm.n, 4), dtype=torch.float64, device=dm.device)
+        self.comps = None
+
+    def start(self, time):
+        self.dm.clock[0] = time
+        self.dm.dt_fresh = False
+
+    def _rms(self):
+        return np.sqrt(self.dm.sum_squares(self.r).cpu().numpy() / self.dm.n)
+
+    def step(self, time, want_residual):
+        """One step from ``time``; returns (new time, reported step). With
+        ``want_residual`` keeps R(start) for the progress report."""
+        dm, s = self.dm, self.s
+        mode = s.time_step_mode
+        if s.time_integration == "Implicit Euler":
+            if want_residual:
+                dm.residual(dm.u, out=self.r)
+            new_time = dm.step_implicit(time)
+            time_step = new_time - time
+        elif mode == "Local (steady)":
+            # this step's local steps: their mean is the reported step and the time advance
+            if not dm.dt_fresh:
+                dm.time_steps(dm.u)
+                dm.dt_fresh = True
+            time_step = float(dm.dt_cell.mean().item())
+            dm.step(residual_out=None)
+            new_time = time + time_step
+        else:
+            dm.step(residual_out=self.r if want_residual else None)
+            clock = dm.clock.cpu().numpy()
+            new_time, time_step = float(clock[0]), float(clock[1])
+        dm.check_status()
+        return new_time, time_step
+
+    def steady_residual(self):
+        self.dm.residual(self.dm.u, out=self.r)
+        self.dm.check_status()
+        self.comps = self._rms()
+        return math.sqrt(float(np.mean(self.comps**2)))
+
+    def components(self, local):
+        return self.comps if local else self._rms()
+
+    def primitive(self):
+        return conservative_to_primitive(self.dm.state(), self.s.gamma)
+
+    def final_residual(self):
+        self.dm.residual(self.dm.u, out=self.r)
+        self.dm.check_status()
+        return math.sqrt(float(np.mean(self._rms() ** 2)))
+
+    def close(self):
+        if self.own:
+            self.dm.close()
+
+
+class _SeamRun:
+    """The reference's step sequence (`solver.py:437-812`) through this
+    module's seams, on host arrays: used when a seam has been replaced."""
+
+    def __init__(self, case, conservative):
+        from .geometry import build_geometry
+
+        self.geometry = build_geometry(case)
+        self.s = case.numerics
+        self.u = np.asarray(conservative, dtype=np.float64).copy()
+        self.derivative = None
+
+    def start(self, time):
+        pass
+
+    def step(self, time, want_residual):
+        s, g = self.s, self.geometry
+        gamma = s.gamma
+        mode = s.time_step_mode
+        primitive = conservative_to_primitive(self.u, gamma)
+        if mode == "Local (steady)":
+            sizes = _stable_time_steps(primitive, g, s)
+            time_step = float(np.mean(sizes))
+        elif mode == "Global fixed":
+            time_step = min(s.fixed_time_step, s.end_time - time)
+            sizes = time_step
+        else:
+            time_step = min(_stable_time_step(primitive, g, s), s.end_time - time)
+            sizes = time_step
+        if not math.isfinite(time_step) or time_step <= 0.0:
+            raise SolverError("The stable time step is non-positive or non-finite.")
+        start = self.u
+        derivative = _residual(start, g, s)
+        ti = s.time_integration
+        if ti == "Implicit Euler":
+            new, accepted = _implicit_euler_step(start, sizes, g, s,
+                                                 allow_time_step_reduction=mode != "Global fixed")
+            time_step = float(np.mean(np.asarray(accepted)))
+        elif ti == "Explicit RK4":
+            new = _explicit_rk4_step(start, sizes, g, s)
+        elif ti == "Explicit Euler":
+            new = start + _scale_cell_rate(derivative, sizes)
+        else:
+            one = start + _scale_cell_rate(derivative, sizes)
+            conservative_to_primitive(one, gamma)
+            d1 = _residual(one, g, s)
+            if ti == "Explicit RK2":
+                new = 0.5 * start + 0.5 * (one + _scale_cell_rate(d1, sizes))
+            else:
+                two = 0.75 * start + 0.25 * (one + _scale_cell_rate(d1, sizes))
+                conservative_to_primitive(two, gamma)
+                d2 = _residual(two, g, s)
+                new = (1.0 / 3.0) * start + (2.0 / 3.0) * (two + _scale_cell_rate(d2, sizes))
+        conservative_to_primitive(new, gamma)
+        self.u = new
+        self.derivative = derivative
+        return time + time_step, time_step
+
+    def steady_residual(self):
+        self.derivative = _residual(self.u, self.geometry, self.s)
+        return float(np.sqrt(np.mean(self.derivative**2)))
+
+    def components(self, local):
+        return np.sqrt(np.mean(self.derivative**2, axis=0))
+
+    def primitive(self):
+        return conservative_to_primitive(self.u, self.s.gamma)
+
+    def final_residual(self):
+        return float(np.sqrt(np.mean(_residual(self.u, self.geometry, self.s) ** 2)))
+
+    def close(self):
+        pass
+
+
 def solve_case(
     case: Case,
     initial_states: tuple[InitialState, ...] | np.ndarray | None = None,
@@ -181,9 +486,10 @@ def solve_case(
     `DeviceMesh` of the same case may be passed as ``mesh_device``.
     ``lusgs_ordering`` "multicolour" sweeps the implicit preconditioner colour
     by colour (a different preconditioner than the reference's natural
-    order, with the same converged solutions; much faster on large meshes)."""
-    from .device import DeviceMesh
-
+    order, with the same converged solutions; much faster on large meshes).
+    When a caller has replaced one of this module's seams (`_residual`,
+    `_stable_time_steps`, ...), every step goes through the seams by name, as
+    in the reference."""
     cpu_start = thread_time()
     validate(case, start_time, step_offset, solution_interval, cpu_time_offset)
     s = case.numerics
@@ -204,86 +510,51 @@ def solve_case(
         raise SolverError(f"Invalid initial state: {error}") from error
     if lusgs_ordering not in ("natural", "multicolour"):
         raise SolverError(f"Unsupported LU-SGS ordering '{lusgs_ordering}'.")
-    dm = mesh_device if mesh_device is not None else DeviceMesh(case, device)
-    if getattr(dm, "_implicit", None) is None:
-        dm.lusgs_ordering = lusgs_ordering
-    elif dm.lusgs_ordering != lusgs_ordering:
-        raise SolverError("The device mesh's implicit structure was built with another LU-SGS ordering.")
-    dm.set_state(conservative)
-    n = dm.n
-    implicit = s.time_integration == "Implicit Euler"
-    r = torch.empty((n, 4), dtype=torch.float64, device=dm.device)
-
-    def rms_components(res):
-        sums = dm.sum_squares(res).cpu().numpy()
-        return np.sqrt(sums / n)
-
-    def host_primitive():
-        return conservative_to_primitive(dm.state(), s.gamma)
+    if seams_replaced():
+        run = _SeamRun(case, conservative)
+    else:
+        run = _DeviceRun(case, conservative, device, mesh_device, lusgs_ordering)
 
     time = start_time
     completed = False
     steps = step_offset
-    dm.clock[0] = time
-    dm.dt_fresh = False
+    run.start(time)
     for local_step in range(1, s.max_iterations + 1):
         step = step_offset + local_step
         if pause_callback is not None:
             pause_callback()
         if cancel_callback is not None and cancel_callback():
             break
-        report_due = progress_callback is not None and step % s.convergence_history_interval == 0
         try:
-            want_r = progress_callback is not None and not local
-            if implicit:
-                if want_r:
-                    dm.residual(dm.u, out=r)
-                new_time = dm.step_implicit(time)
-                time_step = new_time - time
-            elif local:
-                # this step's local steps: their mean is the reported step and the time advance
-                if not dm.dt_fresh:
-                    dm.time_steps(dm.u)
-                    dm.dt_fresh = True
-                time_step = float(dm.dt_cell.mean().item())
-                dm.step(residual_out=None)
-                new_time = time + time_step
-            else:
-                dm.step(residual_out=r if want_r else None)
-                clock = dm.clock.cpu().numpy()
-                new_time, time_step = float(clock[0]), float(clock[1])
-            dm.check_status()
-        except SolverError as error:
+            time, time_step = run.step(time, progress_callback is not None and not local)
+        except (FloatingPointError, ValueError, ZeroDivisionError) as error:
             raise SolverError(f"Solver failed at step {step}: {error}") from error
-        time = new_time
         steps = step
         steady_converged = False
         if local:
-            dm.residual(dm.u, out=r)
             try:
-                dm.check_status()
+                steady_residual = run.steady_residual()
             except SolverError as error:
                 raise SolverError(f"Solver failed at step {step}: {error}") from error
-            comps = rms_components(r)
-            steady_residual = math.sqrt(float(np.mean(comps**2)))
             steady_converged = steady_residual <= s.residual_tolerance
         end_time_reached = not local and time >= s.end_time * (1.0 - 1e-13)
         final_step = (end_time_reached or steady_converged or local_step == s.max_iterations
                       or (cancel_callback is not None and cancel_callback()))
-        report_residual = progress_callback is not None and (report_due or final_step)
+        report_residual = progress_callback is not None and (step % s.convergence_history_interval == 0
+                                                             or final_step)
         progress_states = None
         if progress_callback is not None:
             if live_solution_updates and (local_step == 1 or step % 10 == 0 or end_time_reached or steady_converged
                                           or local_step == s.max_iterations):
-                progress_states = _states(host_primitive())
+                progress_states = _states(run.primitive())
             if report_residual or progress_states is not None:
                 component_residuals = ()
                 if report_residual:
-                    component_residuals = tuple(float(x) for x in (comps if local else rms_components(r)))
+                    component_residuals = tuple(float(x) for x in run.components(local))
                 progress_callback(SolverProgress(step, time, time_step, component_residuals, progress_states, None,
                                                  cpu_time_offset + thread_time() - cpu_start, None))
         if solution_callback is not None and step % solution_interval == 0:
-            snapshot = progress_states if progress_states is not None else _states(host_primitive())
+            snapshot = progress_states if progress_states is not None else _states(run.primitive())
             solution_callback(step, time, snapshot)
         if end_time_reached:
             time = s.end_time
@@ -292,14 +563,11 @@ def solve_case(
         if steady_converged:
             completed = True
             break
-    final = host_primitive()
-    dm.residual(dm.u, out=r)
+    final = run.primitive()
     try:
-        dm.check_status()
+        residual = run.final_residual()
     except SolverError as error:
         raise SolverError(f"Could not evaluate final residual: {error}") from error
-    residual = math.sqrt(float(np.mean(rms_components(r) ** 2)))
-    if mesh_device is None:
-        dm.close()
+    run.close()
     return SolverResult(_states(final), time, steps, residual, completed,
                         cpu_time_offset + thread_time() - cpu_start, final)
