@@ -538,6 +538,15 @@ int mcfd_batch_case_steps(McfdMesh* mesh, int32_t n_cases, const int32_t* case_p
 /* Per-component sum of R^2 over cells into sums[4] (device), deterministic. */
 int mcfd_sum_squares(McfdMesh* mesh, const double* r, double* sums, void* stream);
 
+/* mcfd_sum_squares without a mesh and without allocating: r (C,4) and
+ * sums[4] on the device, n_cells <= 16,777,216 (else an error). No
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (HIP graph capture, serving loops). The block partials live in one
+ * static 2 MiB device array per GPU, held until the process ends: calls that
+ * overlap on one GPU (concurrent streams or threads) race on it. Same bits as
+ * mcfd_sum_squares on a mesh of n_cells cells. */
+int mcfd_sum_squares_noalloc(const double* r, int32_t n_cells, double* sums, void* stream);
+
 /* dst[0:count] = src[0:count] (device, count even): a plain streaming copy,
  * the reference kernel for HBM byte counters (FETCH_SIZE / WRITE_SIZE). */
 int mcfd_copy(const double* src, double* dst, size_t count, void* stream);

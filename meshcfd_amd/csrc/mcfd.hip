@@ -21,6 +21,7 @@
 #include <cstring>
 #include <string>
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "../../include/mcfd.h"
@@ -4533,7 +4534,7 @@ int upload(McfdMesh* m, T** dst, const T* src, size_t count) {
 }
 
 template <typename T>
-int scratch(McfdMesh* m, T** dst, size_t count) {
+int alloc_scratch(McfdMesh* m, T** dst, size_t count) {
   void* p = nullptr;
   HIP_CHECK(hipMalloc(&p, sizeof(T) * (count ? count : 1)));
   m->allocations.push_back(p);
@@ -5442,12 +5443,12 @@ int mcfd_mesh_create(const McfdMeshDesc* d, McfdMesh** out) {
   rc |= upload(m, const_cast<double**>(&fa.bc_wall), d->bc_wall, 2 * (size_t)d->n_bc);
   m->dts = DtArrays{m->dts_ptr, m->dts_nx, m->dts_ny, m->dts_len,
                     m->dts_bc,  m->bc_kind, fa.bc_state, m->area};
-  rc |= scratch(m, &m->W, 4 * C);
-  rc |= scratch(m, &m->G, 8 * C);
-  rc |= scratch(m, &m->PHI, 4 * C);
-  rc |= scratch(m, &m->FLUX, 4 * F);
-  rc |= scratch(m, &m->partial, 4 * (size_t)blocks(d->n_cells) + 4);
-  rc |= scratch(m, &m->status, 2);
+  rc |= alloc_scratch(m, &m->W, 4 * C);
+  rc |= alloc_scratch(m, &m->G, 8 * C);
+  rc |= alloc_scratch(m, &m->PHI, 4 * C);
+  rc |= alloc_scratch(m, &m->FLUX, 4 * F);
+  rc |= alloc_scratch(m, &m->partial, 4 * (size_t)blocks(d->n_cells) + 4);
+  rc |= alloc_scratch(m, &m->status, 2);
   if (rc) {
     std::string msg = g_last_error;
     free_mesh(m);
@@ -5689,6 +5690,56 @@ int mcfd_sum_squares(McfdMesh* m, const double* r, double* sums, void* stream) {
   int nb = blocks(m->n_cells);
   hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(kBlock), 0, s, m->n_cells, r, m->partial);
   hipLaunchKernelGGL(k_sumsq_final, dim3(1), dim3(256), 0, s, nb, m->partial, sums);
+  HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free residual norms: the block partials of k_sumsq live in a
+// static device array instead of a mesh's scratch. Each GPU holds its own
+// copy of the array (2 MiB) until the process ends; its address is looked up
+// once per device, with that device current, on the first call there.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocMaxCells = 1 << 24;                 // 16,777,216 cells
+constexpr int kNoAllocBlocks = kNoAllocMaxCells / kBlock;  // block partials
+static __device__ __align__(256) double scratch[4 * kNoAllocBlocks];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+double* g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mutex;
+
+// The device address of `scratch` on the current device (host code must
+// never pass the array's name itself to a kernel).
+int scratch_address(double** out) {
+  int dev = -1;
+  HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail("mcfd_sum_squares_noalloc: device id out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch[dev];
+  return 0;
+}
+
+}  // namespace
+
+int mcfd_sum_squares_noalloc(const double* r, int32_t n_cells, double* sums, void* stream) {
+  if (!r || !sums) return fail("mcfd_sum_squares_noalloc: NULL argument");
+  if (n_cells < 0 || n_cells > kNoAllocMaxCells)
+    return fail("mcfd_sum_squares_noalloc: n_cells must be in [0, 16777216], the capacity of its static scratch");
+  double* partial = nullptr;
+  if (scratch_address(&partial)) return -1;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = blocks(n_cells);
+  // every partial that k_sumsq_final reads is written first by k_sumsq in
+  // stream order, so nothing is cleared
+  if (nb > 0) hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(kBlock), 0, s, n_cells, r, partial);
+  hipLaunchKernelGGL(k_sumsq_final, dim3(1), dim3(256), 0, s, nb, partial, sums);
   HIP_CHECK(hipGetLastError());
   return 0;
 }

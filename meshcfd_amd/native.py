@@ -213,6 +213,7 @@ EXPORTS = {
     "mcfd_batch_case_steps": (
         C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p],
     ),
+    "mcfd_sum_squares_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "mcfd_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "mcfd_status": (C.c_int, [C.c_void_p, C.c_void_p, _ip, _ip]),
     "mcfd_timing": (C.c_int, [C.c_void_p, C.c_int32, _dp, _ip]),
