@@ -1474,12 +1474,36 @@ __device__ __forceinline__ P4 rec_lds(const double* sW, const double* sG, const 
   return o;
 }
 
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+// Copy n items of SIZE bytes from global memory to LDS with direct-to-LDS
+// loads (global_load_lds): wave w of the workgroup moves 64-item chunks w,
+// w + 4, ...; lane l of chunk c moves item 64c + l (the data never passes
+// through VGPRs). Completion is awaited by the s_waitcnt before the next
+// barrier.
+template <int SIZE>  // 4 or 16
+__device__ __forceinline__ void lds_copy(const void* src, void* dst, int n) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int c = w; c * 64 < n; c += kBlock / 64) {
+    const int e = c * 64 + lane;
+    if (e < n) {
+      const void* g = static_cast<const char*>(src) + (size_t)e * SIZE;
+      lds_void_t* l = (lds_void_t*)(static_cast<char*>(dst) + (size_t)c * 64 * SIZE);
+      if constexpr (SIZE == 16)
+        __builtin_amdgcn_global_load_lds(g, l, 16, 0, 0);
+      else
+        __builtin_amdgcn_global_load_lds(g, l, 4, 0, 0);
+    }
+  }
+}
+
 // Instantiated per flux scheme, reconstruction, far-field presence and
 // next-step dt so that each launch carries only the code it runs (the full
 // kernel is ~48 KB, most of a CU pair's instruction cache).
 template <int SCHEME, bool MUSCL, int LIM, bool FARFIELD, bool DTNEXT>
 __global__ void __launch_bounds__(kBlock)
-    k_stage_fused(int max_acc, int stage_off, int max_local, int max_grad, int max_faces, BlockArrays ba,
+    k_stage_fused(int max_acc, int stage_off, int max_local, int max_grad, int max_faces, int max_lsq, int max_lim,
+                  BlockArrays ba,
                   GeomArrays ga, FaceArrays fa, DtArrays da, double g, double ax, double ay,
                   StageArgs s, int* __restrict__ status) {
   constexpr int limiter = LIM;
@@ -1492,10 +1516,11 @@ __global__ void __launch_bounds__(kBlock)
   const int first = inf[8], ls = inf[9];
   const int n_loc = n_own + n_r1 + n_r2;
   const int n_grad = n_own + n_r1;
-  // LDS: W of every local cell, then one region that holds the gradients
-  // and limiter factors of own + ring-1 cells until the face fluxes are
-  // computed, and the face fluxes and lengths afterwards (and the block
-  // reduction of the next time step at the end).
+  // LDS: W of every local cell, then one region that holds in turn the
+  // block's LSQ and limiter entries (MUSCL: copied in at kernel start), the
+  // gradients and limiter factors of own + ring-1 cells until the face
+  // fluxes are computed, and the face fluxes and lengths afterwards (and the
+  // block reduction of the next time step at the end).
   double* sW = smem;
   double* sG = sW + 4 * max_local;
   double* sP = sG + 8 * max_grad;
@@ -1522,6 +1547,17 @@ __global__ void __launch_bounds__(kBlock)
     l0 = ba.lptr[gps + t];
     l1 = ba.lptr[gps + t + 1];
   }
+  // The block's LSQ coefficients, limiter offsets and LSQ neighbour ids go
+  // straight to LDS while P0 runs, so P1 reads only LDS.
+  const double2* sGE = reinterpret_cast<const double2*>(sG);  // [max_lsq]
+  const double2* sLE = sGE + max_lsq;                          // [max_lim]
+  const unsigned short* sGN = reinterpret_cast<const unsigned short*>(sLE + max_lim) + (gs & 1);
+  if constexpr (MUSCL) {
+    const int n_lsq = ba.gptr[gps + n_grad], n_lim = ba.lptr[gps + n_grad];
+    lds_copy<16>(ba.gent + gs, const_cast<double2*>(sGE), n_lsq);
+    lds_copy<16>(ba.lent + ls, const_cast<double2*>(sLE), n_lim);
+    lds_copy<4>(ba.gnb + (gs & ~1), const_cast<double2*>(sLE + max_lim), ((gs & 1) + n_lsq + 1) >> 1);
+  }
 
   // P0: primitive states (euler.py:20-40)
   for (int j = t; j < n_loc; j += kBlock) {
@@ -1543,16 +1579,20 @@ __global__ void __launch_bounds__(kBlock)
   __syncthreads();
 
   // P1: LSQ gradient (_kernels.py:252-262) + limiter (_kernels.py:9-67)
+  // P2's face record: read from HBM while the block waits at P1's barriers.
+  const int f = t;
+  FaceRec fr{};
   if constexpr (MUSCL) {
+    double gx[4] = {0.0, 0.0, 0.0, 0.0};
+    double gy[4] = {0.0, 0.0, 0.0, 0.0};
+    double phi[4] = {1.0, 1.0, 1.0, 1.0};
     if (t < n_grad) {
       const int j = t;
       double wv[4] = {sW[4 * j], sW[4 * j + 1], sW[4 * j + 2], sW[4 * j + 3]};
-      double gx[4] = {0.0, 0.0, 0.0, 0.0};
-      double gy[4] = {0.0, 0.0, 0.0, 0.0};
       double mn[4] = {wv[0], wv[1], wv[2], wv[3]};
       double mx[4] = {wv[0], wv[1], wv[2], wv[3]};
-      const unsigned short* nbl = ba.gnb + gs;
-      const double2* cxy = ba.gent + gs;
+      const unsigned short* nbl = sGN;
+      const double2* cxy = sGE;
       // entries in batches of 4: the batch's loads are issued together, the
       // sums still run in entry order
       for (int base = e0; base < e1; base += 4) {
@@ -1583,8 +1623,7 @@ __global__ void __launch_bounds__(kBlock)
           }
         }
       }
-      double phi[4] = {1.0, 1.0, 1.0, 1.0};
-      const double2* lxy = ba.lent + ls;
+      const double2* lxy = sLE;
       for (int base = l0; base < l1; base += 4) {
         double dxb[4], dyb[4];
 #pragma unroll
@@ -1605,11 +1644,16 @@ __global__ void __launch_bounds__(kBlock)
         }
         }
       }
+    }
+    if (f < n_faces) fr = ba.frec[fs + f];
+    // the gradients and limiter factors overwrite the entries
+    __syncthreads();
+    if (t < n_grad) {
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        sG[8 * j + 2 * k] = gx[k];
-        sG[8 * j + 2 * k + 1] = gy[k];
-        sP[4 * j + k] = phi[k];
+        sG[8 * t + 2 * k] = gx[k];
+        sG[8 * t + 2 * k + 1] = gy[k];
+        sP[4 * t + k] = phi[k];
       }
     }
     __syncthreads();
@@ -1620,9 +1664,8 @@ __global__ void __launch_bounds__(kBlock)
   // the gradient slots they overwrite.
   F4 fl{0.0, 0.0, 0.0, 0.0};
   double flen = 0.0, fnx = 0.0, fny = 0.0;
-  const int f = t;
   if (f < n_faces) {
-    const FaceRec fr = ba.frec[fs + f];
+    if constexpr (!MUSCL) fr = ba.frec[fs + f];
     const int kind = fr.kind_bc & 15;
     const int l = fr.lr & 0xFFFF, r = fr.lr >> 16;
     const double nx = fr.nx, ny = fr.ny;
@@ -4454,6 +4497,7 @@ struct McfdMesh {
   bool fused = false;
   int nb = 0, block_cells = 0, max_local = 0, max_grad = 0, max_faces = 0;
   int max_acc = 0, max_dts = 0;  // most accumulation / spectral-radius items of a block's own cells
+  int max_lsq = 0, max_lim = 0;  // most LSQ / limiter entries of a block's own + ring-1 cells
   double* block_partial = nullptr;  // [nb] per-block minima of the fused final stage
   BlockArrays blocks{};
   std::vector<void*> block_allocations;
@@ -4568,9 +4612,15 @@ int upload_block(McfdMesh* m, T** dst, const T* src, size_t count) {
 // then the own cells' accumulation coefficients and items and their
 // spectral-radius items, staged for the gather.
 // The staging is written after P2, over W when it fits there.
+// LSQ coefficients, limiter offsets and (dword-aligned, 16-bit) LSQ
+// neighbour ids of a block, copied into the region at kernel start.
+size_t fused_entry_doubles(const McfdMesh* m) {
+  return 2 * (size_t)m->max_lsq + 2 * (size_t)m->max_lim + ((size_t)m->max_lsq + 2 + 3) / 4;
+}
+
 size_t fused_region(const McfdMesh* m, bool muscl) {
   size_t region = 7 * (size_t)m->max_faces;
-  if (muscl) region = std::max(region, 12 * (size_t)m->max_grad);
+  if (muscl) region = std::max(region, std::max(12 * (size_t)m->max_grad, fused_entry_doubles(m)));
   return std::max(region, (size_t)kBlock);
 }
 
@@ -4765,7 +4815,7 @@ void launch_fused_t(McfdMesh* m, size_t lds, hipStream_t s, const GeomArrays& ga
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                             hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024);
   hipLaunchKernelGGL(kern, dim3(m->nb), dim3(kBlock), lds, s, m->max_acc, fused_stage_offset(m, MUSCL), m->max_local,
-                     m->max_grad, m->max_faces, m->blocks, ga, m->faces, m->dts, st->gamma,
+                     m->max_grad, m->max_faces, m->max_lsq, m->max_lim, m->blocks, ga, m->faces, m->dts, st->gamma,
                      st->acceleration_x, st->acceleration_y, a, m->status);
 }
 
@@ -5504,6 +5554,16 @@ int mcfd_mesh_set_blocks(McfdMesh* m, const McfdBlockDesc* d) {
       m->max_acc = std::max(m->max_acc, aptr[inf[8] + inf[1]] - aptr[inf[8]]);
       m->max_dts = std::max(m->max_dts, dptr[inf[8] + inf[1]] - dptr[inf[8]]);
     }
+    // entries per block: the block-relative offsets after its last gradient cell
+    m->max_lsq = 0;
+    m->max_lim = 0;
+    for (int b = 0; b < d->n_blocks; ++b) {
+      const int32_t* inf = d->info + (size_t)kBlockInfo * b;
+      const long end = (long)inf[7] + inf[1] + inf[2];
+      if (end >= d->n_gptr) return fail("mcfd_mesh_set_blocks: gptr too short");
+      m->max_lsq = std::max(m->max_lsq, d->gptr[end]);
+      m->max_lim = std::max(m->max_lim, d->lptr[end]);
+    }
   }
   if (fused_lds_bytes(m, true) > 64 * 1024) {
     free_blocks(m);
@@ -5514,7 +5574,13 @@ int mcfd_mesh_set_blocks(McfdMesh* m, const McfdBlockDesc* d) {
   rc |= upload_block(m, const_cast<int**>(&b.info), d->info, kBlockInfo * (size_t)d->n_blocks);
   rc |= upload_block(m, const_cast<int**>(&b.cells), d->cells, (size_t)d->n_local);
   rc |= upload_block(m, const_cast<int**>(&b.gptr), d->gptr, (size_t)d->n_gptr);
-  rc |= upload_block(m, const_cast<unsigned short**>(&b.gnb), d->gnb, (size_t)d->n_gnb);
+  {
+    // two spare entries: a block's ids are copied as dwords from the even
+    // entry at or before its first one to the dword holding its last one
+    std::vector<uint16_t> gnb(d->gnb, d->gnb + d->n_gnb);
+    gnb.resize((size_t)d->n_gnb + 2, 0);
+    rc |= upload_block(m, const_cast<unsigned short**>(&b.gnb), gnb.data(), gnb.size());
+  }
   rc |= upload_block(m, reinterpret_cast<double**>(const_cast<double2**>(&b.gent)), d->gent,
                      2 * (size_t)d->n_gnb);
   rc |= upload_block(m, const_cast<int**>(&b.lptr), d->lptr, (size_t)d->n_gptr);
