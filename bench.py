@@ -7,10 +7,14 @@ transmissive boundaries. One "step" = one SSP-RK3 step (CFL time step + three
 residual/stage evaluations) over the whole mesh.
 
 Prints one JSON line. ``value`` = cells * steps / seconds / 1e6
-(Mcell-updates/s) with the state resident in HBM; ``roofline`` reports the
+(Mcell-updates/s) with the state resident in HBM, from the median of
+``--repeats`` timed windows of ``--steps`` steps; ``roofline`` reports the
 dominant kernel against the 8 TB/s HBM peak from its algorithmic bytes; the
 ``stage`` figures use B_alg per cell-update over the whole step;
-``cpu_baseline`` times the CPU oracle on one host core on a bounded sample.
+``cpu_baseline`` times the CPU oracle on one host core on a bounded sample
+(median of five runs); ``batch`` times 256 cases of uneven size advanced
+together (HIP-graph replays); ``host`` names the CPU, GPU and library
+versions.
 """
 
 from __future__ import annotations
@@ -105,11 +109,46 @@ def adjoint_rate(dm, torch, repeats: int = 20):
     }
 
 
-def cpu_baseline(case, geometry, primitive, steps: int):
+def host_info() -> dict:
+    """CPU model, NumPy/SciPy/torch versions, GPU name and ROCm version."""
+    import scipy
+    import torch
+
+    model = platform.processor() or platform.machine()
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    rocm = None
+    try:
+        rocm = Path("/opt/rocm/.info/version").read_text().strip()
+    except OSError:
+        pass
+    return {
+        "cpu_model": model,
+        "host_cores": os.cpu_count(),
+        "affinity_cores": len(os.sched_getaffinity(0)),
+        "numpy": np.__version__,
+        "scipy": scipy.__version__,
+        "torch": torch.__version__,
+        "hip_runtime": getattr(torch.version, "hip", None),
+        "rocm": rocm,
+        "gpu": torch.cuda.get_device_name(0) if torch.cuda.is_available() else None,
+        "gpu_arch": getattr(torch.cuda.get_device_properties(0), "gcnArchName", None)
+        if torch.cuda.is_available() else None,
+        "gpu_cus": torch.cuda.get_device_properties(0).multi_processor_count if torch.cuda.is_available() else None,
+    }
+
+
+def cpu_baseline(case, geometry, primitive, steps: int, samples: int = 5):
     """Time the reference-equivalent CPU path on one core: the NumPy/SciPy
     restatement of the solver with single-threaded C stand-ins for the
     reference's numba kernels (oracle/mcfd_oracle.c), as the reference runs
-    them compiled."""
+    them compiled. ``samples`` runs of ``steps`` SSP-RK3 steps each from the
+    initial state after one warm-up residual; the rate is the median."""
     from oracle import compiled
     from oracle.euler import primitive_to_conservative
     from oracle.solver import Oracle
@@ -123,14 +162,18 @@ def cpu_baseline(case, geometry, primitive, steps: int):
     os.sched_setaffinity(0, {core})
     try:
         oracle = Oracle(geometry, case.numerics)
-        u = primitive_to_conservative(primitive, case.numerics.gamma)
-        oracle.residual(u)  # warm-up residual (benchmarks/run_benchmarks.py:266-267)
-        t0 = time.perf_counter()
-        t = 0.0
-        for _ in range(steps):
-            u, dt, _, _ = oracle.step(u, t)
-            t += dt
-        elapsed = time.perf_counter() - t0
+        u0 = primitive_to_conservative(primitive, case.numerics.gamma)
+        oracle.residual(u0)  # warm-up residual (benchmarks/run_benchmarks.py:266-267)
+        times = []
+        for _ in range(samples):
+            u = u0
+            t0 = time.perf_counter()
+            t = 0.0
+            for _ in range(steps):
+                u, dt, _, _ = oracle.step(u, t)
+                t += dt
+            times.append(time.perf_counter() - t0)
+        elapsed = float(np.median(times))
     finally:
         os.sched_setaffinity(0, old)
         if use_c:
@@ -141,12 +184,12 @@ def cpu_baseline(case, geometry, primitive, steps: int):
         "unit": "Mcell-updates/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{steps} SSP-RK3 step(s) of the full {cells}-cell mesh after one warm-up residual; "
+        "sample": f"median of {samples} runs of {steps} SSP-RK3 step(s) of the full {cells}-cell mesh after one "
+        "warm-up residual; "
         + ("NumPy/SciPy oracle with C stand-ins for the numba kernels" if use_c else "NumPy/SciPy oracle")
         + ", pinned to 1 core",
         "seconds": elapsed,
-        "host_cores": os.cpu_count(),
-        "cpu_model": platform.processor() or platform.machine(),
+        "sample_seconds": [round(x, 3) for x in times],
     }
 
 
@@ -216,15 +259,15 @@ def cylinder_bench(args) -> int:
     return 0
 
 
-def batch_bench(args) -> int:
-    """A batch of ``--cases`` cases advanced together (cases alternate an
+def batch_leg(cases: int, steps: int, warmup: int, graph_steps: int, block_cells: int = 128) -> dict:
+    """A batch of ``cases`` cases advanced together (cases alternate an
     isentropic vortex and a moving shock tube; cell counts log-uniform in
     [2e3, 2e5] and Mach uniform in [0.1, 2.0] from seed 2; HLLC +
     MUSCL-linear BJ, SSP-RK3, each case its own CFL 0.5 step) on one
     concatenated device mesh: per-case steps by segmented minima, then the
-    three fused stages over all cases. Times ``--steps`` steps eagerly and
-    as replays of a HIP graph of ``--graph-steps`` steps; prints one JSON line
-    with Mcell-updates/s = sum(C) x steps / t of the graph replays."""
+    three fused stages over all cases. Times ``steps`` steps eagerly and as
+    replays of a HIP graph of ``graph_steps`` steps; Mcell-updates/s =
+    sum(C) x steps / t of the graph replays."""
     import torch
 
     from meshcfd_amd import batch as mb
@@ -234,30 +277,30 @@ def batch_bench(args) -> int:
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
     t0 = time.perf_counter()
-    specs = mb.batch_specs(args.cases, seed=2)
+    specs = mb.batch_specs(cases, seed=2)
     b = mb.build_batch(specs)
     note(f"{b.n_cases} cases, {b.geometry.cell_count} cells built in {time.perf_counter() - t0:.1f}s")
-    dm = DeviceMesh.from_batch(b, "cuda:0", block_cells=args.block_cells)
+    dm = DeviceMesh.from_batch(b, "cuda:0", block_cells=block_cells)
     setup_s = time.perf_counter() - t0
     gamma = b.numerics.gamma
     prim = np.concatenate(b.primitives)
     rho, u, v, p = prim.T
     dm.set_state(np.column_stack((rho, rho * u, rho * v, p / (gamma - 1.0) + 0.5 * rho * (u**2 + v**2))))
     note(f"setup {setup_s:.1f}s")
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         dm.step()
     dm.check_status("warm-up: ")
     torch.cuda.synchronize()
     start = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         dm.step()
     torch.cuda.synchronize()
     eager_s = time.perf_counter() - start
     dm.check_status("eager steps: ")
-    note(f"eager: {eager_s / args.steps * 1e3:.3f} ms per step")
-    g = max(1, args.graph_steps)
+    note(f"eager: {eager_s / steps * 1e3:.3f} ms per step")
+    g = max(1, graph_steps)
     dm.capture(g)
-    replays = max(1, args.steps // g)
+    replays = max(1, steps // g)
     dm.replay()
     torch.cuda.synchronize()
     start = time.perf_counter()
@@ -266,16 +309,18 @@ def batch_bench(args) -> int:
     torch.cuda.synchronize()
     graph_s = time.perf_counter() - start
     dm.check_status("graph steps: ")
+    eager_steps = steps
     steps = replays * g
     cells = dm.n
     clocks = dm.case_clock.view(-1, 2)[:, 0].cpu().numpy()
-    line = {
+    dm.close()
+    return {
         "metric": "batched Mcell-updates/s (HLLC+MUSCL RK3, FP64, per-case CFL)",
         "value": round(cells * steps / graph_s / 1e6, 2),
         "unit": "Mcell-updates/s",
         "n_gpus": 1,
         "steps": steps,
-        "warmup": args.warmup,
+        "warmup": warmup,
         "ms_per_step": round(graph_s / steps * 1e3, 4),
         "higher_is_better": True,
         "dtype": "f64",
@@ -283,11 +328,16 @@ def batch_bench(args) -> int:
         "config": {"workload": f"{b.n_cases} cases (vortex / moving shock tube), cells log-uniform in [2e3, 2e5], "
                                "Mach uniform in [0.1, 2.0], seed 2; HLLC + MUSCL-linear (BJ), SSP-RK3, CFL 0.5 per case",
                    "cases": b.n_cases, "cells": cells, "graph_steps": g},
-        "eager": {"ms_per_step": round(eager_s / args.steps * 1e3, 4),
-                  "value": round(cells * args.steps / eager_s / 1e6, 2)},
+        "eager": {"ms_per_step": round(eager_s / eager_steps * 1e3, 4),
+                  "value": round(cells * eager_steps / eager_s / 1e6, 2)},
         "case_time_range": [float(clocks.min()), float(clocks.max())],
         "setup_s": round(setup_s, 1),
     }
+
+
+def batch_bench(args) -> int:
+    """`batch_leg` alone, printed as one JSON line."""
+    line = batch_leg(args.cases, args.steps, args.warmup, args.graph_steps, args.block_cells)
     print(json.dumps(line), flush=True)
     return 0
 
@@ -298,7 +348,9 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=1000, help="quads per side")
-    ap.add_argument("--cpu-steps", type=int, default=20, help="SSP-RK3 steps timed on the CPU oracle (0 = skip)")
+    ap.add_argument("--repeats", type=int, default=5, help="timed windows of --steps steps; value from the median")
+    ap.add_argument("--cpu-steps", type=int, default=4,
+                    help="SSP-RK3 steps per CPU-oracle sample, 5 samples (0 = skip)")
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--unfused", action="store_true", help="four-kernel path instead of the fused stage kernel")
     ap.add_argument("--block-cells", type=int, default=128, help="cells per workgroup of the fused stage kernel")
@@ -308,6 +360,8 @@ def main(argv=None) -> int:
                     "O-grid cylinder (Roe + MUSCL parabolic + Sutherland, LU-SGS + GMRES); batch: 256 cases of "
                     "uneven size advanced together")
     ap.add_argument("--cases", type=int, default=256, help="batch: number of cases")
+    ap.add_argument("--batch-cases", type=int, default=256,
+                    help="riemann: also time a batch of this many cases as the JSON line's 'batch' key (0 = skip)")
     ap.add_argument("--graph-steps", type=int, default=10, help="batch: steps per captured HIP graph")
     ap.add_argument("--n-theta", type=int, default=2048, help="cylinder: cells around")
     ap.add_argument("--n-r", type=int, default=2048, help="cylinder: cells outward")
@@ -385,19 +439,23 @@ def main(argv=None) -> int:
     dm.check_status("warm-up: ")
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    start.record(stream)
-    for _ in range(args.steps):
-        dm.step()
-    stop.record(stream)
-    torch.cuda.synchronize()
-    ms = start.elapsed_time(stop)
-    dm.check_status("timed run: ")
+    windows = []
+    for _ in range(max(1, args.repeats)):
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        start.record(stream)
+        for _ in range(args.steps):
+            dm.step()
+        stop.record(stream)
+        torch.cuda.synchronize()
+        windows.append(start.elapsed_time(stop))
+        dm.check_status("timed run: ")
+    ms = float(np.median(windows))
     cells = layout.n_cells
     value = cells * args.steps / (ms / 1e3) / 1e6
-    print(f"[bench] {args.steps} steps in {ms:.2f} ms", file=sys.stderr, flush=True)
+    print(f"[bench] {len(windows)} windows of {args.steps} steps: {[round(w, 3) for w in windows]} ms",
+          file=sys.stderr, flush=True)
 
     b_alg, kernel_bytes = algorithmic_bytes(layout, case.numerics.reconstruction, case.numerics.time_integration)
     stage_gbs = b_alg * cells * args.steps / (ms / 1e3) / 1e9
@@ -441,6 +499,12 @@ def main(argv=None) -> int:
         adj = adjoint_rate(dm, torch)
         print(f"[bench] adjoint {adj}", file=sys.stderr, flush=True)
 
+    dm.close()
+    batch = None
+    if args.batch_cases > 0:
+        print("[bench] batch leg ...", file=sys.stderr, flush=True)
+        batch = batch_leg(args.batch_cases, 100, 10, 10)
+
     cpu = None
     if args.cpu_steps > 0:
         print("[bench] CPU baseline ...", file=sys.stderr, flush=True)
@@ -454,6 +518,8 @@ def main(argv=None) -> int:
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms / args.steps, 5),
+        "repeats": len(windows),
+        "window_ms": [round(w, 4) for w in windows],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -480,10 +546,11 @@ def main(argv=None) -> int:
         "kernel_ms": {k: round(t / c, 5) for k, (t, c) in kernel_times.items()},
         "cpu_baseline": cpu,
         "adjoint": adj,
+        "batch": batch,
+        "host": host_info(),
         "setup_s": round(setup_s, 2),
     }
     print(json.dumps(line), flush=True)
-    dm.close()
     return 0
 
 
