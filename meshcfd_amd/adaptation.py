@@ -1,0 +1,969 @@
+"""Adaptive refinement and coarsening of polygon meshes (host side).
+
+Follows the reference's `src/meshcfd/core/adaptation.py`:
+
+* `select_adaptation` (`adaptation.py:40-171`): a distance-weighted
+  least-squares gradient of the indicator variable per cell (face neighbours
+  and periodic partners), the indicator max(|grad| sqrt(A), largest jump) /
+  value scale, refinement marks above the threshold below the maximum level
+  (anisotropic "x" / "y" in a quadrilateral's local directions when the
+  variation along one is more than twice the other and the split passes the
+  quality gate, else "xy"), coarsening of refinement records whose children
+  all fall below 0.3 x threshold, then the 2:1 balance and the quality gate
+  until both hold;
+* `balance_plan` (`adaptation.py:252-451`): periodic partners share marks and
+  coarsening, projected levels and per-axis (directional) levels differ by at
+  most one across every face, resolving the first violation in cell order
+  and scanning again, with the reference's choices at each violation;
+* `adapt_mesh` (`adaptation.py:680-795`): leaves in cell order, coarsened
+  siblings replaced by their parent at the first child's position, marked
+  cells split by `split_polygon` (`adaptation.py:830-938`: quadrilaterals at
+  edge midpoints and the vertex mean, curved boundary edges at their
+  parametric midpoints; other polygons clipped at their bounding-box
+  middles), then `_rebuild_mesh` (`adaptation.py:1032-1190`): every leaf edge
+  is cut at the mesh points lying on it, so a coarse cell next to refined
+  ones gets hanging-node subfaces, faces numbered at first visit;
+* `transfer_cell_values` (`solver.py:1594-1617`): area-weighted averages of
+  the source cells, rescaled so each source group keeps its integral.
+
+Geometry is exact in the same floating-point steps as the reference (edge
+midpoints 0.5 (a + b), quad centres by summing the corners and dividing by
+four, points keyed by rounding to 14 decimals), so adapted meshes carry the
+reference's coordinates and numbering. Pure Python over the leaves, like the
+reference: meant for meshes of up to a few hundred thousand cells.
+"""
+
+from __future__ import annotations
+
+import json
+import math
+from bisect import bisect_left, bisect_right
+from dataclasses import dataclass
+
+import numpy as np
+
+from .curves import evaluate_boundary_curve
+from .model import BoundaryCurve, FiniteVolumeMesh, MeshPatch, RefinementRecord
+
+TOL = 1e-11  # the reference's geometry tolerance (adaptation.py:25)
+AXES = ("x", "y")
+
+
+class AdaptationBalanceError(ValueError):
+    """A refinement plan that cannot satisfy the balance constraints
+    (reference `AdaptationBalanceError`, adaptation.py:27-28)."""
+
+
+@dataclass(frozen=True)
+class Leaf:
+    cell_id: str
+    level: int
+    vertices: tuple
+    sources: tuple
+    curves: tuple = ()
+
+
+# ---------------------------------------------------------------------------
+# Mesh access
+# ---------------------------------------------------------------------------
+
+
+def cell_vertices(mesh: FiniteVolumeMesh, cell: int) -> tuple:
+    """The cell's vertices as (x, y) float tuples, in its CCW order."""
+    ids = mesh.cells[cell, : mesh.cell_sizes[cell]]
+    return tuple((float(mesh.points[n, 0]), float(mesh.points[n, 1])) for n in ids)
+
+
+def _cell_ids(mesh):
+    return tuple(mesh.cell_ids) if len(mesh.cell_ids) else tuple(f"r{k}" for k in range(mesh.cell_count))
+
+
+def _cell_levels(mesh):
+    return tuple(int(v) for v in mesh.cell_levels) if len(mesh.cell_levels) else (0,) * mesh.cell_count
+
+
+def cell_areas(mesh: FiniteVolumeMesh) -> np.ndarray:
+    """|signed area| of every cell (reference `cell_areas`, adaptation.py:512-513)."""
+    return np.array([abs(signed_area(cell_vertices(mesh, c))) for c in range(mesh.cell_count)])
+
+
+# ---------------------------------------------------------------------------
+# Planar helpers (reference adaptation.py:967-1249), same float steps
+# ---------------------------------------------------------------------------
+
+
+def point_key(p):
+    return round(p[0], 14), round(p[1], 14)
+
+
+def edge_key(a, b):
+    return tuple(sorted((point_key(a), point_key(b))))
+
+
+def dist2(a, b):
+    return (a[0] - b[0]) ** 2 + (a[1] - b[1]) ** 2
+
+
+def signed_area(vertices) -> float:
+    n = len(vertices)
+    total = 0
+    for k in range(n):
+        a, b = vertices[k], vertices[(k + 1) % n]
+        total = total + (a[0] * b[1] - b[0] * a[1])
+    return 0.5 * total
+
+
+def clean_polygon(poly) -> tuple:
+    """Drop repeated consecutive points (and a closing repeat)."""
+    out = []
+    for p in poly:
+        if not out or dist2(out[-1], p) > 1e-26:
+            out.append(p)
+    if len(out) > 1 and dist2(out[0], out[-1]) <= 1e-26:
+        out.pop()
+    return tuple(out)
+
+
+def drop_collinear(poly) -> tuple:
+    """Remove vertices between two collinear edges, first one found each pass."""
+    v = list(clean_polygon(poly))
+    while len(v) > 3:
+        for k, cur in enumerate(v):
+            prev, nxt = v[k - 1], v[(k + 1) % len(v)]
+            e0 = (cur[0] - prev[0], cur[1] - prev[1])
+            e1 = (nxt[0] - cur[0], nxt[1] - cur[1])
+            cross = e0[0] * e1[1] - e0[1] * e1[0]
+            if abs(cross) <= TOL * max(math.hypot(*e0) * math.hypot(*e1), 1e-30):
+                v.pop(k)
+                break
+        else:
+            break
+    return tuple(v)
+
+
+def clip_polygon(poly, axis: int, value: float, keep_less: bool) -> tuple:
+    """Sutherland-Hodgman clip against the line coordinate[axis] = value."""
+    if not poly:
+        return ()
+
+    def inside(p):
+        d = p[axis] - value
+        return d <= TOL if keep_less else d >= -TOL
+
+    out = []
+    n = len(poly)
+    for k in range(n):
+        a, b = poly[k], poly[(k + 1) % n]
+        ia, ib = inside(a), inside(b)
+        if ia:
+            out.append(a)
+        if ia != ib:
+            frac = (value - a[axis]) / (b[axis] - a[axis])
+            out.append((a[0] + frac * (b[0] - a[0]), a[1] + frac * (b[1] - a[1])))
+    return tuple(out)
+
+
+def on_segment(p, a, b) -> bool:
+    ex, ey = b[0] - a[0], b[1] - a[1]
+    ox, oy = p[0] - a[0], p[1] - a[1]
+    length = math.hypot(ex, ey)
+    if length <= 1e-15:
+        return dist2(p, a) <= TOL**2
+    cross = abs(ex * oy - ey * ox) / length
+    t = (ox * ex + oy * ey) / (length * length)
+    return cross <= TOL and -TOL <= t <= 1 + TOL
+
+
+def segment_parameter(p, a, b) -> float:
+    dx, dy = b[0] - a[0], b[1] - a[1]
+    return ((p[0] - a[0]) * dx + (p[1] - a[1]) * dy) / (dx * dx + dy * dy)
+
+
+def aspect_ratio(vertices) -> float:
+    """sqrt of the second-moment eigenvalue ratio of the corners
+    (adaptation.py:189-197)."""
+    pts = np.asarray(drop_collinear(vertices), dtype=float)
+    if len(pts) < 3:
+        return math.inf
+    c = pts - np.mean(pts, axis=0)
+    ev = np.linalg.eigvalsh(c.T @ c / len(pts))
+    if ev[0] <= TOL**2:
+        return math.inf
+    return math.sqrt(float(ev[-1] / ev[0]))
+
+
+def local_directions(corners):
+    """Unit mean directions of a quadrilateral's two edge pairs
+    (adaptation.py:174-181)."""
+    p = np.asarray(corners, dtype=float)
+    first = 0.5 * ((p[1] - p[0]) + (p[2] - p[3]))
+    second = 0.5 * ((p[3] - p[0]) + (p[2] - p[1]))
+    if np.linalg.norm(first) <= TOL or np.linalg.norm(second) <= TOL:
+        return np.asarray((1.0, 0.0)), np.asarray((0.0, 1.0))
+    return first / np.linalg.norm(first), second / np.linalg.norm(second)
+
+
+def span_along(vertices, direction) -> float:
+    return float(np.ptp(np.asarray(vertices, dtype=float) @ direction))
+
+
+# ---------------------------------------------------------------------------
+# Boundary curves on cell edges
+# ---------------------------------------------------------------------------
+
+
+def curve_midpoint(curve: BoundaryCurve):
+    return evaluate_boundary_curve(curve.geometry, 0.5 * (curve.start_parameter + curve.end_parameter))
+
+
+def _reversed(curve: BoundaryCurve) -> BoundaryCurve:
+    return BoundaryCurve(curve.geometry, curve.end_parameter, curve.start_parameter, curve.patch_id)
+
+
+def cell_edge_curves(mesh: FiniteVolumeMesh, cell: int, by_edge: dict | None) -> tuple:
+    """Per edge of the cell (start -> end): its boundary curve oriented along
+    the edge, or None; () when the mesh has no curves (adaptation.py:516-545)."""
+    if by_edge is None:
+        return ()
+    ids = [int(n) for n in mesh.cells[cell, : mesh.cell_sizes[cell]]]
+    out = []
+    for k, a in enumerate(ids):
+        b = ids[(k + 1) % len(ids)]
+        hit = by_edge.get((min(a, b), max(a, b)))
+        if hit is None:
+            out.append(None)
+            continue
+        face, curve = hit
+        out.append(curve if (int(mesh.faces[face, 0]), int(mesh.faces[face, 1])) == (a, b) else _reversed(curve))
+    return tuple(out)
+
+
+def _curves_by_edge(mesh: FiniteVolumeMesh):
+    if not mesh.face_curves:
+        return None
+    return {
+        (min(int(mesh.faces[f, 0]), int(mesh.faces[f, 1])), max(int(mesh.faces[f, 0]), int(mesh.faces[f, 1]))):
+        (f, c)
+        for f, c in enumerate(mesh.face_curves)
+        if c is not None
+    }
+
+
+def _is_curved(corners, curve_of_edge: dict) -> bool:
+    """An edge whose curve midpoint leaves the chord midpoint (adaptation.py:653-671)."""
+    n = len(corners)
+    for k in range(n):
+        a, b = corners[k], corners[(k + 1) % n]
+        curve = curve_of_edge.get(edge_key(a, b))
+        if curve is None:
+            continue
+        m = curve_midpoint(curve)
+        chord = (0.5 * (a[0] + b[0]), 0.5 * (a[1] + b[1]))
+        scale = max(1.0, abs(a[0]), abs(a[1]), abs(b[0]), abs(b[1]))
+        if dist2(m, chord) > (scale * 1e-12) ** 2:
+            return True
+    return False
+
+
+def child_edge_curves(vertices, curves, children) -> tuple:
+    """Curves of the children's edges that lie on a curved parent edge: the
+    parent curve's parameters at its end points and midpoint
+    (adaptation.py:598-644)."""
+    if len(curves) != len(vertices):
+        return tuple(() for _ in children)
+    segments = []
+    n = len(vertices)
+    for k in range(n):
+        curve = curves[k]
+        if curve is None:
+            continue
+        a, b = vertices[k], vertices[(k + 1) % n]
+        params = {
+            point_key(a): curve.start_parameter,
+            point_key(curve_midpoint(curve)): 0.5 * (curve.start_parameter + curve.end_parameter),
+            point_key(b): curve.end_parameter,
+        }
+        segments.append((curve.geometry, curve.patch_id, params))
+    result = []
+    for child in children:
+        edges = []
+        m = len(child)
+        for k in range(m):
+            ka, kb = point_key(child[k]), point_key(child[(k + 1) % m])
+            found = None
+            for geometry, patch, params in segments:
+                if ka in params and kb in params:
+                    found = BoundaryCurve(geometry, params[ka], params[kb], patch)
+                    break
+            edges.append(found)
+        result.append(tuple(edges))
+    return tuple(result)
+
+
+def merged_edge_curves(vertices, leaves) -> tuple:
+    """The parent's edge curves from its children's (adaptation.py:548-595)."""
+    grouped: dict = {}
+    for leaf in leaves:
+        if len(leaf.curves) != len(leaf.vertices):
+            continue
+        m = len(leaf.vertices)
+        for k in range(m):
+            curve = leaf.curves[k]
+            if curve is None:
+                continue
+            key = (json.dumps(curve.geometry, sort_keys=True, separators=(",", ":")), curve.patch_id)
+            geometry, params = grouped.setdefault(key, (curve.geometry, {}))
+            params.setdefault(point_key(leaf.vertices[k]), curve.start_parameter)
+            params.setdefault(point_key(leaf.vertices[(k + 1) % m]), curve.end_parameter)
+    out = []
+    n = len(vertices)
+    for k in range(n):
+        ka, kb = point_key(vertices[k]), point_key(vertices[(k + 1) % n])
+        hits = [(g, key[1], p) for key, (g, p) in grouped.items() if ka in p and kb in p]
+        if len(hits) != 1:
+            out.append(None)
+            continue
+        g, patch, p = hits[0]
+        out.append(BoundaryCurve(g, p[ka], p[kb], patch))
+    return tuple(out)
+
+
+# ---------------------------------------------------------------------------
+# Splitting
+# ---------------------------------------------------------------------------
+
+
+def _check_area(parent, pieces, curved: bool = False):
+    """Children must tile the parent (adaptation.py:941-964)."""
+    parent_area = abs(signed_area(parent))
+    child_area = sum(abs(signed_area(p)) for p in pieces)
+    if curved:
+        orientation = math.copysign(1.0, signed_area(parent))
+        if any(signed_area(p) * orientation <= 1e-14 for p in pieces):
+            raise ValueError("Curved boundary splitting produced an invalid cell.")
+        return pieces
+    if not math.isclose(parent_area, child_area, rel_tol=1e-10, abs_tol=max(parent_area, 1.0) * 1e-12):
+        raise ValueError("Adaptive splitting requires a convex cell that can be partitioned by axis-aligned cuts.")
+    return pieces
+
+
+def split_polygon(polygon, axis: str, curves=()) -> tuple:
+    """Children of a cell split along ``axis`` ("x", "y" or "xy")
+    (adaptation.py:830-938). Quadrilaterals split at edge midpoints (a curved
+    boundary edge at its parametric midpoint): "x"/"y" across the edge pair
+    whose midpoints differ most in y/x, "xy" into four around the corner
+    mean; other polygons are clipped at their bounding-box middles."""
+    corners = drop_collinear(polygon)
+    if len(corners) == 4:
+        curve_of_edge = {}
+        if len(curves) == len(polygon):
+            n = len(polygon)
+            for k in range(n):
+                if curves[k] is not None:
+                    curve_of_edge[edge_key(polygon[k], polygon[(k + 1) % n])] = curves[k]
+        mids = []
+        for k in range(4):
+            a, b = corners[k], corners[(k + 1) % 4]
+            c = curve_of_edge.get(edge_key(a, b))
+            mids.append(curve_midpoint(c) if c is not None else (0.5 * (a[0] + b[0]), 0.5 * (a[1] + b[1])))
+        curved = _is_curved(corners, curve_of_edge)
+        if axis in AXES:
+            coord = 1 if axis == "x" else 0
+            first, opposite = max(((0, 2), (1, 3)), key=lambda pr: abs(mids[pr[0]][coord] - mids[pr[1]][coord]))
+            half_a = [mids[first]]
+            k = (first + 1) % 4
+            while k != (opposite + 1) % 4:
+                half_a.append(corners[k])
+                k = (k + 1) % 4
+            half_a.append(mids[opposite])
+            half_b = [mids[opposite]]
+            while k != (first + 1) % 4:
+                half_b.append(corners[k])
+                k = (k + 1) % 4
+            half_b.append(mids[first])
+            return _check_area(corners, (tuple(half_a), tuple(half_b)), curved)
+        if axis == "xy":
+            sx = 0
+            sy = 0
+            for p in corners:
+                sx = sx + p[0]
+                sy = sy + p[1]
+            centre = (sx / 4.0, sy / 4.0)
+            pieces = tuple((corners[k], mids[k], centre, mids[(k - 1) % 4]) for k in range(4))
+            return _check_area(corners, pieces, curved)
+    xs = [p[0] for p in polygon]
+    ys = [p[1] for p in polygon]
+    mx = 0.5 * (min(xs) + max(xs))
+    my = 0.5 * (min(ys) + max(ys))
+    if axis == "x":
+        pieces = (clip_polygon(polygon, 0, mx, True), clip_polygon(polygon, 0, mx, False))
+    elif axis == "y":
+        pieces = (clip_polygon(polygon, 1, my, True), clip_polygon(polygon, 1, my, False))
+    elif axis == "xy":
+        pieces = tuple(clip_polygon(clip_polygon(polygon, 0, mx, left), 1, my, bottom)
+                       for left, bottom in ((True, True), (False, True), (True, False), (False, False)))
+    else:
+        raise ValueError(f"Unsupported refinement split axis '{axis}'.")
+    valid = []
+    for piece in pieces:
+        c = clean_polygon(piece)
+        if len(c) >= 3 and abs(signed_area(c)) > 1e-14:
+            valid.append(c)
+    return _check_area(polygon, tuple(valid))
+
+
+def split_passes_quality(vertices, axis: str, max_aspect: float = 6.0) -> bool:
+    """The quality gate (adaptation.py:200-213): every child keeps >= 10% of
+    the parent's area and an aspect ratio <= max_aspect."""
+    try:
+        children = split_polygon(vertices, axis)
+    except ValueError:
+        return False
+    parent = abs(signed_area(vertices))
+    if parent <= TOL:
+        return False
+    for child in children:
+        if abs(signed_area(child)) / parent < 0.10 or aspect_ratio(child) > max_aspect:
+            return False
+    return True
+
+
+def gated_axis(vertices, axis: str) -> str:
+    return axis if axis == "xy" or split_passes_quality(vertices, axis) else "xy"
+
+
+# ---------------------------------------------------------------------------
+# Directional levels and the balance
+# ---------------------------------------------------------------------------
+
+
+def directional_levels(vertices, base) -> tuple:
+    """(x, y) refinement level of a cell against its root cell: the rounded
+    log2 of the bounding-box span ratio per axis (adaptation.py:493-509)."""
+    out = []
+    for axis in (0, 1):
+        span = max(p[axis] for p in vertices) - min(p[axis] for p in vertices)
+        base_span = max(p[axis] for p in base) - min(p[axis] for p in base)
+        out.append(0 if span <= TOL or base_span <= TOL else max(0, round(math.log2(base_span / span))))
+    return out[0], out[1]
+
+
+def _root_vertices(records):
+    return {r.parent_id: r.vertices for r in records if "." not in r.parent_id}
+
+
+def cell_directional_levels(mesh, cell_ids, records) -> tuple:
+    roots = _root_vertices(records)
+    out = []
+    for k, cid in enumerate(cell_ids):
+        v = cell_vertices(mesh, k)
+        out.append(directional_levels(v, roots.get(cid.split(".", 1)[0], v)))
+    return tuple(out)
+
+
+def parent_directional_levels(records) -> dict:
+    roots = _root_vertices(records)
+    return {r.parent_id: directional_levels(r.vertices, roots.get(r.parent_id.split(".", 1)[0], r.vertices))
+            for r in records}
+
+
+def _least_disruptive_axis(cell, neighbours, projected, levels_dir) -> int:
+    """The split axis whose projected directional levels jump least against
+    the neighbours (max jump, then total, then the cell's own level;
+    adaptation.py:225-249)."""
+    own = projected[cell]
+
+    def cost(a):
+        jumps = [abs(own[c] + int(c == a) - projected[nb][c]) for nb in neighbours[cell] for c in (0, 1)]
+        return max(jumps), sum(jumps), levels_dir[cell][a]
+
+    return min((0, 1), key=cost)
+
+
+def balance_plan(marks: dict, coarsen: set, neighbours, cell_ids, levels, levels_dir, parent_dir: dict,
+                 maximum_level: int, records, periodic_pairs) -> None:
+    """Adjust ``marks`` and ``coarsen`` in place until projected levels and
+    projected directional levels differ by at most one across every face
+    (adaptation.py:252-451); raises `AdaptationBalanceError` when that is
+    impossible or the plan cycles."""
+    by_parent = {r.parent_id: r for r in records}
+    parent_of = {child: r.parent_id for r in records for child in r.child_ids}
+    seen = set()
+
+    def projected():
+        lev, dirs = [], []
+        for k, cid in enumerate(cell_ids):
+            parent = parent_of.get(cid)
+            if parent in coarsen:
+                lev.append(by_parent[parent].level)
+                dirs.append(parent_dir[parent])
+            else:
+                lev.append(levels[k] + int(k in marks))
+                m = marks.get(k, "")
+                dirs.append((levels_dir[k][0] + int("x" in m), levels_dir[k][1] + int("y" in m)))
+        return lev, dirs
+
+    def fix(lower, higher, axis):
+        """Resolve one violation between a coarser ``lower`` and a finer
+        ``higher`` cell (axis None: levels; else one direction)."""
+        parent = parent_of.get(cell_ids[lower])
+        if parent in coarsen:
+            coarsen.remove(parent)
+            return
+        if lower in marks:
+            if axis is None or axis in marks[lower]:
+                if higher in marks:
+                    del marks[higher]
+                    return
+                raise AdaptationBalanceError(
+                    "The current mesh cannot satisfy 2:1 refinement balance in one adaptation step."
+                    if axis is None else
+                    "The current mesh cannot satisfy anisotropic transition balance in one adaptation step."
+                )
+            both = set(marks[lower]) | {axis}
+            marks[lower] = "xy" if len(both) == 2 else next(iter(both))
+            return
+        if levels[lower] < maximum_level:
+            if axis is None:
+                marks[lower] = AXES[_least_disruptive_axis(lower, neighbours, proj_dir, levels_dir)]
+            else:
+                marks[lower] = axis
+            return
+        if higher in marks:
+            del marks[higher]
+            return
+        raise AdaptationBalanceError(
+            "The current mesh cannot satisfy 2:1 refinement balance within the configured maximum level."
+            if axis is None else
+            "The current mesh cannot satisfy anisotropic transition balance within the configured maximum level."
+        )
+
+    changed = True
+    proj_dir = None
+    while changed:
+        changed = False
+        signature = hash((frozenset(marks.items()), frozenset(coarsen)))
+        if signature in seen:
+            raise AdaptationBalanceError("The adaptive refinement plan cycles while balancing mesh transitions.")
+        seen.add(signature)
+        # periodic partners: coarsen both or neither, refine both the same way
+        for a, b, _ in periodic_pairs:
+            pa, pb = parent_of.get(cell_ids[a]), parent_of.get(cell_ids[b])
+            if pa in coarsen and pb not in coarsen:
+                coarsen.remove(pa)
+                changed = True
+            if pb in coarsen and pa not in coarsen:
+                coarsen.remove(pb)
+                changed = True
+        for a, b, _ in periodic_pairs:
+            if a in marks and b not in marks:
+                if levels[b] >= maximum_level:
+                    del marks[a]
+                else:
+                    marks[b] = marks[a]
+                changed = True
+            elif b in marks and a not in marks:
+                if levels[a] >= maximum_level:
+                    del marks[b]
+                else:
+                    marks[a] = marks[b]
+                changed = True
+            elif a in marks and b in marks and marks[a] != marks[b]:
+                if marks[a] != "xy" or marks[b] != "xy":
+                    marks[a] = marks[b] = "xy"
+                    changed = True
+        proj_lev, proj_dir = projected()
+        for k in range(len(cell_ids)):
+            for nb in neighbours[k]:
+                if nb <= k:
+                    continue
+                d = proj_lev[k] - proj_lev[nb]
+                if abs(d) > 1:
+                    fix(k if d < 0 else nb, nb if d < 0 else k, None)
+                    changed = True
+                    break
+                for ai, axis in enumerate(AXES):
+                    d = proj_dir[k][ai] - proj_dir[nb][ai]
+                    if abs(d) <= 1:
+                        continue
+                    fix(k if d < 0 else nb, nb if d < 0 else k, axis)
+                    changed = True
+                    break
+                if changed:
+                    break
+            if changed:
+                break
+    # final verification (adaptation.py:427-451)
+    proj_lev, proj_dir = projected()
+    for k in range(len(cell_ids)):
+        for nb in neighbours[k]:
+            if nb <= k:
+                continue
+            if abs(proj_lev[k] - proj_lev[nb]) > 1:
+                raise AdaptationBalanceError("The projected adaptive mesh violates 2:1 refinement balance.")
+            if any(abs(x - y) > 1 for x, y in zip(proj_dir[k], proj_dir[nb])):
+                raise AdaptationBalanceError("The projected adaptive mesh violates anisotropic transition balance.")
+
+
+# ---------------------------------------------------------------------------
+# Neighbour structure, indicator and marks
+# ---------------------------------------------------------------------------
+
+
+def periodic_cell_pairs(mesh, pairs) -> list:
+    """(owner of face a, owner of face b, shift) per matched periodic face
+    pair (adaptation.py:798-815)."""
+    from .geometry import match_periodic_faces
+
+    out = []
+    for pair in pairs:
+        fa, fb, shift = match_periodic_faces(mesh, pair.patch_a_id, pair.patch_b_id)
+        s = (float(shift[0]), float(shift[1]))
+        out.extend((int(mesh.face_owners[a]), int(mesh.face_owners[b]), s) for a, b in zip(fa, fb))
+    return out
+
+
+def _neighbour_sets(mesh, periodic):
+    nbrs = [set() for _ in range(mesh.cell_count)]
+    for o, n in zip(mesh.face_owners.tolist(), mesh.face_neighbours.tolist()):
+        if n < 0:
+            continue
+        nbrs[o].add(n)
+        nbrs[n].add(o)
+    for a, b, _ in periodic:
+        nbrs[a].add(b)
+        nbrs[b].add(a)
+    return nbrs
+
+
+def active_periodic_pairs(case) -> tuple:
+    kinds = {bc.patch_id: bc.kind for bc in case.boundary_conditions}
+    return tuple(p for p in case.periodic_pairs
+                 if kinds.get(p.patch_a_id) == "Periodic" and kinds.get(p.patch_b_id) == "Periodic")
+
+
+def indicator(mesh, centers, areas, values, links) -> tuple:
+    """Per cell (indicator, gradient): the distance-weighted least-squares
+    gradient of ``values`` over the links (lstsq of the sqrt(1/d^2)-weighted
+    system, rcond 1e-12; cells with fewer than two samples keep zero), and
+    max(|grad| sqrt(A) / scale, largest |jump| / scale) with scale =
+    max(range, 1e-8 max|v|, 1e-30) (adaptation.py:91-122)."""
+    n = mesh.cell_count
+    samples = [[] for _ in range(n)]
+    for a, b, sx, sy in links:
+        dx = centers[b, 0] + sx - centers[a, 0]
+        dy = centers[b, 1] + sy - centers[a, 1]
+        dv = values[b] - values[a]
+        samples[a].append((dx, dy, dv))
+        samples[b].append((-dx, -dy, -dv))
+    grads = np.zeros((n, 2))
+    kmax = max((len(s) for s in samples), default=0)
+    if kmax >= 2:
+        A = np.zeros((n, kmax, 2))
+        rhs = np.zeros((n, kmax))
+        use = np.zeros(n, dtype=bool)
+        for c, s in enumerate(samples):
+            if len(s) < 2:
+                continue
+            arr = np.asarray(s, dtype=float)
+            off = arr[:, :2]
+            w = 1.0 / np.maximum(np.sum(off * off, axis=1), TOL**2)
+            A[c, : len(s)] = off * np.sqrt(w)[:, None]
+            rhs[c, : len(s)] = arr[:, 2] * np.sqrt(w)
+            use[c] = True
+        # minimum-norm least squares per cell (zero rows change nothing)
+        grads[use] = np.einsum("cij,cj->ci", np.linalg.pinv(A[use], rcond=1e-12), rhs[use])
+    value_range = float(np.ptp(values))
+    scale = max(value_range, float(np.max(np.abs(values))) * 1e-8, 1e-30)
+    grad_ind = np.linalg.norm(grads, axis=1) * np.sqrt(areas) / scale
+    jump = np.zeros(n)
+    for a, b, _, _ in links:
+        jump[a] = max(jump[a], abs(float(values[b] - values[a])) / scale)
+    return np.maximum(grad_ind, jump), grads
+
+
+def select_adaptation(mesh, case, centers, areas, values, threshold: float, maximum_level: int):
+    """Refinement marks {cell: "x" | "y" | "xy"} and parent ids to coarsen
+    (reference `select_adaptation`, adaptation.py:40-171)."""
+    cell_ids = _cell_ids(mesh)
+    levels = _cell_levels(mesh)
+    values = np.asarray(values, dtype=float)
+    if not (len(centers) == len(areas) == len(values) == mesh.cell_count):
+        raise ValueError("Adaptation indicators must match the mesh cell count.")
+    if not np.isfinite(values).all():
+        raise ValueError("Adaptation indicator values must be finite.")
+    centers = np.asarray(centers, dtype=float)
+    links = [(int(o), int(n), 0.0, 0.0) for o, n in zip(mesh.face_owners, mesh.face_neighbours) if n >= 0]
+    periodic = periodic_cell_pairs(mesh, active_periodic_pairs(case))
+    nbrs = _neighbour_sets(mesh, ())
+    for a, b, s in periodic:
+        nbrs[a].add(b)
+        nbrs[b].add(a)
+        links.append((a, b, -s[0], -s[1]))
+        links.append((b, a, s[0], s[1]))
+    ind, grads = indicator(mesh, centers, np.asarray(areas, dtype=float), values, links)
+    marks = {}
+    for c in range(mesh.cell_count):
+        if ind[c] <= threshold or levels[c] >= maximum_level:
+            continue
+        verts = cell_vertices(mesh, c)
+        corners = drop_collinear(verts)
+        axis = "xy"
+        if len(corners) == 4 and aspect_ratio(corners) < 3.0:
+            lx, ly = local_directions(corners)
+            vx = abs(float(np.dot(grads[c], lx))) * span_along(corners, lx)
+            vy = abs(float(np.dot(grads[c], ly))) * span_along(corners, ly)
+            if vx > 2.0 * max(vy, 1e-30):
+                axis = "x"
+            elif vy > 2.0 * max(vx, 1e-30):
+                axis = "y"
+            if axis != "xy" and not split_passes_quality(verts, axis):
+                axis = "xy"
+        marks[c] = axis
+    index = {cid: k for k, cid in enumerate(cell_ids)}
+    coarsen = {r.parent_id for r in mesh.refinement_records
+               if all(ch in index and ind[index[ch]] < threshold * 0.3 for ch in r.child_ids)}
+    levels_dir = cell_directional_levels(mesh, cell_ids, mesh.refinement_records)
+    parent_dir = parent_directional_levels(mesh.refinement_records)
+    # balance, then the quality gate on marks the balance added, until stable
+    while True:
+        balance_plan(marks, coarsen, nbrs, cell_ids, levels, levels_dir, parent_dir, maximum_level,
+                     mesh.refinement_records, periodic)
+        adjusted = False
+        for c, axis in tuple(marks.items()):
+            final = gated_axis(cell_vertices(mesh, c), axis)
+            if final != axis:
+                marks[c] = final
+                adjusted = True
+        if not adjusted:
+            return marks, coarsen
+
+
+# ---------------------------------------------------------------------------
+# Adapting and rebuilding
+# ---------------------------------------------------------------------------
+
+
+def adapt_mesh(mesh: FiniteVolumeMesh, marks: dict, coarsen: set, periodic_pairs=()):
+    """The adapted mesh and, per new cell, the old cells it covers
+    (reference `adapt_mesh`, adaptation.py:680-795)."""
+    if mesh.cell_count == 0:
+        raise ValueError("Cannot adapt a mesh without cells.")
+    cell_ids = _cell_ids(mesh)
+    levels = _cell_levels(mesh)
+    if len(cell_ids) != mesh.cell_count or len(levels) != mesh.cell_count:
+        raise ValueError("Mesh refinement metadata does not match its cells.")
+    marks = dict(marks)
+    coarsen = set(coarsen)
+    periodic = periodic_cell_pairs(mesh, periodic_pairs)
+    nbrs = _neighbour_sets(mesh, periodic)
+    maximum_level = max((*levels, *(levels[c] + 1 for c in marks)), default=0)
+    balance_plan(marks, coarsen, nbrs, cell_ids, levels,
+                 cell_directional_levels(mesh, cell_ids, mesh.refinement_records),
+                 parent_directional_levels(mesh.refinement_records), maximum_level, mesh.refinement_records,
+                 periodic)
+    records = {r.parent_id: r for r in mesh.refinement_records}
+    curves = _curves_by_edge(mesh)
+    old = [Leaf(cell_ids[k], levels[k], cell_vertices(mesh, k), (k,), cell_edge_curves(mesh, k, curves))
+           for k in range(mesh.cell_count)]
+    position = {leaf.cell_id: k for k, leaf in enumerate(old)}
+    merged_children: set = set()
+    parent_at_first_child: dict = {}
+    for parent in sorted(coarsen):
+        rec = records.get(parent)
+        if rec is None or any(ch not in position for ch in rec.child_ids):
+            continue
+        kids = [old[position[ch]] for ch in rec.child_ids]
+        if any(kid.level != rec.level + 1 for kid in kids):
+            continue
+        merged_children.update(rec.child_ids)
+        parent_at_first_child[rec.child_ids[0]] = Leaf(
+            parent, rec.level, rec.vertices, tuple(s for kid in kids for s in kid.sources),
+            merged_edge_curves(rec.vertices, kids))
+        for key in tuple(records):
+            if key == parent or key.startswith(parent + "."):
+                del records[key]
+    leaves, new_records = [], []
+    for k, leaf in enumerate(old):
+        if leaf.cell_id in parent_at_first_child:
+            leaves.append(parent_at_first_child[leaf.cell_id])
+            continue
+        if leaf.cell_id in merged_children:
+            continue
+        axis = marks.get(k)
+        if axis is None:
+            leaves.append(leaf)
+            continue
+        axis = gated_axis(leaf.vertices, axis)
+        children = split_polygon(leaf.vertices, axis, leaf.curves)
+        if len(children) < 2:
+            leaves.append(leaf)
+            continue
+        kid_curves = child_edge_curves(leaf.vertices, leaf.curves, children)
+        kid_ids = tuple(f"{leaf.cell_id}.{j}" for j in range(len(children)))
+        new_records.append(RefinementRecord(leaf.cell_id, kid_ids, leaf.vertices, leaf.level))
+        leaves.extend(Leaf(cid, leaf.level + 1, poly, leaf.sources, cc)
+                      for cid, poly, cc in zip(kid_ids, children, kid_curves))
+    if not merged_children and not new_records:
+        return mesh, tuple((k,) for k in range(mesh.cell_count))
+    rebuilt = rebuild_mesh(mesh, leaves, tuple(records.values()) + tuple(new_records))
+    lv = np.asarray(rebuilt.cell_levels)
+    inner = rebuilt.face_neighbours >= 0
+    bad = np.flatnonzero(inner & (np.abs(lv[rebuilt.face_owners] - lv[np.maximum(rebuilt.face_neighbours, 0)]) > 1))
+    if len(bad):
+        raise ValueError(f"Adapted face {int(bad[0])} violates the 2:1 refinement balance.")
+    return rebuilt, tuple(leaf.sources for leaf in leaves)
+
+
+def _boundary_patch(a, b, boundary_edges) -> int:
+    mid = (0.5 * (a[0] + b[0]), 0.5 * (a[1] + b[1]))
+    for p, q, patch in boundary_edges:
+        if on_segment(mid, p, q):
+            return patch
+    return -1
+
+
+def rebuild_mesh(old: FiniteVolumeMesh, leaves, records) -> FiniteVolumeMesh:
+    """Faces, owners and neighbours of the leaves (reference `_rebuild_mesh`,
+    adaptation.py:1032-1190): each leaf edge is cut at every mesh point on
+    it, faces are numbered at first visit (owner = first visiting leaf), new
+    boundary faces take the patch of the old boundary edge under their
+    midpoint or, on a curve, the curve's patch and sub-range."""
+    distinct = {}
+    for leaf in leaves:
+        for p in leaf.vertices:
+            distinct.setdefault(point_key(p), p)
+    known = tuple(distinct.values())
+    by_x = sorted(known, key=lambda p: p[0])
+    by_y = sorted(known, key=lambda p: p[1])
+    xs = [p[0] for p in by_x]
+    ys = [p[1] for p in by_y]
+    boundary_edges = []
+    for f in np.flatnonzero(old.face_neighbours < 0):
+        a, b = old.faces[f]
+        boundary_edges.append(((float(old.points[a, 0]), float(old.points[a, 1])),
+                               (float(old.points[b, 0]), float(old.points[b, 1])), int(old.face_patch_ids[f])))
+    points, pid = [], {}
+    faces, owners, neighbours, patches_of, curves_of = [], [], [], [], []
+    cell_faces, cell_signs, cell_nodes = [], [], []
+    face_of_edge = {}
+
+    def point_id(p):
+        key = point_key(p)
+        if key not in pid:
+            pid[key] = len(points)
+            points.append(p)
+        return pid[key]
+
+    for c, leaf in enumerate(leaves):
+        fids, signs, loop = [], [], []
+        n = len(leaf.vertices)
+        lc = leaf.curves if len(leaf.curves) == n else (None,) * n
+        for k in range(n):
+            a, b = leaf.vertices[k], leaf.vertices[(k + 1) % n]
+            curve = lc[k]
+            axis = 0 if abs(b[0] - a[0]) >= abs(b[1] - a[1]) else 1
+            ordered, coords = (by_x, xs) if axis == 0 else (by_y, ys)
+            lo = min(a[axis], b[axis]) - TOL
+            hi = max(a[axis], b[axis]) + TOL
+            on_edge = [p for p in ordered[bisect_left(coords, lo): bisect_right(coords, hi)] if on_segment(p, a, b)]
+            on_edge.sort(key=lambda p: segment_parameter(p, a, b))
+            for p, q in zip(on_edge, on_edge[1:]):
+                if dist2(p, q) <= 1e-26:
+                    continue
+                i, j = point_id(p), point_id(q)
+                loop.append(i)
+                key = (min(i, j), max(i, j))
+                f = face_of_edge.get(key)
+                if f is None:
+                    f = len(faces)
+                    face_of_edge[key] = f
+                    faces.append((i, j))
+                    owners.append(c)
+                    neighbours.append(-1)
+                    if curve is None:
+                        patches_of.append(_boundary_patch(p, q, boundary_edges))
+                        curves_of.append(None)
+                    else:
+                        t0 = segment_parameter(p, a, b)
+                        t1 = segment_parameter(q, a, b)
+                        span = curve.end_parameter - curve.start_parameter
+                        patches_of.append(curve.patch_id)
+                        curves_of.append(BoundaryCurve(curve.geometry, curve.start_parameter + t0 * span,
+                                                       curve.start_parameter + t1 * span, curve.patch_id))
+                    sign = 1
+                else:
+                    if neighbours[f] >= 0:
+                        raise ValueError("Adaptation produced a non-manifold face.")
+                    neighbours[f] = c
+                    patches_of[f] = -1
+                    curves_of[f] = None
+                    sign = 1 if faces[f] == (i, j) else -1
+                fids.append(f)
+                signs.append(sign)
+        cell_faces.append(fids)
+        cell_signs.append(signs)
+        cell_nodes.append(loop)
+    width = max(len(x) for x in cell_nodes)
+    cells = np.full((len(cell_nodes), width), -1, dtype=np.int64)
+    sizes = np.array([len(x) for x in cell_nodes], dtype=np.int64)
+    cf = np.full((len(cell_faces), width), -1, dtype=np.int64)
+    cs = np.zeros((len(cell_faces), width), dtype=np.int64)
+    for c, (loop, fids, sg) in enumerate(zip(cell_nodes, cell_faces, cell_signs)):
+        cells[c, : len(loop)] = loop
+        cf[c, : len(fids)] = fids
+        cs[c, : len(sg)] = sg
+    patch_arr = np.asarray(patches_of, dtype=np.int64)
+    patches = tuple(MeshPatch(p.id, p.name, np.flatnonzero(patch_arr == p.id)) for p in old.patches)
+    return FiniteVolumeMesh(
+        points=np.asarray(points, dtype=np.float64).reshape(-1, 2),
+        cells=cells,
+        cell_sizes=sizes,
+        faces=np.asarray(faces, dtype=np.int64).reshape(-1, 2),
+        face_owners=np.asarray(owners, dtype=np.int64),
+        face_neighbours=np.asarray(neighbours, dtype=np.int64),
+        face_patch_ids=patch_arr,
+        patches=patches,
+        cell_faces=cf,
+        cell_face_signs=cs,
+        cell_ids=tuple(leaf.cell_id for leaf in leaves),
+        cell_levels=tuple(leaf.level for leaf in leaves),
+        refinement_records=tuple(records),
+        face_curves=tuple(curves_of) if any(c is not None for c in curves_of) else (),
+    )
+
+
+# ---------------------------------------------------------------------------
+# Solution transfer and the solver's indicator variables
+# ---------------------------------------------------------------------------
+
+
+def transfer_cell_values(values, old_areas, new_areas, source_groups) -> np.ndarray:
+    """Area-weighted averages of each new cell's source cells, rescaled so
+    that every group of new cells sharing a source group keeps the group's
+    integral (reference `_transfer_cell_values`, solver.py:1594-1617)."""
+    values = np.asarray(values, dtype=float)
+    old_areas = np.asarray(old_areas, dtype=float)
+    new_areas = np.asarray(new_areas, dtype=float)
+    out = np.asarray([np.average(values[np.asarray(g, dtype=int)], axis=0, weights=old_areas[np.asarray(g, dtype=int)])
+                      for g in source_groups], dtype=float)
+    outputs: dict = {}
+    for k, g in enumerate(source_groups):
+        outputs.setdefault(tuple(g), []).append(k)
+    for g, ks in outputs.items():
+        old_area = float(np.sum(old_areas[np.asarray(g, dtype=int)]))
+        new_area = float(np.sum(new_areas[np.asarray(ks, dtype=int)]))
+        if not math.isfinite(new_area) or new_area <= 0.0:
+            raise ValueError("Adapted cells must have positive finite areas.")
+        out[ks] *= old_area / new_area
+    return out
+
+
+INDICATOR_VARIABLES = {"Density": 0, "Velocity X": 1, "Velocity Y": 2, "Pressure": 3}
+
+
+def indicator_values(primitive, variable: str) -> np.ndarray:
+    """The adaptation variable of the primitive state (solver.py:1570-1591)."""
+    if variable not in INDICATOR_VARIABLES:
+        raise ValueError(f"Unsupported adaptive mesh indicator '{variable}'.")
+    return np.asarray(primitive)[:, INDICATOR_VARIABLES[variable]]

@@ -18,6 +18,7 @@ import numpy as np
 
 from .model import (
     BoundaryCondition,
+    BoundaryCurve,
     Case,
     FiniteVolumeMesh,
     MeshPatch,
@@ -506,7 +507,38 @@ def bump_channel(nx: int, ny: int, h: float, length: float = 3.0, height: float 
         raise ValueError("bump height must be in (0, height/2)")
     base = _grid_mesh(nx, ny, 0.0, length, 0.0, height, 0.0, None)
     base.points = bump_nodes(base.points, h, height, x0, chord)
+    base.face_curves = bump_face_curves(base, h, x0, chord)
     return base
+
+
+def bump_arc(h: float, x0: float = 1.0, chord: float = 1.0) -> dict:
+    """The bump's circular arc as a MeshCAD arc descriptor, run
+    counter-clockwise from its right end over the top to its left end."""
+    radius = (0.25 * chord * chord + h * h) / (2.0 * h)
+    xc, yc = x0 + 0.5 * chord, h - radius
+    right = math.atan2(-yc, 0.5 * chord)
+    return {"type": "arc", "center": [xc, yc], "radius": radius, "start_angle": right,
+            "end_angle": math.pi - right}
+
+
+def bump_face_curves(mesh: FiniteVolumeMesh, h: float, x0: float = 1.0, chord: float = 1.0) -> tuple:
+    """Per face: the bump arc with the parameters of its two nodes for bottom
+    faces on the bump (so that refinement splits them on the arc), None
+    elsewhere."""
+    arc = bump_arc(h, x0, chord)
+    xc, yc = arc["center"]
+    a0, a1 = arc["start_angle"], arc["end_angle"]
+    pts = mesh.points
+    out = []
+    for f, (n0, n1) in enumerate(mesh.faces):
+        on_bump = (mesh.face_patch_ids[f] == 0 and x0 - 1e-12 <= pts[n0, 0] <= x0 + chord + 1e-12
+                   and x0 - 1e-12 <= pts[n1, 0] <= x0 + chord + 1e-12)
+        if not on_bump:
+            out.append(None)
+            continue
+        t = [min(1.0, max(0.0, (math.atan2(pts[n, 1] - yc, pts[n, 0] - xc) - a0) / (a1 - a0))) for n in (n0, n1)]
+        out.append(BoundaryCurve(arc, float(t[0]), float(t[1]), 0))
+    return tuple(out)
 
 
 def bump_case(nx: int, ny: int, h: float, mach: float = 0.675, integrator: str = "Implicit Euler",
