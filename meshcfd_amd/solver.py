@@ -9,14 +9,20 @@ callbacks, `SolverProgress` / `SolverResult` (`solver.py:58-77`) and
 The state stays on the device; the host reads the clock and the status word
 once per step and pulls states only for callbacks and the result.
 
-Not covered (raise `SolverError`): SST turbulence, mesh adaptation, the ILU
-preconditioner, and evaluating `case.initial_conditions` (the caller passes
-the initial states).
+With ``adaptive_mesh`` the mesh is adapted every ``adaptation_interval``
+steps on the host (`adaptation.py`: indicator, refinement and coarsening, the
+conservative transfer of the state) and the device copy is rebuilt
+(`solver.py:816-884`).
+
+Not covered (raise `SolverError`): SST turbulence, the ILU preconditioner,
+and evaluating `case.initial_conditions` (the caller passes the initial
+states).
 """
 
 from __future__ import annotations
 
 import math
+import warnings
 import weakref
 from dataclasses import dataclass
 from time import thread_time
@@ -25,6 +31,7 @@ from typing import Callable
 import numpy as np
 import torch
 
+from .adaptation import AdaptationBalanceError
 from .geometry import SolverError
 from .implicit import halve_until_solved
 from .model import Case, FiniteVolumeMesh, InitialState, NumericalSettings
@@ -96,8 +103,6 @@ def validate(case: Case, start_time: float, step_offset: int, solution_interval:
     s = case.numerics
     if getattr(s, "turbulence_model", "None") != "None":
         raise SolverError("The Menter k-omega SST model is not implemented by the GPU path.")
-    if getattr(s, "adaptive_mesh", False):
-        raise SolverError("Adaptive mesh refinement is not implemented by the GPU path.")
     if s.reconstruction not in ("First order", "MUSCL linear", "MUSCL parabolic"):
         raise SolverError(f"Unsupported reconstruction '{s.reconstruction}'.")
     if s.limiter not in ("Barth-Jespersen", "Venkatakrishnan", "Minmod", "Van Leer"):
@@ -143,6 +148,21 @@ def validate(case: Case, start_time: float, step_offset: int, solution_interval:
         raise SolverError("Convergence history interval must be a positive integer.")
     if not math.isfinite(cpu_time_offset) or cpu_time_offset < 0.0:
         raise SolverError("CPU time offset must be finite and non-negative.")
+    if not isinstance(s.adaptive_mesh, bool):
+        raise SolverError("Adaptive mesh setting must be boolean.")
+    if isinstance(s.adaptation_interval, bool) or not isinstance(s.adaptation_interval, int) \
+            or s.adaptation_interval <= 0:
+        raise SolverError("Adaptation interval must be a positive integer.")
+    if isinstance(s.adaptation_max_level, bool) or not isinstance(s.adaptation_max_level, int) \
+            or not 1 <= s.adaptation_max_level <= 12:
+        raise SolverError("Maximum adaptation level must be an integer from 1 to 12.")
+    if not math.isfinite(s.adaptation_refine_threshold) or s.adaptation_refine_threshold <= 0.0:
+        raise SolverError("Adaptation threshold must be positive and finite.")
+    if s.adaptation_variable not in ("Density", "Velocity X", "Velocity Y", "Pressure", "Turbulent k",
+                                     "Specific dissipation rate"):
+        raise SolverError("Unsupported adaptive mesh indicator variable.")
+    if s.adaptive_mesh and s.adaptation_variable in ("Turbulent k", "Specific dissipation rate"):
+        raise SolverError("Turbulence-based adaptation indicators require the SST model.")
     if not math.isfinite(s.gamma) or s.gamma <= 1.0:
         raise SolverError("Specific heat ratio must be finite and greater than 1.")
     if not math.isfinite(s.gas_constant) or s.gas_constant <= 0.0:
@@ -375,6 +395,28 @@ class _DeviceRun:
     def components(self, local):
         return self.comps if local else self._rms()
 
+    @property
+    def geometry(self):
+        return self.dm.geometry
+
+    def conservative(self):
+        return self.dm.state()
+
+    def remesh(self, case, conservative, device):
+        """Continue on an adapted mesh: a new device copy of its geometry,
+        the transferred state."""
+        from .device import DeviceMesh
+
+        ordering = self.dm.lusgs_ordering
+        clock = self.dm.clock.clone()  # [time, dt] of the global modes
+        self.close()
+        self.own = True
+        self.dm = DeviceMesh(case, device)
+        self.dm.lusgs_ordering = ordering
+        self.dm.set_state(conservative)
+        self.dm.clock.copy_(clock)
+        self.r = torch.empty((self.dm.n, 4), dtype=torch.float64, device=self.dm.device)
+
     def primitive(self):
         return conservative_to_primitive(self.dm.state(), self.s.gamma)
 
@@ -453,6 +495,15 @@ class _SeamRun:
     def components(self, local):
         return np.sqrt(np.mean(self.derivative**2, axis=0))
 
+    def conservative(self):
+        return self.u
+
+    def remesh(self, case, conservative, device):
+        from .geometry import build_geometry
+
+        self.geometry = build_geometry(case)
+        self.u = np.asarray(conservative, dtype=np.float64)
+
     def primitive(self):
         return conservative_to_primitive(self.u, self.s.gamma)
 
@@ -461,6 +512,32 @@ class _SeamRun:
 
     def close(self):
         pass
+
+
+def _adapt(case, run, device, local: bool, want_components: bool):
+    """One adaptation of `solve_case` (`solver.py:816-884`): indicator of the
+    current state, marks, the adapted mesh, the transferred state; the run
+    continues on the new mesh. Returns (adapted mesh or None, the residual
+    components of the step's start on the old mesh, for the report)."""
+    from . import adaptation as ad
+
+    s = case.numerics
+    mesh = case.mesh
+    values = ad.indicator_values(run.primitive(), s.adaptation_variable)
+    g = run.geometry
+    marks, coarsen = ad.select_adaptation(mesh, case, g.centers, g.areas, values, s.adaptation_refine_threshold,
+                                          s.adaptation_max_level)
+    if not marks and not coarsen:
+        return None, None
+    old_areas = ad.cell_areas(mesh)
+    adapted, groups = ad.adapt_mesh(mesh, marks, coarsen, periodic_pairs=case.periodic_pairs)
+    if adapted is mesh:
+        return None, None
+    comps = run.components(local) if (want_components and not local) else None
+    state = ad.transfer_cell_values(run.conservative(), old_areas, ad.cell_areas(adapted), groups)
+    case.mesh = adapted
+    run.remesh(case, state, device)
+    return adapted, comps
 
 
 def solve_case(
@@ -518,6 +595,7 @@ def solve_case(
     time = start_time
     completed = False
     steps = step_offset
+    adaptation_disabled = False
     run.start(time)
     for local_step in range(1, s.max_iterations + 1):
         step = step_offset + local_step
@@ -530,6 +608,17 @@ def solve_case(
         except (FloatingPointError, ValueError, ZeroDivisionError) as error:
             raise SolverError(f"Solver failed at step {step}: {error}") from error
         steps = step
+        adapted_mesh = None
+        adaptation_warning = None
+        saved_components = None
+        if s.adaptive_mesh and not adaptation_disabled and step % s.adaptation_interval == 0:
+            try:
+                adapted_mesh, saved_components = _adapt(case, run, device, local, progress_callback is not None)
+            except AdaptationBalanceError as error:
+                adaptation_disabled = True
+                adaptation_warning = f"Adaptive mesh disabled at step {step}: {error}"
+            except (ValueError, FloatingPointError, np.linalg.LinAlgError) as error:
+                raise SolverError(f"Adaptive mesh update failed at step {step}: {error}") from error
         steady_converged = False
         if local:
             try:
@@ -541,18 +630,22 @@ def solve_case(
         final_step = (end_time_reached or steady_converged or local_step == s.max_iterations
                       or (cancel_callback is not None and cancel_callback()))
         report_residual = progress_callback is not None and (step % s.convergence_history_interval == 0
-                                                             or final_step)
+                                                             or adapted_mesh is not None or final_step)
         progress_states = None
         if progress_callback is not None:
-            if live_solution_updates and (local_step == 1 or step % 10 == 0 or end_time_reached or steady_converged
-                                          or local_step == s.max_iterations):
+            if (live_solution_updates and (local_step == 1 or step % 10 == 0 or end_time_reached or steady_converged
+                                           or local_step == s.max_iterations)) or adapted_mesh is not None:
                 progress_states = _states(run.primitive())
-            if report_residual or progress_states is not None:
+            if report_residual or progress_states is not None or adaptation_warning:
                 component_residuals = ()
                 if report_residual:
-                    component_residuals = tuple(float(x) for x in run.components(local))
-                progress_callback(SolverProgress(step, time, time_step, component_residuals, progress_states, None,
-                                                 cpu_time_offset + thread_time() - cpu_start, None))
+                    comps = saved_components if saved_components is not None else run.components(local)
+                    component_residuals = tuple(float(x) for x in comps)
+                progress_callback(SolverProgress(step, time, time_step, component_residuals, progress_states,
+                                                 adapted_mesh, cpu_time_offset + thread_time() - cpu_start,
+                                                 adaptation_warning))
+        if adaptation_warning and progress_callback is None:
+            warnings.warn(adaptation_warning, RuntimeWarning, stacklevel=2)
         if solution_callback is not None and step % solution_interval == 0:
             snapshot = progress_states if progress_states is not None else _states(run.primitive())
             solution_callback(step, time, snapshot)

@@ -28,7 +28,11 @@ from meshcfd_amd.solver import SolverProgress, conservative_to_primitive, primit
         ("prandtl_number", 0.0, "Prandtl number must be positive and finite."),
         ("acceleration_x", math.inf, "Body-force acceleration must be finite."),
         ("turbulence_model", "Menter k-omega SST", "SST model is not implemented by the GPU path"),
-        ("adaptive_mesh", True, "Adaptive mesh refinement is not implemented by the GPU path"),
+        ("adaptive_mesh", 1, "Adaptive mesh setting must be boolean."),
+        ("adaptation_interval", 0, "Adaptation interval must be a positive integer."),
+        ("adaptation_max_level", 13, "Maximum adaptation level must be an integer from 1 to 12."),
+        ("adaptation_refine_threshold", math.nan, "Adaptation threshold must be positive and finite."),
+        ("adaptation_variable", "Vorticity", "Unsupported adaptive mesh indicator variable."),
     ],
 )
 def test_settings_are_validated_with_the_reference_messages(field, value, message):
