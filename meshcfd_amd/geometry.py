@@ -256,6 +256,52 @@ def _finite(value, label: str) -> float:
     return value
 
 
+WALL_VELOCITY_KEYS = ("wall_velocity_x_expression", "wall_velocity_y_expression")
+
+
+def _wall_velocities(conditions, patch_ids, neighbours, face_centers, normals) -> np.ndarray:
+    """Per face the prescribed velocity of walls with mechanical condition
+    "Prescribed velocity" (its x and y expressions of the face centre,
+    default "0"), which must be tangential to the wall; zero elsewhere
+    (`solver.py:2110-2156`, with the reference's messages)."""
+    from .expressions import ExpressionError, compile_expression
+
+    out = np.zeros((len(patch_ids), 2))
+    for pid, bc in conditions.items():
+        if bc.kind != "Wall" or bc.mechanical_condition != "Prescribed velocity":
+            continue
+        exprs = []
+        for key in WALL_VELOCITY_KEYS:
+            source = bc.values.get(key, "0")
+            if not isinstance(source, str):
+                raise SolverError(f"Prescribed wall velocity expression '{key}' on '{bc.patch_name}' must be a string.")
+            try:
+                exprs.append(compile_expression(source))
+            except ExpressionError as error:
+                raise SolverError(f"Invalid prescribed wall velocity on '{bc.patch_name}': {error}") from error
+        faces = np.flatnonzero((patch_ids == pid) & (neighbours < 0))
+        if not len(faces):
+            continue
+        x, y = face_centers[faces, 0], face_centers[faces, 1]
+        try:
+            vel = np.column_stack([e.evaluate(x, y) for e in exprs])
+        except ExpressionError:
+            for f in faces:  # the first failing face, as a face-by-face evaluation finds it
+                try:
+                    for e in exprs:
+                        e.evaluate(float(face_centers[f, 0]), float(face_centers[f, 1]))
+                except ExpressionError as error:
+                    raise SolverError(
+                        f"Could not evaluate prescribed wall velocity on '{bc.patch_name}', face {int(f)}: {error}"
+                    ) from error
+            raise
+        normal = vel[:, 0] * normals[faces, 0] + vel[:, 1] * normals[faces, 1]
+        if np.any(np.abs(normal) > 1e-10 * np.maximum(1.0, np.hypot(vel[:, 0], vel[:, 1]))):
+            raise SolverError(f"Prescribed velocity on wall '{bc.patch_name}' must be tangential to the fixed mesh wall.")
+        out[faces] = vel
+    return out
+
+
 def build_geometry(case: Case) -> Geometry:
     mesh = case.mesh
     gas_constant = case.numerics.gas_constant
@@ -288,8 +334,6 @@ def build_geometry(case: Case) -> Geometry:
             raise SolverError(
                 f"Boundary type '{bc.kind}' is not implemented by the GPU residual."
             )
-        if bc.kind == "Wall" and bc.mechanical_condition == "Prescribed velocity":
-            raise SolverError("Prescribed wall velocity is not implemented by the GPU residual.")
     if set(conditions) != {p.id for p in mesh.patches}:
         raise SolverError("Boundary conditions must be defined for every mesh patch.")
 
@@ -484,7 +528,7 @@ def build_geometry(case: Case) -> Geometry:
             if bc.kind != "Wall":
                 continue
             faces = boundary[patch_ids[boundary] == pid]
-            if bc.mechanical_condition not in ("Slip", "No-slip"):
+            if bc.mechanical_condition not in ("Slip", "No-slip", "Prescribed velocity"):
                 raise SolverError(f"Unsupported mechanical condition on wall '{bc.patch_name}'.")
             if bc.thermal_condition == "Temperature":
                 value = _positive(bc.values.get("temperature", 300.0), f"Wall temperature for '{bc.patch_name}'")
@@ -494,7 +538,7 @@ def build_geometry(case: Case) -> Geometry:
                 wall_thermal[faces] = 2
             else:
                 raise SolverError(f"Unsupported thermal condition on wall '{bc.patch_name}'.")
-            wall_no_slip[faces] = bc.mechanical_condition == "No-slip"
+            wall_no_slip[faces] = bc.mechanical_condition in ("No-slip", "Prescribed velocity")
             wall_value[faces] = value
     diffusive = boundary[~is_periodic[boundary]]
     return Geometry(
@@ -504,7 +548,7 @@ def build_geometry(case: Case) -> Geometry:
         face_centers=face_centers,
         face_normals=normals,
         face_lengths=lengths,
-        wall_velocities=np.zeros((len(owners), 2)),
+        wall_velocities=_wall_velocities(conditions, patch_ids, neighbours, face_centers, normals),
         owners=owners,
         neighbours=neighbours,
         patch_ids=patch_ids,
