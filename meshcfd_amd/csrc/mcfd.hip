@@ -43,7 +43,13 @@ int fail(const std::string& msg) {
   } while (0)
 
 constexpr int kBlock = 256;
-constexpr int kBlockInfo = 12;  // ints per fused-kernel block record
+constexpr int kBlockInfo = 12;  // ints per fused-kernel block record (McfdBlockDesc.info)
+// ints per block record on the device: the 12 of McfdBlockDesc.info (field 0
+// rewritten to the block's fixed-stride offset into the local-cell list), then
+// the block's LSQ and limiter entry counts and the first item and count of
+// its own cells' accumulation and spectral-radius lists, so that no load of
+// the fused kernel waits for another load to learn its address
+constexpr int kBlockInfoDev = 16;
 
 struct P4 {
   double r, u, v, p;
@@ -1275,7 +1281,9 @@ __device__ __forceinline__ void visc_gather(int i, const ViscAcc& v, const doubl
 // cell_time_step for the fused final stage: the normals and lengths of the
 // cell's faces come from the block's LDS slots; items flagged 0x8000 (periodic
 // b sides and faces with an exterior state) use the global item as above.
-__device__ double block_time_step(int i, const P4& w, int e0, int e1,
+// FARFIELD false: the mesh has no far-field record, so that branch is left out.
+template <bool FARFIELD>
+__device__ double block_time_step(int i, double area, const P4& w, int e0, int e1,
                                   const unsigned short* dloc, int dbase,
                                   const double* sN, const double* sL, const DtArrays& d, double g,
                                   double cfl, int* status) {
@@ -1305,7 +1313,7 @@ __device__ double block_time_step(int i, const P4& w, int e0, int e1,
         x = P4{st[0], st[1], st[2], st[3]};
       else if (kind == MCFD_FACE_PRESSURE_OUTLET)
         x.p = st[3];
-      else if (kind == MCFD_FACE_FAR_FIELD)
+      else if (FARFIELD && kind == MCFD_FACE_FAR_FIELD)
         x = far_field(w, P4{st[0], st[1], st[2], st[3]}, nx, ny, g, &err);
       if (err) report(status, err, i);
       double xr = fabs(x.u * nx + x.v * ny) + sound(x, g);
@@ -1313,7 +1321,7 @@ __device__ double block_time_step(int i, const P4& w, int e0, int e1,
     }
     sum = sum + rad * len;
   }
-  double dt = cfl * d.area[i] / sum;
+  double dt = cfl * area / sum;
   if (!finite(dt) || !(dt > 0.0)) report(status, MCFD_ERR_LOCAL_DT, i);
   return dt;
 }
@@ -1430,8 +1438,9 @@ struct alignas(16) FaceRec {
 static_assert(sizeof(FaceRec) == 64, "face record is 64 bytes");
 
 struct BlockArrays {
-  const int* info;          // [nb][kBlockInfo]
-  const int* cells;         // local -> global cell
+  const int* info;          // [nb][kBlockInfoDev]
+  const int* cells;         // local -> global cell, block b at b * cstride
+  int cstride;              // entries per block in cells (>= max local cells)
   const int* gptr;          // per block, n_grad + 1 offsets into gnb / gent (block relative)
   const unsigned short* gnb;
   const double2* gent;      // LSQ coefficients (cx, cy)
@@ -1510,12 +1519,21 @@ __global__ void __launch_bounds__(kBlock)
   extern __shared__ double smem[];
   constexpr bool muscl = MUSCL;
   const int b = xcd_block();
-  const int* inf = ba.info + kBlockInfo * b;
-  const int cs = inf[0], n_own = inf[1], n_r1 = inf[2], n_r2 = inf[3];
+  const int t = threadIdx.x;
+  // The block's local-cell list sits at a fixed stride (>= kBlock, padded with
+  // a valid cell id), so local cell t's id is loaded from the block id alone,
+  // in the same round trip as the block record, and P0's state loads follow
+  // after one trip instead of two.
+  const int* __restrict__ cl = ba.cells + (long)ba.cstride * b;
+  const int c_first = cl[t];
+  const int* inf = ba.info + kBlockInfoDev * b;
+  const int n_own = inf[1], n_r1 = inf[2], n_r2 = inf[3];
   const int fs = inf[4], n_faces = inf[5], gs = inf[6], gps = inf[7];
   const int first = inf[8], ls = inf[9];
   const int n_loc = n_own + n_r1 + n_r2;
   const int n_grad = n_own + n_r1;
+  // the id is needed now: the wait overlaps the block record's
+  asm volatile("" ::"v"(c_first));
   // LDS: W of every local cell, then one region that holds in turn the
   // block's LSQ and limiter entries (MUSCL: copied in at kernel start), the
   // gradients and limiter factors of own + ring-1 cells until the face
@@ -1534,7 +1552,6 @@ __global__ void __launch_bounds__(kBlock)
   unsigned short* sAI = reinterpret_cast<unsigned short*>(sAC + max_acc);
   unsigned short* sDI = sAI + max_acc;
   const double* __restrict__ U = s.u;
-  const int t = threadIdx.x;
 
   // Index loads of the later phases are issued now, so that their latency
   // overlaps P0 instead of adding a round trip to each phase. Thread t works
@@ -1553,7 +1570,7 @@ __global__ void __launch_bounds__(kBlock)
   const double2* sLE = sGE + max_lsq;                          // [max_lim]
   const unsigned short* sGN = reinterpret_cast<const unsigned short*>(sLE + max_lim) + (gs & 1);
   if constexpr (MUSCL) {
-    const int n_lsq = ba.gptr[gps + n_grad], n_lim = ba.lptr[gps + n_grad];
+    const int n_lsq = inf[10], n_lim = inf[11];
     lds_copy<16>(ba.gent + gs, const_cast<double2*>(sGE), n_lsq);
     lds_copy<16>(ba.lent + ls, const_cast<double2*>(sLE), n_lim);
     lds_copy<4>(ba.gnb + (gs & ~1), const_cast<double2*>(sLE + max_lim), ((gs & 1) + n_lsq + 1) >> 1);
@@ -1561,7 +1578,7 @@ __global__ void __launch_bounds__(kBlock)
 
   // P0: primitive states (euler.py:20-40)
   for (int j = t; j < n_loc; j += kBlock) {
-    int c = (j < n_own) ? first + j : ba.cells[cs + j];
+    int c = (j == t) ? c_first : cl[j];
     P4 q = load4(U, c);
     double rho = q.r;
     double u = q.u / rho;
@@ -1739,8 +1756,8 @@ __global__ void __launch_bounds__(kBlock)
   // per thread in registers), stored to LDS after the barrier, so that P3's
   // loop reads LDS instead of a chain of dependent global loads.
   constexpr int kStageItems = 3;
-  const int abase = ga.acc_ptr[first];
-  const int n_acc = ga.acc_ptr[first + n_own] - abase;
+  const int abase = inf[12];
+  const int n_acc = inf[13];
   double ac_r[kStageItems];
   unsigned short ai_r[kStageItems];
 #pragma unroll
@@ -1752,8 +1769,8 @@ __global__ void __launch_bounds__(kBlock)
   int dbase = 0, n_dts = 0;
   unsigned short di_r[kStageItems];
   if (DTNEXT) {
-    dbase = da.ptr[first];
-    n_dts = da.ptr[first + n_own] - dbase;
+    dbase = inf[14];
+    n_dts = inf[15];
 #pragma unroll
     for (int q = 0; q < kStageItems; ++q) {
       const int k = t + q * kBlock;
@@ -1865,7 +1882,7 @@ __global__ void __launch_bounds__(kBlock)
         double p = (g - 1.0) * (o3 - 0.5 * (o1 * o1 + o2 * o2) / rho);
         if (!finite(rho) || !finite(uu) || !finite(vv) || !finite(p) || !(rho > 0.0) || !(p > 0.0))
           report(status, MCFD_ERR_STATE, i);
-        double dt = block_time_step(i, P4{rho, uu, vv, p}, d0, d1, sDI, dbase, sN, sL, da, g,
+        double dt = block_time_step<FARFIELD>(i, A, P4{rho, uu, vv, p}, d0, d1, sDI, dbase, sN, sL, da, g,
                                     s.cfl, status);
         if (s.dt_next) s.dt_next[i] = dt;
         dt_local = dt;
@@ -3985,7 +4002,7 @@ __global__ void __launch_bounds__(kBlock)
                 double* __restrict__ SBAR, int* __restrict__ status) {
   extern __shared__ double smem[];
   const int b = xcd_block();
-  const int* inf = ba.info + kBlockInfo * b;
+  const int* inf = ba.info + kBlockInfoDev * b;
   const int cs = inf[0], n_own = inf[1], n_r1 = inf[2], n_r2 = inf[3];
   const int fs = inf[4], n_faces = inf[5], gs = inf[6], gps = inf[7];
   const int first = inf[8], ls = inf[9];
@@ -5571,8 +5588,41 @@ int mcfd_mesh_set_blocks(McfdMesh* m, const McfdBlockDesc* d) {
   }
   int rc = 0;
   BlockArrays& b = m->blocks;
-  rc |= upload_block(m, const_cast<int**>(&b.info), d->info, kBlockInfo * (size_t)d->n_blocks);
-  rc |= upload_block(m, const_cast<int**>(&b.cells), d->cells, (size_t)d->n_local);
+  {
+    // device block records and the local-cell list at a fixed stride per
+    // block (padded with each block's first cell), so that the fused kernel
+    // addresses a block's cells from its block id alone
+    const int nb = d->n_blocks;
+    const int stride = std::max(d->max_local, kBlock);
+    std::vector<int32_t> info((size_t)kBlockInfoDev * nb);
+    std::vector<int32_t> cells((size_t)stride * nb);
+    const std::vector<int>& aptr = m->host_acc_ptr;
+    const std::vector<int>& dptr = m->host_dts_ptr;
+    for (int k = 0; k < nb; ++k) {
+      const int32_t* in = d->info + (size_t)kBlockInfo * k;
+      int32_t* out = info.data() + (size_t)kBlockInfoDev * k;
+      const int n_loc = in[1] + in[2] + in[3];
+      if (in[0] < 0 || n_loc > d->max_local || (long)in[0] + n_loc > d->n_local) {
+        rc = fail("mcfd_mesh_set_blocks: local-cell lists out of range");
+        break;
+      }
+      std::copy(in, in + kBlockInfo, out);
+      out[0] = stride * k;
+      const long end = (long)in[7] + in[1] + in[2];
+      out[10] = d->gptr[end];
+      out[11] = d->lptr[end];
+      out[12] = aptr[in[8]];
+      out[13] = aptr[in[8] + in[1]] - aptr[in[8]];
+      out[14] = dptr[in[8]];
+      out[15] = dptr[in[8] + in[1]] - dptr[in[8]];
+      int32_t* c = cells.data() + (size_t)stride * k;
+      std::copy(d->cells + in[0], d->cells + in[0] + n_loc, c);
+      std::fill(c + n_loc, c + stride, d->cells[in[0]]);
+    }
+    b.cstride = stride;
+    if (!rc) rc |= upload_block(m, const_cast<int**>(&b.info), info.data(), info.size());
+    if (!rc) rc |= upload_block(m, const_cast<int**>(&b.cells), cells.data(), cells.size());
+  }
   rc |= upload_block(m, const_cast<int**>(&b.gptr), d->gptr, (size_t)d->n_gptr);
   {
     // two spare entries: a block's ids are copied as dwords from the even
