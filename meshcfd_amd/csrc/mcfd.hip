@@ -342,6 +342,49 @@ __device__ __forceinline__ void limiter_entry(double inc, double wk, double mxk,
   if (f < phik) phik = f;
 }
 
+// limiter_entry with the cell's per-variable constants hoisted (dmx = max - w,
+// dmn = min - w, tol = 1e-14 max(|w|, 1): the same operations, so the same
+// bits) and its branches turned into selects; only the division stays behind
+// a branch, which a wave skips when none of its lanes needs it.
+template <int CODE>
+__device__ __forceinline__ void limiter_entry_sel(double inc, double dmx, double dmn, double tol,
+                                                  double& phik) {
+  const bool up = inc > tol, down = inc < -tol;
+  const double num = up ? dmx : dmn;
+  bool need = up || down;
+  if constexpr (CODE == 3) {
+    // |inc| <= tol: ratio 1
+    const double eps = 1e-12;
+    double f1 = (1.0 * 1.0 + 2.0 * 1.0 + eps) / (1.0 * 1.0 + 1.0 + 2.0 + eps);
+    f1 = (f1 < 1.0) ? f1 : 1.0;
+    if (!need && f1 < phik) phik = f1;
+  }
+  if constexpr (CODE <= 1) {
+    const bool same = (num > 0.0 && inc > 0.0) || (num < 0.0 && inc < 0.0);
+    need = need && !(same && fabs(num) > phik * fabs(inc) * (1.0 + 1e-14));
+  }
+  if (need) {
+    double ratio = num / inc;
+    ratio = (0.0 > ratio) ? 0.0 : ratio;
+    double f;
+    if constexpr (CODE <= 1) {
+      f = (ratio < 1.0) ? ratio : 1.0;
+    } else if constexpr (CODE == 2) {
+      if (ratio > 0.0) {
+        double t = 2.0 * ratio / (1.0 + ratio);
+        f = (t < 1.0) ? t : 1.0;
+      } else {
+        f = 0.0;
+      }
+    } else {
+      const double eps = 1e-12;
+      f = (ratio * ratio + 2.0 * ratio + eps) / (ratio * ratio + ratio + 2.0 + eps);
+      f = (f < 1.0) ? f : 1.0;
+    }
+    if (f < phik) phik = f;
+  }
+}
+
 // LSQ gradient (_kernels.py:252-262) + limiter factor (_kernels.py:9-67).
 // With ARGS (for the adjoint) also per variable k: ARG[4i+k] = limiting
 // limiter entry | max entry << 8 | min entry << 16, entries counted from the
@@ -1640,6 +1683,13 @@ __global__ void __launch_bounds__(kBlock)
           }
         }
       }
+      double dmx[4], dmn[4], tol[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        dmx[k] = mx[k] - wv[k];
+        dmn[k] = mn[k] - wv[k];
+        tol[k] = 1e-14 * vmax(fabs(wv[k]), 1.0);
+      }
       const double2* lxy = sLE;
       for (int base = l0; base < l1; base += 4) {
         double dxb[4], dyb[4];
@@ -1657,7 +1707,7 @@ __global__ void __launch_bounds__(kBlock)
         double dx = dxb[q], dy = dyb[q];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          limiter_entry(gx[k] * dx + gy[k] * dy, wv[k], mx[k], mn[k], limiter, phi[k]);
+          limiter_entry_sel<LIM>(gx[k] * dx + gy[k] * dy, dmx[k], dmn[k], tol[k], phi[k]);
         }
         }
       }
