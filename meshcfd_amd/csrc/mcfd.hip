@@ -163,24 +163,26 @@ __device__ F4 hllc(const P4& L, const P4& R, double nx, double ny, double g, boo
   double den = a.rho * (sl - a.un) - b.rho * (sr - b.un);
   if (!(fabs(den) > 1e-14)) den = copysign(1e-14, den);
   double star = (b.p - a.p + a.rho * a.un * (sl - a.un) - b.rho * b.un * (sr - b.un)) / den;
-  F4 f;
+  // Branch choice first (both orders of euler.py:96-166 / _kernels.py:136-159),
+  // then one evaluation on the chosen side: lanes of a wave that take the
+  // left and the right star state share one pass.
+  bool left, use_star;
   if (sl >= 0.0) {
-    f = F4{a.F[0], a.F[1], a.F[2], a.F[3]};
+    left = true;
+    use_star = false;
   } else if (fused_order) {
-    if (sr <= 0.0)
-      f = F4{b.F[0], b.F[1], b.F[2], b.F[3]};
-    else if (star >= 0.0)
-      f = hllc_star(a, sl, star, nx, ny);
-    else
-      f = hllc_star(b, sr, star, nx, ny);
+    left = !(sr <= 0.0) && star >= 0.0;
+    use_star = !(sr <= 0.0);
   } else {
-    if (star >= 0.0)
-      f = hllc_star(a, sl, star, nx, ny);
-    else if (sr > 0.0)
-      f = hllc_star(b, sr, star, nx, ny);
-    else
-      f = F4{b.F[0], b.F[1], b.F[2], b.F[3]};
+    left = star >= 0.0;
+    use_star = left || sr > 0.0;
   }
+  const Side& s = left ? a : b;
+  F4 f;
+  if (use_star)
+    f = hllc_star(s, left ? sl : sr, star, nx, ny);
+  else
+    f = F4{s.F[0], s.F[1], s.F[2], s.F[3]};
   return f;
 }
 
