@@ -1560,7 +1560,6 @@ __global__ void __launch_bounds__(kBlock)
                   BlockArrays ba,
                   GeomArrays ga, FaceArrays fa, DtArrays da, double g, double ax, double ay,
                   StageArgs s, int* __restrict__ status) {
-  constexpr int limiter = LIM;
   extern __shared__ double smem[];
   constexpr bool muscl = MUSCL;
   const int b = xcd_block();
@@ -2240,7 +2239,8 @@ __global__ void __launch_bounds__(kBlock)
 // Block b reduces partial[b*nb .. b*nb+nb) into out[b]; bit b of sqrt_mask
 // takes the square root.
 __global__ void __launch_bounds__(kBlock)
-    k_dot_final(int nb, const double* __restrict__ partial, double* __restrict__ out, int sqrt_mask) {
+    k_dot_final(int nb, const double* __restrict__ partial, double* __restrict__ out, int sqrt_mask,
+                double* __restrict__ host_out = nullptr) {
   __shared__ double red[kBlock];
   const double* part = partial + (long)blockIdx.x * nb;
   double acc = 0.0;
@@ -2251,7 +2251,13 @@ __global__ void __launch_bounds__(kBlock)
     if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[blockIdx.x] = ((sqrt_mask >> blockIdx.x) & 1) ? sqrt(red[0]) : red[0];
+  if (threadIdx.x == 0) {
+    double v = ((sqrt_mask >> blockIdx.x) & 1) ? sqrt(red[0]) : red[0];
+    out[blockIdx.x] = v;
+    // pinned host mirror: the host reads it after synchronising the stream,
+    // without a copy launch
+    if (host_out) host_out[blockIdx.x] = v;
+  }
 }
 
 // (w, w) and (a, w) in one pass: partials of the first at partial[0..grid),
@@ -2361,27 +2367,40 @@ __global__ void __launch_bounds__(kBlock)
   x[i] = x[i] + acc;
 }
 
-// perturbed = s + (eps*sigma_k) * d   (solver.py:1553-1556)
+// perturbed = s + (eps*sigma_k) * d   (solver.py:1553-1556), eps = num / ||d||
+// with ||d|| from the device (solver.py:1549-1552); ||d|| = 0: the state
+// itself (the caller's product is then d)
 __global__ void __launch_bounds__(kBlock)
-    k_perturb(int n, const double* __restrict__ s, const double* __restrict__ d, double eps,
-              double s0, double s1, double s2, double s3, double* __restrict__ out) {
+    k_perturb(int n, const double* __restrict__ s, const double* __restrict__ d, double num,
+              const double* __restrict__ dnorm, double s0, double s1, double s2, double s3,
+              double* __restrict__ out) {
   int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const double dn = dnorm[0];
+  const double eps = (dn == 0.0) ? 0.0 : num / dn;
   P4 a = load4(s, i), b = load4(d, i);
   store4(out, i, a.r + eps * s0 * b.r, a.u + eps * s1 * b.u, a.v + eps * s2 * b.v,
          a.p + eps * s3 * b.p);
 }
 
-// out = d - (h * ((rp - r) / eps)) / sigma   (solver.py:1557-1563)
+// out = d - (h * ((rp - r) / eps)) / sigma   (solver.py:1557-1563); ||d|| = 0:
+// out = d (solver.py:1550-1551)
 __global__ void __launch_bounds__(kBlock)
     k_jv_finish(int n, const double* __restrict__ d, const double* __restrict__ rp,
-                const double* __restrict__ r, double eps, const double* __restrict__ dt,
-                int dt_per_cell, double s0, double s1, double s2, double s3,
-                double* __restrict__ out) {
+                const double* __restrict__ r, double num, const double* __restrict__ dnorm,
+                const double* __restrict__ dt, int dt_per_cell, double s0, double s1, double s2,
+                double s3, double* __restrict__ out) {
   int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const double dn = dnorm[0];
+  P4 a = load4(d, i);
+  if (dn == 0.0) {
+    store4(out, i, a.r, a.u, a.v, a.p);
+    return;
+  }
+  const double eps = num / dn;
   double h = dt_per_cell ? dt[i] : dt[0];
-  P4 a = load4(d, i), p = load4(rp, i), q = load4(r, i);
+  P4 p = load4(rp, i), q = load4(r, i);
   store4(out, i, a.r - h * ((p.r - q.r) / eps) / s0, a.u - h * ((p.u - q.u) / eps) / s1,
          a.v - h * ((p.v - q.v) / eps) / s2, a.p - h * ((p.p - q.p) / eps) / s3);
 }
@@ -4585,6 +4604,8 @@ struct McfdMesh {
   double* gm_rp = nullptr;      // 4C
   double* gm_scalars = nullptr; // 64 fixed slots, then 3 + m per-column dots, then m coefficients
   double* gm_y_host = nullptr;  // pinned, m coefficients of the basis combination
+  double* gm_scalars_host = nullptr;  // pinned mirror of gm_scalars, written by k_dot_final
+  size_t gm_scalars_n = 0;
   double* gm_partial = nullptr; // dot partials
   // MUSCL parabolic
   bool parabolic = false;
@@ -4714,6 +4735,8 @@ void free_implicit(McfdMesh* m) {
   m->implicit_allocations.clear();
   if (m->gm_y_host) (void)hipHostFree(m->gm_y_host);
   m->gm_y_host = nullptr;
+  if (m->gm_scalars_host) (void)hipHostFree(m->gm_scalars_host);
+  m->gm_scalars_host = nullptr;
   m->implicit = false;
   m->lusgs = LusgsArrays{};
   m->gmres_m = 0;
@@ -5005,16 +5028,30 @@ namespace {
 
 constexpr int kDotBlocks = 1024;
 
+// The pinned host mirror of a GMRES scalar slot (nullptr outside gm_scalars).
+double* host_slot(McfdMesh* m, const double* dev) {
+  if (!m->gm_scalars_host || dev < m->gm_scalars || dev >= m->gm_scalars + m->gm_scalars_n) return nullptr;
+  return m->gm_scalars_host + (dev - m->gm_scalars);
+}
+
 int dot_dev(McfdMesh* m, long n, const double* a, const double* b, double* out, bool sqrt_out,
             hipStream_t s) {
   hipLaunchKernelGGL(k_dot_partial, dim3(kDotBlocks), dim3(kBlock), 0, s, n, a, b, m->gm_partial);
   hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kBlock), 0, s, kDotBlocks, m->gm_partial, out,
-                     sqrt_out ? 1 : 0);
+                     sqrt_out ? 1 : 0, host_slot(m, out));
   HIP_CHECK(hipGetLastError());
   return 0;
 }
 
-int read_scalars(double* host, const double* dev, int count, hipStream_t s) {
+// Scalars written by k_dot_final into gm_scalars are read from the pinned
+// mirror after a stream synchronisation (no copy launch); others by a copy.
+int read_scalars(McfdMesh* m, double* host, const double* dev, int count, hipStream_t s) {
+  const double* h = host_slot(m, dev);
+  if (h && host_slot(m, dev + count - 1)) {
+    HIP_CHECK(hipStreamSynchronize(s));
+    std::memcpy(host, h, sizeof(double) * count);
+    return 0;
+  }
   HIP_CHECK(hipMemcpyAsync(host, dev, sizeof(double) * count, hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
   return 0;
@@ -5031,9 +5068,10 @@ int resolve_state_norm(McfdMesh* m, McfdGmres* g, hipStream_t s) {
   double* out = m->gm_scalars + 63;
   hipLaunchKernelGGL(k_scaled_sq_partial, dim3(kDotBlocks), dim3(kBlock), 0, s, 4L * m->n_cells, g->state,
                      g->sigma[0], g->sigma[1], g->sigma[2], g->sigma[3], m->gm_partial);
-  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kBlock), 0, s, kDotBlocks, m->gm_partial, out, 1);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kBlock), 0, s, kDotBlocks, m->gm_partial, out, 1,
+                     host_slot(m, out));
   HIP_CHECK(hipGetLastError());
-  return read_scalars(&g->state_norm, out, 1, s);
+  return read_scalars(m, &g->state_norm, out, 1, s);
 }
 
 void lusgs_launch(McfdMesh* m, const double* v, double* out, hipStream_t s, const int* te, bool wide) {
@@ -5132,6 +5170,10 @@ int ensure_gmres(McfdMesh* m, int krylov) {
   if (m->gm_y_host) (void)hipHostFree(m->gm_y_host);
   m->gm_y_host = nullptr;
   HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&m->gm_y_host), sizeof(double) * krylov));
+  if (m->gm_scalars_host) (void)hipHostFree(m->gm_scalars_host);
+  m->gm_scalars_host = nullptr;
+  HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&m->gm_scalars_host), sizeof(double) * sizes[6]));
+  m->gm_scalars_n = sizes[6];
   m->gmres_m = krylov;
   return 0;
 }
@@ -5190,21 +5232,21 @@ int op_precond(JacobianOp& op, const double* v, double* out, hipStream_t s) {
 int jacobian_apply(JacobianOp& op, const double* v, double* out, hipStream_t s) {
   McfdMesh* m = op.m;
   long N = 4L * m->n_cells;
-  double dn = 0.0;
-  if (dot_dev(m, N, v, v, m->gm_scalars + 40, true, s)) return -1;
-  if (read_scalars(&dn, m->gm_scalars + 40, 1, s)) return -1;
-  if (dn == 0.0) {
-    HIP_CHECK(hipMemcpyAsync(out, v, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
-    return 0;
-  }
+  // ||v|| stays on the device: eps = (fd * max(base, 1)) / ||v|| is formed
+  // by the perturbation and finishing kernels (same operations), so the
+  // product needs no host round trip
+  double* dn = m->gm_scalars + 40;
+  hipLaunchKernelGGL(k_dot_partial, dim3(kDotBlocks), dim3(kBlock), 0, s, N, v, v, m->gm_partial);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kBlock), 0, s, kDotBlocks, m->gm_partial, dn, 1,
+                     (double*)nullptr);
   const double fd = 1.4901161193847656e-08;  // sqrt(2^-52)
-  double eps = fd * (op.base > 1.0 ? op.base : 1.0) / dn;
+  const double num = fd * (op.base > 1.0 ? op.base : 1.0);
   int nb = blocks(m->n_cells);
-  hipLaunchKernelGGL(k_perturb, dim3(nb), dim3(kBlock), 0, s, m->n_cells, op.state, v, eps,
+  hipLaunchKernelGGL(k_perturb, dim3(nb), dim3(kBlock), 0, s, m->n_cells, op.state, v, num, dn,
                      op.sigma[0], op.sigma[1], op.sigma[2], op.sigma[3], m->gm_pert);
   if (mcfd_residual(m, op.st, m->gm_pert, m->gm_rp, s)) return -1;
   hipLaunchKernelGGL(k_jv_finish, dim3(nb), dim3(kBlock), 0, s, m->n_cells, v, m->gm_rp, op.rate,
-                     eps, op.dt, op.per_cell, op.sigma[0], op.sigma[1], op.sigma[2], op.sigma[3],
+                     num, dn, op.dt, op.per_cell, op.sigma[0], op.sigma[1], op.sigma[2], op.sigma[3],
                      out);
   HIP_CHECK(hipGetLastError());
   return 0;
@@ -5243,7 +5285,7 @@ int gmres_cycle(JacobianOp& op, const double* b, double* x, double rtol, int res
   inner = 0;
   if (dot_dev(m, N, b, b, sc + 0, true, s)) return -1;
   double bnrm2 = 0.0;
-  if (read_scalars(&bnrm2, sc, 1, s)) return -1;
+  if (read_scalars(m, &bnrm2, sc, 1, s)) return -1;
   if (bnrm2 == 0.0) {
     HIP_CHECK(hipMemcpyAsync(x, b, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
     info = 0;
@@ -5255,7 +5297,7 @@ int gmres_cycle(JacobianOp& op, const double* b, double* x, double rtol, int res
   if (op_precond(op, b, m->gm_w, s)) return -1;
   if (dot_dev(m, N, m->gm_w, m->gm_w, sc + 1, true, s)) return -1;
   double mb = 0.0;
-  if (read_scalars(&mb, sc + 1, 1, s)) return -1;
+  if (read_scalars(m, &mb, sc + 1, 1, s)) return -1;
   double ratio = atol / bnrm2;
   double ptol = mb * (ratio < 1.0 ? ratio : 1.0);
   const double eps = 2.220446049250313e-16;
@@ -5266,7 +5308,7 @@ int gmres_cycle(JacobianOp& op, const double* b, double* x, double rtol, int res
     hipLaunchKernelGGL(k_axpby, dim3(nbl), dim3(kBlock), 0, s, N, -1.0, m->gm_av, 1.0, m->gm_r);
     if (dot_dev(m, N, m->gm_r, m->gm_r, sc + 2, true, s)) return -1;
     double r0 = 0.0;
-    if (read_scalars(&r0, sc + 2, 1, s)) return -1;
+    if (read_scalars(m, &r0, sc + 2, 1, s)) return -1;
     if (r0 < atol) {
       info = 0;
       rnorm = r0;
@@ -5279,7 +5321,7 @@ int gmres_cycle(JacobianOp& op, const double* b, double* x, double rtol, int res
   if (op_precond(op, m->gm_r, V, s)) return -1;
   if (dot_dev(m, N, V, V, sc + 3, true, s)) return -1;
   double t0 = 0.0;
-  if (read_scalars(&t0, sc + 3, 1, s)) return -1;
+  if (read_scalars(m, &t0, sc + 3, 1, s)) return -1;
   hipLaunchKernelGGL(k_scale, dim3(nbl), dim3(kBlock), 0, s, N, V, 1.0 / t0, V);
   std::vector<double> h((size_t)restart * (restart + 1), 0.0);  // h[col*(restart+1) + row]
   std::vector<double> S(restart + 1, 0.0), gc(restart, 0.0), gs(restart, 0.0);
@@ -5297,17 +5339,20 @@ int gmres_cycle(JacobianOp& op, const double* b, double* x, double rtol, int res
     // (and the last in the pass forming ||w||), same arithmetic as one dot and
     // one axpy per step.
     hipLaunchKernelGGL(k_dot2_partial, dim3(kDotBlocks), dim3(kBlock), 0, s, N, V, m->gm_w, m->gm_partial);
-    hipLaunchKernelGGL(k_dot_final, dim3(2), dim3(kBlock), 0, s, kDotBlocks, m->gm_partial, cs, 1);
+    hipLaunchKernelGGL(k_dot_final, dim3(2), dim3(kBlock), 0, s, kDotBlocks, m->gm_partial, cs, 1,
+                       host_slot(m, cs));
     for (int k = 1; k <= col; ++k) {
       hipLaunchKernelGGL(k_axpy_dot_partial<false>, dim3(kDotBlocks), dim3(kBlock), 0, s, N,
                          V + (long)(k - 1) * N, m->gm_w, cs + 1 + k - 1, V + (long)k * N, m->gm_partial);
-      hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kBlock), 0, s, kDotBlocks, m->gm_partial, cs + 1 + k, 0);
+      hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kBlock), 0, s, kDotBlocks, m->gm_partial, cs + 1 + k, 0,
+                         host_slot(m, cs + 1 + k));
     }
     hipLaunchKernelGGL(k_axpy_dot_partial<true>, dim3(kDotBlocks), dim3(kBlock), 0, s, N, V + (long)col * N,
                        m->gm_w, cs + 1 + col, (const double*)nullptr, m->gm_partial);
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kBlock), 0, s, kDotBlocks, m->gm_partial, cs + 1 + col + 1, 1);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kBlock), 0, s, kDotBlocks, m->gm_partial, cs + 1 + col + 1, 1,
+                       host_slot(m, cs + 1 + col + 1));
     HIP_CHECK(hipGetLastError());
-    if (read_scalars(host.data(), cs, col + 3, s)) return -1;
+    if (read_scalars(m, host.data(), cs, col + 3, s)) return -1;
     double h0 = host[0];
     double* hc = h.data() + (size_t)col * (restart + 1);
     for (int k = 0; k <= col; ++k) hc[k] = host[1 + k];
@@ -5362,7 +5407,7 @@ int gmres_cycle(JacobianOp& op, const double* b, double* x, double rtol, int res
   HIP_CHECK(hipMemcpyAsync(m->gm_r, b, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL(k_axpby, dim3(nbl), dim3(kBlock), 0, s, N, -1.0, m->gm_av, 1.0, m->gm_r);
   if (dot_dev(m, N, m->gm_r, m->gm_r, sc + 2, true, s)) return -1;
-  if (read_scalars(&rnorm, sc + 2, 1, s)) return -1;
+  if (read_scalars(m, &rnorm, sc + 2, 1, s)) return -1;
   info = (rnorm <= atol) ? 0 : 1;
   HIP_CHECK(hipGetLastError());
   return 0;
