@@ -1037,14 +1037,33 @@ __global__ void __launch_bounds__(kBlock)
   if (!finite(T) || !(T > 0.0)) report(status, MCFD_ERR_TEMPERATURE, i);
   double val[3] = {w.u, w.v, T};
   double rx[3] = {0.0, 0.0, 0.0}, ry[3] = {0.0, 0.0, 0.0};
-  for (int e = grad_ptr[i]; e < grad_ptr[i + 1]; ++e) {
-    P4 o = load4(W, grad_nb[e]);
-    double d[3] = {o.u - w.u, o.v - w.v, o.p / (o.r * vp.R) - T};
-    double wx = va.wl_wx[e], wy = va.wl_wy[e];
+  // entries in batches of 4: the batch's loads are issued together, the sums
+  // still run in entry order
+  const int g0 = grad_ptr[i], g1 = grad_ptr[i + 1];
+  for (int base = g0; base < g1; base += 4) {
+    int nb[4];
+    double wxb[4], wyb[4];
+    P4 ob[4];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      rx[k] = rx[k] + wx * d[k];
-      ry[k] = ry[k] + wy * d[k];
+    for (int q = 0; q < 4; ++q) {
+      const bool ok = base + q < g1;
+      nb[q] = ok ? grad_nb[base + q] : i;
+      wxb[q] = ok ? va.wl_wx[base + q] : 0.0;
+      wyb[q] = ok ? va.wl_wy[base + q] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) ob[q] = load4(W, nb[q]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (base + q >= g1) break;
+      const P4& o = ob[q];
+      double d[3] = {o.u - w.u, o.v - w.v, o.p / (o.r * vp.R) - T};
+      double wx = wxb[q], wy = wyb[q];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        rx[k] = rx[k] + wx * d[k];
+        ry[k] = ry[k] + wy * d[k];
+      }
     }
   }
   double mu = visc_mu(T, vp, status, i);
@@ -1290,34 +1309,65 @@ struct ViscAcc {
   const double* VF;
 };
 
+// Items in batches of 4: the batch's item, coefficient and flux loads are
+// issued together; the sums still run in item order.
 __device__ __forceinline__ void visc_gather(int i, const ViscAcc& v, const double* flen, double A,
                                             double r[4]) {
   double s[4] = {0.0, 0.0, 0.0, 0.0};
-  int e = v.ptr[i], e1 = v.ptr[i + 1];
-  for (; e < e1; ++e) {
-    int item = v.item[e];
-    if ((item & 3) != 0) break;
-    P4 F = load4(v.VF, item >> 2);
-    double c = v.coef[e];
-    s[0] = s[0] + c * F.r;
-    s[1] = s[1] + c * F.u;
-    s[2] = s[2] + c * F.v;
-    s[3] = s[3] + c * F.p;
-  }
-  double d[4] = {0.0 - s[0], 0.0 - s[1], 0.0 - s[2], 0.0 - s[3]};
-  for (; e < e1; ++e) {
-    int item = v.item[e];
-    int f = item >> 2;
-    P4 F = load4(v.VF, f);
-    double L = flen[f];
-    double x[4] = {F.r * L, F.u * L, F.v * L, F.p * L};
-    if ((item & 3) == 1) {
+  double d[4] = {0.0, 0.0, 0.0, 0.0};
+  bool interior = true;
+  const int e0 = v.ptr[i], e1 = v.ptr[i + 1];
+  for (int base = e0; base < e1; base += 4) {
+    int it[4];
+    double cb[4], lb[4];
+    P4 Fb[4];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) d[k] = d[k] + x[k] / A;
-    } else {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) d[k] = d[k] + (-x[k]) / A;
+    for (int q = 0; q < 4; ++q) {
+      const bool ok = base + q < e1;
+      it[q] = ok ? v.item[base + q] : 0;
+      cb[q] = ok ? v.coef[base + q] : 0.0;
     }
+    // lengths of the items from the first periodic or boundary item on
+    bool after = !interior;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const bool ok = base + q < e1;
+      after = after || (ok && (it[q] & 3) != 0);
+      Fb[q] = ok ? load4(v.VF, it[q] >> 2) : P4{0.0, 0.0, 0.0, 0.0};
+      lb[q] = (ok && after) ? flen[it[q] >> 2] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (base + q >= e1) break;
+      const P4& F = Fb[q];
+      // interior items (0 - S) first, then the periodic and boundary items
+      if (interior && (it[q] & 3) == 0) {
+        double c = cb[q];
+        s[0] = s[0] + c * F.r;
+        s[1] = s[1] + c * F.u;
+        s[2] = s[2] + c * F.v;
+        s[3] = s[3] + c * F.p;
+        continue;
+      }
+      if (interior) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) d[k] = 0.0 - s[k];
+        interior = false;
+      }
+      double L = lb[q];
+      double x[4] = {F.r * L, F.u * L, F.v * L, F.p * L};
+      if ((it[q] & 3) == 1) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) d[k] = d[k] + x[k] / A;
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) d[k] = d[k] + (-x[k]) / A;
+      }
+    }
+  }
+  if (interior) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) d[k] = 0.0 - s[k];
   }
 #pragma unroll
   for (int k = 0; k < 4; ++k) r[k] = r[k] + d[k];
@@ -1384,30 +1434,45 @@ __global__ void __launch_bounds__(kBlock)
   double A = area[i];
   double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
   int e0 = aptr[i], e1 = aptr[i + 1];
-  for (int e = e0; e < e1; ++e) {
-    int item = aitem[e];
-    int f = item >> 2;
-    int mode = item & 3;
-    P4 F = load4(FLUX, f);
-    if (mode <= MCFD_ACC_NEIGHBOUR) {
-      double c = acoef[e];  // (-L)/A or L/A, precomputed like the reference's CSR data
-      r0 = r0 + c * F.r;
-      r1 = r1 + c * F.u;
-      r2 = r2 + c * F.v;
-      r3 = r3 + c * F.p;
-      continue;
+  // items in batches of 4: the batch's loads are issued together, the sums
+  // still run in item order
+  for (int base = e0; base < e1; base += 4) {
+    int it[4];
+    double cb[4];
+    P4 Fb[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const bool ok = base + q < e1;
+      it[q] = ok ? aitem[base + q] : 0;
+      cb[q] = ok ? acoef[base + q] : 0.0;
     }
-    double L = flen[f];
-    if (mode == MCFD_ACC_BOUNDARY) {
-      r0 = r0 + (-(F.r * L)) / A;
-      r1 = r1 + (-(F.u * L)) / A;
-      r2 = r2 + (-(F.v * L)) / A;
-      r3 = r3 + (-(F.p * L)) / A;
-    } else {
-      r0 = r0 + (F.r * L) / A;
-      r1 = r1 + (F.u * L) / A;
-      r2 = r2 + (F.v * L) / A;
-      r3 = r3 + (F.p * L) / A;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) Fb[q] = (base + q < e1) ? load4(FLUX, it[q] >> 2) : P4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (base + q >= e1) break;
+      const int mode = it[q] & 3;
+      const P4& F = Fb[q];
+      if (mode <= MCFD_ACC_NEIGHBOUR) {
+        double c = cb[q];  // (-L)/A or L/A, precomputed like the reference's CSR data
+        r0 = r0 + c * F.r;
+        r1 = r1 + c * F.u;
+        r2 = r2 + c * F.v;
+        r3 = r3 + c * F.p;
+        continue;
+      }
+      double L = flen[it[q] >> 2];
+      if (mode == MCFD_ACC_BOUNDARY) {
+        r0 = r0 + (-(F.r * L)) / A;
+        r1 = r1 + (-(F.u * L)) / A;
+        r2 = r2 + (-(F.v * L)) / A;
+        r3 = r3 + (-(F.p * L)) / A;
+      } else {
+        r0 = r0 + (F.r * L) / A;
+        r1 = r1 + (F.u * L) / A;
+        r2 = r2 + (F.v * L) / A;
+        r3 = r3 + (F.p * L) / A;
+      }
     }
   }
   if (vacc.ptr) {  // derivative += _viscous_residual(...) (solver.py:2705-2708)
