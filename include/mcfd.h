@@ -477,11 +477,16 @@ int mcfd_mesh_set_parabolic_adjoint(McfdMesh* mesh, const int32_t* par_rev);
 
 /* ubar = lambda^T dR/dU at state u: the hand-written reverse of `_residual`
  * (solver.py:2437-2718) through fluxes, ghost states, reconstruction,
- * limiter, LSQ gradient and primitive conversion. If sbar is not NULL it
- * receives per boundary record [n_bc*4] lambda^T dR/d(exterior state) for
- * velocity-inlet and far-field records (0 for the others), for gradients
- * with respect to inflow conditions. No counterpart in the reference (which
- * has no adjoint). */
+ * limiter, LSQ gradient and primitive conversion. Pressure-inlet and
+ * mass-flow records (solver.py:4412-4577) are differentiated through their
+ * exterior states: the pressure inlet's Mach number by the implicit-function
+ * derivative of its invariant match, the mass-flow inlet's patch speed by
+ * that of its flow balance (a per-patch reduction), the mass-flow outlet in
+ * closed form; needs mcfd_mesh_set_inflow like the residual. If sbar is not
+ * NULL it receives per boundary record [n_bc*4] lambda^T dR/d(exterior
+ * state) for velocity-inlet and far-field records (0 for the others), for
+ * gradients with respect to inflow conditions. No counterpart in the
+ * reference (which has no adjoint). */
 int mcfd_residual_vjp(McfdMesh* mesh, const McfdSettings* settings, const double* u,
                       const double* lambda, double* ubar, double* sbar, void* stream);
 

@@ -825,6 +825,49 @@ __device__ P4 mass_flow_outlet_state(const P4& in, double nx, double ny, const d
   return P4{in.r, un * nx + ut * tx, un * ny + ut * ty, in.p};
 }
 
+// Reverse of one inflow record's exterior state e = ext(in), the adjoint eb
+// of e added into the interior primitive adjoint inb.
+// Pressure inlet: e depends on in through the outgoing invariant
+// J = u.n + 2 c / (g-1) alone; the Mach number M solves inv(M) = J, so
+// dM/dJ = 1 / inv'(M) with inv(M) = c0 th^(-1/2) (dn M + 2/(g-1)),
+// th = 1 + (g-1)/2 M^2, and along M: d rho/dM = -rho M / th,
+// d speed/dM = a / th (a = sqrt(g R T)), dp/dM = -g p M / th (M, a and th
+// from e itself). Mass-flow outlet: rho and p copied, the normal velocity
+// mdot / (L rho) and the tangential velocity u.t. Mass-flow inlet: the face's
+// own pressure (rho = p / (R T) and p); the patch speed's dependence on every
+// face's pressure is added per patch by k_mass_flow_inlet_adjoint.
+__device__ void inflow_state_b(int kind, const P4& in, const P4& e, double nx, double ny, const double* par,
+                               double dn, double g, double R, const P4& eb, P4& inb) {
+  if (kind == MCFD_FACE_PRESSURE_INLET) {
+    const double h = 0.5 * (g - 1.0);
+    const double speed = sqrt(e.u * e.u + e.v * e.v);
+    const double a = sqrt(g * e.p / e.r);
+    const double M = speed / a;
+    const double th = 1.0 + h * (M * M);
+    const double c0 = sqrt(g * R * par[1]);
+    const double dinv = c0 / sqrt(th) * (dn - h * M * (dn * M + 2.0 / (g - 1.0)) / th);
+    const double Mb = eb.r * (-e.r * M / th) + (eb.u * par[2] + eb.v * par[3]) * (a / th) +
+                      eb.p * (-g * e.p * M / th);
+    const double Jb = Mb / dinv;
+    const double c_in = sqrt(g * in.p / in.r);
+    inb.u += Jb * nx;
+    inb.v += Jb * ny;
+    inb.p += Jb * g / ((g - 1.0) * in.r * c_in);
+    inb.r -= Jb * c_in / ((g - 1.0) * in.r);
+  } else if (kind == MCFD_FACE_MASS_FLOW_INLET) {
+    inb.p += eb.r * (e.r / e.p) + eb.p;
+  } else {
+    const double un = par[0] / (par[1] * in.r);
+    const double tx = -ny, ty = nx;
+    const double unb = eb.u * nx + eb.v * ny;
+    const double utb = eb.u * tx + eb.v * ty;
+    inb.r += eb.r - unb * un / in.r;
+    inb.u += utb * tx;
+    inb.v += utb * ty;
+    inb.p += eb.p;
+  }
+}
+
 __global__ void __launch_bounds__(kBlock)
     k_inflow_records(InflowArrays ia, FaceArrays fa, const double* __restrict__ W,
                      const double* __restrict__ G, const double* __restrict__ PHI,
@@ -855,6 +898,31 @@ __global__ void __launch_bounds__(kBlock)
   }
   if (err) report(status, err, r);
   store4(ia.ext, r, e.r, e.u, e.v, e.p);
+}
+
+// Reverse of the convective faces' inflow states: per record, the exterior
+// adjoint in the right slot of the face's FADJ record goes through
+// inflow_state_b into the left slot (the reconstructed left state's adjoint).
+__global__ void __launch_bounds__(kBlock)
+    k_inflow_adjoint(InflowArrays ia, FaceArrays fa, const double* __restrict__ W,
+                     const double* __restrict__ G, const double* __restrict__ PHI,
+                     const double* __restrict__ K, int muscl, double g, double R, double* __restrict__ FADJ) {
+  int r = xcd_block() * blockDim.x + threadIdx.x;
+  if (r >= ia.n) return;
+  const int k = ia.face[r];
+  const int kind = fa.kind[k];
+  const double nx = fa.nx[k], ny = fa.ny[k];
+  const P4 in = K ? reconstruct_q(W, G, K, PHI, fa.left[k], fa.lx[k], fa.ly[k])
+                  : reconstruct(W, G, PHI, fa.left[k], fa.lx[k], fa.ly[k], muscl);
+  const P4 e = load4(ia.ext, r);
+  double* q = FADJ + 8 * (long)k;
+  const P4 eb{q[4], q[5], q[6], q[7]};
+  P4 inb{0.0, 0.0, 0.0, 0.0};
+  inflow_state_b(kind, in, e, nx, ny, ia.params + 4 * (long)r, ia.proj[r], g, R, eb, inb);
+  q[0] += inb.r;
+  q[1] += inb.u;
+  q[2] += inb.v;
+  q[3] += inb.p;
 }
 
 // Sum over a mass-flow-inlet patch of (p_i / (R T)) * speed * proj_i * L_i
@@ -956,6 +1024,90 @@ __global__ void __launch_bounds__(256)
     int r = a + i;
     double pr = ia.ext[4 * (long)r + 3];
     store4(ia.ext, r, pr / RT, u, v, pr);
+  }
+}
+
+// The patch speed s of a mass-flow inlet solves
+// G(s, p) = sum_g p_g / (R T(s)) s w_g - mdot = 0, T = t0 - s^2 / (2 cp),
+// w_g = (-n.d)_g L_g. With S = sum_g eb_g . d e_g/ds (d rho_g/ds =
+// p_g s / (R cp T^2), d(u, v)/ds = d), each face's pressure adjoint gains
+// -S (dG/dp_f) / (dG/ds), dG/dp_f = s w_f / (R T),
+// dG/ds = sum_g p_g w_g / R (1/T + s^2 / (cp T^2)). Sums over the patch by
+// a fixed tree (deterministic). Returns S / (dG/ds) * (s / (R T)) to every
+// thread; the caller multiplies by -w_f.
+__device__ double mass_flow_patch_factor(const InflowArrays& ia, const double* __restrict__ ext,
+                                         const double* __restrict__ face_len, const double* __restrict__ ebar,
+                                         int ebar_stride, int ebar_off, const int* __restrict__ ebar_face,
+                                         int a, int n, double g, double R, double* red) {
+  const double* par = ia.params + 4 * (long)a;
+  const double t0 = par[1], d0 = par[2], d1 = par[3];
+  const double cp = g * R / (g - 1.0);
+  const double* e0 = ext + 4 * (long)a;
+  const double s = (fabs(d0) >= fabs(d1)) ? e0[1] / d0 : e0[2] / d1;
+  const double T = t0 - s * s / (2.0 * cp);
+  double dg = 0.0, sb = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int r = a + i;
+    const double p = ext[4 * (long)r + 3];
+    const double w = ia.proj[r] * face_len[ia.face[r]];
+    const long row = ebar_face ? (long)ebar_face[r] : (long)r;
+    const double* eb = ebar + ebar_stride * row + ebar_off;
+    dg += p * w / R * (1.0 / T + s * s / (cp * T * T));
+    sb += eb[0] * (p * s / (R * cp * T * T)) + eb[1] * d0 + eb[2] * d1;
+  }
+  red[threadIdx.x] = dg;
+  red[blockDim.x + threadIdx.x] = sb;
+  __syncthreads();
+  for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) {
+      red[threadIdx.x] += red[threadIdx.x + k];
+      red[blockDim.x + threadIdx.x] += red[blockDim.x + threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  const double f = red[blockDim.x] / red[0] * (s / (R * T));
+  __syncthreads();
+  return f;
+}
+
+// Convective faces: one workgroup per mass-flow-inlet patch; the exterior
+// state adjoints are the right slots of the faces' FADJ records, the
+// pressure adjoint goes to the left slot (reconstructed left state).
+__global__ void __launch_bounds__(256)
+    k_mass_flow_inlet_adjoint(InflowArrays ia, const double* __restrict__ face_len, double g, double R,
+                              double* __restrict__ FADJ) {
+  __shared__ double red[512];
+  const int p = blockIdx.x;
+  const int a = ia.patch_rec[p], n = ia.patch_rec[p + 1] - a;
+  const double f = mass_flow_patch_factor(ia, ia.ext, face_len, FADJ, 8, 4, ia.face, a, n, g, R, red);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int r = a + i;
+    const int k = ia.face[r];
+    FADJ[8 * (long)k + 3] += -f * ia.proj[r] * face_len[k];
+  }
+}
+
+// Viscous boundary values (exterior states from the owner cells): the
+// exterior adjoints were gathered per record into vbar; the pressure adjoint
+// goes to the owner cells' primitive adjoints. One workgroup walks the
+// patches and then their records in turn, so cells owning several records
+// are updated in a fixed order.
+__global__ void __launch_bounds__(256)
+    k_mass_flow_inlet_visc_adjoint(InflowArrays ia, int patches, const double* __restrict__ ext_v,
+                                   const double* __restrict__ face_len, const int* __restrict__ owner,
+                                   const double* __restrict__ vbar, double g, double R,
+                                   double* __restrict__ wbar) {
+  __shared__ double red[512];
+  for (int p = 0; p < patches; ++p) {
+    const int a = ia.patch_rec[p], n = ia.patch_rec[p + 1] - a;
+    const double f = mass_flow_patch_factor(ia, ext_v, face_len, vbar, 4, 0, nullptr, a, n, g, R, red);
+    if (threadIdx.x == 0)
+      for (int i = 0; i < n; ++i) {
+        const int r = a + i;
+        const int k = ia.face[r];
+        wbar[4 * (long)owner[k] + 3] += -f * ia.proj[r] * face_len[k];
+      }
+    __syncthreads();
   }
 }
 
@@ -3140,6 +3292,8 @@ __global__ void __launch_bounds__(kBlock)
     } else if (kind == MCFD_FACE_FAR_FIELD) {
       int err = 0;
       wr = far_field(wl, P4{st[0], st[1], st[2], st[3]}, nx, ny, g, &err);
+    } else if (kind >= MCFD_FACE_PRESSURE_INLET) {
+      wr = load4(fa.inflow, b - fa.inflow_first);
     }
   }
   P4 lb, rb;
@@ -3166,6 +3320,10 @@ __global__ void __launch_bounds__(kBlock)
     } else if (kind == MCFD_FACE_FAR_FIELD) {
       const double* st = fa.bc_state + 4 * (long)b;
       far_field_b(wl, P4{st[0], st[1], st[2], st[3]}, nx, ny, g, rb, lb, ext, nbx, nby);
+    } else if (kind >= MCFD_FACE_PRESSURE_INLET) {
+      // the exterior state's adjoint stays in the right slot for
+      // k_inflow_adjoint (and the mass-flow-inlet patch pass)
+      ext = rb;
     } else {  // transmissive: exterior = interior
       lb.r += rb.r;
       lb.u += rb.u;
@@ -3179,7 +3337,11 @@ __global__ void __launch_bounds__(kBlock)
   q[1] = make_double2(lb.v, lb.p);
   q[2] = make_double2(rb.r, rb.u);
   q[3] = make_double2(rb.v, rb.p);
-  if (SBAR && kind > MCFD_FACE_PERIODIC) store4(SBAR, b, ext.r, ext.u, ext.v, ext.p);
+  // inflow records: no state of their own to differentiate (zero)
+  if (SBAR && kind > MCFD_FACE_PERIODIC) {
+    if (kind >= MCFD_FACE_PRESSURE_INLET) ext = P4{0.0, 0.0, 0.0, 0.0};
+    store4(SBAR, b, ext.r, ext.u, ext.v, ext.p);
+  }
 }
 
 // Geometry adjoint of the face terms, lambda^T dR/d(face geometry): per
@@ -3357,6 +3519,9 @@ __global__ void __launch_bounds__(kBlock)
       DR.p = dconst(st[3]);
     } else if (kind == MCFD_FACE_FAR_FIELD) {
       DR = far_field_d(DL, seed(P4{st[0], st[1], st[2], st[3]}, 4), nx, ny, g);
+    } else if (kind >= MCFD_FACE_PRESSURE_INLET) {
+      // directions 4-7: the inflow record's exterior state (k_inflow_adjoint)
+      DR = seed(load4(fa.inflow, b - fa.inflow_first), 4);
     } else {
       DR = DL;
     }
@@ -3683,7 +3848,29 @@ __device__ __forceinline__ void uvT_to_prim_b(const P4& w, double R, double ub, 
 // state adjoint (rho, u, v, p), added to what the convective faces stored.
 __device__ __forceinline__ void visc_inlet_b(int code, const P4& w, const double* st, double nx, double ny,
                                              const ViscParams& v, const double xb[3], double wb[4],
-                                             double* sb) {
+                                             double* sb, int kind = 0, const double* par = nullptr,
+                                             double proj = 0.0, double* vib = nullptr) {
+  if (code == kVisInflow) {
+    // pressure / mass-flow inlet: st is the record's exterior state from the
+    // owner cell w; its (u, v, T) adjoint goes through inflow_state_b into w,
+    // and for a mass-flow inlet also into the record's slot of vib for the
+    // patch pass (k_mass_flow_inlet_visc_adjoint)
+    const P4 e{st[0], st[1], st[2], st[3]};
+    const P4 eb{-xb[2] * e.p / (e.r * e.r * v.R), xb[0], xb[1], xb[2] / (e.r * v.R)};
+    P4 inb{0.0, 0.0, 0.0, 0.0};
+    inflow_state_b(kind, w, e, nx, ny, par, proj, v.gamma, v.R, eb, inb);
+    wb[0] += inb.r;
+    wb[1] += inb.u;
+    wb[2] += inb.v;
+    wb[3] += inb.p;
+    if (kind == MCFD_FACE_MASS_FLOW_INLET && vib) {
+      vib[0] += eb.r;
+      vib[1] += eb.u;
+      vib[2] += eb.v;
+      vib[3] += eb.p;
+    }
+    return;
+  }
   if (code == kVisInlet) {
     if (sb) {
       sb[0] += -xb[2] * st[3] / (st[0] * st[0] * v.R);
@@ -3718,7 +3905,7 @@ __global__ void __launch_bounds__(kBlock)
                         const double* __restrict__ VG, ViscArrays va, ViscParams vp,
                         const double* __restrict__ flen, const double* __restrict__ area,
                         const double* __restrict__ lam, double* __restrict__ VFA,
-                        double* __restrict__ SBAR) {
+                        double* __restrict__ SBAR, InflowArrays ia, double* __restrict__ VIB) {
   int f = xcd_block() * blockDim.x + threadIdx.x;
   if (f >= nf) return;
   int kind = fa.kind[f];
@@ -3826,9 +4013,9 @@ __global__ void __launch_bounds__(kBlock)
       g[5] = g[5] - gt * ny;
       bv[0] = bv[0] - un * nx;
       bv[1] = bv[1] - un * ny;
-    } else if (code == kVisInlet || code == kVisFarField) {
+    } else if (code == kVisInlet || code == kVisInflow || code == kVisFarField) {
       int dummy = 0;
-      visc_inlet(code, w, fa.bc_state + 4 * (long)b, nx, ny, vp, x, &dummy, o);
+      visc_inlet(code, w, visc_inlet_state(code, b, va, fa), nx, ny, vp, x, &dummy, o);
       double cv[3] = {w.u, w.v, T};
       for (int c = 0; c < 3; ++c) {
         bv[c] = x[c];
@@ -3922,7 +4109,7 @@ __global__ void __launch_bounds__(kBlock)
       }
       uvTb[0] += unb * nx;
       uvTb[1] += unb * ny;
-    } else if (code == kVisInlet || code == kVisFarField) {
+    } else if (code == kVisInlet || code == kVisInflow || code == kVisFarField) {
       double xb[3];
       for (int c = 0; c < 3; ++c) {
         double sb = gb[2 * c] * nx + gb[2 * c + 1] * ny;
@@ -3931,7 +4118,11 @@ __global__ void __launch_bounds__(kBlock)
         xb[c] = sb / dn + bvb[c];
         uvTb[c] += -sb / dn;
       }
-      visc_inlet_b(code, w, fa.bc_state + 4 * (long)b, nx, ny, vp, xb, wb, SBAR ? SBAR + 4 * (long)b : nullptr);
+      const int ir = b - va.inf_first;
+      const bool rec = code == kVisInflow;
+      visc_inlet_b(code, w, visc_inlet_state(code, b, va, fa), nx, ny, vp, xb, wb,
+                   SBAR ? SBAR + 4 * (long)b : nullptr, kind, rec ? ia.params + 4 * (long)ir : nullptr,
+                   rec ? ia.proj[ir] : 0.0, rec && VIB ? VIB + 4 * (long)ir : nullptr);
     } else {
       for (int c = 0; c < 3; ++c) {
         gb[2 * c] += bvb[c] * offx;
@@ -3964,7 +4155,8 @@ __global__ void __launch_bounds__(kBlock)
                         const int* __restrict__ acc_ptr, const int* __restrict__ acc_item,
                         const int* __restrict__ grad_ptr, const int* __restrict__ grad_rev,
                         ViscArrays va, FaceArrays fa, ViscParams vp, const double* __restrict__ VFA,
-                        double* __restrict__ wbar, double* __restrict__ vbar, double* __restrict__ SBAR) {
+                        double* __restrict__ wbar, double* __restrict__ vbar, double* __restrict__ SBAR,
+                        InflowArrays ia, double* __restrict__ VIB) {
   int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double G[6] = {0, 0, 0, 0, 0, 0}, mub = 0.0, kb = 0.0, wb[4] = {0.0, 0.0, 0.0, 0.0};
@@ -4038,10 +4230,14 @@ __global__ void __launch_bounds__(kBlock)
       valb[0] += ghb[0] - t * nx;
       valb[1] += ghb[1] - t * ny;
       valb[2] += ghb[2];
-    } else if (code == kVisInlet || code == kVisFarField) {
+    } else if (code == kVisInlet || code == kVisInflow || code == kVisFarField) {
       double xb[3] = {2.0 * ghb[0], 2.0 * ghb[1], 2.0 * ghb[2]};
       for (int k = 0; k < 3; ++k) valb[k] -= ghb[k];
-      visc_inlet_b(code, w, fa.bc_state + 4 * (long)b, nx, ny, vp, xb, wb, SBAR ? SBAR + 4 * (long)b : nullptr);
+      const int ir = b - va.inf_first;
+      const bool rec = code == kVisInflow;
+      visc_inlet_b(code, w, visc_inlet_state(code, b, va, fa), nx, ny, vp, xb, wb,
+                   SBAR ? SBAR + 4 * (long)b : nullptr, fa.kind[f], rec ? ia.params + 4 * (long)ir : nullptr,
+                   rec ? ia.proj[ir] : 0.0, rec && VIB ? VIB + 4 * (long)ir : nullptr);
     } else {
       for (int k = 0; k < 3; ++k) valb[k] += ghb[k];
     }
@@ -4703,6 +4899,7 @@ struct McfdMesh {
   bool inflow = false;        // mcfd_mesh_set_inflow done
   InflowArrays inf{};
   double* inf_ext_v = nullptr;  // [4 * inflow records] exterior states for the viscous terms
+  double* inf_vbar = nullptr;   // [4 * inflow records] their adjoints (viscous mass-flow-inlet patch pass)
   int inflow_patches = 0, inflow_max_n = 0, inflow_max_leaves = 0;
   std::vector<void*> inflow_allocations;
   std::vector<int> host_face_bc;  // [n_faces] for validating the inflow records
@@ -4897,6 +5094,7 @@ void free_inflow(McfdMesh* m) {
   m->inflow = false;
   m->inf = InflowArrays{};
   m->inf_ext_v = nullptr;
+  m->inf_vbar = nullptr;
   m->faces.inflow = nullptr;
   m->dts.inflow = nullptr;
   m->inflow_patches = m->inflow_max_n = m->inflow_max_leaves = 0;
@@ -5521,12 +5719,11 @@ int mcfd_residual_vjp_impl(McfdMesh* m, const McfdSettings* st, const double* u,
     return fail("mcfd_residual_vjp: MUSCL parabolic needs mcfd_mesh_set_parabolic_adjoint first");
   if (st->viscosity && !(m->viscous && m->VFA))
     return fail("mcfd_residual_vjp: viscous settings need mcfd_mesh_set_viscous before mcfd_mesh_set_adjoint");
-  if (m->needs_inflow)
-    return fail("mcfd_residual_vjp: pressure-inlet and mass-flow boundaries have no adjoint yet");
+  if (check_inflow(m, st)) return -1;
   int n = m->n_cells;
   bool muscl = (st->reconstruction == 1 || para) && m->n_grad > 0;
   size_t vjp_lds = m->fused ? vjp_fused_lds_bytes(m) : 0;
-  if (m->fused && !para && !st->viscosity && st->flux != 2 && vjp_lds <= 64 * 1024) {
+  if (m->fused && !para && !st->viscosity && st->flux != 2 && !m->needs_inflow && vjp_lds <= 64 * 1024) {
     // one workgroup per block (reverse-mode fluxes), then the neighbour pass
     GeomArrays ga{m->area,   m->grad_ptr, m->grad_cx,  m->grad_cy, m->lim_ptr,
                   m->lim_dx, m->lim_dy,   m->acc_ptr,  m->face_len, m->acc_coef};
@@ -5561,6 +5758,10 @@ int mcfd_residual_vjp_impl(McfdMesh* m, const McfdSettings* st, const double* u,
                          m->grad_nb, m->grad_cx, m->grad_cy, m->lim_ptr, m->lim_dx, m->lim_dy,
                          st->limiter, m->G, m->PHI, m->adj.arg);
   }
+  const bool inflow = m->inflow && m->inf.n > 0;
+  // pressure / mass-flow boundaries: the forward exterior states from the
+  // reconstructed face states (what the residual's fluxes saw)
+  if (inflow && inflow_states(m, st, s, true, para, muscl)) return -1;
   if (m->n_faces) {
     // Rusanov and HLLC in reverse mode; Roe with duals
     bool reverse = st->flux != 2;
@@ -5573,20 +5774,43 @@ int mcfd_residual_vjp_impl(McfdMesh* m, const McfdSettings* st, const double* u,
       hipLaunchKernelGGL(k_flux_adjoint, dim3(blocks(m->n_faces)), dim3(kBlock), 0, s, m->n_faces,
                          m->faces, m->W, m->G, m->PHI, Kq, m->face_len, m->area, lam, st->flux,
                          muscl ? 1 : 0, st->gamma, m->FADJ, sbar, 0);
+    if (inflow) {
+      // exterior-state adjoints (right slots) -> reconstructed left states
+      hipLaunchKernelGGL(k_inflow_adjoint, dim3(blocks(m->inf.n)), dim3(kBlock), 0, s, m->inf, m->faces, m->W,
+                         m->G, m->PHI, Kq, muscl ? 1 : 0, st->gamma, st->gas_constant, m->FADJ);
+      if (m->inflow_patches)
+        hipLaunchKernelGGL(k_mass_flow_inlet_adjoint, dim3(m->inflow_patches), dim3(256), 0, s, m->inf,
+                           m->face_len, st->gamma, st->gas_constant, m->FADJ);
+    }
   }
   hipLaunchKernelGGL(k_cell_adjoint1, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->G, m->PHI,
                      m->FADJ, aa, muscl ? 1 : 0, st->limiter);
   aa.vbar = nullptr;
   if (st->viscosity) {
     ViscParams vp = visc_params(st);
+    ViscArrays va = m->visc;
+    double* vib = nullptr;
+    if (inflow) {
+      // the viscous boundary values of the inlets: exterior states from the
+      // owner cells, as in the forward residual
+      if (inflow_states(m, st, s, false, false, false, m->inf_ext_v)) return -1;
+      va.inf_ext = m->inf_ext_v;
+      va.inf_first = m->inf.first;
+      vib = m->inf_vbar;
+      hipLaunchKernelGGL(k_fill, dim3(lblocks(4L * m->inf.n)), dim3(kBlock), 0, s, 4L * m->inf.n, 0.0, vib);
+    }
     hipLaunchKernelGGL(k_visc_gradient, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->grad_ptr,
-                       m->grad_nb, m->visc, m->faces, vp, m->VG, m->status);
+                       m->grad_nb, va, m->faces, vp, m->VG, m->status);
     if (m->n_faces)
       hipLaunchKernelGGL(k_visc_face_adjoint, dim3(blocks(m->n_faces)), dim3(kBlock), 0, s, m->n_faces,
-                         m->faces, m->W, m->VG, m->visc, vp, m->face_len, m->area, lam, m->VFA, sbar);
+                         m->faces, m->W, m->VG, va, vp, m->face_len, m->area, lam, m->VFA, sbar, m->inf, vib);
     hipLaunchKernelGGL(k_visc_cell_adjoint, dim3(blocks(n)), dim3(kBlock), 0, s, n, m->W, m->VG,
-                       m->acc_ptr, m->acc_item, m->grad_ptr, m->grad_rev, m->visc, m->faces, vp, m->VFA,
-                       m->adj.wbar, m->VBAR, sbar);
+                       m->acc_ptr, m->acc_item, m->grad_ptr, m->grad_rev, va, m->faces, vp, m->VFA,
+                       m->adj.wbar, m->VBAR, sbar, m->inf, vib);
+    if (inflow && m->inflow_patches)
+      hipLaunchKernelGGL(k_mass_flow_inlet_visc_adjoint, dim3(1), dim3(256), 0, s, m->inf, m->inflow_patches,
+                         m->inf_ext_v, m->face_len, m->faces.left, vib, st->gamma, st->gas_constant,
+                         m->adj.wbar);
     aa.vbar = m->VBAR;
   }
   hipLaunchKernelGGL(k_cell_adjoint2, dim3(blocks(n)), dim3(kBlock), 0, s, n, u, m->W, lam,
@@ -6438,6 +6662,7 @@ int mcfd_mesh_set_inflow(McfdMesh* m, const McfdInflowDesc* d) {
   rc |= alloc_inflow(m, &program, d->program, (size_t)d->n_program);
   rc |= alloc_inflow(m, &ia.ext, (const double*)nullptr, 4 * (size_t)n);
   rc |= alloc_inflow(m, &m->inf_ext_v, (const double*)nullptr, 4 * (size_t)n);
+  rc |= alloc_inflow(m, &m->inf_vbar, (const double*)nullptr, 4 * (size_t)n);
   if (rc) {
     std::string msg = g_last_error;
     free_inflow(m);

@@ -342,3 +342,130 @@ def test_gpu_inflow_errors_name_the_patch(cuda_device):
     with pytest.raises(SolverError, match="Pressure inlet 'left' needs Mach number to define a supersonic inflow"):
         dm.check_status()
     dm.close()
+
+
+# --- adjoint of the pressure-inlet and mass-flow boundaries -------------------
+
+
+def _vjp_setup(cuda_device, case, seed=5):
+    from meshcfd_amd.adjoint import residual_vjp
+    from meshcfd_amd.device import DeviceMesh
+
+    dm = DeviceMesh(case, cuda_device, block_cells=64)
+    # flow_state's v depends on x alone, so cells with equal centroid x (the
+    # straight outlet column) tie in v, where the limiter's extrema are not
+    # differentiable; a 1% random factor removes the ties
+    prim = flow_state(case, seed=seed)
+    prim = prim * (1.0 + 0.01 * np.random.default_rng(seed + 100).random(prim.shape))
+    u = euler.primitive_to_conservative(prim, 1.4)
+    lam = np.random.default_rng(11).normal(size=u.shape)
+    ubar = dm.to_mesh_order(residual_vjp(dm, dm.to_device_order(u), dm.to_device_order(lam)))
+    dm.check_status()
+    return dm, u, lam, ubar
+
+
+def _inflow_owners(g):
+    faces = np.concatenate((g.pressure_inlet_faces, g.mass_flow_inlet_faces, g.mass_flow_outlet_faces))
+    return np.unique(g.owners[faces])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("flux,viscous", [("HLLC", False), ("Rusanov", False), ("Roe", False), ("HLLC", True)])
+@pytest.mark.parametrize("reconstruction", ["First order", "MUSCL linear"])
+@pytest.mark.parametrize("kinds", CASES)
+def test_gpu_inflow_vjp_matches_central_difference_of_oracle(cuda_device, kinds, flux, reconstruction, viscous):
+    """lam^T (dR/dU) v of the device VJP against central differences of the
+    oracle residual (step 1e-7), for v over all cells and for v on the owner
+    cells of the inlet and outlet faces alone (where the inflow states'
+    derivatives are most of the product): 1e-6 of |fd|. The exterior states
+    are bisection roots (64 / 80 halvings, resolved to rounding), so the
+    differences see their implicit-function derivatives."""
+    case = channel(kinds[0], kinds[1], flux=flux, reconstruction=reconstruction, viscous=viscous)
+    print(f"case: inflow VJP {case.name} {flux} {reconstruction} viscous={viscous}", flush=True)
+    dm, u, lam, ubar = _vjp_setup(cuda_device, case)
+    o = Oracle(dm.geometry, case.numerics)
+    rng = np.random.default_rng(13)
+    scale = np.max(np.abs(u), axis=0)
+    owners = _inflow_owners(dm.geometry)
+    for where in ("all", "inflow owners"):
+        v = rng.normal(size=u.shape) * scale
+        if where != "all":
+            mask = np.zeros(len(u), dtype=bool)
+            mask[owners] = True
+            v[~mask] = 0.0
+        eps = 1e-7
+        fd = float(np.sum(lam * (o.residual(u + eps * v) - o.residual(u - eps * v)) / (2 * eps)))
+        ad = float(np.sum(ubar * v))
+        print(where, "fd", fd, "adjoint", ad)
+        assert abs(ad - fd) <= 1e-6 * max(abs(fd), 1.0), (where, fd, ad)
+    dm.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("viscous", [False, True])
+@pytest.mark.parametrize("kinds", CASES)
+def test_gpu_inflow_vjp_entries_of_the_boundary_cells(cuda_device, kinds, viscous):
+    """Every VJP entry of the inlet and outlet faces' owner cells equals a
+    central difference of the oracle residual (first order, HLLC; step 1e-6
+    of the entry: the mass-flow bisection's rounding makes differences at
+    1e-7 scatter by a few 1e-6), to 1e-6: a wrong term of one boundary
+    kind's chain fails here even where a random product would average it
+    out."""
+    case = channel(kinds[0], kinds[1], n=4, reconstruction="First order", viscous=viscous)
+    print(f"case: inflow VJP entries {case.name} viscous={viscous}", flush=True)
+    dm, u, lam, ubar = _vjp_setup(cuda_device, case, seed=7)
+    o = Oracle(dm.geometry, case.numerics)
+    for i in _inflow_owners(dm.geometry):
+        for k in range(4):
+            e = np.zeros_like(u)
+            e[i, k] = 1e-6 * max(abs(u[i, k]), 1.0)
+            fd = np.sum(lam * (o.residual(u + e) - o.residual(u - e))) / (2 * e[i, k])
+            assert abs(ubar[i, k] - fd) <= 1e-6 * max(abs(fd), 1.0), (i, k, ubar[i, k], fd)
+    dm.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("inlet", ["Pressure Inlet", "Mass Flow Inlet"])
+def test_gpu_inflow_steady_adjoint_matches_reconverged_differences(cuda_device, inlet):
+    """A channel with a pressure or mass-flow inlet and a pressure outlet
+    (HLLC, first order) under a body force a_x: d(outlet total-pressure
+    loss)/d(a_x) from one adjoint solve through the inflow VJP,
+    lambda . dR/d(a_x) with dR/d(a_x) = (0, rho, 0, rho u) per cell, equals
+    central differences of re-converged steady states (a_x = 0.05 +- 1e-3)
+    to 1e-5 of the derivative."""
+    import torch
+
+    from meshcfd_amd.device import DeviceMesh
+    from meshcfd_amd.steady_adjoint import OutletTotalPressureLoss, SteadyAdjoint, objective_gradient, solve_steady
+
+    def steady(ax, warm=None):
+        case = channel(inlet, "Pressure Outlet", reconstruction="First order", integrator="Implicit Euler")
+        case.numerics.time_step_mode = "Local (steady)"
+        case.numerics.cfl = 50.0
+        case.numerics.gmres_tolerance = 1e-3
+        case.numerics.acceleration_x = ax
+        dm = DeviceMesh(case, cuda_device, block_cells=64)
+        dm.set_state(euler.primitive_to_conservative(flow_state(case), 1.4) if warm is None else warm)
+        hist = solve_steady(dm, tol=1e-11, max_steps=400, raise_cfl=2.0)
+        print(f"  steady {inlet} a_x={ax}: {len(hist) - 1} steps, rms {hist[-1]:.3e}", flush=True)
+        assert hist[-1] <= 1e-11, hist[-5:]
+        loss = OutletTotalPressureLoss(dm, T0**3.5)
+        return dm, loss, float(loss(dm.u).item())
+
+    print(f"case: inflow steady adjoint {inlet}", flush=True)
+    ax, h = 0.05, 1e-3
+    dm, loss, j0 = steady(ax)
+    u_star = dm.state()
+    sa = SteadyAdjoint(dm, cfl=1e3, rtol=1e-4)
+    _, dj_du = objective_gradient(loss, dm.u)
+    lam, ahist = sa.solve(dj_du, tol=1e-11, max_iterations=80)
+    assert ahist[-1] <= 1e-11, ahist[-5:]
+    grad = float(torch.sum(lam[:, 1] * dm.u[:, 0] + lam[:, 3] * dm.u[:, 1]).item())
+    dm.close()
+    jp = steady(ax + h, u_star)
+    jm = steady(ax - h, u_star)
+    for d in (jp[0], jm[0]):
+        d.close()
+    fd = (jp[2] - jm[2]) / (2 * h)
+    print("adjoint", grad, "fd", fd, "rel", abs(grad - fd) / abs(fd))
+    assert abs(grad - fd) <= 1e-5 * abs(fd)
