@@ -162,9 +162,9 @@ class NumericalSettings:
     prandtl_number: float = 0.72
     acceleration_x: float = 0.0
     acceleration_y: float = 0.0
-    # carried by case files with the reference's defaults; adaptation and the
-    # SST model are outside this package (solve_case rejects a turbulence
-    # model other than "None" and ignores the adaptation settings)
+    # carried by case files with the reference's defaults; solve_case adapts
+    # the mesh with these settings (adaptation.py) and rejects a turbulence
+    # model other than "None" (the SST model is outside this package)
     adaptive_mesh: bool = False
     adaptation_interval: int = 20
     adaptation_max_level: int = 3
