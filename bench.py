@@ -230,9 +230,25 @@ def cylinder_bench(args) -> int:
     n0, g0 = st.newton_iterations, st.gmres_iterations
     torch.cuda.synchronize()
     start = time.perf_counter()
-    for k in range(args.steps):
-        t = dm.step_implicit(t)
-        note(f"step {k + 1}/{args.steps}: {time.perf_counter() - start:.1f}s")
+    history = []
+    if args.converge > 0:
+        # steps until the case's residual tolerance or the time limit
+        dm._implicit.on_newton = None
+        steps = 0
+        rms = r0
+        while rms > case.numerics.residual_tolerance and time.perf_counter() - start < args.converge:
+            t = dm.step_implicit(t)
+            steps += 1
+            rms = residual_rms(dm)
+            if steps % 20 == 0 or steps == 1:
+                history.append((steps, round(time.perf_counter() - start, 1), rms,
+                                st.newton_iterations - n0, st.gmres_iterations - g0))
+                note(f"step {steps}: rms {rms:.4e} at {time.perf_counter() - start:.1f}s")
+        args.steps = max(steps, 1)
+    else:
+        for k in range(args.steps):
+            t = dm.step_implicit(t)
+            note(f"step {k + 1}/{args.steps}: {time.perf_counter() - start:.1f}s")
     torch.cuda.synchronize()
     secs = time.perf_counter() - start
     r1 = residual_rms(dm)
@@ -255,6 +271,10 @@ def cylinder_bench(args) -> int:
         "residual_rms": [r0, r1],
         "setup_s": round(setup_s, 1),
     }
+    if history:
+        line["convergence"] = {"limit_s": args.converge, "tolerance": case.numerics.residual_tolerance,
+                               "history": [{"step": a, "s": b, "rms": c, "newton": d, "gmres": e}
+                                           for a, b, c, d, e in history]}
     print(json.dumps(line), flush=True)
     return 0
 
@@ -365,6 +385,8 @@ def main(argv=None) -> int:
     ap.add_argument("--graph-steps", type=int, default=10, help="batch: steps per captured HIP graph")
     ap.add_argument("--n-theta", type=int, default=2048, help="cylinder: cells around")
     ap.add_argument("--n-r", type=int, default=2048, help="cylinder: cells outward")
+    ap.add_argument("--converge", type=float, default=0.0, metavar="SECONDS",
+                    help="cylinder: step until the case's residual tolerance or this many seconds (0: --steps steps)")
     ap.add_argument("--lusgs-ordering", choices=("natural", "multicolour"), default="natural",
                     help="cylinder: LU-SGS ordering (natural: the reference's; multicolour: few parallel levels)")
     ap.add_argument("--no-autotune", action="store_true",
