@@ -284,6 +284,18 @@ int mcfd_mesh_destroy(McfdMesh* mesh);
  * No counterpart in the reference (its residual is whole-array NumPy passes). */
 int mcfd_mesh_set_blocks(McfdMesh* mesh, const McfdBlockDesc* blocks);
 
+/* First-order viscous stages on the fused path: per block LSQ entry (the
+ * order of McfdBlockDesc.gent, n_vent entries) the viscous WLSQ weights
+ * (wx, wy) as vent[2e], vent[2e+1]; per item of the viscous accumulation list
+ * of mcfd_mesh_set_viscous (n_vacc items) its face as the local face slot*4
+ * + mode of the block owning the item's cell. Needs mcfd_mesh_set_blocks and
+ * mcfd_mesh_set_viscous first (again after either is replaced). Then a
+ * first-order viscous residual or stage (`_residual` with viscous terms,
+ * solver.py:2437-2718, and the implicit path's first-order Jacobian
+ * products, :1532-1567) runs as one launch with the unfused kernels' bits. */
+int mcfd_mesh_set_block_viscous(McfdMesh* mesh, const double* vent, int64_t n_vent, const uint16_t* vacc_local,
+                                int64_t n_vacc);
+
 /* R = residual(U) (replaces `_residual`, solver.py:2437-2718). */
 int mcfd_residual(McfdMesh* mesh, const McfdSettings* settings, const double* u, double* r,
                   void* stream);

@@ -97,6 +97,9 @@ class BlockLayout:
     max_local: int
     max_faces: int
     max_grad_cells: int
+    # first-order viscous fused stages (build_blocks with viscous lists)
+    vent: np.ndarray | None = None  # float64 (n, 2) WLSQ weights (wx, wy), same order as gnb
+    vacc_local: np.ndarray | None = None  # uint16, per global viscous acc item: local face slot * 4 + mode
 
     def nbytes(self) -> int:
         return int(sum(v.nbytes for v in vars(self).values() if isinstance(v, np.ndarray)))
@@ -109,7 +112,12 @@ def _lookup(keys_sorted: np.ndarray, values: np.ndarray, query: np.ndarray) -> n
     return values[pos]
 
 
-def build_blocks(L: DeviceLayout, block_cells: int = 128) -> BlockLayout:
+def build_blocks(L: DeviceLayout, block_cells: int = 128, viscous: dict | None = None) -> BlockLayout:
+    """Blocks of at most ``block_cells`` cells. With ``viscous`` (the device
+    lists of `viscous.build_viscous`: wl_wx, wl_wy, va_ptr, va_item) also the
+    first-order viscous fused stage's arrays: the WLSQ weights of every block
+    LSQ entry (gent's order) and every viscous accumulation item as a local
+    face slot * 4 + mode of its cell's block."""
     n = L.n_cells
     B = block_cells
     if B < 1:
@@ -171,8 +179,8 @@ def build_blocks(L: DeviceLayout, block_cells: int = 128) -> BlockLayout:
     gnb = _lookup(keys, vals, g_b * n + g_nb)
     if gnb.max(initial=0) >= 0xFFFF:
         raise ValueError("block too large for 16-bit local ids")
-    gent = np.column_stack((L.grad_cx[np.repeat(gptr[gc_c], gcnt) + g_off],
-                            L.grad_cy[np.repeat(gptr[gc_c], gcnt) + g_off]))
+    g_idx = np.repeat(gptr[gc_c], gcnt) + g_off  # global LSQ entry of each block entry
+    gent = np.column_stack((L.grad_cx[g_idx], L.grad_cy[g_idx]))
     n_gcells = n_own + n_r1
     # per block n_gcells + 1 offsets (block relative) into the entry lists
     gstart_cells = np.concatenate(([0], np.cumsum(n_gcells + 1)[:-1]))
@@ -238,6 +246,14 @@ def build_blocks(L: DeviceLayout, block_cells: int = 128) -> BlockLayout:
     if dslot.max(initial=0) >= (1 << 15):
         raise ValueError("too many faces per block for 15-bit slots")
     dts_local = dslot | np.where(L.dts_own, 0, 1 << 15)
+    vent = vacc_local = None
+    if viscous is not None:
+        vent = np.column_stack((np.asarray(viscous["wl_wx"])[g_idx], np.asarray(viscous["wl_wy"])[g_idx]))
+        vptr = np.asarray(viscous["va_ptr"], dtype=np.int64)
+        vitem = np.asarray(viscous["va_item"], dtype=np.int64)
+        vcell = np.repeat(np.arange(n), np.diff(vptr))
+        vslot = _lookup(fkeys_s, slots_s, blk[vcell] * nf + (vitem >> 2))
+        vacc_local = ((vslot << 2) | (vitem & 3)).astype(np.uint16)
 
     info = np.zeros((nb, INFO_FIELDS), dtype=np.int64)
     info[:, 0] = cstart
@@ -273,6 +289,8 @@ def build_blocks(L: DeviceLayout, block_cells: int = 128) -> BlockLayout:
         frec=frec,
         acc_local=acc_local.astype(np.uint16),
         dts_local=dts_local.astype(np.uint16),
+        vent=None if vent is None else np.ascontiguousarray(vent),
+        vacc_local=vacc_local,
         max_local=int(n_loc.max()),
         max_faces=int(nfb.max()),
         max_grad_cells=int(n_gcells.max()),

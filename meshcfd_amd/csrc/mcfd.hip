@@ -1173,6 +1173,67 @@ __device__ __forceinline__ void visc_inlet(int code, const P4& w, const double* 
   out[2] = x.p / (x.r * v.R);
 }
 
+// Second half of one cell's viscous gradient (shared by k_visc_gradient and
+// the fused stage kernel): the viscosity and conductivity at the cell, the
+// ghost entries of its non-periodic boundary faces in list order, then the
+// 2x2 solve. out = {du/dx, du/dy, dv/dx, dv/dy, dT/dx, dT/dy, mu, k}.
+__device__ __forceinline__ void visc_gradient_finish(int i, const P4& w, const double val[3], double rx[3],
+                                                     double ry[3], const ViscArrays& va, const FaceArrays& fa,
+                                                     const ViscParams& vp, int* status, double out[8]) {
+  const double T = val[2];
+  double mu = visc_mu(T, vp, status, i);
+  double kc = visc_k(mu, vp);
+  for (int e = va.gh_ptr[i]; e < va.gh_ptr[i + 1]; ++e) {
+    int f = va.gh_face[e];
+    int b = fa.bc[f];
+    int code = va.bv_code[b];
+    double nx = fa.nx[f], ny = fa.ny[f];
+    double gh[3] = {val[0], val[1], val[2]};
+    if (code == kVisWall) {
+      int flags = va.bv_flags[b];
+      const double* bg = va.bv_geom + 4 * (long)b;
+      if (flags & 1) {
+        const double* wv = fa.bc_wall + 2 * (long)b;
+        gh[0] = 2.0 * wv[0] - gh[0];
+        gh[1] = 2.0 * wv[1] - gh[1];
+      } else {
+        double t = 2.0 * (gh[0] * nx + gh[1] * ny);
+        gh[0] = gh[0] - t * nx;
+        gh[1] = gh[1] - t * ny;
+      }
+      if (((flags >> 1) & 3) == 1)
+        gh[2] = 2.0 * bg[3] - val[2];
+      else
+        gh[2] = gh[2] + 2.0 * bg[0] * bg[3] / kc;
+    } else if (code == kVisSymmetry) {
+      double t = 2.0 * (val[0] * nx + val[1] * ny);
+      gh[0] = gh[0] - t * nx;
+      gh[1] = gh[1] - t * ny;
+    } else if (code == kVisInlet || code == kVisInflow || code == kVisFarField) {
+      double x[3];
+      visc_inlet(code, w, visc_inlet_state(code, b, va, fa), nx, ny, vp, x, status, i);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) gh[k] = 2.0 * x[k] - gh[k];
+    }
+    double wx = va.gh_wx[e], wy = va.gh_wy[e];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double d = gh[k] - val[k];
+      rx[k] = rx[k] + wx * d;
+      ry[k] = ry[k] + wy * d;
+    }
+  }
+  const double* m = va.wl_cell + 4 * (long)i;
+  double xx = m[0], xy = m[1], yy = m[2], inv = m[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    out[2 * k] = (inv != 0.0) ? (rx[k] * yy - ry[k] * xy) * inv : 0.0;
+    out[2 * k + 1] = (inv != 0.0) ? (ry[k] * xx - rx[k] * xy) * inv : 0.0;
+  }
+  out[6] = mu;
+  out[7] = kc;
+}
+
 // Weighted least-squares gradients of (u, v, T) with one ghost entry per
 // non-periodic boundary face (_viscous_gradients + the numba kernel
 // weighted_least_squares_gradients, _kernels.py:163-207), then the cell's
@@ -1218,58 +1279,8 @@ __global__ void __launch_bounds__(kBlock)
       }
     }
   }
-  double mu = visc_mu(T, vp, status, i);
-  double kc = visc_k(mu, vp);
-  for (int e = va.gh_ptr[i]; e < va.gh_ptr[i + 1]; ++e) {
-    int f = va.gh_face[e];
-    int b = fa.bc[f];
-    int code = va.bv_code[b];
-    double nx = fa.nx[f], ny = fa.ny[f];
-    double gh[3] = {val[0], val[1], val[2]};
-    if (code == kVisWall) {
-      int flags = va.bv_flags[b];
-      const double* bg = va.bv_geom + 4 * (long)b;
-      if (flags & 1) {
-        const double* wv = fa.bc_wall + 2 * (long)b;
-        gh[0] = 2.0 * wv[0] - gh[0];
-        gh[1] = 2.0 * wv[1] - gh[1];
-      } else {
-        double t = 2.0 * (gh[0] * nx + gh[1] * ny);
-        gh[0] = gh[0] - t * nx;
-        gh[1] = gh[1] - t * ny;
-      }
-      if (((flags >> 1) & 3) == 1)
-        gh[2] = 2.0 * bg[3] - val[2];
-      else
-        gh[2] = gh[2] + 2.0 * bg[0] * bg[3] / kc;
-    } else if (code == kVisSymmetry) {
-      double t = 2.0 * (val[0] * nx + val[1] * ny);
-      gh[0] = gh[0] - t * nx;
-      gh[1] = gh[1] - t * ny;
-    } else if (code == kVisInlet || code == kVisInflow || code == kVisFarField) {
-      double x[3];
-      visc_inlet(code, w, visc_inlet_state(code, b, va, fa), nx, ny, vp, x, status, i);
-#pragma unroll
-      for (int k = 0; k < 3; ++k) gh[k] = 2.0 * x[k] - gh[k];
-    }
-    double wx = va.gh_wx[e], wy = va.gh_wy[e];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      double d = gh[k] - val[k];
-      rx[k] = rx[k] + wx * d;
-      ry[k] = ry[k] + wy * d;
-    }
-  }
-  const double* m = va.wl_cell + 4 * (long)i;
-  double xx = m[0], xy = m[1], yy = m[2], inv = m[3];
   double out[8];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    out[2 * k] = (inv != 0.0) ? (rx[k] * yy - ry[k] * xy) * inv : 0.0;
-    out[2 * k + 1] = (inv != 0.0) ? (ry[k] * xx - rx[k] * xy) * inv : 0.0;
-  }
-  out[6] = mu;
-  out[7] = kc;
+  visc_gradient_finish(i, w, val, rx, ry, va, fa, vp, status, out);
   double2* q = reinterpret_cast<double2*>(VG + 8 * (long)i);
 #pragma unroll
   for (int k = 0; k < 4; ++k) q[k] = make_double2(out[2 * k], out[2 * k + 1]);
@@ -1292,38 +1303,17 @@ __device__ __forceinline__ void visc_flux(const double g[6], double u, double v,
   F[3] = u * tx + v * ty + q;
 }
 
-// One viscous flux per flux face: face averages on interior faces and
-// periodic pairs, the reference's boundary gradient corrections on boundary
-// faces (solver.py:3822-4112).
-__global__ void __launch_bounds__(kBlock)
-    k_visc_flux(int nf, FaceArrays fa, const double* __restrict__ W,
-                const double* __restrict__ VG, ViscArrays va, ViscParams vp,
-                double* __restrict__ VF, int* __restrict__ status) {
-  int f = xcd_block() * blockDim.x + threadIdx.x;
-  if (f >= nf) return;
-  int kind = fa.kind[f];
-  double nx = fa.nx[f], ny = fa.ny[f];
-  int o = fa.left[f];
-  const double* go = VG + 8 * (long)o;
-  double F[4];
-  if (kind == MCFD_FACE_INTERIOR || kind == MCFD_FACE_PERIODIC) {
-    int r = fa.right[f];
-    const double* gr = VG + 8 * (long)r;
-    double g[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) g[k] = 0.5 * (go[k] + gr[k]);
-    P4 a = load4(W, o), c = load4(W, r);
-    visc_flux(g, 0.5 * (a.u + c.u), 0.5 * (a.v + c.v), 0.5 * (go[6] + gr[6]),
-              0.5 * (go[7] + gr[7]), nx, ny, F);
-    store4(VF, f, F[0], F[1], F[2], F[3]);
-    return;
-  }
-  int b = fa.bc[f];
+// Viscous flux through a non-periodic boundary face of record b (shared by
+// k_visc_flux and the fused stage kernel): the owner's gradients go with the
+// reference's boundary corrections (solver.py:3822-4112), boundary values
+// from the owner state w, face offset (offx, offy) from the owner centroid.
+__device__ __forceinline__ void visc_boundary_flux(int b, double nx, double ny, double offx, double offy,
+                                                   const P4& w, const double* go, const ViscArrays& va,
+                                                   const FaceArrays& fa, const ViscParams& vp, int* status,
+                                                   int o, double F[4]) {
   int code = va.bv_code[b];
   const double* bg = va.bv_geom + 4 * (long)b;
   double dn = bg[0], tx = bg[1], ty = bg[2];
-  double offx = fa.lx[f], offy = fa.ly[f];
-  P4 w = load4(W, o);
   double T = w.p / (w.r * vp.R);
   double g[6];
 #pragma unroll
@@ -1412,6 +1402,35 @@ __global__ void __launch_bounds__(kBlock)
   double fmu = visc_mu(bv[2], vp, status, o);
   visc_flux(g, bv[0], bv[1], fmu, visc_k(fmu, vp), nx, ny, F);
   if (code == kVisWall && thermal == 2) F[3] = bg[3];
+}
+
+// One viscous flux per flux face: face averages on interior faces and
+// periodic pairs, the reference's boundary gradient corrections on boundary
+// faces (solver.py:3822-4112).
+__global__ void __launch_bounds__(kBlock)
+    k_visc_flux(int nf, FaceArrays fa, const double* __restrict__ W,
+                const double* __restrict__ VG, ViscArrays va, ViscParams vp,
+                double* __restrict__ VF, int* __restrict__ status) {
+  int f = xcd_block() * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  int kind = fa.kind[f];
+  double nx = fa.nx[f], ny = fa.ny[f];
+  int o = fa.left[f];
+  const double* go = VG + 8 * (long)o;
+  double F[4];
+  if (kind == MCFD_FACE_INTERIOR || kind == MCFD_FACE_PERIODIC) {
+    int r = fa.right[f];
+    const double* gr = VG + 8 * (long)r;
+    double g[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) g[k] = 0.5 * (go[k] + gr[k]);
+    P4 a = load4(W, o), c = load4(W, r);
+    visc_flux(g, 0.5 * (a.u + c.u), 0.5 * (a.v + c.v), 0.5 * (go[6] + gr[6]),
+              0.5 * (go[7] + gr[7]), nx, ny, F);
+    store4(VF, f, F[0], F[1], F[2], F[3]);
+    return;
+  }
+  visc_boundary_flux(fa.bc[f], nx, ny, fa.lx[f], fa.ly[f], load4(W, o), go, va, fa, vp, status, o, F);
   store4(VF, f, F[0], F[1], F[2], F[3]);
 }
 
@@ -1712,6 +1731,11 @@ struct BlockArrays {
   const FaceRec* frec;
   const unsigned short* acc_local;
   const unsigned short* dts_local;
+  // first-order viscous stages: WLSQ weights (wx, wy) of the block's LSQ
+  // entries (same order as gent) and, per viscous accumulation item of the
+  // global list, its face as a local slot*4 + mode of the owning cell's block
+  const double2* vent;
+  const unsigned short* vacc_local;
 };
 
 struct GeomArrays {
@@ -1771,12 +1795,17 @@ __device__ __forceinline__ void lds_copy(const void* src, void* dst, int n) {
 // Instantiated per flux scheme, reconstruction, far-field presence and
 // next-step dt so that each launch carries only the code it runs (the full
 // kernel is ~48 KB, most of a CU pair's instruction cache).
-template <int SCHEME, bool MUSCL, int LIM, bool FARFIELD, bool DTNEXT>
+// With VISC (first-order reconstruction only) P1 computes the viscous
+// gradients of own and ring-1 cells (k_visc_gradient's arithmetic, from LDS),
+// P2 adds each face's viscous flux (k_visc_flux's) and P3 the viscous gather
+// (k_update's), so a first-order viscous residual is one launch.
+template <int SCHEME, bool MUSCL, int LIM, bool FARFIELD, bool DTNEXT, bool VISC = false>
 __global__ void __launch_bounds__(kBlock)
     k_stage_fused(int max_acc, int stage_off, int max_local, int max_grad, int max_faces, int max_lsq, int max_lim,
                   BlockArrays ba,
                   GeomArrays ga, FaceArrays fa, DtArrays da, double g, double ax, double ay,
-                  StageArgs s, int* __restrict__ status) {
+                  StageArgs s, int* __restrict__ status, ViscArrays va, ViscParams vp, ViscAcc vacc) {
+  static_assert(!(VISC && (MUSCL || DTNEXT)), "viscous fused stages are first order without the next dt");
   extern __shared__ double smem[];
   constexpr bool muscl = MUSCL;
   const int b = xcd_block();
@@ -1806,6 +1835,7 @@ __global__ void __launch_bounds__(kBlock)
   double* sF = sG;
   double* sL = sF + 4 * max_faces;
   double* sN = sL + max_faces;  // face normals (final stage with the next step's dt)
+  double* sV = sL + max_faces;  // viscous face fluxes (VISC, which has no next-step dt)
   // own cells' accumulation coefficients and items, and spectral-radius
   // items, for P3: written after P2's last read of W and the gradients, over
   // W when they fit there (stage_off 0), else after the shared region
@@ -1819,9 +1849,11 @@ __global__ void __launch_bounds__(kBlock)
   // on gradient cell t in P1, face t in P2 and own cell t in P3 (blocks have
   // at most kBlock of each).
   int e0 = 0, e1 = 0, l0 = 0, l1 = 0;
-  if (MUSCL && t < n_grad) {
+  if ((MUSCL || VISC) && t < n_grad) {
     e0 = ba.gptr[gps + t];
     e1 = ba.gptr[gps + t + 1];
+  }
+  if (MUSCL && t < n_grad) {
     l0 = ba.lptr[gps + t];
     l1 = ba.lptr[gps + t + 1];
   }
@@ -1943,12 +1975,54 @@ __global__ void __launch_bounds__(kBlock)
     }
     __syncthreads();
   }
+  // P1 (viscous): WLSQ gradients of (u, v, T) of own and ring-1 cells
+  // (k_visc_gradient: face entries in LSQ order, then the ghost entries),
+  // viscosity and conductivity -> sG[8 j .. 8 j + 7]
+  if constexpr (VISC) {
+    if (t < n_grad) {
+      const int i = cl[t];
+      const P4 w = lds4(sW, t);
+      const double T = w.p / (w.r * vp.R);
+      if (t < n_own && (!finite(T) || !(T > 0.0))) report(status, MCFD_ERR_TEMPERATURE, i);
+      double val[3] = {w.u, w.v, T};
+      double rx[3] = {0.0, 0.0, 0.0}, ry[3] = {0.0, 0.0, 0.0};
+      const unsigned short* nbl = ba.gnb + gs;
+      const double2* went = ba.vent + gs;
+      for (int base = e0; base < e1; base += 4) {
+        int ob[4];
+        double2 wb[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const bool ok = base + q < e1;
+          ob[q] = ok ? (int)nbl[base + q] : t;
+          wb[q] = ok ? went[base + q] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (base + q >= e1) break;
+          const P4 o = lds4(sW, ob[q]);
+          double d[3] = {o.u - w.u, o.v - w.v, o.p / (o.r * vp.R) - T};
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            rx[k] = rx[k] + wb[q].x * d[k];
+            ry[k] = ry[k] + wb[q].y * d[k];
+          }
+        }
+      }
+      double out[8];
+      visc_gradient_finish(i, w, val, rx, ry, va, fa, vp, status, out);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) sG[8 * t + k] = out[k];
+    }
+    __syncthreads();
+  }
 
   // P2: face fluxes, one face per thread (a block touches at most kBlock
   // faces); the results wait in registers until every thread is done with
   // the gradient slots they overwrite.
   F4 fl{0.0, 0.0, 0.0, 0.0};
   double flen = 0.0, fnx = 0.0, fny = 0.0;
+  double vf[4] = {0.0, 0.0, 0.0, 0.0};  // viscous flux (VISC)
   if (f < n_faces) {
     if constexpr (!MUSCL) fr = ba.frec[fs + f];
     const int kind = fr.kind_bc & 15;
@@ -1996,6 +2070,22 @@ __global__ void __launch_bounds__(kBlock)
     }
     if (err) report(status, err, ba.faces[fs + f]);
     flen = fr.len;
+    if constexpr (VISC) {
+      // k_visc_flux: face averages on interior and periodic faces, the
+      // boundary corrections elsewhere
+      const double* go = sG + 8 * l;
+      if (kind <= MCFD_FACE_PERIODIC) {
+        const double* gr = sG + 8 * r;
+        double gq[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) gq[k] = 0.5 * (go[k] + gr[k]);
+        const P4 a = lds4(sW, l), c = lds4(sW, r);
+        visc_flux(gq, 0.5 * (a.u + c.u), 0.5 * (a.v + c.v), 0.5 * (go[6] + gr[6]), 0.5 * (go[7] + gr[7]), nx, ny,
+                  vf);
+      } else {
+        visc_boundary_flux(fr.kind_bc >> 4, nx, ny, fr.lx, fr.ly, lds4(sW, l), go, va, fa, vp, status, cl[l], vf);
+      }
+    }
   }
   // P3's per-cell loads, issued before the barrier. P3 runs two threads per
   // own cell: thread t handles components 2*(t&1) and 2*(t&1)+1 of own cell
@@ -2078,6 +2168,10 @@ __global__ void __launch_bounds__(kBlock)
       sN[2 * f] = fnx;
       sN[2 * f + 1] = fny;
     }
+    if constexpr (VISC) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) sV[4 * f + k] = vf[k];
+    }
   }
   __syncthreads();
 
@@ -2106,6 +2200,44 @@ __global__ void __launch_bounds__(kBlock)
         ra = ra + (F.x * L) / A;
         rb = rb + (F.y * L) / A;
       }
+    }
+    if constexpr (VISC) {
+      // k_update's viscous gather (visc_gather): 0 - S over the interior
+      // items, then the periodic and boundary items
+      double s0 = 0.0, s1 = 0.0, dv0 = 0.0, dv1 = 0.0;
+      bool interior = true;
+      const int v0 = vacc.ptr[i], v1 = vacc.ptr[i + 1];
+      for (int e = v0; e < v1; ++e) {
+        const int it = ba.vacc_local[e];
+        const int slot = it >> 2, mode = it & 3;
+        const double2 F = reinterpret_cast<const double2*>(sV + 4 * slot)[hf];
+        if (interior && mode == 0) {
+          const double c = vacc.coef[e];
+          s0 = s0 + c * F.x;
+          s1 = s1 + c * F.y;
+          continue;
+        }
+        if (interior) {
+          dv0 = 0.0 - s0;
+          dv1 = 0.0 - s1;
+          interior = false;
+        }
+        const double L = sL[slot];
+        const double x0 = F.x * L, x1 = F.y * L;
+        if (mode == 1) {
+          dv0 = dv0 + x0 / A;
+          dv1 = dv1 + x1 / A;
+        } else {
+          dv0 = dv0 + (-x0) / A;
+          dv1 = dv1 + (-x1) / A;
+        }
+      }
+      if (interior) {
+        dv0 = 0.0 - s0;
+        dv1 = 0.0 - s1;
+      }
+      ra = ra + dv0;
+      rb = rb + dv1;
     }
     if (ax != 0.0 || ay != 0.0) {
       if (hf == 0) {
@@ -4848,6 +4980,9 @@ struct McfdMesh {
   int max_acc = 0, max_dts = 0;  // most accumulation / spectral-radius items of a block's own cells
   int max_lsq = 0, max_lim = 0;  // most LSQ / limiter entries of a block's own + ring-1 cells
   double* block_partial = nullptr;  // [nb] per-block minima of the fused final stage
+  bool fused_visc = false;  // per-block viscous arrays uploaded (mcfd_mesh_set_block_viscous)
+  long block_n_gnb = 0;     // LSQ entries of the block layout
+  long n_vacc_visc = 0;     // viscous accumulation items (mcfd_mesh_set_viscous)
   BlockArrays blocks{};
   std::vector<void*> block_allocations;
   // implicit
@@ -4942,6 +5077,7 @@ void free_blocks(McfdMesh* m) {
   for (void* p : m->block_allocations) (void)hipFree(p);
   m->block_allocations.clear();
   m->fused = false;
+  m->fused_visc = false;
   m->blocks = BlockArrays{};
   m->block_partial = nullptr;
 }
@@ -4970,9 +5106,12 @@ size_t fused_entry_doubles(const McfdMesh* m) {
   return 2 * (size_t)m->max_lsq + 2 * (size_t)m->max_lim + ((size_t)m->max_lsq + 2 + 3) / 4;
 }
 
-size_t fused_region(const McfdMesh* m, bool muscl) {
+size_t fused_region(const McfdMesh* m, bool muscl, bool visc = false) {
   size_t region = 7 * (size_t)m->max_faces;
   if (muscl) region = std::max(region, std::max(12 * (size_t)m->max_grad, fused_entry_doubles(m)));
+  // viscous: the gradients, viscosity and conductivity of own + ring-1
+  // cells, then per face the convective flux, length and viscous flux
+  if (visc) region = std::max(9 * (size_t)m->max_faces, 8 * (size_t)m->max_grad);
   return std::max(region, (size_t)kBlock);
 }
 
@@ -4981,14 +5120,14 @@ size_t fused_stage_doubles(const McfdMesh* m) {
   return (size_t)m->max_acc + (shorts * sizeof(unsigned short) + 7) / 8;
 }
 
-int fused_stage_offset(const McfdMesh* m, bool muscl) {
+int fused_stage_offset(const McfdMesh* m, bool muscl, bool visc = false) {
   if (fused_stage_doubles(m) <= 4 * (size_t)m->max_local) return 0;
-  return (int)(4 * (size_t)m->max_local + fused_region(m, muscl));
+  return (int)(4 * (size_t)m->max_local + fused_region(m, muscl, visc));
 }
 
-size_t fused_lds_bytes(const McfdMesh* m, bool muscl) {
-  size_t doubles = 4 * (size_t)m->max_local + fused_region(m, muscl);
-  if (fused_stage_offset(m, muscl) != 0) doubles += fused_stage_doubles(m);
+size_t fused_lds_bytes(const McfdMesh* m, bool muscl, bool visc = false) {
+  size_t doubles = 4 * (size_t)m->max_local + fused_region(m, muscl, visc);
+  if (fused_stage_offset(m, muscl, visc) != 0) doubles += fused_stage_doubles(m);
   return doubles * sizeof(double);
 }
 
@@ -5051,6 +5190,7 @@ void free_viscous(McfdMesh* m) {
   for (void* p : m->viscous_allocations) (void)hipFree(p);
   m->viscous_allocations.clear();
   m->viscous = false;
+  m->fused_visc = false;  // the fused kernel reads the viscous lists
   m->visc = ViscArrays{};
   m->vacc = ViscAcc{};
   m->vdt = ViscDt{};
@@ -5171,7 +5311,7 @@ void launch_fused_t(McfdMesh* m, size_t lds, hipStream_t s, const GeomArrays& ga
                             hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024);
   hipLaunchKernelGGL(kern, dim3(m->nb), dim3(kBlock), lds, s, m->max_acc, fused_stage_offset(m, MUSCL), m->max_local,
                      m->max_grad, m->max_faces, m->max_lsq, m->max_lim, m->blocks, ga, m->faces, m->dts, st->gamma,
-                     st->acceleration_x, st->acceleration_y, a, m->status);
+                     st->acceleration_x, st->acceleration_y, a, m->status, ViscArrays{}, ViscParams{}, ViscAcc{});
 }
 
 template <int SCHEME, bool MUSCL, int LIM>
@@ -5195,6 +5335,33 @@ void launch_fused_s(McfdMesh* m, bool muscl, int lim, bool far, bool dtn, size_t
   if (lim <= 1) return launch_fused_l<SCHEME, true, 0>(m, far, dtn, lds, s, ga, st, a);
   if (lim == 2) return launch_fused_l<SCHEME, true, 2>(m, far, dtn, lds, s, ga, st, a);
   return launch_fused_l<SCHEME, true, 3>(m, far, dtn, lds, s, ga, st, a);
+}
+
+// First-order viscous stages (no next-step dt): one launch per stage.
+template <int SCHEME, bool FARFIELD>
+void launch_fused_visc_t(McfdMesh* m, size_t lds, hipStream_t s, const GeomArrays& ga, const McfdSettings* st,
+                         const StageArgs& a) {
+  auto kern = k_stage_fused<SCHEME, false, 0, FARFIELD, false, true>;
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024);
+  hipLaunchKernelGGL(kern, dim3(m->nb), dim3(kBlock), lds, s, m->max_acc, fused_stage_offset(m, false, true),
+                     m->max_local, m->max_grad, m->max_faces, m->max_lsq, m->max_lim, m->blocks, ga, m->faces,
+                     m->dts, st->gamma, st->acceleration_x, st->acceleration_y, a, m->status, m->visc,
+                     visc_params(st), m->vacc);
+}
+
+void launch_fused_visc(McfdMesh* m, int scheme, bool far, size_t lds, hipStream_t s, const GeomArrays& ga,
+                       const McfdSettings* st, const StageArgs& a) {
+  if (scheme == 1) {
+    if (far) launch_fused_visc_t<1, true>(m, lds, s, ga, st, a);
+    else launch_fused_visc_t<1, false>(m, lds, s, ga, st, a);
+  } else if (scheme == 2) {
+    if (far) launch_fused_visc_t<2, true>(m, lds, s, ga, st, a);
+    else launch_fused_visc_t<2, false>(m, lds, s, ga, st, a);
+  } else {
+    if (far) launch_fused_visc_t<0, true>(m, lds, s, ga, st, a);
+    else launch_fused_visc_t<0, false>(m, lds, s, ga, st, a);
+  }
 }
 
 void launch_fused(McfdMesh* m, int scheme, bool muscl, bool far, bool dtn, size_t lds,
@@ -6041,7 +6208,28 @@ int mcfd_mesh_set_blocks(McfdMesh* m, const McfdBlockDesc* d) {
     free_blocks(m);
     return fail(msg);
   }
+  m->block_n_gnb = d->n_gnb;
   m->fused = true;
+  return 0;
+}
+
+int mcfd_mesh_set_block_viscous(McfdMesh* m, const double* vent, int64_t n_vent, const uint16_t* vacc_local,
+                                int64_t n_vacc) {
+  if (!m) return fail("mcfd_mesh_set_block_viscous: NULL mesh");
+  if (!m->fused) return fail("mcfd_mesh_set_block_viscous: call mcfd_mesh_set_blocks first");
+  if (!m->viscous) return fail("mcfd_mesh_set_block_viscous: call mcfd_mesh_set_viscous first");
+  if (!vent || !vacc_local) return fail("mcfd_mesh_set_block_viscous: missing array");
+  if (n_vent != m->block_n_gnb || n_vacc != m->n_vacc_visc)
+    return fail("mcfd_mesh_set_block_viscous: array lengths do not match the blocks and viscous lists");
+  if (fused_lds_bytes(m, false, true) > 64 * 1024)
+    return fail("mcfd_mesh_set_block_viscous: blocks too large for the viscous fused kernel's LDS");
+  (void)hipDeviceSynchronize();
+  BlockArrays& b = m->blocks;
+  int rc = 0;
+  rc |= upload_block(m, reinterpret_cast<double**>(const_cast<double2**>(&b.vent)), vent, 2 * (size_t)n_vent);
+  rc |= upload_block(m, const_cast<unsigned short**>(&b.vacc_local), vacc_local, (size_t)n_vacc);
+  if (rc) return -1;
+  m->fused_visc = true;
   return 0;
 }
 
@@ -6084,6 +6272,20 @@ int mcfd_stage(McfdMesh* m, const McfdSettings* st, const McfdStage* sg, void* s
   if (check_viscous(m, st)) return -1;
   if (sg->dt_min && (st->viscosity || st->reconstruction == 2))
     return fail("mcfd_stage: the next step's dt needs the fused stage (inviscid, no MUSCL parabolic)");
+  if (m->fused && m->fused_visc && st->viscosity && st->reconstruction == 0 && !m->needs_inflow &&
+      !a.dt_partial) {
+    // first-order viscous residual or stage (the Jacobian products of the
+    // implicit path): one fused launch
+    size_t lds = fused_lds_bytes(m, false, true);
+    GeomArrays ga{m->area,   m->grad_ptr, m->grad_cx,  m->grad_cy, m->lim_ptr,
+                  m->lim_dx, m->lim_dy,   m->acc_ptr,  m->face_len, m->acc_coef};
+    {
+      TimedLaunch t(m, s, MCFD_KERNEL_STAGE);
+      launch_fused_visc(m, st->flux, m->has_far_field, lds, s, ga, st, a);
+    }
+    HIP_CHECK(hipGetLastError());
+    return 0;
+  }
   if (m->fused && st->reconstruction != 2 && !st->viscosity && !m->needs_inflow) {
     bool muscl = st->reconstruction == 1 && m->n_grad > 0;
     size_t lds = fused_lds_bytes(m, muscl);
@@ -6569,6 +6771,7 @@ int mcfd_mesh_set_viscous(McfdMesh* m, const McfdViscousDesc* d) {
   rc |= alloc_viscous(m, &m->vacc.ptr, d->va_ptr, C + 1);
   rc |= alloc_viscous(m, &m->vacc.item, d->va_item, (size_t)d->n_vacc);
   rc |= alloc_viscous(m, &m->vacc.coef, d->va_coef, (size_t)d->n_vacc);
+  m->n_vacc_visc = d->n_vacc;
   rc |= alloc_viscous(m, &m->vdt.ptr, d->vd_ptr, C + 1);
   rc |= alloc_viscous(m, &m->vdt.nb, d->vd_nb, (size_t)d->n_vdt);
   rc |= alloc_viscous(m, &m->vdt.len, d->vd_len, (size_t)d->n_vdt);

@@ -95,6 +95,13 @@ class DeviceMesh:
         self.handle = handle
         self.fuse_time_step = fuse_time_step
         self.blocks = None
+        # viscous lists first: the fused kernel's first-order viscous path
+        # packs their weights and items per block
+        self._visc_lists = None
+        if self.settings.viscosity_model != "Inviscid":
+            from . import viscous
+
+            self._visc_lists = viscous.upload(self.handle, self.geometry, self.layout)
         # pressure-inlet / mass-flow boundaries: exterior states from their own
         # kernels, read by the unfused flux and time-step kernels
         self.inflow = inflow.has_inflow(self.layout)
@@ -105,10 +112,6 @@ class DeviceMesh:
         self.native_settings = native_settings(self.settings)
         if self.settings.reconstruction == "MUSCL parabolic":
             self.set_parabolic()
-        if self.settings.viscosity_model != "Inviscid":
-            from . import viscous
-
-            viscous.upload(self.handle, self.geometry, self.layout)
         opts = dict(dtype=torch.float64, device=self.device)
         n = self.n
         self.u = torch.empty((n, 4), **opts)
@@ -203,10 +206,17 @@ class DeviceMesh:
             native.check(self.lib.mcfd_mesh_set_blocks(self.handle, None))
             self.blocks = None
             return
-        self.blocks = build_blocks(self.layout, block_cells)
+        self.blocks = build_blocks(self.layout, block_cells, viscous=self._visc_lists)
         desc, keep = native.block_desc(self.blocks)
         native.check(self.lib.mcfd_mesh_set_blocks(self.handle, C.byref(desc)))
         del keep
+        if self.blocks.vent is not None:
+            # first-order viscous residuals (the implicit path's Jacobian
+            # products) as one fused launch
+            vent = np.ascontiguousarray(self.blocks.vent, dtype=np.float64)
+            vloc = np.ascontiguousarray(self.blocks.vacc_local, dtype=np.uint16)
+            native.check(self.lib.mcfd_mesh_set_block_viscous(
+                self.handle, C.c_void_p(vent.ctypes.data), len(vent), C.c_void_p(vloc.ctypes.data), len(vloc)))
 
     def autotune(self, candidates=(64, 96, 112, 128), steps: int = 3, repeats: int = 2) -> int:
         """Pick the fused kernel's block size: for each candidate (cells per

@@ -188,6 +188,7 @@ EXPORTS = {
     "mcfd_mesh_create": (C.c_int, [C.POINTER(McfdMeshDesc), C.POINTER(C.c_void_p)]),
     "mcfd_mesh_destroy": (C.c_int, [C.c_void_p]),
     "mcfd_mesh_set_blocks": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mcfd_mesh_set_block_viscous": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64]),
     "mcfd_residual": (C.c_int, [C.c_void_p, C.POINTER(McfdSettings), C.c_void_p, C.c_void_p, C.c_void_p]),
     "mcfd_time_steps": (
         C.c_int,

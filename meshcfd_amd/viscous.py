@@ -224,9 +224,13 @@ def viscous_desc(v: dict) -> tuple[McfdViscousDesc, list]:
     return desc, keep
 
 
-def upload(handle, g: Geometry, L: DeviceLayout) -> None:
-    """Build the viscous lists of `g` and upload them to the mesh `handle`."""
+def upload(handle, g: Geometry, L: DeviceLayout) -> dict:
+    """Build the viscous lists of `g`, upload them to the mesh `handle` and
+    return what the fused kernel's block packing needs (the WLSQ weights of
+    the LSQ entries and the accumulation lists)."""
     lib = native.load()
-    desc, keep = viscous_desc(build_viscous(g, L))
+    lists = build_viscous(g, L)
+    desc, keep = viscous_desc(lists)
     native.check(lib.mcfd_mesh_set_viscous(handle, C.byref(desc)))
     del keep
+    return {k: lists[k] for k in ("wl_wx", "wl_wy", "va_ptr", "va_item")}

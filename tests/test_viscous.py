@@ -507,3 +507,56 @@ def test_gpu_viscous_boundary_state_adjoint_matches_fd(cuda_device, kind, recons
     print(kind, reconstruction, "fd", fd, "adjoint", ad)
     assert abs(ad - fd) <= 1e-6 * max(abs(fd), 1.0)
     dm.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("flux", ["Rusanov", "HLLC", "Roe"])
+@pytest.mark.parametrize("kind", CASES)
+def test_gpu_fused_first_order_viscous_stage_equals_four_kernels(cuda_device, kind, flux):
+    """A first-order viscous residual on the fused path (one launch: WLSQ
+    gradients of own and ring-1 cells, convective and viscous face fluxes and
+    both gathers out of LDS) has the four-kernel path's bits, and so do three
+    explicit RK3 steps."""
+    import torch
+
+    case = viscous_case(kind, reconstruction="First order", flux=flux)
+    a = _device(case, cuda_device, fused=True)
+    b = _device(case, cuda_device, fused=False)
+    assert a.blocks is not None and a.blocks.vent is not None
+    prim = viscous_primitive(case)
+    u = euler.primitive_to_conservative(prim, case.numerics.gamma)
+    ra = a.residual(a.to_device_order(u)).clone()
+    rb = b.residual(b.to_device_order(u)).clone()
+    a.check_status()
+    b.check_status()
+    assert torch.equal(ra, rb), float((ra - rb).abs().max())
+    for dm in (a, b):
+        dm.set_state(u)
+        dm.run(3)
+    assert np.array_equal(a.state(), b.state())
+    a.close()
+    b.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["channel", "farfield", "sutherland"])
+def test_gpu_fused_viscous_jacobian_products_give_the_same_implicit_step(cuda_device, kind):
+    """Implicit Euler on a viscous case: the Newton loop's Jacobian products
+    (first-order viscous residuals) run fused on a mesh with blocks and as
+    four kernels without; the implicit step is the same to the last bit."""
+    import torch
+
+    case = viscous_case(kind, integrator="Implicit Euler", flux="Roe" if kind == "sutherland" else "HLLC")
+    case.numerics.time_step_mode = "Local (steady)"
+    case.numerics.cfl = 20.0
+    prim = viscous_primitive(case)
+    u = euler.primitive_to_conservative(prim, case.numerics.gamma)
+    out = []
+    for fused in (True, False):
+        dm = _device(case, cuda_device, fused=fused)
+        dm.set_state(u)
+        dm.step_implicit(0.0)
+        dm.check_status()
+        out.append(dm.u.clone())
+        dm.close()
+    assert torch.equal(out[0], out[1]), float((out[0] - out[1]).abs().max())
