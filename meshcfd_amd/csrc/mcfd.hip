@@ -683,7 +683,18 @@ struct StageArgs {
   double* dt_next;                 // per-cell stable step of the updated state, or NULL
   double* dt_partial;              // per-block minimum of the next step's dt (fused final stage), or NULL
   double cfl;
+  // kOpJacobian (fused stage only): u = state, u1 = direction v, u0 = rate
+  // R1(state), out = v - dt (R1(state + eps sigma v) - rate) / (eps sigma)
+  // with eps = jv_num / ||v|| (||v|| = jv_norm[0] on the device; 0: out = v)
+  const double* jv_norm;
+  double jv_num;
+  double jv_sigma[4];
 };
+
+// Internal stage op of the fused kernel (not an McfdStage op): the
+// implicit path's Jacobian product in one launch (k_perturb, the first-order
+// residual and k_jv_finish together, same operations).
+constexpr int kOpJacobian = 100;
 
 // Arrays of the spectral-radius sum (solver.py:4132-4343, inviscid part).
 struct DtArrays {
@@ -1873,6 +1884,14 @@ __global__ void __launch_bounds__(kBlock)
   for (int j = t; j < n_loc; j += kBlock) {
     int c = (j == t) ? c_first : cl[j];
     P4 q = load4(U, c);
+    if (s.op == kOpJacobian) {
+      // perturbed = s + (eps*sigma_k) * v   (k_perturb, solver.py:1553-1556)
+      const P4 dv = load4(s.u1, c);
+      const double dn = s.jv_norm[0];
+      const double eps = (dn == 0.0) ? 0.0 : s.jv_num / dn;
+      q = P4{q.r + eps * s.jv_sigma[0] * dv.r, q.u + eps * s.jv_sigma[1] * dv.u, q.v + eps * s.jv_sigma[2] * dv.v,
+             q.p + eps * s.jv_sigma[3] * dv.p};
+    }
     double rho = q.r;
     double u = q.u / rho;
     double v = q.v / rho;
@@ -2105,7 +2124,8 @@ __global__ void __launch_bounds__(kBlock)
     }
     if (s.op != MCFD_OP_RESIDUAL) {
       h = s.dt_per_cell ? s.dt[i] : s.dt[0];
-      u2 = reinterpret_cast<const double2*>(U + 4 * (long)i)[hf];
+      // the Jacobian op: u2 = direction v, z2 = rate
+      u2 = reinterpret_cast<const double2*>((s.op == kOpJacobian ? s.u1 : U) + 4 * (long)i)[hf];
       if (s.op != MCFD_OP_EULER) z2 = reinterpret_cast<const double2*>(s.u0 + 4 * (long)i)[hf];
     }
   }
@@ -2261,6 +2281,17 @@ __global__ void __launch_bounds__(kBlock)
       double k = s.c2 * h;
       oa = s.c0 * z2.x + s.c1 * u2.x + k * ra;
       ob = s.c0 * z2.y + s.c1 * u2.y + k * rb;
+    } else if (s.op == kOpJacobian) {
+      // out = v - (h * ((rp - r) / eps)) / sigma   (k_jv_finish, solver.py:1557-1563)
+      const double dn = s.jv_norm[0];
+      if (dn == 0.0) {
+        oa = u2.x;
+        ob = u2.y;
+      } else {
+        const double eps = s.jv_num / dn;
+        oa = u2.x - h * ((ra - z2.x) / eps) / s.jv_sigma[2 * hf];
+        ob = u2.y - h * ((rb - z2.y) / eps) / s.jv_sigma[2 * hf + 1];
+      }
     } else if (s.op == MCFD_OP_RK4_FINAL) {
       double2 y = reinterpret_cast<const double2*>(s.u1 + 4 * (long)i)[hf];
       double2 q = reinterpret_cast<const double2*>(s.r1 + 4 * (long)i)[hf];
@@ -5671,6 +5702,31 @@ int jacobian_apply(JacobianOp& op, const double* v, double* out, hipStream_t s) 
                      (double*)nullptr);
   const double fd = 1.4901161193847656e-08;  // sqrt(2^-52)
   const double num = fd * (op.base > 1.0 ? op.base : 1.0);
+  const McfdSettings* st = op.st;
+  if (m->fused && !m->needs_inflow && st->reconstruction == 0 && (!st->viscosity || m->fused_visc)) {
+    // perturbation, first-order residual and finish in one fused launch
+    StageArgs a{};
+    a.op = kOpJacobian;
+    a.dt_per_cell = op.per_cell;
+    a.u = op.state;
+    a.u0 = op.rate;
+    a.u1 = v;
+    a.dt = op.dt;
+    a.out = out;
+    a.cfl = st->cfl;
+    a.jv_norm = dn;
+    a.jv_num = num;
+    for (int k = 0; k < 4; ++k) a.jv_sigma[k] = op.sigma[k];
+    if (check_viscous(m, st)) return -1;
+    GeomArrays ga{m->area,   m->grad_ptr, m->grad_cx,  m->grad_cy, m->lim_ptr,
+                  m->lim_dx, m->lim_dy,   m->acc_ptr,  m->face_len, m->acc_coef};
+    if (st->viscosity)
+      launch_fused_visc(m, st->flux, m->has_far_field, fused_lds_bytes(m, false, true), s, ga, st, a);
+    else
+      launch_fused(m, st->flux, false, m->has_far_field, false, fused_lds_bytes(m, false), s, ga, st, a);
+    HIP_CHECK(hipGetLastError());
+    return 0;
+  }
   int nb = blocks(m->n_cells);
   hipLaunchKernelGGL(k_perturb, dim3(nb), dim3(kBlock), 0, s, m->n_cells, op.state, v, num, dn,
                      op.sigma[0], op.sigma[1], op.sigma[2], op.sigma[3], m->gm_pert);
