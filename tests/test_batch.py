@@ -90,9 +90,15 @@ def test_graph_replay_equals_eager_steps(cuda_device):
 
 
 @pytest.mark.gpu
-def test_per_instance_strength_gradients(cuda_device):
+@pytest.mark.parametrize("reconstruction,tol", [("MUSCL linear", 1e-4), ("First order", 1e-6)])
+def test_per_instance_strength_gradients(cuda_device, reconstruction, tol):
     """One backward pass through the batched mesh gives d KE_k / d beta_k for
-    every vortex instance; each matches a central difference."""
+    every vortex instance; each matches a central difference (step 1e-4).
+    The rollout takes a fixed step (0.02): the gradient is that of the
+    frozen-dt rollout, and the differences use the same step. First order
+    is smooth, so the differences agree to 1e-6; with the Barth-Jespersen
+    limiter the rollout is only piecewise smooth and differences of step
+    1e-4 agree to 1e-4."""
     import torch
 
     from meshcfd_amd.adjoint import kinetic_energy, rk3_rollout, vortex_state
@@ -100,6 +106,8 @@ def test_per_instance_strength_gradients(cuda_device):
 
     specs = [s for s in mb.batch_specs(8, seed=2, cells=(200, 800)) if s.kind == "vortex"][:3]
     b = mb.build_batch(specs)
+    b.numerics.reconstruction = reconstruction
+    print(f"case: batch strength gradients {reconstruction}", flush=True)
     dm = DeviceMesh.from_batch(b, cuda_device, block_cells=64)
     cen = dm.to_device_order(b.geometry.centers)
     area = dm.to_device_order(b.geometry.areas)
@@ -131,8 +139,8 @@ def test_per_instance_strength_gradients(cuda_device):
             d = torch.zeros(len(specs), dtype=torch.float64, device=cuda_device)
             d[k] = 1e-4
             fd = float((energies(5.0 + d)[k] - energies(5.0 - d)[k]) / 2e-4)
-            print(k, grad[k], fd)
-            assert abs(grad[k] - fd) <= 1e-4 * abs(fd)
+            print(k, grad[k], fd, abs(grad[k] - fd) / abs(fd))
+            assert abs(grad[k] - fd) <= tol * abs(fd)
     dm.close()
 
 

@@ -235,7 +235,7 @@ def _sutherland_cylinder(n_theta=32, n_r=16, mach=0.2, alpha=5.0, reconstruction
     return case, prim
 
 
-def _steady_force(mach, alpha, warm=None, lift=False, laminar=False, sutherland=False):
+def _steady_force(mach, alpha, warm=None, lift=False, laminar=False, sutherland=False, tighten=False):
     from meshcfd_amd.device import DeviceMesh
     from meshcfd_amd.steady_adjoint import WallForce, solve_steady
 
@@ -256,6 +256,11 @@ def _steady_force(mach, alpha, warm=None, lift=False, laminar=False, sutherland=
     # residual near 1e-10 at CFL 200
     hist = solve_steady(dm, tol=2e-10, max_steps=300, progress=progress)
     assert hist[-1] <= 2e-10, hist[-5:]
+    if tighten:
+        # then raise the CFL number whenever a step stalls, so the last steps
+        # are Newton steps on R = 0 (solve_steady's raise_cfl): the laminar
+        # case converges to ~1e-14
+        hist = hist + solve_steady(dm, tol=1e-14, max_steps=40, raise_cfl=2.0, max_cfl=1e6, progress=progress)
     force = WallForce(dm, alpha, lift=lift)
     return dm, force, float(force(dm.u).item()), hist
 
@@ -265,8 +270,9 @@ def _steady_force(mach, alpha, warm=None, lift=False, laminar=False, sutherland=
 def test_gpu_wall_force_inflow_gradients_match_fd(cuda_device, lift):
     """Inviscid cylinder, HLLC first order, M 0.3, alpha 5 deg, 32x16 O-grid:
     d(force)/dM and d(force)/d(alpha) from one adjoint solve equal central
-    differences of re-converged steady states (step 1e-3; steady residual
-    2e-10): drag to 2e-5, lift to 5e-4 of the larger derivative."""
+    differences of re-converged steady states (step 5e-4; steady residual
+    2e-10): drag to 1e-6 of the larger derivative (measured 4.3e-7 and
+    1.9e-8), lift to 5e-4."""
     from meshcfd_amd.steady_adjoint import SteadyAdjoint, free_stream_derivatives, objective_gradient
 
     print(f"case: cylinder wall force gradients lift={lift}", flush=True)
@@ -285,7 +291,7 @@ def test_gpu_wall_force_inflow_gradients_match_fd(cuda_device, lift):
     g_alpha = float(sbar @ d_alpha) + force.d_alpha(dm.u)
     print("adjoint", g_mach, g_alpha, "iterations", len(ahist), "gmres", sa.gmres_iterations)
     dm.close()
-    h = 1e-3
+    h = 5e-4
     fd = []
     for dmach, dalpha in ((h, 0.0), (-h, 0.0), (0.0, math.degrees(h)), (0.0, -math.degrees(h))):
         d2, _, j, _ = _steady_force(mach + dmach, alpha + dalpha, warm=u_star, lift=lift)
@@ -294,13 +300,15 @@ def test_gpu_wall_force_inflow_gradients_match_fd(cuda_device, lift):
     fd_mach = (fd[0] - fd[1]) / (2 * h)
     fd_alpha = (fd[2] - fd[3]) / (2 * h)
     print("fd", fd_mach, fd_alpha)
-    # The FD error is absolute (steady residual floor / step), so both
-    # derivatives are held to a fraction of the larger one: 2e-5 for drag.
-    # Lift measures the circulation, the one steady mode inviscid flow
-    # damps only through numerical dissipation; the residual floor leaves it
-    # least converged, so its differences are good to 5e-4 only.
+    # The FD error is absolute (steady residual floor / step, plus the
+    # step's truncation), so both derivatives are held to a fraction of the
+    # larger one: 1e-6 for drag. Lift measures the circulation, the one
+    # steady mode inviscid flow damps only through numerical dissipation; the
+    # residual floor leaves it least converged (and Newton steps at a raised
+    # CFL do not converge with HLLC's branch switches), so its differences
+    # are good to 5e-4 only (1.2e-4 at step 1e-3, 1.3e-3 at 1e-4).
     scale = max(abs(fd_mach), abs(fd_alpha))
-    frac = 5e-4 if lift else 2e-5
+    frac = 5e-4 if lift else 1e-6
     assert abs(g_mach - fd_mach) <= frac * scale
     assert abs(g_alpha - fd_alpha) <= frac * scale
 
@@ -311,39 +319,41 @@ def test_gpu_laminar_drag_gradients_match_fd(cuda_device):
     order): the drag includes the viscous wall traction; d(drag)/dM and
     d(drag)/d(alpha) from one adjoint solve (viscous reverse kernels in the
     adjoint operator) equal central differences of re-converged steady
-    states (step 1e-3) to 2e-5 of the larger derivative."""
+    states to 1e-7 of the larger derivative (measured 5e-9 and 1.6e-9): the
+    states are converged to ~1e-14 by Newton steps at a raised CFL, so the
+    differences can take a step of 1e-4."""
     from meshcfd_amd.steady_adjoint import SteadyAdjoint, free_stream_derivatives, objective_gradient
 
     print("case: laminar cylinder drag gradients", flush=True)
     mach, alpha = 0.3, 5.0
-    dm, force, j0, hist = _steady_force(mach, alpha, laminar=True)
+    dm, force, j0, hist = _steady_force(mach, alpha, laminar=True, tighten=True)
     fp, fv = force.components(dm.u)
     print("steady", len(hist), "steps, rms", hist[-1], "drag", j0, "pressure", fp.tolist(), "viscous", fv.tolist())
     assert abs(float(fv[0])) > 0.1 * abs(j0)
     u_star = dm.state()
     sa = SteadyAdjoint(dm, cfl=1e3, rtol=1e-4)
     _, dj_du = objective_gradient(force, dm.u)
-    lam, ahist = sa.solve(dj_du, tol=1e-11, max_iterations=80,
+    lam, ahist = sa.solve(dj_du, tol=1e-12, max_iterations=100,
                           progress=lambda k, rel: print(f"  adjoint iteration {k}: {rel:.3e}", flush=True))
-    assert ahist[-1] <= 1e-11, ahist[-5:]
+    assert ahist[-1] <= 1e-12, ahist[-5:]
     sbar = sa.inflow_gradient(lam)
     d_mach, d_alpha = free_stream_derivatives(mach, alpha, 1.4, 1.0, 1.0 / 1.4)
     g_mach = float(sbar @ d_mach)
     g_alpha = float(sbar @ d_alpha) + force.d_alpha(dm.u)
     print("adjoint", g_mach, g_alpha, "iterations", len(ahist), "gmres", sa.gmres_iterations)
     dm.close()
-    h = 1e-3
+    h = 1e-4
     fd = []
     for dmach, dalpha in ((h, 0.0), (-h, 0.0), (0.0, math.degrees(h)), (0.0, -math.degrees(h))):
-        d2, _, j, _ = _steady_force(mach + dmach, alpha + dalpha, warm=u_star, laminar=True)
+        d2, _, j, _ = _steady_force(mach + dmach, alpha + dalpha, warm=u_star, laminar=True, tighten=True)
         d2.close()
         fd.append(j)
     fd_mach = (fd[0] - fd[1]) / (2 * h)
     fd_alpha = (fd[2] - fd[3]) / (2 * h)
-    print("fd", fd_mach, fd_alpha)
     scale = max(abs(fd_mach), abs(fd_alpha))
-    assert abs(g_mach - fd_mach) <= 2e-5 * scale
-    assert abs(g_alpha - fd_alpha) <= 2e-5 * scale
+    print("fd", fd_mach, fd_alpha, "rel", abs(g_mach - fd_mach) / scale, abs(g_alpha - fd_alpha) / scale)
+    assert abs(g_mach - fd_mach) <= 1e-7 * scale
+    assert abs(g_alpha - fd_alpha) <= 1e-7 * scale
 
 
 @pytest.mark.gpu
@@ -352,8 +362,10 @@ def test_gpu_roe_sutherland_drag_gradients_match_fd(cuda_device):
     viscosity, Pr 0.72, M 0.2, Re 40, alpha 5 deg; 32x16 O-grid, first-order
     reconstruction): d(drag)/dM and d(drag)/d(alpha) from one adjoint solve
     (Roe and Sutherland reverse kernels) equal central differences of
-    re-converged steady states (step 1e-3) to 5e-6 of the larger derivative
-    (measured: 1.0e-6 for d/dM, 2e-8 for d/d(alpha))."""
+    re-converged steady states (step 5e-4) to 1e-6 of the larger derivative
+    (measured: 4.5e-7 for d/dM, 2.9e-8 for d/d(alpha); the Roe flux's
+    entropy fix keeps Newton steps at a raised CFL from converging, so the
+    states stay at the 2e-10 floor)."""
     from meshcfd_amd.steady_adjoint import SteadyAdjoint, free_stream_derivatives, objective_gradient
 
     print("case: Roe + Sutherland cylinder drag gradients, M 0.2 Re 40", flush=True)
@@ -374,7 +386,7 @@ def test_gpu_roe_sutherland_drag_gradients_match_fd(cuda_device):
     g_alpha = float(sbar @ d_alpha) + force.d_alpha(dm.u)
     print("adjoint", g_mach, g_alpha, "iterations", len(ahist), "gmres", sa.gmres_iterations)
     dm.close()
-    h = 1e-3
+    h = 5e-4
     fd = []
     for dmach, dalpha in ((h, 0.0), (-h, 0.0), (0.0, math.degrees(h)), (0.0, -math.degrees(h))):
         d2, _, j, _ = _steady_force(mach + dmach, alpha + dalpha, warm=u_star, sutherland=True)
@@ -385,5 +397,5 @@ def test_gpu_roe_sutherland_drag_gradients_match_fd(cuda_device):
     print("fd", fd_mach, fd_alpha, "rel", abs(g_mach - fd_mach) / max(abs(fd_mach), abs(fd_alpha)),
           abs(g_alpha - fd_alpha) / max(abs(fd_mach), abs(fd_alpha)))
     scale = max(abs(fd_mach), abs(fd_alpha))
-    assert abs(g_mach - fd_mach) <= 5e-6 * scale
-    assert abs(g_alpha - fd_alpha) <= 5e-6 * scale
+    assert abs(g_mach - fd_mach) <= 1e-6 * scale
+    assert abs(g_alpha - fd_alpha) <= 1e-6 * scale
