@@ -2495,9 +2495,20 @@ __device__ __forceinline__ void lusgs_fetch(const LusgsArrays& la, int k, int K,
   }
 }
 
-template <bool FORWARD>
-__device__ __forceinline__ void lusgs_finish(const LusgsArrays& la, const LusgsSlot& sl, int K,
-                                             const double* src, double* dst, const int* te) {
+// y_j = v_j / d_j: the value a cell of the first forward level (no lower
+// items) stores, formed where it is read by the two-level sweep
+__device__ __forceinline__ P4 lusgs_first_level(const LusgsArrays& la, const double* v, int j) {
+  const P4 o = load4(v, j);
+  const double d = la.diag[j];
+  return P4{o.r / d, o.u / d, o.v / d, o.p / d};
+}
+
+// One position of a sweep: (own - sum c * src[column]) / d forward,
+// (max(d, 1e-14) own - sum c * src[column]) / d backward. ONFLY: the column
+// values are the first forward level's, formed from src = v.
+template <bool FORWARD, bool ONFLY = false>
+__device__ __forceinline__ P4 lusgs_solve(const LusgsArrays& la, const LusgsSlot& sl, int K, const double* src,
+                                          const int* te) {
   P4 s;
   if (FORWARD) {
     s = sl.v;
@@ -2510,7 +2521,7 @@ __device__ __forceinline__ void lusgs_finish(const LusgsArrays& la, const LusgsS
   for (int q = 0; q < kLusgsK; ++q) {
     if (q >= lim || q >= sl.cnt) break;
     double c = sl.c[q];
-    P4 o = load4(src, sl.nb[q]);
+    P4 o = ONFLY ? lusgs_first_level(la, src, sl.nb[q]) : load4(src, sl.nb[q]);
     s.r = s.r - c * o.r;
     s.u = s.u - c * o.u;
     s.v = s.v - c * o.v;
@@ -2522,7 +2533,7 @@ __device__ __forceinline__ void lusgs_finish(const LusgsArrays& la, const LusgsS
     const int e1 = FORWARD ? la.off_split[i] : la.off_ptr[i + 1];
     for (int e = e0; e < e1; ++e) {
       double c = te ? la.off_val[te[e]] : la.off_val[e];
-      P4 o = load4(src, la.off_nb[e]);
+      P4 o = ONFLY ? lusgs_first_level(la, src, la.off_nb[e]) : load4(src, la.off_nb[e]);
       s.r = s.r - c * o.r;
       s.u = s.u - c * o.u;
       s.v = s.v - c * o.v;
@@ -2530,7 +2541,14 @@ __device__ __forceinline__ void lusgs_finish(const LusgsArrays& la, const LusgsS
     }
   }
   const double d = sl.d;
-  store4(dst, sl.cell, s.r / d, s.u / d, s.v / d, s.p / d);
+  return P4{s.r / d, s.u / d, s.v / d, s.p / d};
+}
+
+template <bool FORWARD>
+__device__ __forceinline__ void lusgs_finish(const LusgsArrays& la, const LusgsSlot& sl, int K,
+                                             const double* src, double* dst, const int* te) {
+  const P4 o = lusgs_solve<FORWARD>(la, sl, K, src, te);
+  store4(dst, sl.cell, o.r, o.u, o.v, o.p);
 }
 
 // Forward: own = v, src = dst = y. Backward: own = src = dst = x (in place).
@@ -2598,6 +2616,38 @@ __global__ void __launch_bounds__(kBlock)
   lusgs_finish<FORWARD>(la, sl, K, y, y, te);
 }
 
+// Sweeps of two levels whose forward levels are the backward levels swapped
+// (a 2-colouring): two launches instead of four, same bits. The first runs
+// the forward's second level, reading its lower neighbours' first-level
+// values as v / d, and the backward's first level on the result (no upper
+// items: x = (max(d, 1e-14) y) / d); the second runs the backward's second
+// level with its own first-level value v / d.
+__global__ void __launch_bounds__(kBlock)
+    k_lusgs_two_first(LusgsArrays la, const double* __restrict__ v, double* x, const int* __restrict__ te, int a,
+                      int b) {
+  const int k = a + blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= b) return;
+  const double* pval = te ? la.pval_ft : la.pval_f;
+  LusgsSlot sl;
+  lusgs_fetch<true>(la, k, la.kf, la.lvl_cells_f, la.pcnt_f, la.pnb_f, pval, la.pdiag_f, v, sl);
+  const P4 y = lusgs_solve<true, true>(la, sl, la.kf, v, te);
+  const double d = sl.d, dm = vmax(d, 1e-14);
+  store4(x, sl.cell, (dm * y.r) / d, (dm * y.u) / d, (dm * y.v) / d, (dm * y.p) / d);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_lusgs_two_second(LusgsArrays la, const double* __restrict__ v, double* x, const int* __restrict__ te, int a,
+                       int b) {
+  const int k = a + blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= b) return;
+  const double* pval = te ? la.pval_bt : la.pval_b;
+  LusgsSlot sl;
+  lusgs_fetch<false>(la, k, la.kb, la.lvl_cells_b, la.pcnt_b, la.pnb_b, pval, la.pdiag_b, v, sl);
+  const double d = sl.d;
+  sl.v = P4{sl.v.r / d, sl.v.u / d, sl.v.v / d, sl.v.p / d};
+  lusgs_finish<false>(la, sl, la.kb, x, x, te);
+}
+
 // Deterministic dot product: fixed-shape block partials, then one block.
 __global__ void __launch_bounds__(kBlock)
     k_dot_partial(long n, const double* __restrict__ a, const double* __restrict__ b,
@@ -2618,21 +2668,37 @@ __global__ void __launch_bounds__(kBlock)
 
 // Block b reduces partial[b*nb .. b*nb+nb) into out[b]; bit b of sqrt_mask
 // takes the square root.
-__global__ void __launch_bounds__(kBlock)
-    k_dot_final(int nb, const double* __restrict__ partial, double* __restrict__ out, int sqrt_mask,
-                double* __restrict__ host_out = nullptr) {
-  __shared__ double red[kBlock];
-  const double* part = partial + (long)blockIdx.x * nb;
+// Workgroups of the deterministic reductions (fixed, so every dot has one
+// order whatever the mesh)
+constexpr int kDotBlocks = 1024;
+// elements per thread loaded ahead of a kernel's leading reduction
+constexpr int kPrefetch = 4;
+
+// Sum of nb block partials, returned to every thread of a kBlock workgroup:
+// thread t adds part[t], part[t + kBlock], ... in order, then a fixed tree.
+// Every reduction of dot partials goes through this one order, so a result
+// is the same bits whichever kernel finishes it.
+__device__ double block_sum(const double* __restrict__ part, int nb, double* red) {
   double acc = 0.0;
-  for (int j = threadIdx.x; j < nb; j += blockDim.x) acc += part[j];
+  for (int j = threadIdx.x; j < nb; j += kBlock) acc += part[j];
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int s = kBlock / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
+  const double v = red[0];
+  __syncthreads();  // red is free for the caller again
+  return v;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_dot_final(int nb, const double* __restrict__ partial, double* __restrict__ out, int sqrt_mask,
+                double* __restrict__ host_out = nullptr) {
+  __shared__ double red[kBlock];
+  const double sum = block_sum(partial + (long)blockIdx.x * nb, nb, red);
   if (threadIdx.x == 0) {
-    double v = ((sqrt_mask >> blockIdx.x) & 1) ? sqrt(red[0]) : red[0];
+    double v = ((sqrt_mask >> blockIdx.x) & 1) ? sqrt(sum) : sum;
     out[blockIdx.x] = v;
     // pinned host mirror: the host reads it after synchronising the stream,
     // without a copy launch
@@ -2692,19 +2758,55 @@ __global__ void __launch_bounds__(kBlock)
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-// One modified Gram-Schmidt step fused with the next one's dot: w -= (*coef) * p
-// with a device-resident coefficient, then the block partial of (q, w) over the updated w, or of
-// (w, w) with SELF; per-thread order and tree as k_dot_partial.
+// One modified Gram-Schmidt step fused with the next one's dot. Each
+// workgroup first finishes the previous dot(s) from their block partials
+// (nprev slots of kDotBlocks at prev, bit j of sqrt_mask taking the square
+// root; workgroup 0 stores them to out[0..nprev) and the pinned mirror), the
+// last of which is the coefficient c; then w -= c * p and the block partial
+// of (q, w) over the updated w, or of (w, w) with SELF; per-thread order and
+// tree as k_dot_partial.
 template <bool SELF>
 __global__ void __launch_bounds__(kBlock)
     k_axpy_dot_partial(long n, const double* __restrict__ p, double* __restrict__ w,
-                       const double* __restrict__ coef, const double* __restrict__ q,
+                       const double* __restrict__ prev, int nprev, int sqrt_mask, double* __restrict__ out,
+                       double* __restrict__ host_out, const double* __restrict__ q,
                        double* __restrict__ partial) {
   __shared__ double red[kBlock];
-  const double c = coef[0];
+  // the thread's first kPrefetch elements are loaded before the coefficient
+  // is finished, so their latency overlaps the reduction
+  const long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long stride = (long)gridDim.x * blockDim.x;
+  double pw[kPrefetch], pp[kPrefetch], pq[kPrefetch];
+#pragma unroll
+  for (int r = 0; r < kPrefetch; ++r) {
+    const long i = i0 + r * stride;
+    if (i < n) {
+      pw[r] = w[i];
+      pp[r] = p[i];
+      pq[r] = SELF ? 0.0 : q[i];
+    }
+  }
+  double c = 0.0;
+  for (int j = 0; j < nprev; ++j) {
+    double v = block_sum(prev + (long)j * kDotBlocks, kDotBlocks, red);
+    if ((sqrt_mask >> j) & 1) v = sqrt(v);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      out[j] = v;
+      if (host_out) host_out[j] = v;
+    }
+    c = v;
+  }
   double acc = 0.0;
-  long stride = (long)gridDim.x * blockDim.x;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+#pragma unroll
+  for (int r = 0; r < kPrefetch; ++r) {
+    const long i = i0 + r * stride;
+    if (i < n) {
+      double wi = pw[r] - c * pp[r];
+      w[i] = wi;
+      acc += SELF ? wi * wi : pq[r] * wi;
+    }
+  }
+  for (long i = i0 + kPrefetch * stride; i < n; i += stride) {
     double wi = w[i] - c * p[i];
     w[i] = wi;
     acc += SELF ? wi * wi : q[i] * wi;
@@ -2730,11 +2832,72 @@ __global__ void __launch_bounds__(kBlock) k_fill(long n, double v, double* __res
   if (i < n) y[i] = v;
 }
 
-// out = x * s
+// out = x * s over a grid of kDotBlocks workgroups, with the block partials of
+// (out, out) in k_dot_partial's order and shape (x may be out)
 __global__ void __launch_bounds__(kBlock)
-    k_scale(long n, const double* __restrict__ x, double s, double* __restrict__ out) {
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = x[i] * s;
+    k_scale_sq_partial(long n, const double* x, double s, double* out, double* __restrict__ partial) {
+  __shared__ double red[kBlock];
+  double acc = 0.0;
+  long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double o = x[i] * s;
+    out[i] = o;
+    acc += o * o;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int t = kBlock / 2; t > 0; t >>= 1) {
+    if (threadIdx.x < t) red[threadIdx.x] += red[threadIdx.x + t];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// Arnoldi normalisation over a grid of kDotBlocks workgroups: h = ||w|| from
+// the block partials at prev (square root of block_sum), stored by
+// workgroup 0 to out[0] and the pinned mirror; v = w * (1 / h), or v = w on
+// breakdown (h <= eps * h0, h0 = ||w|| before the projections, at *h0p);
+// partial = the block partials of (v, v) in k_dot_partial's order and shape.
+__global__ void __launch_bounds__(kBlock)
+    k_arnoldi_scale(long n, const double* __restrict__ w, const double* __restrict__ prev,
+                    const double* __restrict__ h0p, double eps, double* __restrict__ out,
+                    double* __restrict__ host_out, double* __restrict__ v, double* __restrict__ partial) {
+  __shared__ double red[kBlock];
+  const long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long stride = (long)gridDim.x * blockDim.x;
+  double pw[kPrefetch];
+#pragma unroll
+  for (int r = 0; r < kPrefetch; ++r)
+    if (i0 + r * stride < n) pw[r] = w[i0 + r * stride];
+  const double h = sqrt(block_sum(prev, kDotBlocks, red));
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    out[0] = h;
+    if (host_out) host_out[0] = h;
+  }
+  const bool breakdown = h <= eps * h0p[0];
+  const double s = 1.0 / h;
+  double acc = 0.0;
+#pragma unroll
+  for (int r = 0; r < kPrefetch; ++r) {
+    const long i = i0 + r * stride;
+    if (i < n) {
+      double o = breakdown ? pw[r] : pw[r] * s;
+      v[i] = o;
+      acc += o * o;
+    }
+  }
+  for (long i = i0 + kPrefetch * stride; i < n; i += stride) {
+    double o = breakdown ? w[i] : w[i] * s;
+    v[i] = o;
+    acc += o * o;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int t = kBlock / 2; t > 0; t >>= 1) {
+    if (threadIdx.x < t) red[threadIdx.x] += red[threadIdx.x + t];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
 // x += sum_k y_k V_k, k = 0..m-1 in order (y in device memory)
@@ -5021,6 +5184,8 @@ struct McfdMesh {
   LusgsArrays lusgs{};
   std::vector<void*> implicit_allocations;
   std::vector<int> host_lvl_f, host_lvl_b;  // level pointers of the LU-SGS sweeps
+  // two levels each, the backward's the forward's swapped (a 2-colouring)
+  bool lusgs_two = false;
   int lusgs_path = -1;  // -1 undecided, 0 one-workgroup sweeps, 1 one launch per level
   int gmres_m = 0;
   double* gm_V = nullptr;       // (m+1) * 4C Krylov basis
@@ -5487,8 +5652,6 @@ int residual_front(McfdMesh* m, const McfdSettings* st, const double* u, hipStre
 
 namespace {
 
-constexpr int kDotBlocks = 1024;
-
 // The pinned host mirror of a GMRES scalar slot (nullptr outside gm_scalars).
 double* host_slot(McfdMesh* m, const double* dev) {
   if (!m->gm_scalars_host || dev < m->gm_scalars || dev >= m->gm_scalars + m->gm_scalars_n) return nullptr;
@@ -5536,7 +5699,16 @@ int resolve_state_norm(McfdMesh* m, McfdGmres* g, hipStream_t s) {
 }
 
 void lusgs_launch(McfdMesh* m, const double* v, double* out, hipStream_t s, const int* te, bool wide) {
-  if (wide) {
+  if (wide && m->lusgs_two) {
+    const std::vector<int>& pf = m->host_lvl_f;
+    const std::vector<int>& pb = m->host_lvl_b;
+    if (pf[2] > pf[1])
+      hipLaunchKernelGGL(k_lusgs_two_first, dim3(blocks(pf[2] - pf[1])), dim3(kBlock), 0, s, m->lusgs, v, out, te,
+                         pf[1], pf[2]);
+    if (pb[2] > pb[1])
+      hipLaunchKernelGGL(k_lusgs_two_second, dim3(blocks(pb[2] - pb[1])), dim3(kBlock), 0, s, m->lusgs, v, out, te,
+                         pb[1], pb[2]);
+  } else if (wide) {
     // few, large levels: one launch per level over the whole GPU
     const std::vector<int>& pf = m->host_lvl_f;
     for (int l = 0; l < m->lusgs.n_levels_f; ++l)
@@ -5620,7 +5792,9 @@ int ensure_gmres(McfdMesh* m, int krylov) {
   for (int k = 0; k < 7; ++k) {
     if (upload_implicit(m, &bufs[k], (const double*)nullptr, sizes[k])) return -1;
   }
-  if (upload_implicit(m, &m->gm_partial, (const double*)nullptr, 2 * (size_t)kDotBlocks)) return -1;
+  // dot partials: two alternating 2-slot halves (each MGS kernel reads its
+  // predecessor's while writing its own), then the Krylov vector's (v, v)
+  if (upload_implicit(m, &m->gm_partial, (const double*)nullptr, 5 * (size_t)kDotBlocks)) return -1;
   m->gm_V = bufs[0];
   m->gm_w = bufs[1];
   m->gm_r = bufs[2];
@@ -5679,10 +5853,13 @@ int adjoint_apply(JacobianOp& op, const double* v, double* out, hipStream_t s) {
   return 0;
 }
 
-int jacobian_apply(JacobianOp& op, const double* v, double* out, hipStream_t s);
+int jacobian_apply(JacobianOp& op, const double* v, double* out, hipStream_t s,
+                   const double* vpart = nullptr);
 
-int op_apply(JacobianOp& op, const double* v, double* out, hipStream_t s) {
-  return op.adjoint ? adjoint_apply(op, v, out, s) : jacobian_apply(op, v, out, s);
+// vpart: block partials of (v, v) already formed by the kernel that wrote v
+// (the Jacobian product's ||v|| then needs only their final sum)
+int op_apply(JacobianOp& op, const double* v, double* out, hipStream_t s, const double* vpart = nullptr) {
+  return op.adjoint ? adjoint_apply(op, v, out, s) : jacobian_apply(op, v, out, s, vpart);
 }
 
 int op_precond(JacobianOp& op, const double* v, double* out, hipStream_t s) {
@@ -5690,16 +5867,18 @@ int op_precond(JacobianOp& op, const double* v, double* out, hipStream_t s) {
 }
 
 // out = v - dt * J v / sigma with J v = (R1(s + eps*sigma*v) - R1(s)) / eps
-int jacobian_apply(JacobianOp& op, const double* v, double* out, hipStream_t s) {
+int jacobian_apply(JacobianOp& op, const double* v, double* out, hipStream_t s, const double* vpart) {
   McfdMesh* m = op.m;
   long N = 4L * m->n_cells;
   // ||v|| stays on the device: eps = (fd * max(base, 1)) / ||v|| is formed
   // by the perturbation and finishing kernels (same operations), so the
   // product needs no host round trip
   double* dn = m->gm_scalars + 40;
-  hipLaunchKernelGGL(k_dot_partial, dim3(kDotBlocks), dim3(kBlock), 0, s, N, v, v, m->gm_partial);
-  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kBlock), 0, s, kDotBlocks, m->gm_partial, dn, 1,
-                     (double*)nullptr);
+  if (!vpart) {
+    hipLaunchKernelGGL(k_dot_partial, dim3(kDotBlocks), dim3(kBlock), 0, s, N, v, v, m->gm_partial);
+    vpart = m->gm_partial;
+  }
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kBlock), 0, s, kDotBlocks, vpart, dn, 1, (double*)nullptr);
   const double fd = 1.4901161193847656e-08;  // sqrt(2^-52)
   const double num = fd * (op.base > 1.0 ? op.base : 1.0);
   const McfdSettings* st = op.st;
@@ -5808,7 +5987,9 @@ int gmres_cycle(JacobianOp& op, const double* b, double* x, double rtol, int res
   if (dot_dev(m, N, V, V, sc + 3, true, s)) return -1;
   double t0 = 0.0;
   if (read_scalars(m, &t0, sc + 3, 1, s)) return -1;
-  hipLaunchKernelGGL(k_scale, dim3(nbl), dim3(kBlock), 0, s, N, V, 1.0 / t0, V);
+  // partials of each Krylov vector's (v, v), for the product's ||v||
+  double* vpart = m->gm_partial + 4L * kDotBlocks;
+  hipLaunchKernelGGL(k_scale_sq_partial, dim3(kDotBlocks), dim3(kBlock), 0, s, N, V, 1.0 / t0, V, vpart);
   std::vector<double> h((size_t)restart * (restart + 1), 0.0);  // h[col*(restart+1) + row]
   std::vector<double> S(restart + 1, 0.0), gc(restart, 0.0), gs(restart, 0.0);
   S[0] = t0;
@@ -5818,25 +5999,33 @@ int gmres_cycle(JacobianOp& op, const double* b, double* x, double rtol, int res
   for (col = 0; col < restart; ++col) {
     double* vc = V + (long)col * N;
     double* vn = V + (long)(col + 1) * N;
-    if (op_apply(op, vc, m->gm_av, s)) return -1;
+    if (op_apply(op, vc, m->gm_av, s, vpart)) return -1;
     if (op_precond(op, m->gm_av, m->gm_w, s)) return -1;
     // h0 = ||w||, then modified Gram-Schmidt with device coefficients: each
     // projection w -= h_k v_k runs in the pass that forms h_{k+1} = (v_{k+1}, w)
     // (and the last in the pass forming ||w||), same arithmetic as one dot and
-    // one axpy per step.
-    hipLaunchKernelGGL(k_dot2_partial, dim3(kDotBlocks), dim3(kBlock), 0, s, N, V, m->gm_w, m->gm_partial);
-    hipLaunchKernelGGL(k_dot_final, dim3(2), dim3(kBlock), 0, s, kDotBlocks, m->gm_partial, cs, 1,
-                       host_slot(m, cs));
+    // one axpy per step. Each kernel finishes its predecessor's dot from the
+    // block partials (alternating halves of gm_partial), and the normalisation
+    // finishes the last, so the column takes col + 3 launches and one read.
+    double* part[2] = {m->gm_partial, m->gm_partial + 2L * kDotBlocks};
+    hipLaunchKernelGGL(k_dot2_partial, dim3(kDotBlocks), dim3(kBlock), 0, s, N, V, m->gm_w, part[0]);
     for (int k = 1; k <= col; ++k) {
+      // k = 1 also finishes h0 (square root) into cs[0]
+      double* o = k == 1 ? cs : cs + k;
       hipLaunchKernelGGL(k_axpy_dot_partial<false>, dim3(kDotBlocks), dim3(kBlock), 0, s, N,
-                         V + (long)(k - 1) * N, m->gm_w, cs + 1 + k - 1, V + (long)k * N, m->gm_partial);
-      hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kBlock), 0, s, kDotBlocks, m->gm_partial, cs + 1 + k, 0,
-                         host_slot(m, cs + 1 + k));
+                         V + (long)(k - 1) * N, m->gm_w, part[(k - 1) & 1], k == 1 ? 2 : 1, k == 1 ? 1 : 0, o,
+                         host_slot(m, o), V + (long)k * N, part[k & 1]);
     }
-    hipLaunchKernelGGL(k_axpy_dot_partial<true>, dim3(kDotBlocks), dim3(kBlock), 0, s, N, V + (long)col * N,
-                       m->gm_w, cs + 1 + col, (const double*)nullptr, m->gm_partial);
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kBlock), 0, s, kDotBlocks, m->gm_partial, cs + 1 + col + 1, 1,
-                       host_slot(m, cs + 1 + col + 1));
+    {
+      double* o = col == 0 ? cs : cs + 1 + col;
+      hipLaunchKernelGGL(k_axpy_dot_partial<true>, dim3(kDotBlocks), dim3(kBlock), 0, s, N, V + (long)col * N,
+                         m->gm_w, part[col & 1], col == 0 ? 2 : 1, col == 0 ? 1 : 0, o, host_slot(m, o),
+                         (const double*)nullptr, part[(col + 1) & 1]);
+    }
+    // h1 = ||w||; v_{col+1} = w / h1, or w on breakdown (h1 <= eps h0), with
+    // the partials of its (v, v) for the next product
+    hipLaunchKernelGGL(k_arnoldi_scale, dim3(kDotBlocks), dim3(kBlock), 0, s, N, m->gm_w, part[(col + 1) & 1],
+                       cs, eps, cs + col + 2, host_slot(m, cs + col + 2), vn, vpart);
     HIP_CHECK(hipGetLastError());
     if (read_scalars(m, host.data(), cs, col + 3, s)) return -1;
     double h0 = host[0];
@@ -5847,9 +6036,6 @@ int gmres_cycle(JacobianOp& op, const double* b, double* x, double rtol, int res
     if (h1 <= eps * h0) {
       hc[col + 1] = 0.0;
       breakdown = true;
-      HIP_CHECK(hipMemcpyAsync(vn, m->gm_w, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
-    } else {
-      hipLaunchKernelGGL(k_scale, dim3(nbl), dim3(kBlock), 0, s, N, m->gm_w, 1.0 / h1, vn);
     }
     for (int k = 0; k < col; ++k) {
       double c = gc[k], sn = gs[k];
@@ -6585,6 +6771,8 @@ int mcfd_mesh_set_implicit(McfdMesh* m, const McfdImplicitDesc* d) {
   la.kb = kb;
   m->host_lvl_f = pf;
   m->host_lvl_b = pb;
+  m->lusgs_two = nlf == 2 && nlb == 2;
+  for (int i = 0; i < n && m->lusgs_two; ++i) m->lusgs_two = lev_b[i] == 1 - lev_f[i];
   m->lusgs_path = d->launch_path - 1;  // -1 undecided
   la.n_rec = d->n_records;
   la.n_levels_f = nlf;

@@ -84,6 +84,26 @@ def colour_cells(ptr: np.ndarray, nbr: np.ndarray, seed: int = 0) -> np.ndarray:
         colour[sel] = np.log2(free[sel].astype(np.float64)).astype(np.int64)
 
 
+def two_colouring(ptr: np.ndarray, nbr: np.ndarray):
+    """A 2-colouring of the cell graph (CSR ``ptr``, ``nbr``), or None when
+    the graph has an odd cycle. On the bipartite double cover (cells (v, 0)
+    and (v, 1), each edge u-v joining (u, 0)-(v, 1) and (u, 1)-(v, 0)) a
+    component of the graph splits in two exactly when it is bipartite; a
+    cell takes colour 0 when (v, 0) lies in the lower-numbered half."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+
+    n = len(ptr) - 1
+    src = np.repeat(np.arange(n), np.diff(ptr))
+    rows = np.concatenate((src, src + n))
+    cols = np.concatenate((nbr + n, nbr))
+    graph = coo_matrix((np.ones(len(rows), dtype=np.int8), (rows, cols)), shape=(2 * n, 2 * n)).tocsr()
+    _, lab = connected_components(graph, directed=False)
+    if np.any(lab[:n] == lab[n:]):
+        return None
+    return (lab[:n] > lab[n:]).astype(np.int64)
+
+
 def implicit_desc(g: Geometry, L: DeviceLayout, ordering: str = "natural"):
     """The LU-SGS structure on the device cells. ``ordering`` "natural" is
     the reference's (`solver.py:1309-1323`: triangular solves in mesh
@@ -138,7 +158,9 @@ def implicit_desc(g: Geometry, L: DeviceLayout, ordering: str = "natural"):
         co = np.argsort(o_cells, kind="stable")
         cptr = np.zeros(n + 1, dtype=np.int64)
         np.cumsum(np.bincount(o_cells, minlength=n), out=cptr[1:])
-        colour = colour_cells(cptr, o_cols[co])
+        colour = two_colouring(cptr, o_cols[co])
+        if colour is None:
+            colour = colour_cells(cptr, o_cols[co])
         rank = np.empty(n, dtype=np.int64)
         rank[np.lexsort((np.arange(n), colour))] = np.arange(n)
     # a cell's items: lower (earlier in the sweep order) first

@@ -22,8 +22,8 @@ from oracle.euler import conservative_to_primitive, primitive_to_conservative
 from oracle.solver import Oracle, OracleError
 
 
-def periodic(nx=5, ny=4, **kw):
-    mesh = meshgen.rectangle_mesh(nx, ny, jitter=0.2, seed=21, split_fraction=0.3)
+def periodic(nx=5, ny=4, split=0.3, **kw):
+    mesh = meshgen.rectangle_mesh(nx, ny, jitter=0.2, seed=21, split_fraction=split)
     numerics = NumericalSettings(**kw)
     case = meshgen.periodic_case(mesh, numerics)
     g = build_geometry(case)
@@ -277,28 +277,66 @@ def test_multicolour_ordering_is_a_proper_colouring(maker):
     assert colour.max() + 1 <= np.diff(ptr).max() + 1
 
 
+@pytest.mark.parametrize("split", [0.0, 0.3])
+def test_two_colouring_of_bipartite_meshes_and_none_with_odd_cycles(split):
+    """Quad meshes (even periodic sizes) are bipartite: the red-black
+    colouring is proper and the multicolour sweep order has two colours.
+    Split cells make triangles (odd cycles): no 2-colouring, and the greedy
+    colouring is used. A 5-cycle and a 6-cycle pin the odd/even cases."""
+    from meshcfd_amd.implicit import two_colouring
+
+    ring = lambda k: (np.arange(k + 1) * 2, np.column_stack(((np.arange(k) - 1) % k, (np.arange(k) + 1) % k)).ravel())
+    assert two_colouring(*ring(5)) is None
+    c6 = two_colouring(*ring(6))
+    assert c6 is not None and np.all(c6 != np.roll(c6, 1))
+    _, g, _ = periodic(24, 16, split)
+    n = g.cell_count
+    cells = np.concatenate((g.owners[g.interior_faces], g.neighbours[g.interior_faces], g.owners[g.periodic_faces_a],
+                            g.owners[g.periodic_faces_b]))
+    nbrs = np.concatenate((g.neighbours[g.interior_faces], g.owners[g.interior_faces], g.owners[g.periodic_faces_b],
+                           g.owners[g.periodic_faces_a]))
+    o = np.argsort(cells, kind="stable")
+    ptr = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(np.bincount(cells, minlength=n), out=ptr[1:])
+    colour = two_colouring(ptr, nbrs[o])
+    d, rank, mesh_of_device = _sweep_rank(g, "multicolour")
+    if split == 0.0:
+        assert colour is not None and np.all(colour[cells] != colour[nbrs])
+        # sweep order: all of one colour, then the other
+        assert np.all(np.diff(colour[np.argsort(rank)]) >= 0)
+    else:
+        assert colour is None
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("size", [(9, 7), (100, 60), (230, 150)])
+@pytest.mark.parametrize("size", [(9, 7, 0.3), (100, 60, 0.3), (230, 150, 0.3), (120, 64, 0.0)])
 @pytest.mark.parametrize("transpose", [False, True])
 def test_gpu_multicolour_lusgs_matches_oracle_of_the_permuted_system(cuda_device, size, transpose):
     """The multicolour LU-SGS is the natural-order LU-SGS of the system
     renumbered colour by colour: the oracle's natural-order factors of P S P^T
-    (and their transpose) applied to P v, mapped back. The small mesh runs
-    the one-workgroup sweep, the middle one (7,800 cells, 1,560 per colour)
-    times both launch paths on its first application and keeps the faster,
-    the large one (44,850 cells) a launch per level."""
+    (and their transpose) applied to P v, mapped back. The meshes with split
+    cells (triangles, so odd cycles) take the greedy colouring: the small
+    one runs the one-workgroup sweep, the middle one (7,800 cells, 1,560 per
+    colour) times both launch paths on its first application and keeps the
+    faster, the large one (44,850 cells) a launch per level. The quad mesh
+    (even sizes, so bipartite) takes the red-black colouring: two levels per
+    sweep."""
     import ctypes as C
 
     import torch
 
     from meshcfd_amd.implicit import ImplicitStepper
 
-    case, g, prim = periodic(*size, flux="Rusanov", time_integration="Implicit Euler", cfl=100.0)
+    nx, ny, split = size
+    case, g, prim = periodic(nx, ny, split, flux="Rusanov", time_integration="Implicit Euler", cfl=100.0)
     print(f"case: multicolour LU-SGS {g.cell_count} cells transpose={transpose}", flush=True)
     dm = _dm(case, cuda_device, True)
     dm.lusgs_ordering = "multicolour"
     st = ImplicitStepper(dm)
-    _, rank, _ = _sweep_rank(g, "multicolour")
+    d, rank, _ = _sweep_rank(g, "multicolour")
+    if split == 0.0:
+        # two colours: each cell's items are all lower or all upper
+        assert np.all((d["off_split"] == d["off_ptr"][1:]) | (d["off_split"] == d["off_ptr"][:-1]))
     perm = np.argsort(rank)  # sweep position -> mesh cell
     u = primitive_to_conservative(prim, 1.4)
     h = np.linspace(0.01, 0.05, g.cell_count)
@@ -414,17 +452,19 @@ def test_gpu_lusgs_one_cell_wide_periodic_matches_oracle(cuda_device, size, orde
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("ordering", ["natural", "multicolour"])
-def test_gpu_lusgs_launch_paths_give_identical_bits(cuda_device, ordering):
+@pytest.mark.parametrize("ordering,split", [("natural", 0.3), ("multicolour", 0.3), ("multicolour", 0.0)])
+def test_gpu_lusgs_launch_paths_give_identical_bits(cuda_device, ordering, split):
     """The one-workgroup sweeps and the launch-per-level path compute every
     cell with the same code: forced onto each path, the forward and the
-    transposed applications agree bit for bit."""
+    transposed applications agree bit for bit. On the quad mesh (red-black
+    colouring) the level path is the two-launch sweep, whose first-level
+    values are formed where they are read."""
     import ctypes as C
 
     import torch
     from meshcfd_amd.implicit import ImplicitStepper
 
-    case, g, prim = periodic(100, 60, flux="Rusanov", time_integration="Implicit Euler", cfl=100.0)
+    case, g, prim = periodic(100, 60, split, flux="Rusanov", time_integration="Implicit Euler", cfl=100.0)
     print(f"case: LU-SGS launch paths {g.cell_count} cells {ordering}", flush=True)
     u = primitive_to_conservative(prim, 1.4)
     h = np.linspace(0.01, 0.05, g.cell_count)
