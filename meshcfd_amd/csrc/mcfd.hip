@@ -2672,7 +2672,7 @@ __global__ void __launch_bounds__(kBlock)
 // order whatever the mesh)
 constexpr int kDotBlocks = 1024;
 // elements per thread loaded ahead of a kernel's leading reduction
-constexpr int kPrefetch = 4;
+constexpr int kPrefetch = 16;
 
 // Sum of nb block partials, returned to every thread of a kBlock workgroup:
 // thread t adds part[t], part[t + kBlock], ... in order, then a fixed tree.
