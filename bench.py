@@ -12,7 +12,8 @@ Prints one JSON line. ``value`` = cells * steps / seconds / 1e6
 dominant kernel against the 8 TB/s HBM peak from its algorithmic bytes; the
 ``stage`` figures use B_alg per cell-update over the whole step;
 ``cpu_baseline`` times the CPU oracle on one host core on a bounded sample
-(median of five runs); ``batch`` times 256 cases of uneven size advanced
+(median of five runs), and under ``all_cores`` its OpenMP restatement (same
+bits) on every core of the process; ``batch`` times 256 cases of uneven size advanced
 together (HIP-graph replays); ``host`` names the CPU, GPU and library
 versions.
 """
@@ -190,6 +191,44 @@ def cpu_baseline(case, geometry, primitive, steps: int, samples: int = 5):
         + ", pinned to 1 core",
         "seconds": elapsed,
         "sample_seconds": [round(x, 3) for x in times],
+        "all_cores": cpu_all_cores(case, geometry, primitive, steps, samples),
+    }
+
+
+def cpu_all_cores(case, geometry, primitive, steps: int, samples: int = 5):
+    """The multi-core CPU figure: the OpenMP restatement of the same explicit
+    step (oracle/cpu_omp.c, the oracle's arithmetic and summation orders,
+    equal to it bit for bit) on every core of the process's affinity; same
+    protocol as the one-core figure. None when its library is not built."""
+    from oracle import omp
+    from oracle.euler import primitive_to_conservative
+
+    if not omp.available():
+        return None
+    t0 = time.perf_counter()
+    stepper = omp.OmpStepper(geometry, case.numerics)
+    setup = time.perf_counter() - t0
+    threads = len(os.sched_getaffinity(0))
+    stepper.set_threads(threads)
+    u0 = primitive_to_conservative(primitive, case.numerics.gamma)
+    stepper.residual(u0)  # warm-up residual
+    times = []
+    for _ in range(samples):
+        t0 = time.perf_counter()
+        stepper.run(u0, steps)
+        times.append(time.perf_counter() - t0)
+    elapsed = float(np.median(times))
+    cells = geometry.cell_count
+    return {
+        "value": cells * steps / elapsed / 1e6,
+        "unit": "Mcell-updates/s",
+        "cores": stepper.threads,
+        "kind": "port",
+        "sample": f"median of {samples} runs of {steps} SSP-RK3 step(s) of the full {cells}-cell mesh after one "
+        "warm-up residual; OpenMP C restatement of the oracle's step (same bits) on all affinity cores",
+        "seconds": elapsed,
+        "sample_seconds": [round(x, 3) for x in times],
+        "setup_seconds": round(setup, 2),
     }
 
 

@@ -42,3 +42,58 @@ def test_enabled_oracle_residual_unchanged():
     finally:
         compiled.disable()
     assert np.array_equal(want, got)
+
+
+@pytest.mark.parametrize("limiter", ["Barth-Jespersen", "Minmod", "Van Leer", "Venkatakrishnan"])
+@pytest.mark.parametrize("threads", [1, 4])
+def test_openmp_baseline_steps_equal_the_oracle(limiter, threads, monkeypatch):
+    """The all-core CPU baseline (oracle/cpu_omp.c) is the oracle's explicit
+    step: the four-quadrant Riemann workload's settings with each limiter,
+    residual and 6 SSP-RK3 steps (state and time) bit for bit, on one thread
+    and on four (each sum runs per cell in the oracle's order, so the
+    thread count cannot change a bit). The oracle runs with its C stand-ins
+    of the compiled kernels, whose min/max (like the reference's compiled
+    and Python `min`) drop a NaN wave speed where np.minimum propagates it:
+    with Van Leer two reconstructed face pressures of this grid go negative
+    and only the NumPy restatement turns that into a non-finite residual."""
+    import subprocess
+    import sys
+
+    from oracle import omp
+
+    if not omp.available():
+        pytest.skip("OpenMP baseline library not built")
+    code = f"""
+import numpy as np
+from meshcfd_amd import meshgen
+from meshcfd_amd.geometry import build_geometry
+from oracle import compiled
+from oracle.euler import primitive_to_conservative
+from oracle.omp import OmpStepper
+from oracle.solver import Oracle
+compiled.enable()
+case, prim = meshgen.riemann_workload(24, 6)
+case.numerics.limiter = {limiter!r}
+g = build_geometry(case)
+u0 = primitive_to_conservative(prim, 1.4)
+o, st = Oracle(g, case.numerics), OmpStepper(g, case.numerics)
+assert np.array_equal(o.residual(u0), st.residual(u0))
+ua, ta = o.run(prim, 6)
+ub, tb = st.run(u0, 6)
+assert np.array_equal(ua, ub) and ta == tb, (np.abs(ua - ub).max(), ta, tb)
+print("ok")
+"""
+    env = dict(__import__("os").environ, OMP_NUM_THREADS=str(threads))
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0 and out.stdout.strip() == "ok", out.stderr[-2000:]
+
+
+def test_openmp_baseline_rejects_other_settings():
+    from oracle import omp
+
+    if not omp.available():
+        pytest.skip("OpenMP baseline library not built")
+    case, _ = meshgen.riemann_workload(6, 1)
+    case.numerics.flux = "Roe"
+    with pytest.raises(ValueError, match="covers"):
+        omp.OmpStepper(build_geometry(case), case.numerics)
