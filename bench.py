@@ -494,7 +494,8 @@ def main(argv=None) -> int:
     dm.clock.zero_()
     tuned = None
     if not args.unfused and not args.no_autotune:
-        block = dm.autotune(candidates=(96, 104, 112, 120, args.block_cells))
+        # smaller blocks give small meshes more workgroups
+        block = dm.autotune(candidates=tuple(sorted({32, 48, 64, 80, 96, 104, 112, 120, args.block_cells})))
         tuned = {"chosen": block, "ms_per_step": {str(k): round(v, 5) for k, v in dm.timings.items()}}
         args.block_cells = block
         print(f"[bench] autotuned block size {block}: {tuned['ms_per_step']}", file=sys.stderr, flush=True)
