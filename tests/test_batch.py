@@ -90,15 +90,14 @@ def test_graph_replay_equals_eager_steps(cuda_device):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("reconstruction,tol", [("MUSCL linear", 1e-4), ("First order", 1e-6)])
+@pytest.mark.parametrize("reconstruction,tol", [("MUSCL linear", 1e-6), ("First order", 1e-6)])
 def test_per_instance_strength_gradients(cuda_device, reconstruction, tol):
     """One backward pass through the batched mesh gives d KE_k / d beta_k for
     every vortex instance; each matches a central difference (step 1e-4).
     The rollout takes a fixed step (0.02): the gradient is that of the
-    frozen-dt rollout, and the differences use the same step. First order
-    is smooth, so the differences agree to 1e-6; with the Barth-Jespersen
-    limiter the rollout is only piecewise smooth and differences of step
-    1e-4 agree to 1e-4."""
+    frozen-dt rollout, and the differences use the same step. Held to 1e-6
+    (measured: 1.1e-10 to 6.6e-10 with and without the Barth-Jespersen
+    limiter, whose kinks no difference of this step crosses here)."""
     import torch
 
     from meshcfd_amd.adjoint import kinetic_energy, rk3_rollout, vortex_state
