@@ -23,7 +23,6 @@
 #include <math.h>
 #include <omp.h>
 #include <stdint.h>
-#include <stdlib.h>
 #include <string.h>
 
 typedef struct {
@@ -43,7 +42,7 @@ typedef struct {
   const double *toff, *tnrm, *tlen;
   const int64_t *sptr;       /* time-step faces per cell: owned faces, then interior faces as neighbour */
   const double *snrm, *slen;
-  double *prim, *grads, *factors, *mins, *maxs, *flux, *d0, *d1, *stage;
+  double *prim, *grads, *factors, *flux, *d0, *d1, *stage;
 } OmpCase;
 
 static int to_primitive(const OmpCase *c, const double *u) {

@@ -36,8 +36,7 @@ class _Case(C.Structure):
         ("dptr", _I), ("dcol", _I), ("dval", _P),
         ("tptr", _I), ("toff", _P), ("tnrm", _P), ("tlen", _P),
         ("sptr", _I), ("snrm", _P), ("slen", _P),
-        ("prim", _P), ("grads", _P), ("factors", _P), ("mins", _P), ("maxs", _P), ("flux", _P), ("d0", _P),
-        ("d1", _P), ("stage", _P),
+        ("prim", _P), ("grads", _P), ("factors", _P), ("flux", _P), ("d0", _P), ("d1", _P), ("stage", _P),
     ]
 
 
@@ -113,8 +112,8 @@ class OmpStepper:
         o, ptr = _csr(cells, n)
         c.sptr = arr("sptr", ptr, np.int64)
         c.snrm, c.slen = arr("snrm", g.face_normals[faces[o]], np.float64), arr("slen", g.face_lengths[faces[o]], np.float64)
-        for name, size in (("prim", 4 * n), ("grads", 8 * n), ("factors", 4 * n), ("mins", 4 * n), ("maxs", 4 * n),
-                           ("flux", 4 * max(len(inner), 1)), ("d0", 4 * n), ("d1", 4 * n), ("stage", 4 * n)):
+        for name, size in (("prim", 4 * n), ("grads", 8 * n), ("factors", 4 * n), ("flux", 4 * max(len(inner), 1)),
+                           ("d0", 4 * n), ("d1", 4 * n), ("stage", 4 * n)):
             setattr(c, name, arr(name, np.zeros(size), np.float64))
         self._keep = keep
         self._case = c
