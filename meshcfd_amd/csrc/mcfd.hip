@@ -2385,6 +2385,14 @@ struct LusgsArrays {
   double* pval_bt;
   double* pdiag_f;       // [n]
   double* pdiag_b;
+  // The one-workgroup sweeps' own values in their level-position order, so
+  // a position's data never waits on its cell index: own_f[k] = v at the
+  // forward's position k (gathered before the sweep), own_b[k] = y at the
+  // backward's position k (written by the forward sweep through fb, the
+  // backward position of each forward position's cell).
+  const int* fb;         // [n]
+  double* own_f;         // [n * 4]
+  double* own_b;         // [n * 4]
 };
 
 // Per record: 0.5*max(|u_o.n_o|+c_o, |u_n.n_n|+c_n)*L for face pairs, the
@@ -2472,21 +2480,24 @@ constexpr int kLusgsSlots = 2;
 constexpr int kLusgsWideLevel = 4096;
 
 struct LusgsSlot {
-  int cell, cnt;
+  int cell, cnt, fb;
   double d;
   P4 v;
   int nb[kLusgsK];
   double c[kLusgsK];
 };
 
-template <bool FORWARD>
+// POS: own is in the sweep's position order (indexed by k, not by the cell),
+// and the forward slot also carries its cell's backward position.
+template <bool FORWARD, bool POS = false>
 __device__ __forceinline__ void lusgs_fetch(const LusgsArrays& la, int k, int K, const int* cells,
                                             const int* cnt, const int* pnb, const double* pval,
                                             const double* pdiag, const double* own, LusgsSlot& sl) {
   sl.cell = cells[k];
   sl.cnt = cnt[k];
   sl.d = pdiag[k];
-  sl.v = load4(own, sl.cell);
+  sl.v = POS ? load4(own, k) : load4(own, sl.cell);
+  sl.fb = (POS && FORWARD) ? la.fb[k] : 0;
 #pragma unroll
   for (int q = 0; q < kLusgsK; ++q) {
     bool ok = q < K;
@@ -2551,10 +2562,29 @@ __device__ __forceinline__ void lusgs_finish(const LusgsArrays& la, const LusgsS
   store4(dst, sl.cell, o.r, o.u, o.v, o.p);
 }
 
-// Forward: own = v, src = dst = y. Backward: own = src = dst = x (in place).
+// own_f[k] = v[cell of the forward's position k]
+__global__ void __launch_bounds__(kBlock)
+    k_lusgs_gather_own(int n, LusgsArrays la, const double* __restrict__ v) {
+  const int k = xcd_block() * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const P4 o = load4(v, la.lvl_cells_f[k]);
+  store4(la.own_f, k, o.r, o.u, o.v, o.p);
+}
+
+// Forward: own = own_f (v by position), src = dst = y, and y by backward
+// position into own_b. Backward: own = own_b, src = dst = x (in place).
+template <bool FORWARD>
+__device__ __forceinline__ void lusgs_sweep_finish(const LusgsArrays& la, const LusgsSlot& sl, int K, double* y,
+                                                   const int* te) {
+  const P4 o = lusgs_solve<FORWARD>(la, sl, K, y, te);
+  store4(y, sl.cell, o.r, o.u, o.v, o.p);
+  if (FORWARD) store4(la.own_b, sl.fb, o.r, o.u, o.v, o.p);
+}
+
 template <bool FORWARD>
 __global__ void __launch_bounds__(1024)
-    k_lusgs_sweep(LusgsArrays la, const double* __restrict__ own, double* y, const int* __restrict__ te) {
+    k_lusgs_sweep(LusgsArrays la, double* y, const int* __restrict__ te) {
+  const double* own = FORWARD ? la.own_f : la.own_b;
   const int nl = FORWARD ? la.n_levels_f : la.n_levels_b;
   const int* ptr = FORWARD ? la.lvl_ptr_f : la.lvl_ptr_b;
   const int* cells = FORWARD ? la.lvl_cells_f : la.lvl_cells_b;
@@ -2568,7 +2598,8 @@ __global__ void __launch_bounds__(1024)
   int a = ptr[0], b = ptr[1];
 #pragma unroll
   for (int q = 0; q < kLusgsSlots; ++q)
-    if (a + t + q * T < b) lusgs_fetch<FORWARD>(la, a + t + q * T, K, cells, cnt, pnb, pval, pdiag, own, nxt[q]);
+    if (a + t + q * T < b)
+      lusgs_fetch<FORWARD, true>(la, a + t + q * T, K, cells, cnt, pnb, pval, pdiag, own, nxt[q]);
   for (int lev = 0; lev < nl; ++lev) {
     LusgsSlot cur[kLusgsSlots];
 #pragma unroll
@@ -2579,15 +2610,16 @@ __global__ void __launch_bounds__(1024)
       b = ptr[lev + 2];
 #pragma unroll
       for (int q = 0; q < kLusgsSlots; ++q)
-        if (a + t + q * T < b) lusgs_fetch<FORWARD>(la, a + t + q * T, K, cells, cnt, pnb, pval, pdiag, own, nxt[q]);
+        if (a + t + q * T < b)
+          lusgs_fetch<FORWARD, true>(la, a + t + q * T, K, cells, cnt, pnb, pval, pdiag, own, nxt[q]);
     }
 #pragma unroll
     for (int q = 0; q < kLusgsSlots; ++q)
-      if (ca + t + q * T < cb) lusgs_finish<FORWARD>(la, cur[q], K, y, y, te);
+      if (ca + t + q * T < cb) lusgs_sweep_finish<FORWARD>(la, cur[q], K, y, te);
     for (int k = ca + t + kLusgsSlots * T; k < cb; k += T) {
       LusgsSlot sl;
-      lusgs_fetch<FORWARD>(la, k, K, cells, cnt, pnb, pval, pdiag, own, sl);
-      lusgs_finish<FORWARD>(la, sl, K, y, y, te);
+      lusgs_fetch<FORWARD, true>(la, k, K, cells, cnt, pnb, pval, pdiag, own, sl);
+      lusgs_sweep_finish<FORWARD>(la, sl, K, y, te);
     }
     // workgroup-scope release / acquire: the one workgroup's waves share
     // the CU's write-through L1, so the next level reads these values (an
@@ -5721,8 +5753,9 @@ void lusgs_launch(McfdMesh* m, const double* v, double* out, hipStream_t s, cons
         hipLaunchKernelGGL(k_lusgs_level<false>, dim3(blocks(pb[l + 1] - pb[l])), dim3(kBlock), 0, s, m->lusgs,
                          out, out, te, pb[l], pb[l + 1]);
   } else {
-    hipLaunchKernelGGL(k_lusgs_sweep<true>, dim3(1), dim3(1024), 0, s, m->lusgs, v, out, te);
-    hipLaunchKernelGGL(k_lusgs_sweep<false>, dim3(1), dim3(1024), 0, s, m->lusgs, out, out, te);
+    hipLaunchKernelGGL(k_lusgs_gather_own, dim3(blocks(m->n_cells)), dim3(kBlock), 0, s, m->n_cells, m->lusgs, v);
+    hipLaunchKernelGGL(k_lusgs_sweep<true>, dim3(1), dim3(1024), 0, s, m->lusgs, out, te);
+    hipLaunchKernelGGL(k_lusgs_sweep<false>, dim3(1), dim3(1024), 0, s, m->lusgs, out, te);
   }
 }
 
@@ -6812,6 +6845,14 @@ int mcfd_mesh_set_implicit(McfdMesh* m, const McfdImplicitDesc* d) {
   rc |= upload_implicit(m, &la.pval_bt, (const double*)nullptr, nb_b.size());
   rc |= upload_implicit(m, &la.pdiag_f, (const double*)nullptr, (size_t)n);
   rc |= upload_implicit(m, &la.pdiag_b, (const double*)nullptr, (size_t)n);
+  {
+    std::vector<int> posb(n), fbv(n);
+    for (int k = 0; k < n; ++k) posb[cb[k]] = k;
+    for (int k = 0; k < n; ++k) fbv[k] = posb[cf[k]];
+    rc |= upload_implicit(m, const_cast<int**>(&la.fb), fbv.data(), (size_t)n);
+  }
+  rc |= upload_implicit(m, &la.own_f, (const double*)nullptr, 4 * (size_t)n);
+  rc |= upload_implicit(m, &la.own_b, (const double*)nullptr, 4 * (size_t)n);
   if (rc) {
     std::string msg = g_last_error;
     free_implicit(m);
