@@ -253,7 +253,7 @@ def cylinder_bench(args) -> int:
     # the Newton residuals (MUSCL parabolic + viscous) run the four-kernel
     # path; the blocks serve the first-order viscous Jacobian products, one
     # fused launch each
-    dm = DeviceMesh(case, "cuda:0", fused=True)
+    dm = DeviceMesh(case, "cuda:0", fused=True, block_cells=args.block_cells)
     dm.lusgs_ordering = args.lusgs_ordering
     setup_s = time.perf_counter() - t0
     note(f"setup {setup_s:.1f}s")
