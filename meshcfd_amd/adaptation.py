@@ -37,7 +37,6 @@ from __future__ import annotations
 
 import json
 import math
-from bisect import bisect_left, bisect_right
 from dataclasses import dataclass
 
 import numpy as np
@@ -815,9 +814,45 @@ def adapt_mesh(mesh: FiniteVolumeMesh, marks: dict, coarsen: set, periodic_pairs
     return rebuilt, tuple(leaf.sources for leaf in leaves)
 
 
-def _boundary_patch(a, b, boundary_edges) -> int:
+class _Grid:
+    """Uniform buckets of items with an axis-aligned box each, for the
+    candidate searches of `rebuild_mesh`; a query returns the items whose
+    box meets the query box, in ascending item order, so a search over the
+    candidates visits them in the order a full scan would."""
+
+    def __init__(self, boxes, h: float):
+        self.h = h
+        self.cells = {}
+        for i, (x0, x1, y0, y1) in enumerate(boxes):
+            for key in self._keys(x0, x1, y0, y1):
+                self.cells.setdefault(key, []).append(i)
+
+    def _keys(self, x0, x1, y0, y1):
+        h = self.h
+        for ix in range(math.floor(x0 / h), math.floor(x1 / h) + 1):
+            for iy in range(math.floor(y0 / h), math.floor(y1 / h) + 1):
+                yield ix, iy
+
+    def query(self, x0, x1, y0, y1):
+        found = set()
+        for key in self._keys(x0, x1, y0, y1):
+            found.update(self.cells.get(key, ()))
+        return sorted(found)
+
+
+def _segment_box(a, b):
+    """A box holding every point `on_segment` accepts for segment ab (TOL
+    across it and up to TOL times its length beyond each end), with a
+    factor 2 of margin for rounding."""
+    e = 2.0 * TOL * (1.0 + math.hypot(b[0] - a[0], b[1] - a[1]))
+    return (min(a[0], b[0]) - e, max(a[0], b[0]) + e, min(a[1], b[1]) - e, max(a[1], b[1]) + e)
+
+
+def _boundary_patch(a, b, boundary_edges, grid) -> int:
+    """The patch of the first old boundary edge under the midpoint of ab."""
     mid = (0.5 * (a[0] + b[0]), 0.5 * (a[1] + b[1]))
-    for p, q, patch in boundary_edges:
+    for i in grid.query(mid[0], mid[0], mid[1], mid[1]):
+        p, q, patch = boundary_edges[i]
         if on_segment(mid, p, q):
             return patch
     return -1
@@ -834,15 +869,28 @@ def rebuild_mesh(old: FiniteVolumeMesh, leaves, records) -> FiniteVolumeMesh:
         for p in leaf.vertices:
             distinct.setdefault(point_key(p), p)
     known = tuple(distinct.values())
-    by_x = sorted(known, key=lambda p: p[0])
-    by_y = sorted(known, key=lambda p: p[1])
-    xs = [p[0] for p in by_x]
-    ys = [p[1] for p in by_y]
+    # rank of each point in the x- and y-sorted orders (stable sorts), which
+    # order the candidates on an edge before they are sorted along it
+    order_x = sorted(range(len(known)), key=lambda i: known[i][0])
+    order_y = sorted(range(len(known)), key=lambda i: known[i][1])
+    rank = ([0] * len(known), [0] * len(known))
+    for r, i in enumerate(order_x):
+        rank[0][i] = r
+    for r, i in enumerate(order_y):
+        rank[1][i] = r
+    # bucket size: the median leaf edge length
+    lengths = sorted(math.hypot(leaf.vertices[(k + 1) % len(leaf.vertices)][0] - leaf.vertices[k][0],
+                                leaf.vertices[(k + 1) % len(leaf.vertices)][1] - leaf.vertices[k][1])
+                     for leaf in leaves for k in range(len(leaf.vertices)))
+    h = max(lengths[len(lengths) // 2] if lengths else 1.0, 1e-9)
+    point_grid = _Grid([(p[0], p[0], p[1], p[1]) for p in known], h)
     boundary_edges = []
     for f in np.flatnonzero(old.face_neighbours < 0):
         a, b = old.faces[f]
         boundary_edges.append(((float(old.points[a, 0]), float(old.points[a, 1])),
                                (float(old.points[b, 0]), float(old.points[b, 1])), int(old.face_patch_ids[f])))
+    edge_grid = _Grid([_segment_box(p, q) for p, q, _ in boundary_edges], h)
+    straight = []  # new faces off curves: their patch is looked up once the leaves are done
     points, pid = [], {}
     faces, owners, neighbours, patches_of, curves_of = [], [], [], [], []
     cell_faces, cell_signs, cell_nodes = [], [], []
@@ -863,10 +911,18 @@ def rebuild_mesh(old: FiniteVolumeMesh, leaves, records) -> FiniteVolumeMesh:
             a, b = leaf.vertices[k], leaf.vertices[(k + 1) % n]
             curve = lc[k]
             axis = 0 if abs(b[0] - a[0]) >= abs(b[1] - a[1]) else 1
-            ordered, coords = (by_x, xs) if axis == 0 else (by_y, ys)
+            # points within TOL of the edge's range along its main axis (the
+            # candidates in that axis's sorted order), filtered by on_segment
             lo = min(a[axis], b[axis]) - TOL
             hi = max(a[axis], b[axis]) + TOL
-            on_edge = [p for p in ordered[bisect_left(coords, lo): bisect_right(coords, hi)] if on_segment(p, a, b)]
+            x0, x1, y0, y1 = _segment_box(a, b)
+            if axis == 0:
+                x0, x1 = lo, hi
+            else:
+                y0, y1 = lo, hi
+            cand = [i for i in point_grid.query(x0, x1, y0, y1) if lo <= known[i][axis] <= hi]
+            cand.sort(key=lambda i: rank[axis][i])
+            on_edge = [known[i] for i in cand if on_segment(known[i], a, b)]
             on_edge.sort(key=lambda p: segment_parameter(p, a, b))
             for p, q in zip(on_edge, on_edge[1:]):
                 if dist2(p, q) <= 1e-26:
@@ -882,7 +938,8 @@ def rebuild_mesh(old: FiniteVolumeMesh, leaves, records) -> FiniteVolumeMesh:
                     owners.append(c)
                     neighbours.append(-1)
                     if curve is None:
-                        patches_of.append(_boundary_patch(p, q, boundary_edges))
+                        patches_of.append(-1)
+                        straight.append((f, p, q))
                         curves_of.append(None)
                     else:
                         t0 = segment_parameter(p, a, b)
@@ -904,6 +961,9 @@ def rebuild_mesh(old: FiniteVolumeMesh, leaves, records) -> FiniteVolumeMesh:
         cell_faces.append(fids)
         cell_signs.append(signs)
         cell_nodes.append(loop)
+    for f, p, q in straight:
+        if neighbours[f] < 0:
+            patches_of[f] = _boundary_patch(p, q, boundary_edges, edge_grid)
     width = max(len(x) for x in cell_nodes)
     cells = np.full((len(cell_nodes), width), -1, dtype=np.int64)
     sizes = np.array([len(x) for x in cell_nodes], dtype=np.int64)
