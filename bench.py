@@ -404,6 +404,79 @@ def batch_bench(args) -> int:
     return 0
 
 
+def adaptive_bench(args) -> int:
+    """The adaptive bump channel through `solve_case`: Mach 0.675 inflow,
+    pressure outlet, bump height 0.06 on a --bump-nx x --bump-ny base grid;
+    HLLC + MUSCL linear, SSP-RK3 with local CFL steps (0.8), the density
+    indicator refining up to 3 levels (threshold 0.02) every
+    --adapt-interval steps, --steps steps from the free stream. Prints one
+    JSON line splitting the wall time into the host's adaptations (indicator,
+    marks, new mesh, transfer and the device rebuild) and the rest (device
+    steps and their progress reads)."""
+    import torch
+
+    from meshcfd_amd import meshgen, solver
+
+    case, prim = meshgen.bump_case(args.bump_nx, args.bump_ny, 0.06, integrator="Explicit RK3", cfl=0.8)
+    s = case.numerics
+    s.adaptive_mesh = True
+    s.adaptation_interval = args.adapt_interval
+    s.adaptation_max_level = 3
+    s.adaptation_refine_threshold = 0.02
+    s.adaptation_variable = "Density"
+    s.max_iterations = args.steps
+    s.residual_tolerance = 1e-9
+    s.convergence_history_interval = args.steps + 1
+    base = case.mesh.cell_count
+    log = []  # (seconds, cells after) per adaptation
+    inner = solver._adapt
+
+    def timed(*a, **k):
+        t0 = time.perf_counter()
+        out = inner(*a, **k)
+        torch.cuda.synchronize()
+        log.append((time.perf_counter() - t0, case.mesh.cell_count))
+        print(f"[bench] adaptation {len(log)}: {case.mesh.cell_count} cells, {log[-1][0]:.2f}s", file=sys.stderr,
+              flush=True)
+        return out
+
+    solver._adapt = timed
+    try:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = solver.solve_case(case, prim, device="cuda:0")
+        torch.cuda.synchronize()
+        total = time.perf_counter() - t0
+    finally:
+        solver._adapt = inner
+    # cell-updates: each interval runs on the mesh of the adaptation before it
+    counts = [base] + [c for _, c in log]
+    updates = 0
+    for k in range(res.steps):
+        updates += counts[min(k // args.adapt_interval, len(counts) - 1)]
+    adapt_s = sum(t for t, _ in log)
+    line = {
+        "metric": "adaptive bump channel: device Mcell-updates/s and host adaptation seconds",
+        "value": round(updates / (total - adapt_s) / 1e6, 2),
+        "unit": "Mcell-updates/s (wall time outside the adaptations)",
+        "n_gpus": 1,
+        "steps": res.steps,
+        "higher_is_better": True,
+        "dtype": "f64",
+        "data": "synthetic (bump channel, free-stream start)",
+        "config": {"workload": f"bump channel {args.bump_nx}x{args.bump_ny} base, h 0.06, Mach 0.675, HLLC + MUSCL "
+                   f"linear SSP-RK3 local CFL 0.8, density indicator (0.02) up to 3 levels every "
+                   f"{args.adapt_interval} steps"},
+        "seconds": round(total, 2),
+        "adaptation_seconds": round(adapt_s, 2),
+        "adaptations": [{"seconds": round(t, 2), "cells": c} for t, c in log],
+        "cells": {"base": base, "final": case.mesh.cell_count},
+        "residual": res.residual,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -417,10 +490,13 @@ def main(argv=None) -> int:
     ap.add_argument("--unfused", action="store_true", help="four-kernel path instead of the fused stage kernel")
     ap.add_argument("--block-cells", type=int, default=128, help="cells per workgroup of the fused stage kernel")
     ap.add_argument("--no-adjoint", action="store_true", help="skip timing the adjoint residual (VJP)")
-    ap.add_argument("--workload", choices=("riemann", "cylinder", "batch"), default="riemann",
+    ap.add_argument("--workload", choices=("riemann", "cylinder", "batch", "adaptive"), default="riemann",
                     help="riemann: the explicit headline benchmark; cylinder: implicit steps of the laminar "
                     "O-grid cylinder (Roe + MUSCL parabolic + Sutherland, LU-SGS + GMRES); batch: 256 cases of "
-                    "uneven size advanced together")
+                    "uneven size advanced together; adaptive: the bump channel adapting through solve_case")
+    ap.add_argument("--bump-nx", type=int, default=512, help="adaptive: base cells along the channel")
+    ap.add_argument("--bump-ny", type=int, default=128, help="adaptive: base cells across the channel")
+    ap.add_argument("--adapt-interval", type=int, default=200, help="adaptive: steps between adaptations")
     ap.add_argument("--cases", type=int, default=256, help="batch: number of cases")
     ap.add_argument("--batch-cases", type=int, default=256,
                     help="riemann: also time a batch of this many cases as the JSON line's 'batch' key (0 = skip)")
@@ -453,6 +529,8 @@ def main(argv=None) -> int:
         return cylinder_bench(args)
     if args.workload == "batch":
         return batch_bench(args)
+    if args.workload == "adaptive":
+        return adaptive_bench(args)
 
     import torch
 
