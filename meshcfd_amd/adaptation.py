@@ -73,6 +73,12 @@ def cell_vertices(mesh: FiniteVolumeMesh, cell: int) -> tuple:
     return tuple((float(mesh.points[n, 0]), float(mesh.points[n, 1])) for n in ids)
 
 
+def all_cell_vertices(mesh: FiniteVolumeMesh) -> list:
+    """`cell_vertices` of every cell (the same tuples), in one pass."""
+    pts = [tuple(p) for p in mesh.points.tolist()]
+    return [tuple(pts[n] for n in row[:size]) for row, size in zip(mesh.cells.tolist(), mesh.cell_sizes.tolist())]
+
+
 def _cell_ids(mesh):
     return tuple(mesh.cell_ids) if len(mesh.cell_ids) else tuple(f"r{k}" for k in range(mesh.cell_count))
 
@@ -452,11 +458,33 @@ def _root_vertices(records):
 
 
 def cell_directional_levels(mesh, cell_ids, records) -> tuple:
+    """`directional_levels` of every cell against its root: the bounding-box
+    spans in NumPy (min and max are exact), the log2 of the ratios in Python
+    for the cells whose root box differs from their own."""
     roots = _root_vertices(records)
+    n = mesh.cell_count
+    if n == 0:
+        return ()
+    width = mesh.cells.shape[1]
+    valid = np.arange(width)[None, :] < mesh.cell_sizes[:, None]
+    ids = np.where(valid, mesh.cells, 0)
+    xy = mesh.points[ids]  # (n, width, 2)
+    big = np.inf
+    lo = np.where(valid[:, :, None], xy, big).min(axis=1)
+    hi = np.where(valid[:, :, None], xy, -big).max(axis=1)
+    span = (hi - lo).tolist()
+    base_of = {}
+    for rid, verts in roots.items():
+        base_of[rid] = tuple(max(p[axis] for p in verts) - min(p[axis] for p in verts) for axis in (0, 1))
     out = []
     for k, cid in enumerate(cell_ids):
-        v = cell_vertices(mesh, k)
-        out.append(directional_levels(v, roots.get(cid.split(".", 1)[0], v)))
+        base = base_of.get(cid.split(".", 1)[0])
+        if base is None:  # the cell is its own root: ratio 1, level 0
+            out.append((0, 0))
+            continue
+        own = span[k]
+        out.append(tuple(0 if own[a] <= TOL or base[a] <= TOL else max(0, round(math.log2(base[a] / own[a])))
+                         for a in (0, 1)))
     return tuple(out)
 
 
@@ -484,35 +512,73 @@ def balance_plan(marks: dict, coarsen: set, neighbours, cell_ids, levels, levels
     """Adjust ``marks`` and ``coarsen`` in place until projected levels and
     projected directional levels differ by at most one across every face
     (adaptation.py:252-451); raises `AdaptationBalanceError` when that is
-    impossible or the plan cycles."""
+    impossible or the plan cycles.
+
+    Each round fixes the first violation in scan order (cells ascending,
+    each cell's larger neighbours in its set's order), as a full rescan
+    would; a fix or a periodic adjustment changes the projections of a few
+    cells only, so the projections are updated for those cells and the next
+    scan starts at the first cell of a pair that holds one of them, which
+    finds the same first violation without rescanning the cells before it.
+    States are recognised for the cycle test by an order-free sum of item
+    hashes kept up to date with every change."""
     by_parent = {r.parent_id: r for r in records}
     parent_of = {child: r.parent_id for r in records for child in r.child_ids}
-    seen = set()
+    n = len(cell_ids)
+    index = {cid: k for k, cid in enumerate(cell_ids)}
+    kids_of = {p: [index[ch] for ch in r.child_ids if ch in index] for p, r in by_parent.items()}
+    nb_lists = [list(neighbours[k]) for k in range(n)]
+    mask = (1 << 64) - 1
+    state = {"h": 0}
+    dirty = set()
 
-    def projected():
-        lev, dirs = [], []
-        for k, cid in enumerate(cell_ids):
-            parent = parent_of.get(cid)
-            if parent in coarsen:
-                lev.append(by_parent[parent].level)
-                dirs.append(parent_dir[parent])
-            else:
-                lev.append(levels[k] + int(k in marks))
-                m = marks.get(k, "")
-                dirs.append((levels_dir[k][0] + int("x" in m), levels_dir[k][1] + int("y" in m)))
-        return lev, dirs
+    def item_hash(item):
+        return hash(item) & mask
+
+    for k, axis in marks.items():
+        state["h"] = (state["h"] + item_hash(("m", k, axis))) & mask
+    for parent in coarsen:
+        state["h"] = (state["h"] + item_hash(("c", parent))) & mask
+
+    def set_mark(k, axis):
+        old = marks.get(k)
+        if old is not None:
+            state["h"] = (state["h"] - item_hash(("m", k, old))) & mask
+        marks[k] = axis
+        state["h"] = (state["h"] + item_hash(("m", k, axis))) & mask
+        dirty.add(k)
+
+    def del_mark(k):
+        state["h"] = (state["h"] - item_hash(("m", k, marks.pop(k)))) & mask
+        dirty.add(k)
+
+    def drop_coarsen(parent):
+        coarsen.remove(parent)
+        state["h"] = (state["h"] - item_hash(("c", parent))) & mask
+        dirty.update(kids_of.get(parent, ()))
+
+    def project(k):
+        parent = parent_of.get(cell_ids[k])
+        if parent in coarsen:
+            return by_parent[parent].level, parent_dir[parent]
+        m = marks.get(k, "")
+        return levels[k] + int(k in marks), (levels_dir[k][0] + int("x" in m), levels_dir[k][1] + int("y" in m))
+
+    proj_lev, proj_dir = [0] * n, [None] * n
+    for k in range(n):
+        proj_lev[k], proj_dir[k] = project(k)
 
     def fix(lower, higher, axis):
         """Resolve one violation between a coarser ``lower`` and a finer
         ``higher`` cell (axis None: levels; else one direction)."""
         parent = parent_of.get(cell_ids[lower])
         if parent in coarsen:
-            coarsen.remove(parent)
+            drop_coarsen(parent)
             return
         if lower in marks:
             if axis is None or axis in marks[lower]:
                 if higher in marks:
-                    del marks[higher]
+                    del_mark(higher)
                     return
                 raise AdaptationBalanceError(
                     "The current mesh cannot satisfy 2:1 refinement balance in one adaptation step."
@@ -520,16 +586,16 @@ def balance_plan(marks: dict, coarsen: set, neighbours, cell_ids, levels, levels
                     "The current mesh cannot satisfy anisotropic transition balance in one adaptation step."
                 )
             both = set(marks[lower]) | {axis}
-            marks[lower] = "xy" if len(both) == 2 else next(iter(both))
+            set_mark(lower, "xy" if len(both) == 2 else next(iter(both)))
             return
         if levels[lower] < maximum_level:
             if axis is None:
-                marks[lower] = AXES[_least_disruptive_axis(lower, neighbours, proj_dir, levels_dir)]
+                set_mark(lower, AXES[_least_disruptive_axis(lower, neighbours, proj_dir, levels_dir)])
             else:
-                marks[lower] = axis
+                set_mark(lower, axis)
             return
         if higher in marks:
-            del marks[higher]
+            del_mark(higher)
             return
         raise AdaptationBalanceError(
             "The current mesh cannot satisfy 2:1 refinement balance within the configured maximum level."
@@ -537,11 +603,26 @@ def balance_plan(marks: dict, coarsen: set, neighbours, cell_ids, levels, levels
             "The current mesh cannot satisfy anisotropic transition balance within the configured maximum level."
         )
 
+    def first_violation(k0):
+        for k in range(k0, n):
+            for nb in nb_lists[k]:
+                if nb <= k:
+                    continue
+                d = proj_lev[k] - proj_lev[nb]
+                if abs(d) > 1:
+                    return k, (k if d < 0 else nb), (nb if d < 0 else k), None
+                for ai, axis in enumerate(AXES):
+                    d = proj_dir[k][ai] - proj_dir[nb][ai]
+                    if abs(d) > 1:
+                        return k, (k if d < 0 else nb), (nb if d < 0 else k), axis
+        return None
+
+    seen = set()
     changed = True
-    proj_dir = None
+    restart = 0
     while changed:
         changed = False
-        signature = hash((frozenset(marks.items()), frozenset(coarsen)))
+        signature = (state["h"], len(marks), len(coarsen))
         if signature in seen:
             raise AdaptationBalanceError("The adaptive refinement plan cycles while balancing mesh transitions.")
         seen.add(signature)
@@ -549,53 +630,46 @@ def balance_plan(marks: dict, coarsen: set, neighbours, cell_ids, levels, levels
         for a, b, _ in periodic_pairs:
             pa, pb = parent_of.get(cell_ids[a]), parent_of.get(cell_ids[b])
             if pa in coarsen and pb not in coarsen:
-                coarsen.remove(pa)
+                drop_coarsen(pa)
                 changed = True
             if pb in coarsen and pa not in coarsen:
-                coarsen.remove(pb)
+                drop_coarsen(pb)
                 changed = True
         for a, b, _ in periodic_pairs:
             if a in marks and b not in marks:
                 if levels[b] >= maximum_level:
-                    del marks[a]
+                    del_mark(a)
                 else:
-                    marks[b] = marks[a]
+                    set_mark(b, marks[a])
                 changed = True
             elif b in marks and a not in marks:
                 if levels[a] >= maximum_level:
-                    del marks[b]
+                    del_mark(b)
                 else:
-                    marks[a] = marks[b]
+                    set_mark(a, marks[b])
                 changed = True
             elif a in marks and b in marks and marks[a] != marks[b]:
                 if marks[a] != "xy" or marks[b] != "xy":
-                    marks[a] = marks[b] = "xy"
+                    set_mark(a, "xy")
+                    set_mark(b, "xy")
                     changed = True
-        proj_lev, proj_dir = projected()
-        for k in range(len(cell_ids)):
-            for nb in neighbours[k]:
-                if nb <= k:
-                    continue
-                d = proj_lev[k] - proj_lev[nb]
-                if abs(d) > 1:
-                    fix(k if d < 0 else nb, nb if d < 0 else k, None)
-                    changed = True
-                    break
-                for ai, axis in enumerate(AXES):
-                    d = proj_dir[k][ai] - proj_dir[nb][ai]
-                    if abs(d) <= 1:
-                        continue
-                    fix(k if d < 0 else nb, nb if d < 0 else k, axis)
-                    changed = True
-                    break
-                if changed:
-                    break
-            if changed:
-                break
+        # projections of the changed cells, and the first pair that may have changed
+        for k in dirty:
+            proj_lev[k], proj_dir[k] = project(k)
+            restart = min(restart, k, min(nb_lists[k], default=k))
+        dirty.clear()
+        found = first_violation(restart)
+        if found is None:
+            restart = n
+            continue
+        restart, lower, higher, axis = found
+        fix(lower, higher, axis)
+        changed = True
     # final verification (adaptation.py:427-451)
-    proj_lev, proj_dir = projected()
-    for k in range(len(cell_ids)):
-        for nb in neighbours[k]:
+    for k in dirty:
+        proj_lev[k], proj_dir[k] = project(k)
+    for k in range(n):
+        for nb in nb_lists[k]:
             if nb <= k:
                 continue
             if abs(proj_lev[k] - proj_lev[nb]) > 1:
@@ -648,36 +722,40 @@ def indicator(mesh, centers, areas, values, links) -> tuple:
     max(|grad| sqrt(A) / scale, largest |jump| / scale) with scale =
     max(range, 1e-8 max|v|, 1e-30) (adaptation.py:91-122)."""
     n = mesh.cell_count
-    samples = [[] for _ in range(n)]
-    for a, b, sx, sy in links:
-        dx = centers[b, 0] + sx - centers[a, 0]
-        dy = centers[b, 1] + sy - centers[a, 1]
-        dv = values[b] - values[a]
-        samples[a].append((dx, dy, dv))
-        samples[b].append((-dx, -dy, -dv))
+    values = np.asarray(values, dtype=float)
+    lk = np.asarray(links, dtype=float).reshape(-1, 4)
+    la, lb = lk[:, 0].astype(np.int64), lk[:, 1].astype(np.int64)
+    dx = centers[lb, 0] + lk[:, 2] - centers[la, 0]
+    dy = centers[lb, 1] + lk[:, 3] - centers[la, 1]
+    dv = values[lb] - values[la]
+    # each link gives (dx, dy, dv) to a and the negation to b; a cell's
+    # samples in link order
+    cell = np.column_stack((la, lb)).ravel()
+    sx = np.column_stack((dx, -dx)).ravel()
+    sy = np.column_stack((dy, -dy)).ravel()
+    sv = np.column_stack((dv, -dv)).ravel()
+    order = np.argsort(cell, kind="stable")
+    cell, sx, sy, sv = cell[order], sx[order], sy[order], sv[order]
+    count = np.bincount(cell, minlength=n)
+    slot = np.arange(len(cell)) - np.repeat(np.cumsum(count) - count, count)
     grads = np.zeros((n, 2))
-    kmax = max((len(s) for s in samples), default=0)
+    kmax = int(count.max()) if n else 0
     if kmax >= 2:
         A = np.zeros((n, kmax, 2))
         rhs = np.zeros((n, kmax))
-        use = np.zeros(n, dtype=bool)
-        for c, s in enumerate(samples):
-            if len(s) < 2:
-                continue
-            arr = np.asarray(s, dtype=float)
-            off = arr[:, :2]
-            w = 1.0 / np.maximum(np.sum(off * off, axis=1), TOL**2)
-            A[c, : len(s)] = off * np.sqrt(w)[:, None]
-            rhs[c, : len(s)] = arr[:, 2] * np.sqrt(w)
-            use[c] = True
+        use = count >= 2
+        keep = use[cell]
+        off = np.column_stack((sx, sy))[keep]
+        w = 1.0 / np.maximum(np.sum(off * off, axis=1), TOL**2)
+        A[cell[keep], slot[keep]] = off * np.sqrt(w)[:, None]
+        rhs[cell[keep], slot[keep]] = sv[keep] * np.sqrt(w)
         # minimum-norm least squares per cell (zero rows change nothing)
         grads[use] = np.einsum("cij,cj->ci", np.linalg.pinv(A[use], rcond=1e-12), rhs[use])
     value_range = float(np.ptp(values))
     scale = max(value_range, float(np.max(np.abs(values))) * 1e-8, 1e-30)
     grad_ind = np.linalg.norm(grads, axis=1) * np.sqrt(areas) / scale
     jump = np.zeros(n)
-    for a, b, _, _ in links:
-        jump[a] = max(jump[a], abs(float(values[b] - values[a])) / scale)
+    np.maximum.at(jump, la, np.abs(dv) / scale)
     return np.maximum(grad_ind, jump), grads
 
 
@@ -702,8 +780,8 @@ def select_adaptation(mesh, case, centers, areas, values, threshold: float, maxi
         links.append((b, a, s[0], s[1]))
     ind, grads = indicator(mesh, centers, np.asarray(areas, dtype=float), values, links)
     marks = {}
-    for c in range(mesh.cell_count):
-        if ind[c] <= threshold or levels[c] >= maximum_level:
+    for c in np.flatnonzero(ind > threshold).tolist():
+        if levels[c] >= maximum_level:
             continue
         verts = cell_vertices(mesh, c)
         corners = drop_collinear(verts)
@@ -763,8 +841,15 @@ def adapt_mesh(mesh: FiniteVolumeMesh, marks: dict, coarsen: set, periodic_pairs
                  periodic)
     records = {r.parent_id: r for r in mesh.refinement_records}
     curves = _curves_by_edge(mesh)
-    old = [Leaf(cell_ids[k], levels[k], cell_vertices(mesh, k), (k,), cell_edge_curves(mesh, k, curves))
-           for k in range(mesh.cell_count)]
+    verts = all_cell_vertices(mesh)
+    if curves is None:
+        old = [Leaf(cell_ids[k], levels[k], verts[k], (k,), ()) for k in range(mesh.cell_count)]
+    else:
+        # only the owners of curved (boundary) faces have a curved edge
+        curved = {int(mesh.face_owners[f]) for f, _ in curves.values()}
+        old = [Leaf(cell_ids[k], levels[k], verts[k], (k,),
+                    cell_edge_curves(mesh, k, curves) if k in curved else (None,) * len(verts[k]))
+               for k in range(mesh.cell_count)]
     position = {leaf.cell_id: k for k, leaf in enumerate(old)}
     merged_children: set = set()
     parent_at_first_child: dict = {}
@@ -858,131 +943,223 @@ def _boundary_patch(a, b, boundary_edges, grid) -> int:
     return -1
 
 
+def _edges_with_other_points(P, V, ia, ib, ea, eb, h):
+    """Per leaf edge (endpoints ea, eb; their points ia, ib among the
+    distinct points P): whether any other point of P lies in the box that
+    `_points_on_edge` searches (the edge's range along its main axis, within
+    TOL, and its `_segment_box` across). A superset test in NumPy: edges
+    without such a point hold only their endpoints."""
+    n_e = len(ea)
+    if n_e == 0 or len(P) == 0:
+        return np.zeros(n_e, dtype=bool)
+    d = eb - ea
+    main = np.where(np.abs(d[:, 0]) >= np.abs(d[:, 1]), 0, 1)
+    e = 2.0 * TOL * (1.0 + np.hypot(d[:, 0], d[:, 1]))
+    lo_box = np.minimum(ea, eb) - e[:, None]
+    hi_box = np.maximum(ea, eb) + e[:, None]
+    rows = np.arange(n_e)
+    lo_box[rows, main] = np.minimum(ea, eb)[rows, main] - TOL
+    hi_box[rows, main] = np.maximum(ea, eb)[rows, main] + TOL
+    cell = np.floor(P / h).astype(np.int64)
+    base = cell.min(axis=0)
+    width = int(cell[:, 1].max() - base[1]) + 1
+    bid = (cell[:, 0] - base[0]) * width + (cell[:, 1] - base[1])
+    order = np.argsort(bid, kind="stable")
+    sbid = bid[order]
+    c0 = np.floor(lo_box / h).astype(np.int64) - base
+    c1 = np.floor(hi_box / h).astype(np.int64) - base
+    c0[:, 1] = np.maximum(c0[:, 1], 0)
+    c1[:, 1] = np.minimum(c1[:, 1], width - 1)
+    nx = np.maximum(c1[:, 0] - c0[:, 0] + 1, 0)
+    ny = np.maximum(c1[:, 1] - c0[:, 1] + 1, 0)
+    cnt = nx * ny
+    edge = np.repeat(rows, cnt)
+    off = np.arange(int(cnt.sum())) - np.repeat(np.cumsum(cnt) - cnt, cnt)
+    bx = c0[edge, 0] + off // np.maximum(ny[edge], 1)
+    by = c0[edge, 1] + off % np.maximum(ny[edge], 1)
+    q = bx * width + by
+    s0 = np.searchsorted(sbid, q, "left")
+    s1 = np.searchsorted(sbid, q, "right")
+    m = s1 - s0
+    pe = np.repeat(edge, m)
+    pp = order[np.repeat(s0, m) + (np.arange(int(m.sum())) - np.repeat(np.cumsum(m) - m, m))]
+    inside = np.all((P[pp] >= lo_box[pe]) & (P[pp] <= hi_box[pe]), axis=1) & (pp != ia[pe]) & (pp != ib[pe])
+    found = np.zeros(n_e, dtype=bool)
+    found[pe[inside]] = True
+    return found
+
+
+def _points_on_edge(a, b, known, rank, point_grid):
+    """The distinct points on segment ab in order along it: candidates within
+    TOL of its range along its main axis, taken in that axis's sorted order,
+    kept by `on_segment`, then sorted by their parameter."""
+    axis = 0 if abs(b[0] - a[0]) >= abs(b[1] - a[1]) else 1
+    lo = min(a[axis], b[axis]) - TOL
+    hi = max(a[axis], b[axis]) + TOL
+    x0, x1, y0, y1 = _segment_box(a, b)
+    if axis == 0:
+        x0, x1 = lo, hi
+    else:
+        y0, y1 = lo, hi
+    cand = [i for i in point_grid.query(x0, x1, y0, y1) if lo <= known[i][axis] <= hi]
+    cand.sort(key=lambda i: rank[axis][i])
+    on_edge = [known[i] for i in cand if on_segment(known[i], a, b)]
+    on_edge.sort(key=lambda p: segment_parameter(p, a, b))
+    return on_edge
+
+
 def rebuild_mesh(old: FiniteVolumeMesh, leaves, records) -> FiniteVolumeMesh:
     """Faces, owners and neighbours of the leaves (reference `_rebuild_mesh`,
     adaptation.py:1032-1190): each leaf edge is cut at every mesh point on
     it, faces are numbered at first visit (owner = first visiting leaf), new
     boundary faces take the patch of the old boundary edge under their
-    midpoint or, on a curve, the curve's patch and sub-range."""
+    midpoint or, on a curve, the curve's patch and sub-range.
+
+    The edges whose search box holds no point but their own endpoints (all
+    but those next to a hanging node) are cut nowhere and are resolved in
+    NumPy; the others go through `_points_on_edge`. Points and faces are
+    numbered at first visit over the same sequence of sub-edges."""
+    sizes_in = [len(leaf.vertices) for leaf in leaves]
+    flat = [p for leaf in leaves for p in leaf.vertices]
+    memo = {}  # a vertex shared by several leaves is one tuple, keyed once
+    keys = [memo[p] if p in memo else memo.setdefault(p, point_key(p)) for p in flat]
     distinct = {}
-    for leaf in leaves:
-        for p in leaf.vertices:
-            distinct.setdefault(point_key(p), p)
-    known = tuple(distinct.values())
-    # rank of each point in the x- and y-sorted orders (stable sorts), which
-    # order the candidates on an edge before they are sorted along it
-    order_x = sorted(range(len(known)), key=lambda i: known[i][0])
-    order_y = sorted(range(len(known)), key=lambda i: known[i][1])
-    rank = ([0] * len(known), [0] * len(known))
-    for r, i in enumerate(order_x):
-        rank[0][i] = r
-    for r, i in enumerate(order_y):
-        rank[1][i] = r
-    # bucket size: the median leaf edge length
-    lengths = sorted(math.hypot(leaf.vertices[(k + 1) % len(leaf.vertices)][0] - leaf.vertices[k][0],
-                                leaf.vertices[(k + 1) % len(leaf.vertices)][1] - leaf.vertices[k][1])
-                     for leaf in leaves for k in range(len(leaf.vertices)))
-    h = max(lengths[len(lengths) // 2] if lengths else 1.0, 1e-9)
-    point_grid = _Grid([(p[0], p[0], p[1], p[1]) for p in known], h)
+    vid = np.empty(len(flat), dtype=np.int64)
+    for v, key in enumerate(keys):
+        vid[v] = distinct.setdefault(key, len(distinct))
+    first_of = {}
+    for v, key in enumerate(keys):
+        first_of.setdefault(key, v)
+    known = tuple(flat[first_of[key]] for key in distinct)
+    key_index = distinct
+    P = np.asarray(known, dtype=np.float64).reshape(-1, 2)
+    V = np.asarray(flat, dtype=np.float64).reshape(-1, 2)
+    sizes_arr = np.asarray(sizes_in, dtype=np.int64)
+    starts = np.cumsum(sizes_arr) - sizes_arr
+    leaf_of = np.repeat(np.arange(len(leaves)), sizes_arr)
+    va = np.arange(len(flat))
+    vb = va + 1
+    last = starts + sizes_arr - 1
+    vb[last] = starts
+    ea, eb = V[va], V[vb]
+    lengths = np.hypot(eb[:, 0] - ea[:, 0], eb[:, 1] - ea[:, 1])
+    h = max(float(np.sort(lengths)[len(lengths) // 2]) if len(lengths) else 1.0, 1e-9)
+    ia, ib = vid[va], vid[vb]
+    # an edge needs the exact search when another point may lie on it, when
+    # an endpoint's distinct point is not the endpoint itself, or when it is
+    # degenerate
+    exact = _edges_with_other_points(P, V, ia, ib, ea, eb, h)
+    exact |= np.any(P[ia] != ea, axis=1) | np.any(P[ib] != eb, axis=1)
+    exact |= ((ea[:, 0] - eb[:, 0]) ** 2 + (ea[:, 1] - eb[:, 1]) ** 2) <= 1e-26
+    # the sub-edges in visit order: (leaf edge, first point, second point)
+    sub_edge, sub_i, sub_j = [va], [ia], [ib]
+    slow = np.flatnonzero(exact)
+    if len(slow):
+        order_x = sorted(range(len(known)), key=lambda i: known[i][0])
+        order_y = sorted(range(len(known)), key=lambda i: known[i][1])
+        rank = ([0] * len(known), [0] * len(known))
+        for r, i in enumerate(order_x):
+            rank[0][i] = r
+        for r, i in enumerate(order_y):
+            rank[1][i] = r
+        point_grid = _Grid([(p[0], p[0], p[1], p[1]) for p in known], h)
+        se, si, sj = [], [], []
+        for v in slow.tolist():
+            a, b = flat[v], flat[vb[v]]
+            on_edge = _points_on_edge(a, b, known, rank, point_grid)
+            for p, q in zip(on_edge, on_edge[1:]):
+                if dist2(p, q) <= 1e-26:
+                    continue
+                se.append(v)
+                si.append(key_index[point_key(p)])
+                sj.append(key_index[point_key(q)])
+        keep = ~exact
+        sub_edge = np.concatenate((va[keep], np.asarray(se, dtype=np.int64)))
+        sub_i = np.concatenate((ia[keep], np.asarray(si, dtype=np.int64)))
+        sub_j = np.concatenate((ib[keep], np.asarray(sj, dtype=np.int64)))
+        # visit order: by leaf edge, then along it (stable)
+        o = np.argsort(sub_edge, kind="stable")
+        sub_edge, sub_i, sub_j = sub_edge[o], sub_i[o], sub_j[o]
+    else:
+        sub_edge, sub_i, sub_j = va, ia, ib
+    n_sub = len(sub_edge)
+    # points numbered at first visit over (i0, j0, i1, j1, ...)
+    seq = np.column_stack((sub_i, sub_j)).ravel()
+    uniq, first = np.unique(seq, return_index=True)
+    pid_of_known = np.full(len(known), -1, dtype=np.int64)
+    by_first = np.argsort(first, kind="stable")
+    pid_of_known[uniq[by_first]] = np.arange(len(uniq))
+    points = P[uniq[by_first]]
+    I, J = pid_of_known[sub_i], pid_of_known[sub_j]
+    # faces numbered at first visit; owner the first visitor, neighbour the second
+    lo_p, hi_p = np.minimum(I, J), np.maximum(I, J)
+    fkey = lo_p * (len(points) + 1) + hi_p
+    ukey, ffirst, finv, fcount = np.unique(fkey, return_index=True, return_inverse=True, return_counts=True)
+    if np.any(fcount > 2):
+        raise ValueError("Adaptation produced a non-manifold face.")
+    forder = np.argsort(ffirst, kind="stable")
+    face_id = np.empty(len(ukey), dtype=np.int64)
+    face_id[forder] = np.arange(len(ukey))
+    fid = face_id[finv]
+    n_faces = len(ukey)
+    first_visit = ffirst[forder]  # sub-edge index of each face's creation
+    faces = np.column_stack((I[first_visit], J[first_visit]))
+    sub_leaf = leaf_of[sub_edge]
+    owners = sub_leaf[first_visit]
+    neighbours = np.full(n_faces, -1, dtype=np.int64)
+    second = np.ones(n_sub, dtype=bool)
+    second[first_visit] = False
+    neighbours[fid[second]] = sub_leaf[second]
+    signs = np.where(second & ~((I == faces[fid, 0]) & (J == faces[fid, 1])), -1, 1)
+    # boundary faces: patch (and curve) from the creating sub-edge
     boundary_edges = []
     for f in np.flatnonzero(old.face_neighbours < 0):
         a, b = old.faces[f]
         boundary_edges.append(((float(old.points[a, 0]), float(old.points[a, 1])),
                                (float(old.points[b, 0]), float(old.points[b, 1])), int(old.face_patch_ids[f])))
-    edge_grid = _Grid([_segment_box(p, q) for p, q, _ in boundary_edges], h)
-    straight = []  # new faces off curves: their patch is looked up once the leaves are done
-    points, pid = [], {}
-    faces, owners, neighbours, patches_of, curves_of = [], [], [], [], []
-    cell_faces, cell_signs, cell_nodes = [], [], []
-    face_of_edge = {}
-
-    def point_id(p):
-        key = point_key(p)
-        if key not in pid:
-            pid[key] = len(points)
-            points.append(p)
-        return pid[key]
-
-    for c, leaf in enumerate(leaves):
-        fids, signs, loop = [], [], []
+    edge_grid = None
+    patches_of = np.full(n_faces, -1, dtype=np.int64)
+    curves_of = [None] * n_faces
+    for f in np.flatnonzero(neighbours < 0).tolist():
+        e_sub = int(first_visit[f])
+        v = int(sub_edge[e_sub])
+        c = int(leaf_of[v])
+        leaf = leaves[c]
+        k = v - int(starts[c])
         n = len(leaf.vertices)
-        lc = leaf.curves if len(leaf.curves) == n else (None,) * n
-        for k in range(n):
-            a, b = leaf.vertices[k], leaf.vertices[(k + 1) % n]
-            curve = lc[k]
-            axis = 0 if abs(b[0] - a[0]) >= abs(b[1] - a[1]) else 1
-            # points within TOL of the edge's range along its main axis (the
-            # candidates in that axis's sorted order), filtered by on_segment
-            lo = min(a[axis], b[axis]) - TOL
-            hi = max(a[axis], b[axis]) + TOL
-            x0, x1, y0, y1 = _segment_box(a, b)
-            if axis == 0:
-                x0, x1 = lo, hi
-            else:
-                y0, y1 = lo, hi
-            cand = [i for i in point_grid.query(x0, x1, y0, y1) if lo <= known[i][axis] <= hi]
-            cand.sort(key=lambda i: rank[axis][i])
-            on_edge = [known[i] for i in cand if on_segment(known[i], a, b)]
-            on_edge.sort(key=lambda p: segment_parameter(p, a, b))
-            for p, q in zip(on_edge, on_edge[1:]):
-                if dist2(p, q) <= 1e-26:
-                    continue
-                i, j = point_id(p), point_id(q)
-                loop.append(i)
-                key = (min(i, j), max(i, j))
-                f = face_of_edge.get(key)
-                if f is None:
-                    f = len(faces)
-                    face_of_edge[key] = f
-                    faces.append((i, j))
-                    owners.append(c)
-                    neighbours.append(-1)
-                    if curve is None:
-                        patches_of.append(-1)
-                        straight.append((f, p, q))
-                        curves_of.append(None)
-                    else:
-                        t0 = segment_parameter(p, a, b)
-                        t1 = segment_parameter(q, a, b)
-                        span = curve.end_parameter - curve.start_parameter
-                        patches_of.append(curve.patch_id)
-                        curves_of.append(BoundaryCurve(curve.geometry, curve.start_parameter + t0 * span,
-                                                       curve.start_parameter + t1 * span, curve.patch_id))
-                    sign = 1
-                else:
-                    if neighbours[f] >= 0:
-                        raise ValueError("Adaptation produced a non-manifold face.")
-                    neighbours[f] = c
-                    patches_of[f] = -1
-                    curves_of[f] = None
-                    sign = 1 if faces[f] == (i, j) else -1
-                fids.append(f)
-                signs.append(sign)
-        cell_faces.append(fids)
-        cell_signs.append(signs)
-        cell_nodes.append(loop)
-    for f, p, q in straight:
-        if neighbours[f] < 0:
+        curve = leaf.curves[k] if len(leaf.curves) == n else None
+        p, q = known[int(sub_i[e_sub])], known[int(sub_j[e_sub])]
+        if curve is None:
+            if edge_grid is None:
+                edge_grid = _Grid([_segment_box(a, b) for a, b, _ in boundary_edges], h)
             patches_of[f] = _boundary_patch(p, q, boundary_edges, edge_grid)
-    width = max(len(x) for x in cell_nodes)
-    cells = np.full((len(cell_nodes), width), -1, dtype=np.int64)
-    sizes = np.array([len(x) for x in cell_nodes], dtype=np.int64)
-    cf = np.full((len(cell_faces), width), -1, dtype=np.int64)
-    cs = np.zeros((len(cell_faces), width), dtype=np.int64)
-    for c, (loop, fids, sg) in enumerate(zip(cell_nodes, cell_faces, cell_signs)):
-        cells[c, : len(loop)] = loop
-        cf[c, : len(fids)] = fids
-        cs[c, : len(sg)] = sg
-    patch_arr = np.asarray(patches_of, dtype=np.int64)
-    patches = tuple(MeshPatch(p.id, p.name, np.flatnonzero(patch_arr == p.id)) for p in old.patches)
+        else:
+            a, b = leaf.vertices[k], leaf.vertices[(k + 1) % n]
+            t0 = segment_parameter(p, a, b)
+            t1 = segment_parameter(q, a, b)
+            span = curve.end_parameter - curve.start_parameter
+            patches_of[f] = curve.patch_id
+            curves_of[f] = BoundaryCurve(curve.geometry, curve.start_parameter + t0 * span,
+                                         curve.start_parameter + t1 * span, curve.patch_id)
+    # per leaf: its sub-edges' faces, signs and first points, in visit order
+    per_leaf = np.bincount(sub_leaf, minlength=len(leaves))
+    width = int(per_leaf.max()) if len(per_leaf) else 0
+    slot = np.arange(n_sub) - np.repeat(np.cumsum(per_leaf) - per_leaf, per_leaf)
+    cells = np.full((len(leaves), width), -1, dtype=np.int64)
+    cf = np.full((len(leaves), width), -1, dtype=np.int64)
+    cs = np.zeros((len(leaves), width), dtype=np.int64)
+    cells[sub_leaf, slot] = I
+    cf[sub_leaf, slot] = fid
+    cs[sub_leaf, slot] = signs
+    patches = tuple(MeshPatch(pt.id, pt.name, np.flatnonzero(patches_of == pt.id)) for pt in old.patches)
     return FiniteVolumeMesh(
         points=np.asarray(points, dtype=np.float64).reshape(-1, 2),
         cells=cells,
-        cell_sizes=sizes,
-        faces=np.asarray(faces, dtype=np.int64).reshape(-1, 2),
-        face_owners=np.asarray(owners, dtype=np.int64),
-        face_neighbours=np.asarray(neighbours, dtype=np.int64),
-        face_patch_ids=patch_arr,
+        cell_sizes=per_leaf.astype(np.int64),
+        faces=faces.astype(np.int64).reshape(-1, 2),
+        face_owners=owners.astype(np.int64),
+        face_neighbours=neighbours,
+        face_patch_ids=patches_of,
         patches=patches,
         cell_faces=cf,
         cell_face_signs=cs,
