@@ -1811,7 +1811,7 @@ __device__ __forceinline__ void lds_copy(const void* src, void* dst, int n) {
 // P2 adds each face's viscous flux (k_visc_flux's) and P3 the viscous gather
 // (k_update's), so a first-order viscous residual is one launch.
 template <int SCHEME, bool MUSCL, int LIM, bool FARFIELD, bool DTNEXT, bool VISC = false>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
     k_stage_fused(int max_acc, int stage_off, int max_local, int max_grad, int max_faces, int max_lsq, int max_lim,
                   BlockArrays ba,
                   GeomArrays ga, FaceArrays fa, DtArrays da, double g, double ax, double ay,
