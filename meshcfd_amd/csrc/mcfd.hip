@@ -481,11 +481,11 @@ __global__ void __launch_bounds__(kBlock)
   double phi[4] = {1.0, 1.0, 1.0, 1.0};
   if (pfull[i]) {
     double c[5][4] = {{0}};
-    const int p0 = pptr[i];
-    for (int e = p0; e < pptr[i + 1]; ++e) {
-      P4 o = load4(W, pnb[e]);
+    const int p0 = pptr[i], p1 = pptr[i + 1];
+    // entries in pairs: both entries' loads are issued before either is
+    // summed (sums in entry order)
+    auto entry = [&](int e, const P4& o, const double* pe) {
       double ov[4] = {o.r, o.u, o.v, o.p};
-      const double* pe = pcoef + 5 * (long)e;
       for (int k = 0; k < 4; ++k) {
         double d = ov[k] - wv[k];
         for (int a = 0; a < 5; ++a) c[a][k] = c[a][k] + pe[a] * d;
@@ -498,7 +498,20 @@ __global__ void __launch_bounds__(kBlock)
           if (ARGS) arg[k] = (arg[k] & 0xFF00FFu) | ((unsigned)(e - p0) << 8);
         }
       }
+    };
+    int e = p0;
+    for (; e + 1 < p1; e += 2) {
+      const int n0 = pnb[e], n1 = pnb[e + 1];
+      const P4 o0 = load4(W, n0), o1 = load4(W, n1);
+      double pe0[5], pe1[5];
+      for (int a = 0; a < 5; ++a) {
+        pe0[a] = pcoef[5 * (long)e + a];
+        pe1[a] = pcoef[5 * (long)(e + 1) + a];
+      }
+      entry(e, o0, pe0);
+      entry(e + 1, o1, pe1);
     }
+    for (; e < p1; ++e) entry(e, load4(W, pnb[e]), pcoef + 5 * (long)e);
     for (int k = 0; k < 4; ++k) {
       gx[k] = c[0][k];
       gy[k] = c[1][k];
