@@ -357,3 +357,149 @@ def test_bump_channel_indicator_refines_near_the_bump_and_stays_balanced():
     assert float(g.areas.sum()) - 1e-3 < area < float(g.areas.sum())
     curves = [c for c in refined.face_curves if c is not None]
     assert len(curves) > 4  # the bump's four faces were split
+
+
+# -- the incremental balance plan against full rescans -----------------------
+
+def _balance_by_full_rescans(marks, coarsen, neighbours, cell_ids, levels, levels_dir, parent_dir,
+                             maximum_level, records, periodic_pairs):
+    """The balance loop as the reference writes it (adaptation.py:252-451):
+    every round recomputes all projections and rescans every face from the
+    first cell, fixing the first violation; a plain restatement for
+    checking `ad.balance_plan`, which updates projections in place and
+    resumes its scan."""
+    by_parent = {r.parent_id: r for r in records}
+    parent_of = {child: r.parent_id for r in records for child in r.child_ids}
+    seen = set()
+
+    def projected():
+        lev, dirs = [], []
+        for k, cid in enumerate(cell_ids):
+            parent = parent_of.get(cid)
+            if parent in coarsen:
+                lev.append(by_parent[parent].level)
+                dirs.append(parent_dir[parent])
+            else:
+                lev.append(levels[k] + int(k in marks))
+                m = marks.get(k, "")
+                dirs.append((levels_dir[k][0] + int("x" in m), levels_dir[k][1] + int("y" in m)))
+        return lev, dirs
+
+    def fix(lower, higher, axis, proj_dir):
+        parent = parent_of.get(cell_ids[lower])
+        if parent in coarsen:
+            coarsen.remove(parent)
+            return
+        if lower in marks:
+            if axis is None or axis in marks[lower]:
+                if higher in marks:
+                    del marks[higher]
+                    return
+                raise ad.AdaptationBalanceError("one step")
+            both = set(marks[lower]) | {axis}
+            marks[lower] = "xy" if len(both) == 2 else next(iter(both))
+            return
+        if levels[lower] < maximum_level:
+            marks[lower] = (ad.AXES[ad._least_disruptive_axis(lower, neighbours, proj_dir, levels_dir)]
+                            if axis is None else axis)
+            return
+        if higher in marks:
+            del marks[higher]
+            return
+        raise ad.AdaptationBalanceError("maximum level")
+
+    changed = True
+    while changed:
+        changed = False
+        signature = (frozenset(marks.items()), frozenset(coarsen))
+        if signature in seen:
+            raise ad.AdaptationBalanceError("cycle")
+        seen.add(signature)
+        for a, b, _ in periodic_pairs:
+            pa, pb = parent_of.get(cell_ids[a]), parent_of.get(cell_ids[b])
+            if pa in coarsen and pb not in coarsen:
+                coarsen.remove(pa)
+                changed = True
+            if pb in coarsen and pa not in coarsen:
+                coarsen.remove(pb)
+                changed = True
+        for a, b, _ in periodic_pairs:
+            if a in marks and b not in marks:
+                if levels[b] >= maximum_level:
+                    del marks[a]
+                else:
+                    marks[b] = marks[a]
+                changed = True
+            elif b in marks and a not in marks:
+                if levels[a] >= maximum_level:
+                    del marks[b]
+                else:
+                    marks[a] = marks[b]
+                changed = True
+            elif a in marks and b in marks and marks[a] != marks[b]:
+                if marks[a] != "xy" or marks[b] != "xy":
+                    marks[a] = marks[b] = "xy"
+                    changed = True
+        lev, dirs = projected()
+        for k in range(len(cell_ids)):
+            for nb in neighbours[k]:
+                if nb <= k:
+                    continue
+                d = lev[k] - lev[nb]
+                if abs(d) > 1:
+                    fix(k if d < 0 else nb, nb if d < 0 else k, None, dirs)
+                    changed = True
+                    break
+                for ai, axis in enumerate(ad.AXES):
+                    d = dirs[k][ai] - dirs[nb][ai]
+                    if abs(d) > 1:
+                        fix(k if d < 0 else nb, nb if d < 0 else k, axis, dirs)
+                        changed = True
+                        break
+                if changed:
+                    break
+            if changed:
+                break
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_incremental_balance_plan_equals_full_rescans(seed):
+    """On a twice-adapted bump channel (cells at levels 0-2, hanging nodes)
+    random refinement marks and coarsening sets are balanced by
+    `ad.balance_plan` and by full rescans: the same marks and coarsening
+    sets, or the same error type (12 seeds: 1-9 marks changed and 2-15
+    coarsening parents dropped per plan by the fixes)."""
+    from meshcfd_amd import meshgen
+    from meshcfd_amd.geometry import build_geometry
+
+    case, prim = meshgen.bump_case(24, 8, 0.08)
+    for _ in range(2):
+        g = build_geometry(case)
+        x = g.centers[:, 0]
+        p = np.tile(prim[:1], (case.mesh.cell_count, 1)).astype(float)
+        p[:, 0] *= 1.0 + 0.3 * np.tanh((x - 0.5 * x.max()) / 0.05)
+        values = ad.indicator_values(p, case.numerics.adaptation_variable)
+        marks, coarsen = ad.select_adaptation(case.mesh, case, g.centers, g.areas, values, 0.02, 3)
+        case.mesh, _ = ad.adapt_mesh(case.mesh, marks, coarsen)
+    mesh = case.mesh
+    cell_ids = ad._cell_ids(mesh)
+    levels = ad._cell_levels(mesh)
+    nbrs = ad._neighbour_sets(mesh, ())
+    levels_dir = ad.cell_directional_levels(mesh, cell_ids, mesh.refinement_records)
+    parent_dir = ad.parent_directional_levels(mesh.refinement_records)
+    rng = np.random.default_rng(seed)
+    n = mesh.cell_count
+    marks = {int(k): str(rng.choice(["x", "y", "xy"]))
+             for k in rng.choice(n, size=int(rng.integers(1, n // 3)), replace=False)}
+    parents = sorted(r.parent_id for r in mesh.refinement_records)
+    coarsen = {p for p in parents if rng.random() < 0.3}
+    maximum = int(rng.integers(1, 4))
+    results = []
+    for balance in (ad.balance_plan, _balance_by_full_rescans):
+        m, c = dict(marks), set(coarsen)
+        try:
+            balance(m, c, nbrs, cell_ids, levels, levels_dir, parent_dir, maximum, mesh.refinement_records, [])
+            results.append((m, c))
+        except ad.AdaptationBalanceError as error:
+            results.append(type(error))
+    assert results[0] == results[1]
