@@ -503,3 +503,128 @@ def test_incremental_balance_plan_equals_full_rescans(seed):
         except ad.AdaptationBalanceError as error:
             results.append(type(error))
     assert results[0] == results[1]
+
+
+# -- the vectorised rebuild against the per-edge loop ------------------------
+
+def _rebuild_by_full_scans(old, leaves):
+    """The reference's `_rebuild_mesh` loop (adaptation.py:1032-1190) as a
+    plain restatement: every leaf edge takes the known points within TOL of
+    its range along its main axis (in that axis's sorted order) that lie on
+    it, sorted along it; points and faces numbered at first visit; a new
+    boundary face takes the first old boundary edge under its midpoint, or
+    its curve. For checking `ad.rebuild_mesh`."""
+    distinct = {}
+    for leaf in leaves:
+        for p in leaf.vertices:
+            distinct.setdefault(ad.point_key(p), p)
+    known = tuple(distinct.values())
+    edges = []
+    for f in np.flatnonzero(old.face_neighbours < 0):
+        a, b = old.faces[f]
+        edges.append(((float(old.points[a, 0]), float(old.points[a, 1])),
+                      (float(old.points[b, 0]), float(old.points[b, 1])), int(old.face_patch_ids[f])))
+    points, pid, faces, owners, neighbours, patches, curves = [], {}, [], [], [], [], []
+    cell_faces, cell_signs, cell_nodes, face_of_edge = [], [], [], {}
+
+    def point_id(p):
+        key = ad.point_key(p)
+        if key not in pid:
+            pid[key] = len(points)
+            points.append(p)
+        return pid[key]
+
+    for c, leaf in enumerate(leaves):
+        fids, signs, loop = [], [], []
+        n = len(leaf.vertices)
+        lc = leaf.curves if len(leaf.curves) == n else (None,) * n
+        for k in range(n):
+            a, b = leaf.vertices[k], leaf.vertices[(k + 1) % n]
+            axis = 0 if abs(b[0] - a[0]) >= abs(b[1] - a[1]) else 1
+            lo, hi = min(a[axis], b[axis]) - ad.TOL, max(a[axis], b[axis]) + ad.TOL
+            on_edge = [p for p in sorted(known, key=lambda p: p[axis])
+                       if lo <= p[axis] <= hi and ad.on_segment(p, a, b)]
+            on_edge.sort(key=lambda p: ad.segment_parameter(p, a, b))
+            for p, q in zip(on_edge, on_edge[1:]):
+                if ad.dist2(p, q) <= 1e-26:
+                    continue
+                i, j = point_id(p), point_id(q)
+                loop.append(i)
+                key = (min(i, j), max(i, j))
+                f = face_of_edge.get(key)
+                if f is None:
+                    f = face_of_edge[key] = len(faces)
+                    faces.append((i, j))
+                    owners.append(c)
+                    neighbours.append(-1)
+                    if lc[k] is None:
+                        mid = (0.5 * (p[0] + q[0]), 0.5 * (p[1] + q[1]))
+                        patches.append(next((e[2] for e in edges if ad.on_segment(mid, e[0], e[1])), -1))
+                        curves.append(None)
+                    else:
+                        cv = lc[k]
+                        t0, t1 = ad.segment_parameter(p, a, b), ad.segment_parameter(q, a, b)
+                        span = cv.end_parameter - cv.start_parameter
+                        patches.append(cv.patch_id)
+                        curves.append(BoundaryCurve(cv.geometry, cv.start_parameter + t0 * span,
+                                                    cv.start_parameter + t1 * span, cv.patch_id))
+                    sign = 1
+                else:
+                    neighbours[f] = c
+                    patches[f] = -1
+                    curves[f] = None
+                    sign = 1 if faces[f] == (i, j) else -1
+                fids.append(f)
+                signs.append(sign)
+        cell_faces.append(fids)
+        cell_signs.append(signs)
+        cell_nodes.append(loop)
+    width = max(len(x) for x in cell_nodes)
+    pad = lambda rows, fill: np.array([r + [fill] * (width - len(r)) for r in rows], dtype=np.int64)
+    return {"points": np.asarray(points, dtype=float), "cells": pad(cell_nodes, -1),
+            "cell_sizes": np.array([len(x) for x in cell_nodes]), "faces": np.asarray(faces, dtype=np.int64),
+            "face_owners": np.asarray(owners), "face_neighbours": np.asarray(neighbours),
+            "face_patch_ids": np.asarray(patches), "cell_faces": pad(cell_faces, -1),
+            "cell_face_signs": pad(cell_signs, 0),
+            "face_curves": repr(tuple(curves) if any(c is not None for c in curves) else ())}
+
+
+@pytest.mark.parametrize("which", ["bump", "curved periodic"])
+def test_vectorised_rebuild_equals_the_per_edge_loop(which, monkeypatch):
+    """Every mesh `ad.adapt_mesh` rebuilds equals the per-edge loop's, array
+    for array: a bump channel refined twice (768 -> 848 -> 960 cells, curved
+    faces on the bump) then coarsened and refined at once (128 parents
+    merged, 80 cells split), and the reference's curved periodic mesh split
+    in x, y and both."""
+    from meshcfd_amd import meshgen
+    from meshcfd_amd.geometry import build_geometry
+
+    seen = []
+    real = ad.rebuild_mesh
+
+    def spy(old, leaves, records):
+        out = real(old, leaves, records)
+        seen.append((out, _rebuild_by_full_scans(old, leaves)))
+        return out
+
+    monkeypatch.setattr(ad, "rebuild_mesh", spy)
+    if which == "bump":
+        case, prim = meshgen.bump_case(48, 16, 0.08)
+        for rnd in range(3):
+            g = build_geometry(case)
+            x = g.centers[:, 0]
+            p = np.tile(prim[:1], (case.mesh.cell_count, 1)).astype(float)
+            front = 0.5 * x.max() if rnd < 2 else 0.8 * x.max()  # the third round coarsens
+            p[:, 0] *= 1.0 + 0.3 * np.tanh((x - front) / 0.05)
+            values = ad.indicator_values(p, case.numerics.adaptation_variable)
+            marks, coarsen = ad.select_adaptation(case.mesh, case, g.centers, g.areas, values, 0.02, 3)
+            case.mesh, _ = ad.adapt_mesh(case.mesh, marks, coarsen)
+    else:
+        mesh = curved_periodic_mesh()
+        for axis in ("x", "y", "xy"):
+            ad.adapt_mesh(mesh, {c: axis for c in range(4)}, set())
+    assert seen
+    for out, want in seen:
+        for name, value in want.items():
+            got = repr(out.face_curves) if name == "face_curves" else getattr(out, name)
+            assert (got == value) if name == "face_curves" else np.array_equal(np.asarray(got), value), name
