@@ -1823,9 +1823,12 @@ __device__ __forceinline__ void lds_copy(const void* src, void* dst, int n) {
 // gradients of own and ring-1 cells (k_visc_gradient's arithmetic, from LDS),
 // P2 adds each face's viscous flux (k_visc_flux's) and P3 the viscous gather
 // (k_update's), so a first-order viscous residual is one launch.
+// Waves per SIMD: the MUSCL stages without the next dt target 6 (80 VGPRs, 2
+// spilled; 26 KB of LDS at 128 cells), the final stage 5 (at 6 it spills 16
+// VGPRs), the viscous stages 5 (96 VGPRs).
 template <int SCHEME, bool MUSCL, int LIM, bool FARFIELD, bool DTNEXT, bool VISC = false>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
-    k_stage_fused(int max_acc, int stage_off, int max_local, int max_grad, int max_faces, int max_lsq, int max_lim,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MUSCL && !DTNEXT ? 6 : 5, 8)))
+    k_stage_fused(int max_acc, int stage_off, int max_local, int max_grad, int max_faces, int max_lsq,
                   BlockArrays ba,
                   GeomArrays ga, FaceArrays fa, DtArrays da, double g, double ax, double ay,
                   StageArgs s, int* __restrict__ status, ViscArrays va, ViscParams vp, ViscAcc vacc) {
@@ -1881,16 +1884,16 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 
     l0 = ba.lptr[gps + t];
     l1 = ba.lptr[gps + t + 1];
   }
-  // The block's LSQ coefficients, limiter offsets and LSQ neighbour ids go
-  // straight to LDS while P0 runs, so P1 reads only LDS.
+  // The block's LSQ coefficients and LSQ neighbour ids go straight to LDS
+  // while P0 runs, so the gradient loop reads only LDS. The limiter offsets
+  // are read from HBM by the limiter loop: with them in LDS the block needs
+  // 31 KB at 128 cells and a CU holds 5 workgroups, without them 26 KB and 6.
   const double2* sGE = reinterpret_cast<const double2*>(sG);  // [max_lsq]
-  const double2* sLE = sGE + max_lsq;                          // [max_lim]
-  const unsigned short* sGN = reinterpret_cast<const unsigned short*>(sLE + max_lim) + (gs & 1);
+  const unsigned short* sGN = reinterpret_cast<const unsigned short*>(sGE + max_lsq) + (gs & 1);
   if constexpr (MUSCL) {
-    const int n_lsq = inf[10], n_lim = inf[11];
+    const int n_lsq = inf[10];
     lds_copy<16>(ba.gent + gs, const_cast<double2*>(sGE), n_lsq);
-    lds_copy<16>(ba.lent + ls, const_cast<double2*>(sLE), n_lim);
-    lds_copy<4>(ba.gnb + (gs & ~1), const_cast<double2*>(sLE + max_lim), ((gs & 1) + n_lsq + 1) >> 1);
+    lds_copy<4>(ba.gnb + (gs & ~1), const_cast<double2*>(sGE + max_lsq), ((gs & 1) + n_lsq + 1) >> 1);
   }
 
   // P0: primitive states (euler.py:20-40)
@@ -1972,7 +1975,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 
         dmn[k] = mn[k] - wv[k];
         tol[k] = 1e-14 * vmax(fabs(wv[k]), 1.0);
       }
-      const double2* lxy = sLE;
+      const double2* __restrict__ lxy = ba.lent + ls;
       for (int base = l0; base < l1; base += 4) {
         double dxb[4], dyb[4];
 #pragma unroll
@@ -5217,7 +5220,7 @@ struct McfdMesh {
   bool fused = false;
   int nb = 0, block_cells = 0, max_local = 0, max_grad = 0, max_faces = 0;
   int max_acc = 0, max_dts = 0;  // most accumulation / spectral-radius items of a block's own cells
-  int max_lsq = 0, max_lim = 0;  // most LSQ / limiter entries of a block's own + ring-1 cells
+  int max_lsq = 0;  // most LSQ entries of a block's own + ring-1 cells
   double* block_partial = nullptr;  // [nb] per-block minima of the fused final stage
   bool fused_visc = false;  // per-block viscous arrays uploaded (mcfd_mesh_set_block_viscous)
   long block_n_gnb = 0;     // LSQ entries of the block layout
@@ -5341,10 +5344,10 @@ int upload_block(McfdMesh* m, T** dst, const T* src, size_t count) {
 // then the own cells' accumulation coefficients and items and their
 // spectral-radius items, staged for the gather.
 // The staging is written after P2, over W when it fits there.
-// LSQ coefficients, limiter offsets and (dword-aligned, 16-bit) LSQ
-// neighbour ids of a block, copied into the region at kernel start.
+// LSQ coefficients and (dword-aligned, 16-bit) LSQ neighbour ids of a
+// block, copied into the region at kernel start.
 size_t fused_entry_doubles(const McfdMesh* m) {
-  return 2 * (size_t)m->max_lsq + 2 * (size_t)m->max_lim + ((size_t)m->max_lsq + 2 + 3) / 4;
+  return 2 * (size_t)m->max_lsq + ((size_t)m->max_lsq + 2 + 3) / 4;
 }
 
 size_t fused_region(const McfdMesh* m, bool muscl, bool visc = false) {
@@ -5551,7 +5554,7 @@ void launch_fused_t(McfdMesh* m, size_t lds, hipStream_t s, const GeomArrays& ga
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                             hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024);
   hipLaunchKernelGGL(kern, dim3(m->nb), dim3(kBlock), lds, s, m->max_acc, fused_stage_offset(m, MUSCL), m->max_local,
-                     m->max_grad, m->max_faces, m->max_lsq, m->max_lim, m->blocks, ga, m->faces, m->dts, st->gamma,
+                     m->max_grad, m->max_faces, m->max_lsq, m->blocks, ga, m->faces, m->dts, st->gamma,
                      st->acceleration_x, st->acceleration_y, a, m->status, ViscArrays{}, ViscParams{}, ViscAcc{});
 }
 
@@ -5586,7 +5589,7 @@ void launch_fused_visc_t(McfdMesh* m, size_t lds, hipStream_t s, const GeomArray
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                             hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024);
   hipLaunchKernelGGL(kern, dim3(m->nb), dim3(kBlock), lds, s, m->max_acc, fused_stage_offset(m, false, true),
-                     m->max_local, m->max_grad, m->max_faces, m->max_lsq, m->max_lim, m->blocks, ga, m->faces,
+                     m->max_local, m->max_grad, m->max_faces, m->max_lsq, m->blocks, ga, m->faces,
                      m->dts, st->gamma, st->acceleration_x, st->acceleration_y, a, m->status, m->visc,
                      visc_params(st), m->vacc);
 }
@@ -6414,13 +6417,11 @@ int mcfd_mesh_set_blocks(McfdMesh* m, const McfdBlockDesc* d) {
     }
     // entries per block: the block-relative offsets after its last gradient cell
     m->max_lsq = 0;
-    m->max_lim = 0;
     for (int b = 0; b < d->n_blocks; ++b) {
       const int32_t* inf = d->info + (size_t)kBlockInfo * b;
       const long end = (long)inf[7] + inf[1] + inf[2];
       if (end >= d->n_gptr) return fail("mcfd_mesh_set_blocks: gptr too short");
       m->max_lsq = std::max(m->max_lsq, d->gptr[end]);
-      m->max_lim = std::max(m->max_lim, d->lptr[end]);
     }
   }
   if (fused_lds_bytes(m, true) > 64 * 1024) {
