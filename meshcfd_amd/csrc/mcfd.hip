@@ -1823,11 +1823,10 @@ __device__ __forceinline__ void lds_copy(const void* src, void* dst, int n) {
 // gradients of own and ring-1 cells (k_visc_gradient's arithmetic, from LDS),
 // P2 adds each face's viscous flux (k_visc_flux's) and P3 the viscous gather
 // (k_update's), so a first-order viscous residual is one launch.
-// Waves per SIMD: the MUSCL stages without the next dt target 6 (78 VGPRs;
-// 26 KB of LDS at 128 cells), the final stage 5 (at 6 it spills 16 VGPRs),
-// the viscous stages 5 (96 VGPRs).
+// Waves per SIMD: the MUSCL stages target 6 (78 VGPRs, 80 with the next
+// dt; 26 KB of LDS at 128 cells), the viscous stages 5 (96 VGPRs).
 template <int SCHEME, bool MUSCL, int LIM, bool FARFIELD, bool DTNEXT, bool VISC = false>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MUSCL && !DTNEXT ? 6 : 5, 8)))
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MUSCL ? 6 : 5, 8)))
     k_stage_fused(int max_acc, int stage_off, int max_local, int max_grad, int max_faces, int max_lsq,
                   BlockArrays ba,
                   GeomArrays ga, FaceArrays fa, DtArrays da, double g, double ax, double ay,
@@ -1975,7 +1974,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MUS
         dmn[k] = mn[k] - wv[k];
         tol[k] = 1e-14 * vmax(fabs(wv[k]), 1.0);
       }
-      // offsets in batches of 3 (of 4, the 6-wave stages spill 2 VGPRs)
+      // offsets in batches of 3 (of 4, the 6-wave stages spill)
       const double2* __restrict__ lxy = ba.lent + ls;
       for (int base = l0; base < l1; base += 3) {
         double dxb[3], dyb[3];
