@@ -1974,12 +1974,14 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MUS
         dmn[k] = mn[k] - wv[k];
         tol[k] = 1e-14 * vmax(fabs(wv[k]), 1.0);
       }
-      // offsets in batches of 3 (of 4, the 6-wave stages spill)
+      // offsets in batches of 3, of 2 for limiter 3 (larger batches make the
+      // 6-wave stages spill)
       const double2* __restrict__ lxy = ba.lent + ls;
-      for (int base = l0; base < l1; base += 3) {
-        double dxb[3], dyb[3];
+      constexpr int kLimBatch = (LIM == 3) ? 2 : 3;
+      for (int base = l0; base < l1; base += kLimBatch) {
+        double dxb[kLimBatch], dyb[kLimBatch];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
+        for (int q = 0; q < kLimBatch; ++q) {
           int e = base + q;
           bool ok = e < l1;
           double2 d2 = ok ? lxy[e] : make_double2(0.0, 0.0);
@@ -1987,7 +1989,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MUS
           dyb[q] = d2.y;
         }
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
+        for (int q = 0; q < kLimBatch; ++q) {
         if (base + q >= l1) break;
         double dx = dxb[q], dy = dyb[q];
 #pragma unroll
